@@ -1,0 +1,169 @@
+"""Stacking throughput of the gfx950 search backend on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--cpu-seconds S]
+
+A step is one full stacking pass (every trial motion over every frame and pixel) of
+the workload's synthetic frames, already resident in HBM. The timed region is K
+steps bracketed by device synchronisation; the kernel's own duration is taken with
+HIP events on the stream the passes run on. Prints one JSON line.
+
+Workloads (sizes of the search configurations, 60 s cadence):
+  tiled-6144x4096  60 frames of 6144 x 4096, rates 0.1-2 px/min, grid_tol 0.372 px
+                   -> 19992 trial motions (the bit-exact GPU configuration; default)
+  plain-512        20 frames of 512 x 512, rates 0.2-1.1 px/min, grid_tol 0.125 px
+                   -> 5468 trial motions
+  nulls-2048       40 frames of 2048 x 2048, rates 0.2-1.1 px/min, grid_tol 0.212 px
+                   -> 7996 trial motions
+
+The CPU baseline is the oracle's cache-blocked restatement of the reference CPU
+backend, timed on this host on a band of output rows of the same workload, all
+motions, and reported in the same unit (scaled per unit).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "stacked px·frames·velocities/s (% HBM roofline); null-calibrated searches/s; p99"
+UNIT = "stacked px·frames·velocities/s"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+BYTES_PER_UNIT = 4  # one fp32 frame sample per stacked unit
+
+WORKLOADS = {
+    # name: (n, w, h, vmin_pxh, vmax_pxh, grid_tol_px)
+    "tiled-6144x4096": (60, 6144, 4096, 6.0, 120.0, 0.372),
+    "plain-512": (20, 512, 512, 12.0, 66.0, 0.125),
+    "nulls-2048": (40, 2048, 2048, 12.0, 66.0, 0.212),
+}
+
+
+def workload_shifts(name):
+    import fainttrail_amd as ft
+    n, w, h, vmin, vmax, tol = WORKLOADS[name]
+    t = (np.arange(n) - (n - 1) / 2.0) / 60.0  # hours about the mid epoch, 60 s cadence
+    grid, dv = ft.velocity_grid(float(t[-1] - t[0]), vmin, vmax, tol)
+    tab = ft.linear_shift_table(t, grid)
+    min_count = max(3, int(np.ceil(0.7 * n)))
+    return t, grid, dv, tab, min_count
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), if a summary for this workload exists."""
+    p = ROOT / "profiles" / f"stack_kernel_{workload}_traffic.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(frames_dev, tab, min_count, target_s: float):
+    """Time the oracle's blocked CPU pass on output rows in the middle of the frame."""
+    from oracle import pyoracle as po
+    n, h, w = frames_dev.shape
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    lo_dy, hi_dy = int(tab.dy.min()), int(tab.dy.max())
+
+    def window(y0, y1):
+        r0, r1 = max(0, y0 + lo_dy), min(h, y1 + hi_dy)
+        return r0, frames_dev[:, r0:r1].cpu().numpy()
+
+    y0 = h // 2
+    r0, win = window(y0, y0 + 1)
+    t0 = time.perf_counter()
+    po.stack_blocked_rows(win, h, r0, tab.dx, tab.dy, min_count, y0, y0 + 1, threads)
+    one = time.perf_counter() - t0
+    rows = int(max(1, min(h // 2, round(target_s / max(one, 1e-3)))))
+    r0, win = window(y0, y0 + rows)
+    t0 = time.perf_counter()
+    po.stack_blocked_rows(win, h, r0, tab.dx, tab.dy, min_count, y0, y0 + rows, threads)
+    secs = time.perf_counter() - t0
+    units = float(tab.motions) * n * w * rows
+    return {"value": units / secs, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"output rows {y0}-{y0 + rows - 1} of {h} (all {w} columns, all {tab.motions} motions, "
+                      f"{n} frames): {units:.3e} units in {secs:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tiled-6144x4096", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this backend has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    import fainttrail_amd as ft
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    n, w, h = WORKLOADS[args.workload][:3]
+    t, grid, dv, tab, min_count = workload_shifts(args.workload)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    frames = torch.randn((n, h, w), generator=gen, device=dev, dtype=torch.float32)
+    plan = ft.Plan(n, w, h, tab.dx, tab.dy, min_count)
+    out_m = torch.empty((h, w), dtype=torch.float32, device=dev)
+    out_b = torch.empty((h, w), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    print(f"bench: {args.workload}: {n} x {w}x{h}, {tab.motions} motions, {plan.blocks} blocks, "
+          f"{plan.units:.3e} units/pass", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        plan.run(frames, out_m, out_b, stream)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        plan.run(frames, out_m, out_b, stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_s = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / 1e3 / args.steps
+    units = plan.units
+    value = units * args.steps / wall
+    achieved = BYTES_PER_UNIT * units / kernel_s
+    print(f"bench: {wall / args.steps * 1e3:.1f} ms/pass, {value:.3e} units/s", file=sys.stderr, flush=True)
+    res = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (unit-variance fp32 frames, seed 7)",
+        "config": {"workload": args.workload, "frames": n, "width": w, "height": h, "motions": int(tab.motions),
+                   "grid_tol_px": WORKLOADS[args.workload][5], "dv_pxh": dv, "min_count": min_count,
+                   "units_per_step": units},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles(args.workload),
+                     "kernel_ms": kernel_s * 1e3,
+                     "note": "algorithmic bytes = 4 B (one fp32 sample) per stacked unit; frac > 1 is reuse "
+                             "from LDS-staged frame boxes"},
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline (oracle, blocked)", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(frames, tab, min_count, args.cpu_seconds)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,74 @@
+"""Build the gfx950 backend library (libftk_hip.so) and the CPU oracle in-tree.
+
+hipcc compiles for gfx950 only, with -ffp-contract=off (no fused multiply-add, the
+reference's choice for bit-identical results, CMakeLists.txt:56-60 and 86-91),
+denormals kept, and packed-FP32 instruction selection disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libftk_hip.so"
+
+HIP_SOURCES = ["stack_kernel.hip", "ftk_api.hip", "stack_plan.cpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build the gfx950 backend)")
+
+
+def _run(cmd: list[str], cwd: Path) -> None:
+    res = subprocess.run(cmd, cwd=str(cwd), capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ... (exit {res.returncode})")
+
+
+def build_backend() -> Path:
+    srcs = []
+    for s in HIP_SOURCES:
+        p = CSRC / s
+        # .cpp files are host code; hipcc compiles them as HIP too (-x hip) for one .so
+        srcs += (["-x", "hip", str(p)] if p.suffix == ".cpp" else [str(p)])
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-I", str(CSRC), *srcs, "-o", str(tmp)], CSRC)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8"], ROOT)
+    if with_reference and Path("/root/reference/src").is_dir():
+        _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8", "ref"], ROOT)
+
+
+def build_all() -> None:
+    build_backend()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

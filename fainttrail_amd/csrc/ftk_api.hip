@@ -1,0 +1,435 @@
+// C-ABI of the gfx950 stacking backend (include/ftk_hip.h): options, devices, the
+// synchronous drop-in pass (host frames in, host maps out), the device-resident plan,
+// and the host emulation of the kernel's algorithm.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ftk_hip.h"
+#include "stack_launch.hpp"
+#include "stack_plan.hpp"
+
+namespace ftk {
+
+namespace {
+
+void set_err(char* err, size_t len, const std::string& msg) {
+    if (!err || len == 0) return;
+    const size_t k = std::min(len - 1, msg.size());
+    std::memcpy(err, msg.data(), k);
+    err[k] = '\0';
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+}
+
+struct Cancel {};
+
+template <typename T>
+struct DeviceBuffer {
+    T* p = nullptr;
+    size_t n = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    void reserve(size_t count, const char* what) {
+        if (count <= n && p) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        check(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, count) * sizeof(T)), what);
+        n = count;
+    }
+    void upload(const std::vector<T>& v, const char* what) {
+        reserve(v.size(), what);
+        if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+    }
+};
+
+int resolve_device(int requested) {
+    if (requested >= 0) return requested;
+    if (const char* e = std::getenv("FAINTTRAIL_HIP_DEVICE")) return std::atoi(e);
+    return 0;
+}
+
+ftk_options default_options() {
+    ftk_options o;
+    o.device = -1;
+    o.max_mb = 0.0;
+    // about 10 ms of kernel time per launch at the measured stacking rate
+    o.launch_adds = 1e11;
+    return o;
+}
+
+// Plan tables on the device.
+struct DevicePlan {
+    DeviceBuffer<int> shift, box, ubox, motion;
+    DeviceBuffer<float> inv_sqrt;
+    void upload(const StackPlan& p) {
+        shift.upload(p.shift, "upload shift table");
+        box.upload(p.box, "upload boxes");
+        ubox.upload(p.ubox, "upload boxes");
+        motion.upload(p.motion, "upload motions");
+        inv_sqrt.upload(p.inv_sqrt, "upload 1/sqrt table");
+    }
+};
+
+}  // namespace
+}  // namespace ftk
+
+using namespace ftk;
+
+struct ftk_plan {
+    StackPlan plan;
+    int device = 0;
+    DevicePlan dev;
+    DeviceBuffer<unsigned long long> keys;
+};
+
+extern "C" int ftk_parse_options(const char* spec, ftk_options* out, char* err, size_t err_len) {
+    if (!out) return FTK_ERROR;
+    ftk_options o = default_options();
+    try {
+        std::stringstream ss(spec ? spec : "");
+        std::string item;
+        while (std::getline(ss, item, ',')) {
+            if (item.empty()) continue;
+            const auto eq = item.find('=');
+            const std::string key = item.substr(0, eq), val = eq == std::string::npos ? "" : item.substr(eq + 1);
+            char* end = nullptr;
+            const double d = std::strtod(val.c_str(), &end);
+            if (val.empty() || end == val.c_str() || *end != '\0')
+                throw std::runtime_error("hip backend option '" + item + "': expected key=number");
+            if (key == "device") o.device = static_cast<int>(d);
+            else if (key == "max_mb") o.max_mb = d;
+            else if (key == "launch_adds") o.launch_adds = d;
+            else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds)");
+        }
+        if (o.max_mb < 0 || o.launch_adds <= 0) throw std::runtime_error("hip backend options must be positive");
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+    *out = o;
+    return FTK_OK;
+}
+
+extern "C" int ftk_device_count(char* why, size_t why_len) {
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) {
+        set_err(why, why_len, hipGetErrorString(e));
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (count == 0) set_err(why, why_len, "no HIP device found");
+    return count;
+}
+
+extern "C" int ftk_device_description(int device, char* buf, size_t len) {
+    device = resolve_device(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        set_err(buf, len, "HIP device " + std::to_string(device) + " not present");
+        return FTK_ERROR;
+    }
+    char tmp[512];
+    std::snprintf(tmp, sizeof tmp, "HIP device %d: %s (%s), %.1f GB, %d CUs", device, prop.name, prop.gcnArchName,
+                  prop.totalGlobalMem / 1073741824.0, prop.multiProcessorCount);
+    set_err(buf, len, tmp);
+    return FTK_OK;
+}
+
+extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, float* max_snr, int32_t* best_v,
+                              ftk_timings* timings, ftk_progress_fn progress, void* user, char* err, size_t err_len) {
+    using clk = std::chrono::steady_clock;
+    const auto t_start = clk::now();
+    try {
+        if (!pass || !pass->frames) throw std::runtime_error("stack pass: missing frames");
+        if (!max_snr || !best_v) throw std::runtime_error("stack pass: missing outputs");
+        for (int i = 0; i < pass->n; ++i)
+            if (!pass->frames[i]) throw std::runtime_error("stack pass: missing frames");
+        const ftk_options opt = opt_in ? *opt_in : default_options();
+        StackPlan p = make_stack_plan(pass->n, pass->w, pass->h, pass->nv, pass->dx, pass->dy, pass->min_count);
+        const int n = p.n, w = p.w, h = p.h;
+        const int device = resolve_device(opt.device);
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count)
+            throw std::runtime_error("HIP device " + std::to_string(device) + " not present (" + std::to_string(count) + " found)");
+        check(hipSetDevice(device), "hipSetDevice");
+
+        // memory: the frame rows of a band take the budget; tables are small and fixed
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const double fixed = 4.0 * (p.shift.size() + p.box.size() + p.motion.size()) + 64.0 * 1048576.0;
+        double budget = 0.8 * static_cast<double>(free_b) - fixed;
+        if (opt.max_mb > 0) budget = std::min(budget, opt.max_mb * 1048576.0);
+        // per output row a band also holds keys (8 w B) and the two output maps (8 w B)
+        const double share = static_cast<double>(n) / (n + 4.0);
+        const std::vector<Band> bands = plan_row_bands(w, h, n, p.min_dy, p.max_dy, std::max(1.0, budget * share));
+
+        DevicePlan dev;
+        dev.upload(p);
+        DeviceBuffer<float> d_frames, d_max;
+        DeviceBuffer<int> d_best;
+        DeviceBuffer<unsigned long long> d_keys;
+        double t_up = 0, t_kernel = 0, t_down = 0;
+        const double total_work = static_cast<double>(bands.size()) * p.nb;
+        double done = 0;
+        for (const Band& b : bands) {
+            const int rows = b.r1 - b.r0, out_rows = b.y1 - b.y0;
+            const size_t np = static_cast<size_t>(out_rows) * w;
+            auto t0 = clk::now();
+            d_frames.reserve(static_cast<size_t>(n) * rows * w, "allocate frames");
+            d_keys.reserve(np, "allocate keys");
+            d_max.reserve(np, "allocate output");
+            d_best.reserve(np, "allocate output");
+            for (int i = 0; i < n; ++i)
+                check(hipMemcpy(d_frames.p + static_cast<size_t>(i) * rows * w,
+                                pass->frames[i] + static_cast<size_t>(b.r0) * w, static_cast<size_t>(rows) * w * sizeof(float),
+                                hipMemcpyHostToDevice),
+                      "upload frames");
+            auto t1 = clk::now();
+            t_up += std::chrono::duration<double>(t1 - t0).count();
+
+            plan_tiles(p, out_rows);
+            // launches of about launch_adds units: whole segments of blocks
+            const double units_per_block = static_cast<double>(out_rows) * w * n * kVel;
+            const int blocks_per_launch =
+                static_cast<int>(std::clamp(opt.launch_adds / std::max(1.0, units_per_block), 1.0, static_cast<double>(p.nb)));
+            const int bps = std::min(p.blocks_per_seg, blocks_per_launch);
+            const int nseg_total = (p.nb + bps - 1) / bps;
+            const int segs_per_launch = std::max(1, blocks_per_launch / bps);
+            check(launch_fill_keys(d_keys.p, static_cast<long long>(np), nullptr), "clear keys");
+            for (int s0 = 0; s0 < nseg_total; s0 += segs_per_launch) {
+                StackLaunch L{};
+                L.frames = d_frames.p;
+                L.frame_stride = static_cast<long long>(rows) * w;
+                L.row0 = b.r0;
+                L.row1 = b.r1;
+                L.shift = dev.shift.p;
+                L.box = dev.box.p;
+                L.ubox = dev.ubox.p;
+                L.motion = dev.motion.p;
+                L.inv_sqrt = dev.inv_sqrt.p;
+                L.y_begin = b.y0;
+                L.y_end = b.y1;
+                L.tiles_x = p.tiles_x;
+                L.tiles_y = p.tiles_y;
+                L.bps = bps;
+                L.seg0 = s0;
+                L.nseg = std::min(segs_per_launch, nseg_total - s0);
+                L.keys = d_keys.p;
+                check(launch_stack(p, L, nullptr), "launch stack kernel");
+                check(hipDeviceSynchronize(), "stack kernel");
+                done += std::min(p.nb, (s0 + L.nseg) * bps) - std::min(p.nb, s0 * bps);
+                if (progress && !progress(done / total_work, "stack", user)) throw Cancel{};
+            }
+            check(launch_finalize(d_keys.p, static_cast<long long>(np), d_max.p, d_best.p, nullptr), "finalize");
+            check(hipDeviceSynchronize(), "finalize");
+            auto t2 = clk::now();
+            t_kernel += std::chrono::duration<double>(t2 - t1).count();
+            check(hipMemcpy(max_snr + static_cast<size_t>(b.y0) * w, d_max.p, np * sizeof(float), hipMemcpyDeviceToHost),
+                  "download");
+            check(hipMemcpy(best_v + static_cast<size_t>(b.y0) * w, d_best.p, np * sizeof(int), hipMemcpyDeviceToHost),
+                  "download");
+            t_down += std::chrono::duration<double>(clk::now() - t2).count();
+        }
+        const double total = std::chrono::duration<double>(clk::now() - t_start).count();
+        if (timings) {
+            timings->upload_s = t_up;
+            timings->kernel_s = t_kernel;
+            timings->download_s = t_down;
+            timings->setup_s = std::max(0.0, total - t_up - t_kernel - t_down);
+        }
+        if (progress) progress(1.0, "stack", user);
+        return FTK_OK;
+    } catch (const Cancel&) {
+        set_err(err, err_len, "cancelled");
+        return FTK_CANCELLED;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count,
+                               const ftk_options* opt_in, ftk_plan** out, char* err, size_t err_len) {
+    if (!out) return FTK_ERROR;
+    *out = nullptr;
+    try {
+        const ftk_options opt = opt_in ? *opt_in : default_options();
+        auto* pl = new ftk_plan();
+        try {
+            pl->plan = make_stack_plan(n, w, h, nv, dx, dy, min_count);
+            pl->device = resolve_device(opt.device);
+            check(hipSetDevice(pl->device), "hipSetDevice");
+            pl->dev.upload(pl->plan);
+            pl->keys.reserve(static_cast<size_t>(w) * h, "allocate keys");
+        } catch (...) {
+            delete pl;
+            throw;
+        }
+        *out = pl;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_stride, float* d_max_snr,
+                            int32_t* d_best_v, void* stream, char* err, size_t err_len) {
+    try {
+        if (!pl || !d_frames || !d_max_snr || !d_best_v) throw std::runtime_error("ftk_plan_run: null argument");
+        const StackPlan& p = pl->plan;
+        if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_run: frame stride smaller than a frame");
+        check(hipSetDevice(pl->device), "hipSetDevice");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const long long np = static_cast<long long>(p.w) * p.h;
+        check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
+        StackLaunch L{};
+        L.frames = d_frames;
+        L.frame_stride = static_cast<long long>(frame_stride);
+        L.row0 = 0;
+        L.row1 = p.h;
+        L.shift = pl->dev.shift.p;
+        L.box = pl->dev.box.p;
+        L.ubox = pl->dev.ubox.p;
+        L.motion = pl->dev.motion.p;
+        L.inv_sqrt = pl->dev.inv_sqrt.p;
+        L.y_begin = 0;
+        L.y_end = p.h;
+        L.tiles_x = p.tiles_x;
+        L.tiles_y = p.tiles_y;
+        L.bps = p.blocks_per_seg;
+        L.seg0 = 0;
+        L.nseg = p.nseg;
+        L.keys = pl->keys.p;
+        check(launch_stack(p, L, s), "launch stack kernel");
+        check(launch_finalize(pl->keys.p, np, d_max_snr, d_best_v, s), "finalize");
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" double ftk_plan_units(const ftk_plan* pl) { return pl ? pl->plan.units : 0.0; }
+extern "C" int ftk_plan_blocks(const ftk_plan* pl) { return pl ? pl->plan.nb : 0; }
+extern "C" void ftk_plan_destroy(ftk_plan* pl) { delete pl; }
+
+extern "C" int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream) {
+    return launch_copy(d_src, d_dst, static_cast<long long>(count), static_cast<hipStream_t>(stream)) == hipSuccess
+               ? FTK_OK
+               : FTK_ERROR;
+}
+
+// ------------------------------------------------------------------ host emulation
+// The kernel's algorithm on the CPU: same plan, tiles, boxes, -0.0 padding of reads
+// outside the frame, coverage counted from the full-frame bounds only on tiles that
+// are not interior, and the packed-key maximum. Shares no code with the oracle.
+extern "C" int ftk_emulate_pass(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
+                                int min_count, float* max_snr, int32_t* best_v, char* err, size_t err_len) {
+    try {
+        StackPlan p = make_stack_plan(n, w, h, nv, dx, dy, min_count);
+        const size_t plane = static_cast<size_t>(w) * h;
+        std::vector<unsigned long long> keys(plane, 0ull);
+        const int tiles = p.tiles_x * p.tiles_y;
+        for (int t = 0; t < tiles; ++t) {
+            const int x0 = (t % p.tiles_x) * kTileX, y0 = (t / p.tiles_x) * kTileY;
+            std::vector<float> box(static_cast<size_t>(kLdsRows) * kPitch);
+            std::vector<float> acc(static_cast<size_t>(kVel) * kTileX * kTileY);
+            for (int seg = 0; seg < p.nseg; ++seg) {
+                std::vector<float> best(static_cast<size_t>(kTileX) * kTileY, -std::numeric_limits<float>::infinity());
+                std::vector<int> arg(static_cast<size_t>(kTileX) * kTileY, -1);
+                const int b0 = seg * p.blocks_per_seg, b1 = std::min(p.nb, b0 + p.blocks_per_seg);
+                for (int blk = b0; blk < b1; ++blk) {
+                    const int* ub = &p.ubox[static_cast<size_t>(blk) * 4];
+                    const bool interior = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w && y0 + ub[1] >= 0 &&
+                                          y0 + kTileY - 1 + ub[3] < h;
+                    std::fill(acc.begin(), acc.end(), 0.0f);
+                    for (int i = 0; i < n; ++i) {
+                        const int* bx = &p.box[(static_cast<size_t>(blk) * n + i) * 4];
+                        const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
+                        for (int r = 0; r < kTileY + bx[3]; ++r)
+                            for (int c = 0; c < kTileX + bx[2]; ++c) {
+                                const int gx = bx0 + c, gy = by0 + r;
+                                box[static_cast<size_t>(r) * kPitch + c] =
+                                    (gx >= 0 && gx < w && gy >= 0 && gy < h)
+                                        ? frames[static_cast<size_t>(i) * plane + static_cast<size_t>(gy) * w + gx]
+                                        : -0.0f;
+                            }
+                        const int* sh = &p.shift[(static_cast<size_t>(blk) * n + i) * 2 * kVel];
+                        for (int k = 0; k < kVel; ++k)
+                            for (int ly = 0; ly < kTileY; ++ly)
+                                for (int lx = 0; lx < kTileX; ++lx)
+                                    acc[(static_cast<size_t>(k) * kTileY + ly) * kTileX + lx] +=
+                                        box[static_cast<size_t>(ly + sh[kVel + k] - bx[1]) * kPitch + lx + sh[k] - bx[0]];
+                    }
+                    for (int k = 0; k < kVel; ++k) {
+                        const int j = p.motion[static_cast<size_t>(blk) * kVel + k];
+                        if (j < 0) continue;
+                        for (int ly = 0; ly < kTileY; ++ly)
+                            for (int lx = 0; lx < kTileX; ++lx) {
+                                int c = n;
+                                if (!interior) {
+                                    c = 0;
+                                    for (int i = 0; i < n; ++i) {
+                                        const int* sh = &p.shift[(static_cast<size_t>(blk) * n + i) * 2 * kVel];
+                                        const int sx = x0 + lx + sh[k], sy = y0 + ly + sh[kVel + k];
+                                        c += (sx >= 0 && sx < w && sy >= 0 && sy < h) ? 1 : 0;
+                                    }
+                                }
+                                if (c < min_count) continue;
+                                const float snr = acc[(static_cast<size_t>(k) * kTileY + ly) * kTileX + lx] * p.inv_sqrt[static_cast<size_t>(c)];
+                                const size_t o = static_cast<size_t>(ly) * kTileX + lx;
+                                if (snr > best[o] || (snr == best[o] && j < arg[o])) {
+                                    best[o] = snr;
+                                    arg[o] = j;
+                                }
+                            }
+                    }
+                }
+                for (int ly = 0; ly < kTileY; ++ly)
+                    for (int lx = 0; lx < kTileX; ++lx) {
+                        const int x = x0 + lx, y = y0 + ly;
+                        const size_t o = static_cast<size_t>(ly) * kTileX + lx;
+                        if (arg[o] < 0 || x >= w || y >= h) continue;
+                        const unsigned long long key = pack_key(best[o], arg[o]);
+                        unsigned long long& dst = keys[static_cast<size_t>(y) * w + x];
+                        dst = std::max(dst, key);
+                    }
+            }
+        }
+        for (size_t q = 0; q < plane; ++q) {
+            if (keys[q] == 0ull) {
+                max_snr[q] = 0.0f;
+                best_v[q] = -1;
+                continue;
+            }
+            const float v = from_ordered_bits(static_cast<uint32_t>(keys[q] >> 32));
+            max_snr[q] = std::isfinite(v) ? v : 0.0f;
+            best_v[q] = static_cast<int32_t>(~static_cast<uint32_t>(keys[q] & 0xffffffffull));
+        }
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}

@@ -1,0 +1,78 @@
+// Host planning for the gfx950 stacking kernel: velocity blocks, per-frame staging
+// boxes and work segments. Plain C++17 (no HIP), shared by the device launcher and
+// the host emulation.
+//
+// Work decomposition. The output is cut into tiles of kTileX x kTileY pixels (one
+// wavefront spans a tile row of 64 pixels; each of the 4 wavefronts owns
+// kRowsPerWave rows). The trial motions are grouped into blocks of kVel motions
+// whose shifts lie close together in every frame, so one staged frame box
+// (tile + the block's shift spread) in LDS serves all kVel motions. A workgroup
+// takes one tile and one segment of consecutive blocks, keeps the best S/N per
+// pixel in registers across the segment, and merges it into a packed 64-bit key
+// per pixel with one atomic max (value, then lowest motion index).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ftk {
+
+constexpr int kTileX = 64;                     // pixels per tile row = wavefront width
+constexpr int kWaves = 4;                      // wavefronts per workgroup
+constexpr int kRowsPerWave = 4;                // tile rows per wavefront
+constexpr int kTileY = kWaves * kRowsPerWave;  // 16
+constexpr int kVel = 16;                       // motions per block
+constexpr int kSpreadMax = 32;                 // largest per-frame shift spread of a block, each axis
+constexpr int kPitch = kTileX + kSpreadMax;    // LDS row pitch (floats)
+constexpr int kLdsRows = kTileY + kSpreadMax;  // LDS rows per staged frame
+constexpr int kThreads = kTileX * kWaves;      // 256
+
+struct StackPlan {
+    int n = 0, w = 0, h = 0, nv = 0, min_count = 1;
+    int nb = 0;                      // velocity blocks
+    std::vector<int32_t> motion;     // nb x kVel motion indices, ascending; -1 = padding
+    std::vector<int32_t> shift;      // nb x n x (kVel dx, kVel dy), clamped to +-w, +-h
+    std::vector<int32_t> box;        // nb x n x 4: min dx, min dy, spread x, spread y
+    std::vector<int32_t> ubox;       // nb x 4: min dx, min dy, max dx, max dy over all frames
+    std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))
+    int min_dy = 0, max_dy = 0;      // extremes of the clamped vertical shifts
+    int tiles_x = 0, tiles_y = 0;
+    int blocks_per_seg = 1, nseg = 1;
+    double units = 0.0;              // nv * n * w * h
+};
+
+// Validates like validate_pass (src/backend.cpp:89-108) and builds the plan.
+// Throws std::runtime_error with the reference's messages.
+StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count);
+
+// Tile rows for output rows [y0, y1) (row bands) and the segment size for a grid.
+void plan_tiles(StackPlan& p, int out_rows);
+
+// Packed (value, lowest index) key: larger key = larger S/N, then lower index.
+// Never called with NaN; 0 means "no defined motion".
+inline uint32_t ordered_bits(float f) {
+    uint32_t u;
+    static_assert(sizeof(u) == sizeof(f), "float size");
+    __builtin_memcpy(&u, &f, sizeof u);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+inline float from_ordered_bits(uint32_t o) {
+    const uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    float f;
+    __builtin_memcpy(&f, &u, sizeof f);
+    return f;
+}
+inline uint64_t pack_key(float snr, int32_t j) {
+    return (static_cast<uint64_t>(ordered_bits(snr)) << 32) | static_cast<uint32_t>(~static_cast<uint32_t>(j));
+}
+
+// Row bands whose frame rows fit a byte budget (the reference's plan_bands,
+// src/cuda_stack_core.hpp:90-106, same rule).
+struct Band {
+    int y0 = 0, y1 = 0;  // output rows
+    int r0 = 0, r1 = 0;  // frame rows needed
+};
+std::vector<Band> plan_row_bands(int w, int h, int n, int min_dy, int max_dy, double budget_bytes);
+
+}  // namespace ftk

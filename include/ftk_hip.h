@@ -1,0 +1,119 @@
+/* ftk_hip.h - C-ABI of the MI355X (gfx950) FaintTrail search backend.
+ *
+ * Plain C: POD structs, plain pointers and sizes, integer status codes and a
+ * caller-supplied error buffer. No C++ or torch types cross this boundary.
+ *
+ * Status codes mirror the reference C API (include/fainttrail/fainttrail.h:26-28):
+ * FTK_OK, FTK_ERROR (err receives the message; validation messages follow
+ * validate_pass, src/backend.cpp:89-108) and FTK_CANCELLED (the progress callback
+ * returned 0; the reference throws Cancelled, include/fainttrail/types.hpp:70-72).
+ *
+ * Layouts:
+ *   frames  n frames of w x h row-major fp32 (the reference's Image, types.hpp:20-34)
+ *   shifts  motion-major int32: motion j, frame i at j * n + i (ShiftTable,
+ *           include/fainttrail/backend.hpp:56-69)
+ *   results max_snr fp32 and best_v int32, w x h row-major (StackResult,
+ *           backend.hpp:99-105): 0 / -1 where fewer than min_count frames cover.
+ */
+#ifndef FTK_HIP_H
+#define FTK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTK_OK 0
+#define FTK_ERROR 1
+#define FTK_CANCELLED 2
+
+/* Progress in [0,1] and a stage name; return 0 to cancel.
+ * Same shape as ft_progress_fn (fainttrail.h:71) and ProgressFn (types.hpp:76). */
+typedef int (*ftk_progress_fn)(double fraction, const char* stage, void* user);
+
+/* Backend options, "hip:device=N,max_mb=M,launch_adds=A".
+ * Replaces CudaOptions / parse_cuda_options (src/cuda_plan.hpp:27-32,
+ * src/cuda_plan.cpp:14-33): device -1 = FAINTTRAIL_HIP_DEVICE or 0; max_mb caps the
+ * device memory for frame rows (0 = 80 % of free memory; small values force row
+ * bands); launch_adds = stacked units per kernel launch (progress and cancel
+ * granularity). */
+typedef struct ftk_options {
+    int device;
+    double max_mb;
+    double launch_adds;
+} ftk_options;
+
+/* Parses the part after "hip:" (NULL or "" = defaults). Unknown keys or
+ * non-numbers are errors, as in parse_cuda_options. */
+int ftk_parse_options(const char* spec, ftk_options* out, char* err, size_t err_len);
+
+/* Number of usable HIP devices (0 when none; why receives the reason).
+ * Replaces cuda_device_count (src/backend_cuda.cu:198-209). */
+int ftk_device_count(char* why, size_t why_len);
+
+/* Human-readable device description for SearchBackend::description()
+ * (backend_cuda.cu:89-94). */
+int ftk_device_description(int device, char* buf, size_t len);
+
+/* One stacking pass: the inputs of StackPass (backend.hpp:71-78) with the shifts
+ * always given as a table (a straight-line grid arrives as linear_shift_table,
+ * backend.cpp:114-123, which the contract defines as identical). */
+typedef struct ftk_pass {
+    int n, w, h;
+    const float* const* frames; /* n host pointers, each w * h fp32, read-only */
+    int nv;                     /* trial motions                               */
+    const int32_t* dx;          /* nv * n, motion-major                        */
+    const int32_t* dy;
+    int min_count;
+} ftk_pass;
+
+/* Sub-timings reported in StackResult::timings (backend_cuda.cu:189-192). */
+typedef struct ftk_timings {
+    double upload_s, kernel_s, download_s, setup_s;
+} ftk_timings;
+
+/* SearchBackend::run for the "hip" backend (backend.hpp:107-117; replaces
+ * CudaBackend::run, backend_cuda.cu:99-195). Synchronous; host in, host out.
+ * max_snr and best_v are caller-allocated w * h arrays. Bit-identical to the CPU
+ * backend (backend_cpu.cpp:77-204). timings / progress may be NULL. */
+int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt, float* max_snr, int32_t* best_v,
+                   ftk_timings* timings, ftk_progress_fn progress, void* user, char* err, size_t err_len);
+
+/* ------------------------------------------------------------------ device-resident form
+ * No reference counterpart: the reference uploads the frames on every pass
+ * (backend_cuda.cu:146-150). A plan holds the velocity blocks, shift tables and
+ * scratch on the device; ftk_plan_run stacks frames that are already in HBM.
+ * Used by the batched search and by callers that keep frames resident. */
+typedef struct ftk_plan ftk_plan;
+
+int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count,
+                    const ftk_options* opt, ftk_plan** out, char* err, size_t err_len);
+/* d_frames: n frames of w x h fp32 on the device, frame i at d_frames + i * frame_stride.
+ * d_max_snr / d_best_v: w x h device outputs. stream: a hipStream_t (NULL = default).
+ * Asynchronous with respect to the host. */
+int ftk_plan_run(ftk_plan* plan, const float* d_frames, size_t frame_stride, float* d_max_snr, int32_t* d_best_v,
+                 void* stream, char* err, size_t err_len);
+/* Units (motions x frames x pixels) one ftk_plan_run stacks, and the number of
+ * kernel launches it makes. */
+double ftk_plan_units(const ftk_plan* plan);
+int ftk_plan_blocks(const ftk_plan* plan);
+void ftk_plan_destroy(ftk_plan* plan);
+
+/* Plain device copy (n floats) on `stream`: the known-size kernel that calibrates
+ * HBM traffic counters. */
+int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream);
+
+/* Host emulation of the HIP algorithm (tiles, velocity blocks, -0.0 padding, edge
+ * coverage counting, packed-key maximum) on the CPU, for tests without a GPU; the
+ * analogue of emulate_cuda_pass (src/cuda_plan.cpp:74-117). frames: n x h x w
+ * contiguous. */
+int ftk_emulate_pass(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
+                     int min_count, float* max_snr, int32_t* best_v, char* err, size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTK_HIP_H */

@@ -1,0 +1,253 @@
+// Plain-C entry points over the reference's own C++ API (TEST INFRASTRUCTURE ONLY).
+//
+// Linked with the reference's sources, compiled where they lie (oracle/Makefile,
+// target `ref`), into oracle/_ref/libftref.so. Tests use it to check the oracle's
+// restatement against the reference itself and to write golden vectors; it never
+// enters the product. Each function calls the reference function named in its
+// comment and converts between flat arrays and the reference's types.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "backend_cpu.hpp"
+#include "cuda_plan.hpp"
+#include "fainttrail/backend.hpp"
+#include "fainttrail/search.hpp"
+#include "fainttrail/simulate.hpp"
+#include "internal.hpp"
+
+using namespace fainttrail;
+
+namespace {
+
+std::vector<Image> to_images(int n, int w, int h, const float* frames) {
+    std::vector<Image> v(static_cast<size_t>(n), Image(w, h));
+    const size_t plane = static_cast<size_t>(w) * h;
+    for (int i = 0; i < n; ++i) std::memcpy(v[static_cast<size_t>(i)].data.data(), frames + i * plane, plane * sizeof(float));
+    return v;
+}
+
+ShiftTable to_table(int n, int nv, const int32_t* dx, const int32_t* dy) {
+    ShiftTable t;
+    t.resize(nv, n);
+    std::copy(dx, dx + static_cast<size_t>(nv) * n, t.dx.begin());
+    std::copy(dy, dy + static_cast<size_t>(nv) * n, t.dy.begin());
+    return t;
+}
+
+void from_result(const StackResult& r, float* max_snr, int32_t* best_v) {
+    std::copy(r.max_snr.data.begin(), r.max_snr.data.end(), max_snr);
+    std::copy(r.best_v.begin(), r.best_v.end(), best_v);
+}
+
+}  // namespace
+
+extern "C" {
+
+// run_pass_naive (src/backend.cpp:148-172), explicit shift table
+int ref_stack_naive(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
+                    int min_count, float* max_snr, int32_t* best_v) {
+    const auto f = to_images(n, w, h, frames);
+    const auto t = to_table(n, nv, dx, dy);
+    StackPass p;
+    p.frames = &f;
+    p.shifts = &t;
+    p.min_count = min_count;
+    from_result(run_pass_naive(p), max_snr, best_v);
+    return 0;
+}
+
+// run_pass_simple (src/backend.cpp:178-278)
+int ref_stack_simple(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
+                     int min_count, int threads, float* max_snr, int32_t* best_v) {
+    const auto f = to_images(n, w, h, frames);
+    const auto t = to_table(n, nv, dx, dy);
+    StackPass p;
+    p.frames = &f;
+    p.shifts = &t;
+    p.min_count = min_count;
+    p.threads = threads;
+    from_result(run_pass_simple(p), max_snr, best_v);
+    return 0;
+}
+
+// CpuBackend::run (src/backend_cpu.cpp:77-204) with explicit blocking (0 = automatic)
+int ref_stack_cpu(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy, int min_count,
+                  int threads, int band_rows, int vel_block, float* max_snr, int32_t* best_v) {
+    const auto f = to_images(n, w, h, frames);
+    const auto t = to_table(n, nv, dx, dy);
+    StackPass p;
+    p.frames = &f;
+    p.shifts = &t;
+    p.min_count = min_count;
+    p.threads = threads;
+    detail::CpuBackend be(detail::CpuBlocking{band_rows, vel_block});
+    from_result(be.run(p), max_snr, best_v);
+    return 0;
+}
+
+// emulate_cuda_pass (src/cuda_plan.cpp:74-117)
+int ref_emulate_cuda(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
+                     int min_count, double budget, double launch_adds, float* max_snr, int32_t* best_v) {
+    const auto f = to_images(n, w, h, frames);
+    const auto t = to_table(n, nv, dx, dy);
+    StackPass p;
+    p.frames = &f;
+    p.shifts = &t;
+    p.min_count = min_count;
+    from_result(detail::emulate_cuda_pass(p, budget, launch_adds), max_snr, best_v);
+    return 0;
+}
+
+// linear_shift_table (src/backend.cpp:114-123)
+void ref_linear_shifts(int n, const double* t_h, int nv, const double* vxy, int32_t* dx, int32_t* dy) {
+    std::vector<double> t(t_h, t_h + n);
+    std::vector<std::pair<double, double>> v(static_cast<size_t>(nv));
+    for (int j = 0; j < nv; ++j) v[static_cast<size_t>(j)] = {vxy[2 * j], vxy[2 * j + 1]};
+    const ShiftTable s = linear_shift_table(t, v);
+    std::copy(s.dx.begin(), s.dx.end(), dx);
+    std::copy(s.dy.begin(), s.dy.end(), dy);
+}
+
+// velocity_grid (src/search.cpp:39-58)
+int ref_velocity_grid(double span_h, double vmin, double vmax, double tol, double* vxy, int cap, double* dv) {
+    Prepared p;
+    p.span_h = span_h;
+    SearchParams s;
+    s.vmin_pxh = vmin;
+    s.vmax_pxh = vmax;
+    s.grid_tol_px = tol;
+    try {
+        const auto g = velocity_grid(p, s, dv);
+        for (size_t k = 0; k < g.size() && static_cast<int>(k) < cap; ++k) {
+            vxy[2 * k] = g[k].first;
+            vxy[2 * k + 1] = g[k].second;
+        }
+        return static_cast<int>(g.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+// null_time_orders (src/search.cpp:559-571)
+void ref_null_time_orders(const double* t_h, int n, int runs, uint64_t seed, double* out) {
+    const std::vector<double> t(t_h, t_h + n);
+    const auto o = detail::null_time_orders(t, runs, seed);
+    for (int k = 0; k < runs; ++k) std::copy(o[static_cast<size_t>(k)].begin(), o[static_cast<size_t>(k)].end(), out + static_cast<size_t>(k) * n);
+}
+
+// find_peaks (src/search.cpp:105-135), sorted and truncated
+int ref_find_peaks(const float* max_snr, const int32_t* best_v, int w, int h, float floor, int radius, int edge,
+                   int max_peaks, int32_t* px, int32_t* py, float* psnr, int32_t* pv) {
+    StackResult r;
+    r.max_snr = Image(w, h);
+    std::copy(max_snr, max_snr + static_cast<size_t>(w) * h, r.max_snr.data.begin());
+    r.best_v.assign(best_v, best_v + static_cast<size_t>(w) * h);
+    const auto pk = detail::find_peaks(r, floor, radius, edge, static_cast<size_t>(max_peaks));
+    for (size_t k = 0; k < pk.size(); ++k) {
+        px[k] = pk[k].x;
+        py[k] = pk[k].y;
+        psnr[k] = pk[k].snr;
+        pv[k] = pk[k].vidx;
+    }
+    return static_cast<int>(pk.size());
+}
+
+// set_threshold (src/search.cpp:573-612), automatic calibration with `runs` null runs
+double ref_null_threshold(const float* pooled, int count, int runs, double target, double floor) {
+    Prepared p;
+    SearchParams s;
+    s.target_false_per_run = target;
+    s.min_snr_floor = floor;
+    SearchResult res;
+    const std::vector<float> v(pooled, pooled + count);
+    detail::set_threshold(p, s, 1000, runs, v, res);
+    return res.threshold_snr;
+}
+
+// build_stage (src/preprocess.cpp:286-486), untiled, with a given PSF FWHM
+int ref_build_stage(int n, int w, int h, const float* sub, const double* t_h, double fwhm_px, int static_exclude,
+                    float* filtered, float* static_sky, double* sigma, float* mask, int32_t* cr_pixels) {
+    const auto subs = to_images(n, w, h, sub);
+    Prepared p;
+    p.width = w;
+    p.height = h;
+    p.nframes = n;
+    p.t_h.assign(t_h, t_h + n);
+    p.fwhm_px = fwhm_px;
+    SearchParams s;
+    s.static_exclude = static_exclude;
+    s.fwhm_px = fwhm_px;
+    detail::build_stage(subs, p, s, {}, false);
+    const size_t plane = static_cast<size_t>(w) * h;
+    for (int i = 0; i < n; ++i) std::copy(p.filtered[static_cast<size_t>(i)].data.begin(), p.filtered[static_cast<size_t>(i)].data.end(), filtered + i * plane);
+    if (static_sky) std::copy(p.static_sky.data.begin(), p.static_sky.data.end(), static_sky);
+    if (sigma) std::copy(p.sigma.begin(), p.sigma.end(), sigma);
+    if (mask) std::copy(p.mask.data.begin(), p.mask.data.end(), mask);
+    if (cr_pixels) std::copy(p.cr_pixels.begin(), p.cr_pixels.end(), cr_pixels);
+    return 0;
+}
+
+// prepare (src/preprocess.cpp:589-650) with align = "off": background, blank growth,
+// build_stage. Outputs sub (n planes), background (n) and filtered (n planes).
+int ref_prepare(int n, int w, int h, const float* frames, const double* mjd, double fwhm_px, float* sub,
+                double* background, float* filtered, double* t_h) {
+    std::vector<Frame> fr(static_cast<size_t>(n));
+    const size_t plane = static_cast<size_t>(w) * h;
+    for (int i = 0; i < n; ++i) {
+        fr[static_cast<size_t>(i)].image = Image(w, h);
+        std::memcpy(fr[static_cast<size_t>(i)].image.data.data(), frames + i * plane, plane * sizeof(float));
+        fr[static_cast<size_t>(i)].mjd_mid = mjd[i];
+        fr[static_cast<size_t>(i)].exptime_s = 60.0;
+    }
+    SearchParams s;
+    s.align = "off";
+    s.fwhm_px = fwhm_px;
+    const Prepared p = prepare(fr, s);
+    for (int i = 0; i < n; ++i) {
+        if (sub) std::copy(p.sub[static_cast<size_t>(i)].data.begin(), p.sub[static_cast<size_t>(i)].data.end(), sub + i * plane);
+        if (filtered) std::copy(p.filtered[static_cast<size_t>(i)].data.begin(), p.filtered[static_cast<size_t>(i)].data.end(), filtered + i * plane);
+        if (background) background[i] = p.background[static_cast<size_t>(i)];
+        if (t_h) t_h[i] = p.t_h[static_cast<size_t>(i)];
+    }
+    return 0;
+}
+
+// The inputs of `fainttrail check-backend` (cli/main.cpp:697-742): simulate() with
+// seed 77 + round, prepare() with vmin 2 and vmax 20 / 45, velocity_grid. Returns the
+// number of motions; frames (n planes of the filtered images), t_h (n) and vxy
+// (2 per motion, up to cap) are written when non-NULL. dims receives w, h, n,
+// min_count.
+int ref_check_backend_inputs(int round, int* dims, float* frames, double* t_h, double* vxy, int cap) {
+    SimParams sp;
+    sp.width = round == 0 ? 72 : 384;
+    sp.height = round == 0 ? 52 : 320;
+    sp.nframes = round == 0 ? 12 : 40;
+    sp.nstars = round == 0 ? 20 : 400;
+    sp.seed = 77 + static_cast<std::uint64_t>(round);
+    const SimResult sim = simulate(sp);
+    SearchParams p;
+    p.vmin_pxh = 2;
+    p.vmax_pxh = round == 0 ? 20 : 45;
+    const Prepared prep = prepare(sim.frames, p);
+    const auto grid = velocity_grid(prep, p);
+    dims[0] = prep.width;
+    dims[1] = prep.height;
+    dims[2] = prep.nframes;
+    dims[3] = std::max(3, static_cast<int>(std::ceil(p.min_coverage * prep.nframes)));
+    const size_t plane = static_cast<size_t>(prep.width) * prep.height;
+    if (frames)
+        for (int i = 0; i < prep.nframes; ++i)
+            std::copy(prep.filtered[static_cast<size_t>(i)].data.begin(), prep.filtered[static_cast<size_t>(i)].data.end(), frames + i * plane);
+    if (t_h) std::copy(prep.t_h.begin(), prep.t_h.end(), t_h);
+    if (vxy)
+        for (size_t k = 0; k < grid.size() && static_cast<int>(k) < cap; ++k) {
+            vxy[2 * k] = grid[k].first;
+            vxy[2 * k + 1] = grid[k].second;
+        }
+    return static_cast<int>(grid.size());
+}
+
+}  // extern "C"

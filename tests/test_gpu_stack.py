@@ -1,0 +1,127 @@
+"""Parity of the gfx950 stacking kernel with the reference CPU backend, through the
+C-ABI (ftk_stack_pass / ftk_plan_run). Bit-identical maps and motion indices are the
+bar: the reference's check-backend reports "bit-identical in 4 of 4 checks" for its
+GPU backend (cli/main.cpp:676-746), and so must this one."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from refcases import same_bits
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["contract-random", "contract-constant", "shift-linear", "shift-curved", "cuda-linear", "cuda-table"]
+
+
+def case(golden, name):
+    return {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+
+
+def run_hip(ft, frames, dx, dy, min_count, options=""):
+    be = ft.make_backend("hip:" + options if options else "hip")
+    tab = ft.ShiftTable.make(dx, dy)
+    r = be.run(ft.StackPass(frames=frames, shifts=tab, min_count=int(min_count)))
+    return r.max_snr, r.best_v
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("options", ["", "max_mb=0.01,launch_adds=1", "max_mb=0.003,launch_adds=3000"])
+def test_reference_test_inputs_bit_identical(hip, golden, name, options):
+    """test_backend_contract / test_shift_tables / test_cuda_algorithm inputs (NaN,
+    shifts beyond the frame, 171 motions, curved tables), unbanded and forced into
+    row bands with tiny launches (test_main.cpp:602-609 runs "cuda:max_mb=0.01,
+    launch_adds=1")."""
+    g = case(golden, name)
+    print(f"case {name} options '{options}'", flush=True)
+    m, b = run_hip(hip, g["frames"], g["dx"], g["dy"], g["min_count"], options)
+    assert same_bits(m, g["max_snr"]), name
+    assert np.array_equal(b, g["best_v"]), name
+
+
+def test_check_backend_round0_naive_and_curved(hip, golden):
+    """check-backend checks 1 and 2 (cli/main.cpp:713-720): 72x52, 12 frames vs the
+    naive contract, straight and curved tables."""
+    fr, mc = golden["check0/frames"], int(golden["check0/min_count"])
+    for key in ("check0", "check0c"):
+        m, b = run_hip(hip, fr, golden[f"{key}/dx"], golden[f"{key}/dy"], mc)
+        assert same_bits(m, golden[f"{key}/max_snr"]) and np.array_equal(b, golden[f"{key}/best_v"]), key
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_check_backend_round1_simple_and_bands(hip):
+    """check-backend checks 3 and 4 (cli/main.cpp:721-741): 384x320, 40 frames vs
+    run_pass_simple, and the same forced into row bands (cap 2 MB, 2e6 units/launch)."""
+    fr, t, v, mc = po.ref_check_backend_inputs(1)
+    dx, dy = po.linear_shifts(t, v)
+    rm, rb = po.ref_stack("simple", fr, dx, dy, mc)
+    for opts in ("", "max_mb=2,launch_adds=2e6"):
+        print(f"check-backend round 1 options '{opts}'", flush=True)
+        m, b = run_hip(hip, fr, dx, dy, mc, opts)
+        assert same_bits(m, rm) and np.array_equal(b, rb), opts
+        chk = po.check_backend(fr, dx, dy, mc, (rm, rb), (m, b))
+        assert chk["ok"] and chk["max_abs_diff"] == 0.0
+
+
+def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
+    """512 x 512, 20 frames, the 5468-motion grid of the 512^2 configuration
+    (dv * span = 0.5 px), unit-variance frames: device-resident plan and the host
+    pass both equal the oracle's blocked CPU backend bit for bit."""
+    import torch
+    n, h, w = 20, 512, 512
+    rng = np.random.default_rng(7)
+    frames = rng.standard_normal((n, h, w), dtype=np.float32)
+    t = (np.arange(n) - (n - 1) / 2.0) / 60.0
+    grid, dv = hip.velocity_grid(t[-1] - t[0], 12.0, 66.0, 0.125)
+    assert grid.shape[0] == 5468
+    tab = hip.linear_shift_table(t, grid)
+    mc = max(3, int(np.ceil(0.7 * n)))
+    print("cfg1 shape: oracle", flush=True)
+    om, ob = po.stack_blocked(frames, tab.dx, tab.dy, mc)
+    print("cfg1 shape: device plan", flush=True)
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    d = torch.from_numpy(frames).cuda()
+    m, b = plan.run(d)
+    torch.cuda.synchronize()
+    assert same_bits(m.cpu().numpy(), om) and np.array_equal(b.cpu().numpy(), ob)
+    print("cfg1 shape: host pass", flush=True)
+    m2, b2 = run_hip(hip, frames, tab.dx, tab.dy, mc)
+    assert same_bits(m2, om) and np.array_equal(b2, ob)
+
+
+def test_plan_reuse_and_stride(hip, golden):
+    """A plan runs repeatedly on resident frames; a padded frame stride gives the same maps."""
+    import torch
+    g = case(golden, "cuda-linear")
+    fr = g["frames"]
+    n, h, w = fr.shape
+    plan = hip.Plan(n, w, h, g["dx"], g["dy"], int(g["min_count"]))
+    big = torch.zeros((n, h * w + 77), dtype=torch.float32, device="cuda")
+    big[:, : h * w] = torch.from_numpy(fr.reshape(n, -1)).cuda()
+    m = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    b = torch.empty((h, w), dtype=torch.int32, device="cuda")
+    for _ in range(3):
+        plan.run_ptr(big.data_ptr(), h * w + 77, m.data_ptr(), b.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert same_bits(m.cpu().numpy(), g["max_snr"]) and np.array_equal(b.cpu().numpy(), g["best_v"])
+
+
+def test_cancellation_between_launches(hip, golden):
+    """A progress callback returning false cancels (types.hpp:70-76; backend_cuda.cu:172)."""
+    g = case(golden, "contract-random")
+    be = hip.make_backend("hip:launch_adds=1")
+    calls = []
+
+    def progress(f, stage):
+        calls.append(f)
+        return len(calls) < 3
+
+    with pytest.raises(hip.Cancelled):
+        be.run(hip.StackPass(frames=g["frames"], shifts=hip.ShiftTable.make(g["dx"], g["dy"]), min_count=5), progress)
+    assert len(calls) == 3
+
+
+def test_missing_device_is_an_error(hip, golden):
+    g = case(golden, "contract-random")
+    be = hip.make_backend("hip:device=57")
+    with pytest.raises(hip.FtkError, match="not present"):
+        be.run(hip.StackPass(frames=g["frames"], shifts=hip.ShiftTable.make(g["dx"], g["dy"]), min_count=5))
