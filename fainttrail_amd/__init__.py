@@ -64,6 +64,13 @@ def lib() -> C.CDLL:
         return _lib
     if not _LIB_PATH.exists():
         raise FtkError(f"{_LIB_PATH} is not built; run python -c 'import __graft_entry__ as g; g.build()'")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7. Loading torch
+    # first makes the dynamic loader bind this library to that same copy (equal
+    # SONAME); loading /opt/rocm's copy first leaves torch without a usable device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(_LIB_PATH))
     L.ftk_parse_options.argtypes = [C.c_char_p, C.POINTER(_Options), C.c_char_p, C.c_size_t]
     L.ftk_device_count.argtypes = [C.c_char_p, C.c_size_t]
@@ -79,6 +86,7 @@ def lib() -> C.CDLL:
     L.ftk_plan_blocks.argtypes = [C.c_void_p]
     L.ftk_plan_destroy.argtypes = [C.c_void_p]
     L.ftk_copy_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    L.ftk_probe_lds.argtypes = [C.c_int, C.POINTER(C.c_double)]
     L.ftk_emulate_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32),
                                    C.c_char_p, C.c_size_t]
@@ -103,6 +111,14 @@ def parse_options(spec: str = "") -> _Options:
     o, e = _Options(), _err()
     _raise(lib().ftk_parse_options(spec.encode(), C.byref(o), e, len(e)), e)
     return o
+
+
+def probe_lds(width_bytes: int = 4) -> float:
+    """Measured LDS read bandwidth of the current device, bytes/s (ftk_probe_lds)."""
+    v = C.c_double()
+    if lib().ftk_probe_lds(int(width_bytes), C.byref(v)) != FTK_OK:
+        raise FtkError("LDS probe failed")
+    return v.value
 
 
 def device_count() -> int:

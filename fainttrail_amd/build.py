@@ -17,7 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libftk_hip.so"
 
-HIP_SOURCES = ["stack_kernel.hip", "ftk_api.hip", "stack_plan.cpp"]
+HIP_SOURCES = ["stack_kernel.hip", "probe_kernel.hip", "prep_kernels.hip", "ftk_api.hip", "ftk_prep_api.hip", "stack_plan.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -61,7 +61,7 @@ def build_backend() -> Path:
 def build_oracle(with_reference: bool = True) -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8"], ROOT)
     if with_reference and Path("/root/reference/src").is_dir():
-        _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8", "ref"], ROOT)
+        _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8", "ref", "cli"], ROOT)
 
 
 def build_all() -> None:

@@ -1,0 +1,73 @@
+// Device preprocessing (build_stage and the per-frame background) for gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace ftk {
+
+// One segment of a per-frame order-statistic selection: the regular sample grid of an
+// image (sample_pixels, src/common.hpp:85-100) and the selection state.
+struct SelectSeg {
+    const float* img = nullptr;  // w x h image
+    int w = 0;
+    int border = 0, step = 1, nx = 0, ny = 0;
+    int count = 0;               // valid samples
+    int k[2] = {0, 0};           // remaining ranks of the two targets
+    uint32_t prefix[2] = {0u, 0u};
+    double center = 0.0;         // median (deviation mode centre)
+    double result = 0.0;         // median, or robust sigma
+};
+
+// Parameters of build_stage (SearchParams, include/fainttrail/search.hpp:18-90).
+struct StageParams {
+    double fwhm_px = 3.0;
+    double clip_sigma = 8.0;
+    double cr_sigma = 5.0;
+    int static_exclude = -1;
+    double mask_scatter = 2.5;
+    double weight_scatter = 1.5;
+};
+
+// Scratch and launch sequence for one frame stack of n x h x w.
+class DeviceStage {
+public:
+    DeviceStage(int n, int w, int h);
+    ~DeviceStage();
+    DeviceStage(const DeviceStage&) = delete;
+    DeviceStage& operator=(const DeviceStage&) = delete;
+
+    // Per-frame background (median of the sample grid) and background-subtracted frames
+    // with grown blank areas (prepare_core, src/preprocess.cpp:568-581).
+    void background(const float* raw, long long raw_stride, float* sub, long long sub_stride, double* bg_host,
+                    hipStream_t s);
+
+    // build_stage for the frame times t_h (only their order matters): writes the
+    // normalised matched-filtered frames; optional outputs may be null.
+    void build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm, float* filtered,
+               long long filt_stride, float* static_sky, double* sigma_host, float* mask_out, int* cr_host,
+               hipStream_t s);
+
+    int n, w, h;
+    long long plane;
+
+private:
+    void select(std::vector<SelectSeg>& segs, const float* mask, bool deviation, int unit_floor, hipStream_t s);
+    float* resid_ = nullptr;     // n planes: residuals, then z, then the row-filtered frames
+    unsigned char* flags_ = nullptr;  // n planes of cosmic-ray hit flags / blank flags
+    float* sky_ = nullptr;
+    float* scatter_ = nullptr;
+    float* mask_ = nullptr;
+    SelectSeg* seg_ = nullptr;
+    unsigned int* hist_ = nullptr;
+    float* inv_ = nullptr;
+    double* sigma_ = nullptr;
+    int* cr_ = nullptr;
+    int* rank_ = nullptr;
+    int* any_ = nullptr;
+    float* taps_ = nullptr;
+};
+
+}  // namespace ftk

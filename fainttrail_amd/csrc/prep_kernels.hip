@@ -1,0 +1,625 @@
+// gfx950 kernels of the time-order-dependent preprocessing stage (build_stage,
+// src/preprocess.cpp:286-486) and the per-frame background (:528-585), bit-identical
+// to the reference's CPU arithmetic:
+//   * every float/double conversion sits where the reference has it;
+//   * order statistics are exact whatever the algorithm (per-pixel rank counting in
+//     LDS, per-frame radix select on ordered float bits), so any exact selection
+//     reproduces std::sort / std::nth_element results;
+//   * transcendental constants (filter taps, the cosmic-ray ratio, 1/sigma) come from
+//     the host (glibc); device code uses only IEEE + - * / and comparisons, built with
+//     -ffp-contract=off and correctly rounded fp32 division.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "prep.hpp"
+
+namespace ftk {
+
+namespace {
+
+__device__ inline uint32_t ord_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float unord_bits(uint32_t o) { return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o); }
+__device__ inline bool finitef(float v) { return __builtin_isfinite(v); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ static sky
+// One pixel per thread, 64 threads per workgroup. The pixel's n samples sit in an LDS
+// column (vals[i][lane]); each finite sample's position in the (value, time rank)
+// order is counted (ranks are distinct, so the order is total, as std::sort of the
+// pairs makes it, preprocess.cpp:171), giving the sorted index -> frame map `ord`.
+// Global median: the two middle values added in float, halved in double. Leave-out
+// median for frame i: walk the sorted order skipping frames within `exclude` ranks
+// of i (preprocess.cpp:178-199), a and b in double, st = 0.5 (a + b).
+__global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, long long fstride, int n,
+                                                 long long plane, const int* __restrict__ rank, int exclude,
+                                                 int min_finite, float* __restrict__ sky, float* __restrict__ resid,
+                                                 long long rstride) {
+    extern __shared__ unsigned char smem[];
+    float* vals = reinterpret_cast<float*>(smem);             // [n][64]
+    int* rk = reinterpret_cast<int*>(smem + 4 * 64 * n);      // [n]
+    unsigned char* ord = smem + 4 * 64 * n + 4 * n;           // [n][64]
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += 64) rk[i] = rank[i];
+    __syncthreads();
+    const long long p = static_cast<long long>(blockIdx.x) * 64 + t;
+    if (p >= plane) return;
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const float v = sub[i * fstride + p];
+        vals[i * 64 + t] = v;
+        m += finitef(v) ? 1 : 0;
+    }
+    if (m < min_finite) {
+        sky[p] = __builtin_nanf("");
+        for (int i = 0; i < n; ++i) resid[i * rstride + p] = __builtin_nanf("");
+        return;
+    }
+    for (int i = 0; i < n; ++i) {
+        const float vi = vals[i * 64 + t];
+        if (!finitef(vi)) continue;
+        const int ri = rk[i];
+        int pos = 0;
+        for (int j = 0; j < n; ++j) {
+            const float vj = vals[j * 64 + t];
+            pos += (finitef(vj) && (vj < vi || (vj == vi && rk[j] < ri))) ? 1 : 0;
+        }
+        ord[pos * 64 + t] = static_cast<unsigned char>(i);
+    }
+    const float ga = vals[ord[((m - 1) / 2) * 64 + t] * 64 + t];
+    const float gb = vals[ord[(m / 2) * 64 + t] * 64 + t];
+    const float gsum = ga + gb;
+    const double global = 0.5 * static_cast<double>(gsum);
+    sky[p] = static_cast<float>(global);
+    for (int i = 0; i < n; ++i) {
+        const float v = vals[i * 64 + t];
+        if (!finitef(v)) {
+            resid[i * rstride + p] = __builtin_nanf("");
+            continue;
+        }
+        double st = global;
+        if (exclude > 0) {
+            const int ri = rk[i];
+            int kept = 0;
+            for (int k = 0; k < m; ++k) {
+                const int d = rk[ord[k * 64 + t]] - ri;
+                kept += (d > exclude || d < -exclude) ? 1 : 0;
+            }
+            if (kept >= 5) {
+                const int lo = (kept - 1) / 2, hi = kept / 2;
+                double a = 0.0, b = 0.0;
+                int c = 0;
+                for (int k = 0; k < m; ++k) {
+                    const int f = ord[k * 64 + t];
+                    const int d = rk[f] - ri;
+                    if (d <= exclude && d >= -exclude) continue;
+                    if (c == lo) a = vals[f * 64 + t];
+                    if (c == hi) {
+                        b = vals[f * 64 + t];
+                        break;
+                    }
+                    ++c;
+                }
+                st = 0.5 * (a + b);
+            }
+        }
+        resid[i * rstride + p] = static_cast<float>(static_cast<double>(v) - st);
+    }
+}
+
+// ------------------------------------------------------------------ scatter map
+// scatter = 1.4826 * median_i(float(|r_i| / sigma_i)) over finite residuals, 1e9
+// when fewer than n/2 are finite (preprocess.cpp:371-391). The median's order
+// statistics come from rank counting in an LDS column.
+__global__ __launch_bounds__(64) void scatter_kernel(const float* __restrict__ resid, long long rstride, int n,
+                                                     long long plane, const double* __restrict__ sigma,
+                                                     float* __restrict__ scatter) {
+    extern __shared__ unsigned char smem[];
+    float* col = reinterpret_cast<float*>(smem);  // [n][64]
+    const int t = threadIdx.x;
+    const long long p = static_cast<long long>(blockIdx.x) * 64 + t;
+    if (p >= plane) return;
+    int c = 0;
+    for (int i = 0; i < n; ++i) {
+        const float v = resid[i * rstride + p];
+        if (!finitef(v)) continue;
+        col[c * 64 + t] = static_cast<float>(static_cast<double>(__builtin_fabsf(v)) / sigma[i]);
+        ++c;
+    }
+    if (c < n / 2) {
+        scatter[p] = 1e9f;
+        return;
+    }
+    const int mid = c / 2;
+    float vmid = 0.0f, vlow = 0.0f;
+    for (int j = 0; j < c; ++j) {
+        const float vj = col[j * 64 + t];
+        int lt = 0, le = 0;
+        for (int k = 0; k < c; ++k) {
+            const float vk = col[k * 64 + t];
+            lt += vk < vj ? 1 : 0;
+            le += vk <= vj ? 1 : 0;
+        }
+        if (lt <= mid && mid < le) vmid = vj;
+        if (lt <= mid - 1 && mid - 1 < le) vlow = vj;
+    }
+    double med = vmid;
+    if (c % 2 == 0) med = 0.5 * (med + static_cast<double>(vlow));
+    scatter[p] = static_cast<float>(1.4826 * med);
+}
+
+// mask = 0 within dil pixels (square) of any pixel whose scatter exceeds mask_scatter
+// (preprocess.cpp:393-400; the comparison is float against double).
+__global__ void mask_kernel(const float* __restrict__ scatter, int w, int h, int dil, double mask_scatter,
+                            float* __restrict__ mask) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    float out = 1.0f;
+    for (int yy = max(0, y - dil); yy <= min(h - 1, y + dil) && out != 0.0f; ++yy)
+        for (int xx = max(0, x - dil); xx <= min(w - 1, x + dil); ++xx)
+            if (!(static_cast<double>(scatter[static_cast<long long>(yy) * w + xx]) <= mask_scatter)) {
+                out = 0.0f;
+                break;
+            }
+    mask[static_cast<long long>(y) * w + x] = out;
+}
+
+// z = r * float(1/sigma_i) / e where unmasked and finite, else 0; e = scatter when it
+// exceeds weight_scatter, else 1 (preprocess.cpp:413-425). In place over r.
+__global__ void z_kernel(float* __restrict__ rz, long long rstride, long long plane, const float* __restrict__ mask,
+                         const float* __restrict__ scatter, const float* __restrict__ inv_sigma, float wscat) {
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (p >= plane) return;
+    float* r = rz + i * rstride;
+    const float v = r[p];
+    float z = 0.0f;
+    if (mask[p] != 0.0f && finitef(v)) {
+        const float s = scatter[p];
+        const float e = s > wscat ? s : 1.0f;
+        z = v * inv_sigma[i] / e;
+    }
+    r[p] = z;
+}
+
+// Cosmic-ray hits on the interior: z >= hit and the 4-neighbour mean below ratio * z
+// (preprocess.cpp:428-440). Counted per frame.
+__global__ void hit_kernel(const float* __restrict__ z, long long zstride, int w, int h, float hit, float ratio,
+                           unsigned char* __restrict__ flags, int* __restrict__ cr) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const float* zi = z + i * zstride;
+    const long long k = static_cast<long long>(y) * w + x;
+    unsigned char f = 0;
+    if (x >= 1 && x < w - 1 && y >= 1 && y < h - 1) {
+        const float v = zi[k];
+        if (!(v < hit)) {
+            const float n4 = 0.25f * (zi[k - 1] + zi[k + 1] + zi[k - w] + zi[k + w]);
+            if (!(n4 >= ratio * v)) {
+                f = 1;
+                atomicAdd(&cr[i], 1);
+            }
+        }
+    }
+    flags[i * (static_cast<long long>(w) * h) + k] = f;
+}
+
+// Kill the hits and their 8-neighbours above 2.5 (the kill list is applied after the
+// scan, so neighbours are judged on the unkilled z), then clamp to +-clip
+// (preprocess.cpp:441-451). Writes to `out` (a separate buffer).
+__global__ void kill_clip_kernel(const float* __restrict__ z, long long zstride, int w, int h,
+                                 const unsigned char* __restrict__ flags, float clip, float* __restrict__ out,
+                                 long long ostride) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const long long k = static_cast<long long>(y) * w + x;
+    const long long plane = static_cast<long long>(w) * h;
+    const unsigned char* fl = flags + i * plane;
+    const float v = z[i * zstride + k];
+    bool kill = fl[k] != 0;
+    if (!kill && v > 2.5f) {
+        for (int dy = -1; dy <= 1 && !kill; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx, yy = y + dy;
+                if ((dx || dy) && xx >= 0 && xx < w && yy >= 0 && yy < h && fl[static_cast<long long>(yy) * w + xx]) {
+                    kill = true;
+                    break;
+                }
+            }
+    }
+    float o = kill ? 0.0f : v;
+    o = o < -clip ? -clip : (clip < o ? clip : o);
+    out[i * ostride + k] = o;
+}
+
+// Separable matched filter with taps truncated at the image edges, multiply then add
+// in fp32 in ascending tap order (preprocess.cpp:116-148). taps = float(exp(.)),
+// taps_norm = taps * float(1 / sum taps^2), both from the host.
+__global__ void mf_rows_kernel(const float* __restrict__ in, long long istride, int w, int R,
+                               const float* __restrict__ taps, float* __restrict__ out, long long ostride) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const float* row = in + i * istride + static_cast<long long>(y) * w;
+    const int a = max(0, x - R), b = min(w - 1, x + R);
+    float acc = 0.0f;
+    for (int j = a; j <= b; ++j) acc += row[j] * taps[j - x + R];
+    out[i * ostride + static_cast<long long>(y) * w + x] = acc;
+}
+
+__global__ void mf_cols_kernel(const float* __restrict__ in, long long istride, int w, int h, int R,
+                               const float* __restrict__ taps_norm, float* __restrict__ out, long long ostride) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const float* t = in + i * istride;
+    const int a = max(0, y - R), b = min(h - 1, y + R);
+    float o = 0.0f;
+    for (int j = a; j <= b; ++j) o += taps_norm[j - y + R] * t[static_cast<long long>(j) * w + x];
+    out[i * ostride + static_cast<long long>(y) * w + x] = o;
+}
+
+// g *= float(1 / sg) where sg > 0 (preprocess.cpp:474-477).
+__global__ void scale_kernel(float* __restrict__ g, long long gstride, long long plane, const SelectSeg* __restrict__ seg) {
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (p >= plane) return;
+    const double sg = seg[i].result;
+    if (sg > 0.0) g[i * gstride + p] *= static_cast<float>(1.0 / sg);
+}
+
+// sub = float(v - bg) for finite v, NaN otherwise (preprocess.cpp:233-236)
+__global__ void subtract_kernel(const float* __restrict__ raw, long long fstride, long long plane,
+                                const SelectSeg* __restrict__ seg, float* __restrict__ sub, long long sstride,
+                                unsigned char* __restrict__ bad, int* __restrict__ any_bad) {
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (p >= plane) return;
+    const float v = raw[i * fstride + p];
+    const float s = finitef(v) ? static_cast<float>(static_cast<double>(v) - seg[i].result) : __builtin_nanf("");
+    sub[i * sstride + p] = s;
+    const unsigned char b = finitef(s) ? 0 : 1;
+    bad[i * plane + p] = b;
+    if (b) atomicOr(&any_bad[i], 1);
+}
+
+// Grow non-finite areas by `grow` pixels in a square (preprocess.cpp:237-254).
+__global__ void grow_kernel(const unsigned char* __restrict__ bad, int w, int h, int grow,
+                            const int* __restrict__ any_bad, float* __restrict__ sub, long long sstride) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w || !any_bad[i]) return;
+    const long long plane = static_cast<long long>(w) * h;
+    const unsigned char* b = bad + i * plane;
+    for (int yy = max(0, y - grow); yy <= min(h - 1, y + grow); ++yy)
+        for (int xx = max(0, x - grow); xx <= min(w - 1, x + grow); ++xx)
+            if (b[static_cast<long long>(yy) * w + xx]) {
+                sub[i * sstride + static_cast<long long>(y) * w + x] = __builtin_nanf("");
+                return;
+            }
+}
+
+// ------------------------------------------------------------------ per-frame order statistics
+// Samples of segment s (a frame): the regular grid of sample_pixels (common.hpp:85-100),
+// finite and (optionally) unmasked values. Keys are ordered float bits; excluded
+// samples get 0xffffffff and sort after every finite value. A 4-pass, 8-bit radix
+// select finds the order statistics k_lo and k_hi exactly.
+
+__device__ inline bool sample_key(const SelectSeg& s, long long q, const float* __restrict__ mask, int mode,
+                                  uint32_t& key) {
+    const int gy = static_cast<int>(q / s.nx), gx = static_cast<int>(q % s.nx);
+    const int y = s.border + gy * s.step, x = s.border + gx * s.step;
+    const long long k = static_cast<long long>(y) * s.w + x;
+    const float v = s.img[k];
+    if (!finitef(v) || (mask && mask[k] == 0.0f)) {
+        key = 0xffffffffu;
+        return false;
+    }
+    // mode 1: absolute deviation from the segment's median, rounded to float (common.hpp:80)
+    const float u = mode == 0 ? v : static_cast<float>(__builtin_fabs(static_cast<double>(v) - s.center));
+    key = ord_bits(u);
+    return true;
+}
+
+__global__ void select_count_kernel(SelectSeg* __restrict__ seg, const float* __restrict__ mask) {
+    SelectSeg& s = seg[blockIdx.y];
+    const long long G = static_cast<long long>(s.nx) * s.ny;
+    int c = 0;
+    for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < G;
+         q += static_cast<long long>(gridDim.x) * blockDim.x) {
+        uint32_t key;
+        c += sample_key(s, q, mask, 0, key) ? 1 : 0;
+    }
+    // wave reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s.count, c);
+}
+
+// Targets of median_inplace: element m/2 and, for even m, element m/2 - 1.
+__global__ void select_init_kernel(SelectSeg* __restrict__ seg, int nseg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    SelectSeg& s = seg[i];
+    const int m = s.count;
+    s.k[1] = m / 2;
+    s.k[0] = (m % 2 == 0) ? m / 2 - 1 : m / 2;
+    if (s.k[0] < 0) s.k[0] = 0;
+    s.prefix[0] = s.prefix[1] = 0u;
+}
+
+__global__ __launch_bounds__(256) void select_hist_kernel(SelectSeg* __restrict__ seg, const float* __restrict__ mask,
+                                                          int mode, int shift, unsigned int* __restrict__ hist) {
+    __shared__ unsigned int lh[2][256];
+    for (int d = threadIdx.x; d < 512; d += blockDim.x) lh[d >> 8][d & 255] = 0u;
+    __syncthreads();
+    const SelectSeg& s = seg[blockIdx.y];
+    const long long G = static_cast<long long>(s.nx) * s.ny;
+    const uint32_t hi_mask = shift >= 24 ? 0u : (0xffffffffu << (shift + 8));
+    const uint32_t p0 = s.prefix[0] & hi_mask, p1 = s.prefix[1] & hi_mask;
+    for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < G;
+         q += static_cast<long long>(gridDim.x) * blockDim.x) {
+        uint32_t key;
+        sample_key(s, q, mask, mode, key);
+        const uint32_t d = (key >> shift) & 255u;
+        if ((key & hi_mask) == p0) atomicAdd(&lh[0][d], 1u);
+        if ((key & hi_mask) == p1) atomicAdd(&lh[1][d], 1u);
+    }
+    __syncthreads();
+    unsigned int* gh = hist + static_cast<long long>(blockIdx.y) * 512;
+    for (int d = threadIdx.x; d < 512; d += blockDim.x) {
+        const unsigned int c = lh[d >> 8][d & 255];
+        if (c) atomicAdd(&gh[d], c);
+    }
+}
+
+__global__ void select_digit_kernel(SelectSeg* __restrict__ seg, int nseg, int shift, unsigned int* __restrict__ hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * nseg) return;
+    const int sidx = i >> 1, t = i & 1;
+    SelectSeg& s = seg[sidx];
+    unsigned int* h = hist + static_cast<long long>(sidx) * 512 + t * 256;
+    unsigned int cum = 0;
+    const unsigned int k = static_cast<unsigned int>(s.k[t]);
+    int digit = 255;
+    for (int d = 0; d < 256; ++d) {
+        if (cum + h[d] > k) {
+            digit = d;
+            break;
+        }
+        cum += h[d];
+    }
+    s.prefix[t] |= static_cast<uint32_t>(digit) << shift;
+    s.k[t] = static_cast<int>(k - cum);
+    for (int d = 0; d < 256; ++d) h[d] = 0u;
+}
+
+// median_inplace (common.hpp:64-74): element m/2, averaged in double with element
+// m/2 - 1 for even m; 0 for no samples. stage 0: store as the centre (the median);
+// stage 1: result = 1.4826 * median of the deviations (robust_sigma, common.hpp:77-82),
+// 0 when fewer than 3 samples; with `unit_floor`, results <= 0 become 1
+// (sigma_from_samples, preprocess.cpp:265-268).
+__global__ void select_finish_kernel(SelectSeg* __restrict__ seg, int nseg, int stage, int unit_floor) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    SelectSeg& s = seg[i];
+    const int m = s.count;
+    double med = 0.0;
+    if (m > 0) {
+        med = static_cast<double>(unord_bits(s.prefix[1]));
+        if (m % 2 == 0) med = 0.5 * (med + static_cast<double>(unord_bits(s.prefix[0])));
+    }
+    if (stage == 0) {
+        s.center = med;
+        s.result = med;
+    } else {
+        double r = m < 3 ? 0.0 : 1.4826 * med;
+        if (unit_floor && !(r > 0.0)) r = 1.0;
+        s.result = r;
+    }
+    s.k[0] = s.k[1] = 0;
+    s.prefix[0] = s.prefix[1] = 0u;
+}
+
+// 1/sigma as float for the z kernel (preprocess.cpp:416)
+__global__ void inv_sigma_kernel(const SelectSeg* __restrict__ seg, int nseg, float* __restrict__ inv,
+                                 double* __restrict__ sigma) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    sigma[i] = seg[i].result;
+    inv[i] = static_cast<float>(1.0 / seg[i].result);
+}
+
+}  // namespace ftk
+
+// ------------------------------------------------------------------ host orchestration
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+namespace ftk {
+
+namespace {
+
+constexpr double kFwhmToSigmaD = 1.0 / 2.354820045030949;  // psf.hpp:17
+
+void hcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+T* dalloc(size_t count, const char* what) {
+    T* p = nullptr;
+    hcheck(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, count) * sizeof(T)), what);
+    return p;
+}
+
+// The sample grid of sample_pixels on a w x h image with the given border.
+SelectSeg grid_seg(const float* img, int w, int h, int border) {
+    SelectSeg s;
+    s.img = img;
+    s.w = w;
+    s.border = border;
+    const long long usable = static_cast<long long>(std::max(0, w - 2 * border)) * std::max(0, h - 2 * border);
+    s.step = usable == 0 ? 1 : std::max(1, static_cast<int>(std::sqrt(static_cast<double>(usable) / 200000.0)));
+    s.nx = std::max(0, (w - 2 * border + s.step - 1) / s.step);
+    s.ny = std::max(0, (h - 2 * border + s.step - 1) / s.step);
+    if (usable == 0) s.nx = s.ny = 0;
+    return s;
+}
+
+unsigned blocks_for(long long count, int threads) { return static_cast<unsigned>((count + threads - 1) / threads); }
+
+}  // namespace
+
+DeviceStage::DeviceStage(int n_, int w_, int h_) : n(n_), w(w_), h(h_), plane(static_cast<long long>(w_) * h_) {
+    if (n < 1 || w < 3 || h < 3) throw std::runtime_error("build_stage: bad frame stack size");
+    if (n > 255) throw std::runtime_error("build_stage: at most 255 frames on the device path");
+    const size_t np = static_cast<size_t>(n) * plane;
+    resid_ = dalloc<float>(np, "allocate residuals");
+    flags_ = dalloc<unsigned char>(np, "allocate flags");
+    sky_ = dalloc<float>(plane, "allocate static sky");
+    scatter_ = dalloc<float>(plane, "allocate scatter");
+    mask_ = dalloc<float>(plane, "allocate mask");
+    seg_ = dalloc<SelectSeg>(n, "allocate selection state");
+    hist_ = dalloc<unsigned int>(static_cast<size_t>(n) * 512, "allocate histograms");
+    hcheck(hipMemset(hist_, 0, sizeof(unsigned int) * n * 512), "clear histograms");
+    inv_ = dalloc<float>(n, "allocate 1/sigma");
+    sigma_ = dalloc<double>(n, "allocate sigma");
+    cr_ = dalloc<int>(n, "allocate cosmic-ray counts");
+    rank_ = dalloc<int>(n, "allocate ranks");
+    any_ = dalloc<int>(n, "allocate flags");
+    taps_ = dalloc<float>(2 * 64 + 2, "allocate taps");
+}
+
+DeviceStage::~DeviceStage() {
+    for (void* p : {static_cast<void*>(resid_), static_cast<void*>(flags_), static_cast<void*>(sky_),
+                    static_cast<void*>(scatter_), static_cast<void*>(mask_), static_cast<void*>(seg_),
+                    static_cast<void*>(hist_), static_cast<void*>(inv_), static_cast<void*>(sigma_),
+                    static_cast<void*>(cr_), static_cast<void*>(rank_), static_cast<void*>(any_),
+                    static_cast<void*>(taps_)})
+        if (p) (void)hipFree(p);
+}
+
+void DeviceStage::select(std::vector<SelectSeg>& segs, const float* mask, bool deviation, int unit_floor,
+                         hipStream_t s) {
+    const int ns = static_cast<int>(segs.size());
+    int maxg = 1;
+    for (const auto& g : segs) maxg = std::max(maxg, g.nx * g.ny);
+    const unsigned gx = std::max(1u, std::min(64u, blocks_for(maxg, 256)));
+    if (!deviation) {
+        hcheck(hipMemcpyAsync(seg_, segs.data(), sizeof(SelectSeg) * ns, hipMemcpyHostToDevice, s), "upload segments");
+        hipLaunchKernelGGL(select_count_kernel, dim3(gx, ns), dim3(256), 0, s, seg_, mask);
+        hipLaunchKernelGGL(select_init_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const int mode = pass;
+        if (pass == 1 && !deviation) break;
+        if (pass == 0 && deviation) continue;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            hipLaunchKernelGGL(select_hist_kernel, dim3(gx, ns), dim3(256), 0, s, seg_, mask, mode, shift, hist_);
+            hipLaunchKernelGGL(select_digit_kernel, dim3(blocks_for(2 * ns, 64)), dim3(64), 0, s, seg_, ns, shift,
+                               hist_);
+        }
+        hipLaunchKernelGGL(select_finish_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns, mode, unit_floor);
+        if (mode == 0) hipLaunchKernelGGL(select_init_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns);
+    }
+    hcheck(hipGetLastError(), "selection kernels");
+}
+
+void DeviceStage::background(const float* raw, long long raw_stride, float* sub, long long sub_stride,
+                             double* bg_host, hipStream_t s) {
+    std::vector<SelectSeg> segs;
+    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(raw + i * raw_stride, w, h, 0));
+    select(segs, nullptr, false, 0, s);
+    hcheck(hipMemsetAsync(any_, 0, sizeof(int) * n, s), "clear flags");
+    hipLaunchKernelGGL(subtract_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, raw, raw_stride, plane, seg_,
+                       sub, sub_stride, flags_, any_);
+    hipLaunchKernelGGL(grow_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, flags_, w, h, 3, any_, sub,
+                       sub_stride);
+    hcheck(hipGetLastError(), "background kernels");
+    if (bg_host) {
+        std::vector<SelectSeg> out(static_cast<size_t>(n));
+        hcheck(hipMemcpyAsync(out.data(), seg_, sizeof(SelectSeg) * n, hipMemcpyDeviceToHost, s), "download background");
+        hcheck(hipStreamSynchronize(s), "background");
+        for (int i = 0; i < n; ++i) bg_host[i] = out[static_cast<size_t>(i)].result;
+    }
+}
+
+void DeviceStage::build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm,
+                        float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
+                        float* mask_out, int* cr_host, hipStream_t s) {
+    // time ranks: stable sort of the frame times (preprocess.cpp:319-331)
+    std::vector<int> order(static_cast<size_t>(n)), rank(static_cast<size_t>(n));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t_h[a] < t_h[b]; });
+    for (int r = 0; r < n; ++r) rank[static_cast<size_t>(order[static_cast<size_t>(r)])] = r;
+    const int exclude = prm.static_exclude >= 0 ? prm.static_exclude : std::max(1, static_cast<int>(std::lround(n / 8.0)));
+    const int min_finite = static_cast<int>(std::max<size_t>(5, static_cast<size_t>(0.6 * n)));
+    hcheck(hipMemcpyAsync(rank_, rank.data(), sizeof(int) * n, hipMemcpyHostToDevice, s), "upload ranks");
+
+    // static sky + leave-out residuals
+    float* sky = static_sky ? static_sky : sky_;
+    const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
+    hipLaunchKernelGGL(sky_kernel, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, rank_,
+                       exclude, min_finite, sky, resid_, plane);
+    // per-frame residual noise: robust sigma of the grid samples, border 2
+    std::vector<SelectSeg> segs;
+    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(resid_ + i * plane, w, h, 2));
+    select(segs, nullptr, false, 1, s);
+    select(segs, nullptr, true, 1, s);
+    hipLaunchKernelGGL(inv_sigma_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, seg_, n, inv_, sigma_);
+    // scatter map and mask
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(plane, 64)), dim3(64), static_cast<size_t>(n) * 64 * 4, s,
+                       resid_, plane, n, plane, sigma_, scatter_);
+    const int dil = std::max(1, static_cast<int>(std::lround(0.5 * prm.fwhm_px)));
+    float* mask = mask_out ? mask_out : mask_;
+    hipLaunchKernelGGL(mask_kernel, dim3(blocks_for(w, 128), h), dim3(128), 0, s, scatter_, w, h, dil,
+                       prm.mask_scatter, mask);
+    // z, cosmic rays, clip
+    const double psf_sigma = prm.fwhm_px * kFwhmToSigmaD;
+    hipLaunchKernelGGL(z_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, resid_, plane, plane, mask,
+                       scatter_, inv_, static_cast<float>(prm.weight_scatter));
+    hcheck(hipMemsetAsync(cr_, 0, sizeof(int) * n, s), "clear counts");
+    const float ratio = static_cast<float>(0.5 * std::exp(-0.5 / (psf_sigma * psf_sigma)));
+    hipLaunchKernelGGL(hit_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
+                       static_cast<float>(prm.cr_sigma), ratio, flags_, cr_);
+    hipLaunchKernelGGL(kill_clip_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, flags_,
+                       static_cast<float>(prm.clip_sigma), filtered, filt_stride);
+    // matched filter: taps from the host (glibc exp), truncated at the edges
+    const int R = std::max(1, static_cast<int>(std::ceil(3.0 * psf_sigma)));
+    if (R > 64) throw std::runtime_error("build_stage: PSF too wide for the device filter (R > 64)");
+    std::vector<float> taps(static_cast<size_t>(2 * R + 1)), taps_norm(taps.size());
+    double k2 = 0.0;
+    for (int i = -R; i <= R; ++i) {
+        const double v = std::exp(-0.5 * i * i / (psf_sigma * psf_sigma));
+        taps[static_cast<size_t>(i + R)] = static_cast<float>(v);
+        k2 += v * v;
+    }
+    const float norm = static_cast<float>(1.0 / k2);
+    for (size_t q = 0; q < taps.size(); ++q) taps_norm[q] = taps[q] * norm;
+    std::vector<float> both(taps);
+    both.insert(both.end(), taps_norm.begin(), taps_norm.end());
+    hcheck(hipMemcpyAsync(taps_, both.data(), sizeof(float) * both.size(), hipMemcpyHostToDevice, s), "upload taps");
+    hipLaunchKernelGGL(mf_rows_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, filtered, filt_stride, w, R,
+                       taps_, resid_, plane);
+    hipLaunchKernelGGL(mf_cols_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, R,
+                       taps_ + (2 * R + 1), filtered, filt_stride);
+    // normalisation: robust sigma of the masked grid samples, border ceil(3 sigma) + 1
+    const int fborder = static_cast<int>(std::ceil(3 * psf_sigma)) + 1;
+    segs.clear();
+    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(filtered + i * filt_stride, w, h, fborder));
+    select(segs, mask, false, 0, s);
+    select(segs, mask, true, 0, s);
+    hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, filtered, filt_stride, plane, seg_);
+    hcheck(hipGetLastError(), "build_stage kernels");
+    if (sigma_host || cr_host) {
+        if (sigma_host) hcheck(hipMemcpyAsync(sigma_host, sigma_, sizeof(double) * n, hipMemcpyDeviceToHost, s), "download sigma");
+        if (cr_host) hcheck(hipMemcpyAsync(cr_host, cr_, sizeof(int) * n, hipMemcpyDeviceToHost, s), "download counts");
+        hcheck(hipStreamSynchronize(s), "build_stage");
+    }
+}
+
+}  // namespace ftk

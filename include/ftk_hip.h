@@ -101,9 +101,49 @@ double ftk_plan_units(const ftk_plan* plan);
 int ftk_plan_blocks(const ftk_plan* plan);
 void ftk_plan_destroy(ftk_plan* plan);
 
+/* ------------------------------------------------------------------ preprocessing stage
+ * The time-order-dependent preprocessing of a search, build_stage
+ * (src/preprocess.cpp:286-486), and the per-frame background of prepare_core
+ * (:568-581), on the device and bit-identical to the reference. It replaces the host
+ * prepare() / with_times() calls (internal.hpp:55-58, preprocess.cpp:488-521) that
+ * feed every stacking pass: the null runs rebuild this stage with permuted times. */
+typedef struct ftk_stage ftk_stage;
+
+/* SearchParams fields that build_stage reads (include/fainttrail/search.hpp:18-90);
+ * fwhm_px is the PSF FWHM already estimated on the real data. */
+typedef struct ftk_stage_params {
+    double fwhm_px;
+    double clip_sigma;      /* 8   */
+    double cr_sigma;        /* 5   */
+    int static_exclude;     /* -1 = max(1, lround(n / 8)) */
+    double mask_scatter;    /* 2.5 */
+    double weight_scatter;  /* 1.5 */
+} ftk_stage_params;
+
+/* Device scratch for n frames of w x h (at most 255 frames). */
+int ftk_stage_create(int n, int w, int h, int device, ftk_stage** out, char* err, size_t err_len);
+void ftk_stage_destroy(ftk_stage* stage);
+/* d_raw (n frames, stride raw_stride floats) -> d_sub: float(v - median background),
+ * NaN-grown blank areas; background (host, n doubles, may be NULL). */
+int ftk_stage_background(ftk_stage* stage, const float* d_raw, size_t raw_stride, float* d_sub, size_t sub_stride,
+                         double* background, void* stream, char* err, size_t err_len);
+/* build_stage on the device for the frame times t_h (host, n; only their order
+ * matters): d_filtered receives the normalised matched-filtered frames (the frames of
+ * the stacking pass). Optional outputs (NULL to skip): d_static_sky (w x h device),
+ * sigma (n host doubles), d_mask (w x h device), cr_pixels (n host ints). When sigma
+ * or cr_pixels is given the call synchronises the stream. */
+int ftk_stage_build(ftk_stage* stage, const float* d_sub, size_t sub_stride, const double* t_h,
+                    const ftk_stage_params* params, float* d_filtered, size_t filt_stride, float* d_static_sky,
+                    double* sigma, float* d_mask, int32_t* cr_pixels, void* stream, char* err, size_t err_len);
+
 /* Plain device copy (n floats) on `stream`: the known-size kernel that calibrates
  * HBM traffic counters. */
 int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream);
+
+/* Measured LDS read bandwidth of the current device (bytes/s) for reads of
+ * width_bytes (4, 8 or 16) per lane: the roofline of the stacking kernel's inner
+ * loop, which reads one fp32 sample from LDS per stacked unit. */
+int ftk_probe_lds(int width_bytes, double* bytes_per_s);
 
 /* Host emulation of the HIP algorithm (tiles, velocity blocks, -0.0 padding, edge
  * coverage counting, packed-key maximum) on the CPU, for tests without a GPU; the
