@@ -77,12 +77,13 @@ ftk_options default_options() {
 
 // Plan tables on the device.
 struct DevicePlan {
-    DeviceBuffer<int> shift, box, ubox, motion;
+    DeviceBuffer<int> shift, box, ubox, lds_off, motion;
     DeviceBuffer<float> inv_sqrt;
     void upload(const StackPlan& p) {
         shift.upload(p.shift, "upload shift table");
         box.upload(p.box, "upload boxes");
         ubox.upload(p.ubox, "upload boxes");
+        lds_off.upload(p.lds_off, "upload offsets");
         motion.upload(p.motion, "upload motions");
         inv_sqrt.upload(p.inv_sqrt, "upload 1/sqrt table");
     }
@@ -225,6 +226,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                 L.shift = dev.shift.p;
                 L.box = dev.box.p;
                 L.ubox = dev.ubox.p;
+                L.lds_off = dev.lds_off.p;
                 L.motion = dev.motion.p;
                 L.inv_sqrt = dev.inv_sqrt.p;
                 L.y_begin = b.y0;
@@ -311,6 +313,7 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
         L.shift = pl->dev.shift.p;
         L.box = pl->dev.box.p;
         L.ubox = pl->dev.ubox.p;
+        L.lds_off = pl->dev.lds_off.p;
         L.motion = pl->dev.motion.p;
         L.inv_sqrt = pl->dev.inv_sqrt.p;
         L.y_begin = 0;

@@ -6,8 +6,9 @@
 // max over j with ties to the lowest j. Bit-identical to the CPU backend
 // (src/backend_cpu.cpp:77-204) because
 //   * every (pixel, motion) sum adds the covered samples in ascending frame order,
-//     and an uncovered frame adds -0.0f, the exact identity of IEEE addition
-//     (a sum that starts at +0.0f is never -0.0f, so +0 + -0 = +0 keeps it);
+//     and an uncovered frame adds a zero: a sum that starts at +0.0f is never -0.0f
+//     (IEEE round-to-nearest gives +0 for exact cancellation), and x + (+-0) = x for
+//     every x that is not -0, so the zero is an exact identity;
 //   * the build uses -ffp-contract=off, so no fused multiply-add appears;
 //   * c and 1/sqrt(c) come from the same integer count and the same host table;
 //   * the maximum keeps (value, lowest index), merged across workgroups with a
@@ -17,6 +18,11 @@
 // segment of velocity blocks. Per block and frame, the box the block's shifted tiles
 // can reach is staged in LDS (double buffered, one barrier per frame) and each lane
 // reads its 16 motions x 4 rows from it. No MFMA: this is a gather-sum.
+//
+// Two instantiations: tiles inside the plan's union shift box (every read of every
+// block lies in the frame rows present) stage their boxes with gfx950's direct
+// global->LDS loads and skip all coverage work (c = n); the ring of remaining tiles
+// stages through registers with bounds, and counts coverage per block when needed.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -31,7 +37,12 @@ namespace {
 constexpr int V = kVel;
 constexpr int RPW = kRowsPerWave;
 constexpr int kPreRows = kLdsRows / kWaves;  // staged rows per wavefront, at most
-constexpr int kPreCols = (kPitch + kTileX - 1) / kTileX;
+
+// Plan tables are read at wave-uniform addresses; through the constant address space
+// the compiler issues scalar loads (s_load), which do not wait on the vector memory
+// counter that the in-flight frame prefetch holds.
+using cint = const __attribute__((address_space(4))) int;
+__device__ inline cint* as_const(const int* p) { return (cint*)(p); }
 
 __device__ inline uint32_t d_ordered(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -41,17 +52,50 @@ __device__ inline float d_unordered(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
+// Tile coordinates of workgroup tile index t: interior tiles form the rectangle
+// [tx_lo, tx_hi) x [ty_lo, ty_hi); the ring lists the others row by row.
+struct TileRect {
+    int tiles_x, tiles_y, tx_lo, tx_hi, ty_lo, ty_hi;
+};
+
+__device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int& ty) {
+    if (!ring) {
+        const int wdt = R.tx_hi - R.tx_lo;
+        tx = R.tx_lo + t % wdt;
+        ty = R.ty_lo + t / wdt;
+        return;
+    }
+    const int top = R.ty_lo * R.tiles_x;
+    const int per = R.tx_lo + (R.tiles_x - R.tx_hi);
+    const int mid = (R.ty_hi - R.ty_lo) * per;
+    if (t < top) {
+        ty = t / R.tiles_x;
+        tx = t % R.tiles_x;
+    } else if (t < top + mid) {
+        const int u = t - top;
+        ty = R.ty_lo + u / per;
+        const int c = u % per;
+        tx = c < R.tx_lo ? c : R.tx_hi + (c - R.tx_lo);
+    } else {
+        const int u = t - top - mid;
+        ty = R.ty_hi + u / R.tiles_x;
+        tx = u % R.tiles_x;
+    }
+}
+
 }  // namespace
 
+template <bool RING>
 __global__ __launch_bounds__(kThreads) void stack_kernel(
     const float* __restrict__ frames, long long frame_stride, int row0, int row1, int n, int w, int h,
     const int* __restrict__ shift, const int* __restrict__ box, const int* __restrict__ ubox,
-    const int* __restrict__ motion, const float* __restrict__ inv_sqrt, int min_count, int nb, int bps,
-    int tiles_x, int y_begin, int y_end, int seg0, int nseg, unsigned long long* __restrict__ keys) {
-    __shared__ float lds[2][kLdsRows * kPitch];
+    const int* __restrict__ lds_off, const int* __restrict__ motion, const float* __restrict__ inv_sqrt, int min_count,
+    int nb, int bps, TileRect rect, int y_begin, int y_end, int seg0, int nseg, unsigned long long* __restrict__ keys) {
+    __shared__ float lds0[kLdsRows * kPitch];
+    __shared__ float lds1[kLdsRows * kPitch];
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     // Workgroups go to the 8 XCDs round-robin. Renumber so that each XCD works through
     // a contiguous range of (tile, segment) pairs: all segments of a tile then run on
     // one XCD and share its L2 copy of the tile's frame boxes.
@@ -60,11 +104,19 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     const unsigned logical = xcd * q + (xcd < r8 ? xcd : r8) + (b >> 3);
     const int tile = static_cast<int>(logical / static_cast<unsigned>(nseg));
     const int seg = seg0 + static_cast<int>(logical % static_cast<unsigned>(nseg));
-    const int x0 = (tile % tiles_x) * kTileX;
-    const int y0 = y_begin + (tile / tiles_x) * kTileY;
+    int tx, ty;
+    tile_xy(rect, RING, tile, tx, ty);
+    const int x0 = tx * kTileX;
+    const int y0 = y_begin + ty * kTileY;
     const int x = x0 + lane;
     const int yw = y0 + wave * RPW;
     const float inv_n = inv_sqrt[n];
+    cint* const c_box = as_const(box);
+    cint* const c_ubox = as_const(ubox);
+    cint* const c_off = as_const(lds_off);
+    cint* const c_motion = as_const(motion);
+    cint* const c_shift = as_const(shift);
+    const int lane_base = wave * RPW * kPitch + lane;
 
     float best[RPW];
     int arg[RPW];
@@ -77,10 +129,12 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     const int blk_begin = seg * bps;
     const int blk_end = min(nb, blk_begin + bps);
     for (int blk = blk_begin; blk < blk_end; ++blk) {
-        const int* ub = ubox + 4 * blk;
-        // interior: every staged read of every frame lies in the frame rows present
-        const bool interior = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w && y0 + ub[1] >= row0 &&
-                              y0 + kTileY - 1 + ub[3] < row1;
+        bool interior = true;
+        if (RING) {
+            cint* ub = c_ubox + 4 * blk;
+            interior = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w && y0 + ub[1] >= row0 &&
+                       y0 + kTileY - 1 + ub[3] < row1;
+        }
 
         float acc[V][RPW];
 #pragma unroll
@@ -88,10 +142,27 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
 #pragma unroll
             for (int r = 0; r < RPW; ++r) acc[k][r] = 0.0f;
 
-        float pre[kPreRows][kPreCols];
-        // gather frame i's box into registers (coalesced rows; -0.0f outside the frame)
+        // ---- staging of frame i's box into LDS buffer `dst`
+        float pre[kPreRows][2];
+        // interior: direct global -> LDS loads, one 64-float row chunk per instruction
+        auto stage_direct = [&](int i, float* dst) {
+            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
+            const int sx = bx[2], R = kTileY + bx[3];
+            const float* f = frames + static_cast<long long>(i) * frame_stride +
+                             static_cast<long long>(y0 + bx[1] - row0) * w + (x0 + bx[0]);
+#pragma unroll
+            for (int m = 0; m < kPreRows; ++m) {
+                const int rr = wave + kWaves * m;
+                if (rr >= R) break;
+                const float* src = f + static_cast<long long>(rr) * w;
+                __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
+                if (lane < sx) __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
+            }
+        };
+        // ring: bounds-checked loads into registers (zero outside the frame rows present
+        // and outside [0, w)), written to LDS after the current frame is consumed
         auto gather = [&](int i) {
-            const int* bx = box + 4 * (static_cast<long long>(blk) * n + i);
+            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
             const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
             const int C = kTileX + bx[2], R = kTileY + bx[3];
             const float* f = frames + static_cast<long long>(i) * frame_stride;
@@ -99,73 +170,86 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
             for (int m = 0; m < kPreRows; ++m) {
                 const int rr = wave + kWaves * m;
                 const int gy = by0 + rr;
+                const bool row_ok = rr < R && gy >= row0 && gy < row1;
 #pragma unroll
-                for (int c = 0; c < kPreCols; ++c) {
+                for (int c = 0; c < 2; ++c) {
                     const int cc = lane + kTileX * c;
                     const int gx = bx0 + cc;
-                    float v = -0.0f;
-                    if (rr < R && cc < C) {
-                        if (interior) {
-                            v = f[static_cast<long long>(gy - row0) * w + gx];
-                        } else if (gy >= row0 && gy < row1 && gx >= 0 && gx < w) {
-                            v = f[static_cast<long long>(gy - row0) * w + gx];
-                        }
-                    }
-                    pre[m][c] = v;
+                    const bool ok = row_ok && cc < C && gx >= 0 && gx < w;
+                    const long long off = ok ? static_cast<long long>(gy - row0) * w + gx : 0;
+                    const float v = f[off];
+                    pre[m][c] = ok ? v : 0.0f;
                 }
             }
         };
-        auto deposit = [&](int buf, int i) {
-            const int* bx = box + 4 * (static_cast<long long>(blk) * n + i);
-            const int C = kTileX + bx[2], R = kTileY + bx[3];
+        auto deposit = [&](float* dst, int i) {
+            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
+            const int R = kTileY + bx[3];
 #pragma unroll
             for (int m = 0; m < kPreRows; ++m) {
                 const int rr = wave + kWaves * m;
+                if (rr >= R) break;
+                dst[rr * kPitch + lane] = pre[m][0];
+                if (lane < kPitch - kTileX) dst[rr * kPitch + kTileX + lane] = pre[m][1];
+            }
+        };
+        auto consume = [&](const float* L, int i) {
+            cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
 #pragma unroll
-                for (int c = 0; c < kPreCols; ++c) {
-                    const int cc = lane + kTileX * c;
-                    if (rr < R && cc < C) lds[buf][rr * kPitch + cc] = pre[m][c];
-                }
+            for (int k = 0; k < V; ++k) {
+                const float* p = L + lane_base + lo[k];
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
             }
         };
 
-        gather(0);
-        deposit(0, 0);
-        __syncthreads();
-        for (int i = 0; i < n; ++i) {
-            const int cur = i & 1;
-            if (i + 1 < n) gather(i + 1);
-            const int* bx = box + 4 * (static_cast<long long>(blk) * n + i);
-            const int mdx = bx[0], mdy = bx[1];
-            const int* sh = shift + (static_cast<long long>(blk) * n + i) * (2 * V);
-            const float* L = lds[cur];
-            const int lane_base = wave * RPW * kPitch + lane;
-#pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const int off = lane_base + (sh[V + k] - mdy) * kPitch + (sh[k] - mdx);
-#pragma unroll
-                for (int r = 0; r < RPW; ++r) acc[k][r] += L[off + r * kPitch];
-            }
-            if (i + 1 < n) deposit(cur ^ 1, i + 1);
+        if (!RING || interior) {
+            stage_direct(0, lds0);
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
+            for (int i = 0; i < n; i += 2) {
+                if (i + 1 < n) stage_direct(i + 1, lds1);
+                consume(lds0, i);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (i + 1 >= n) break;
+                if (i + 2 < n) stage_direct(i + 2, lds0);
+                consume(lds1, i + 1);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+        } else {
+            gather(0);
+            deposit(lds0, 0);
+            __syncthreads();
+            for (int i = 0; i < n; i += 2) {
+                if (i + 1 < n) gather(i + 1);
+                consume(lds0, i);
+                if (i + 1 < n) deposit(lds1, i + 1);
+                __syncthreads();
+                if (i + 1 >= n) break;
+                if (i + 2 < n) gather(i + 2);
+                consume(lds1, i + 1);
+                if (i + 2 < n) deposit(lds0, i + 2);
+                __syncthreads();
+            }
         }
 
         // S/N of the block's motions (ascending index) and the running maximum
-        const int* mo = motion + V * blk;
+        cint* mo = c_motion + V * blk;
 #pragma unroll
         for (int k = 0; k < V; ++k) {
             const int j = mo[k];
             if (j < 0) continue;
             int cnt[RPW];
-            if (interior) {
 #pragma unroll
-                for (int r = 0; r < RPW; ++r) cnt[r] = n;
-            } else {
+            for (int r = 0; r < RPW; ++r) cnt[r] = n;
+            if (RING && !interior) {
                 // coverage from the full-frame bounds, frame by frame
 #pragma unroll
                 for (int r = 0; r < RPW; ++r) cnt[r] = 0;
                 for (int i = 0; i < n; ++i) {
-                    const int* sh = shift + (static_cast<long long>(blk) * n + i) * (2 * V);
+                    cint* sh = c_shift + (static_cast<long long>(blk) * n + i) * (2 * V);
                     const int sdx = sh[k], sdy = sh[V + k];
                     const int xin = static_cast<unsigned>(x + sdx) < static_cast<unsigned>(w) ? 1 : 0;
 #pragma unroll
@@ -176,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 if (cnt[r] < min_count) continue;
-                const float snr = acc[k][r] * (interior ? inv_n : inv_sqrt[cnt[r]]);
+                const float snr = acc[k][r] * ((RING && !interior) ? inv_sqrt[cnt[r]] : inv_n);
                 if (snr > best[r] || (snr == best[r] && j < arg[r])) {
                     best[r] = snr;
                     arg[r] = j;
@@ -246,12 +330,38 @@ hipError_t launch_copy(const float* src, float* dst, long long count, hipStream_
     return hipGetLastError();
 }
 
+namespace {
+int ceil_div_pos(int a, int b) { return a <= 0 ? 0 : (a + b - 1) / b; }
+int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+}  // namespace
+
+// Interior tiles: x0 + min dx >= 0, x0 + 63 + max dx < w, y0 + min dy >= row0 and
+// y0 + 15 + max dy < row1 for the union box of every block.
+void interior_rect(const StackPlan& p, const StackLaunch& L, int& tx_lo, int& tx_hi, int& ty_lo, int& ty_hi) {
+    const int* u = p.all_box;
+    tx_lo = std::max(0, ceil_div_pos(-u[0], kTileX));
+    tx_hi = std::min(L.tiles_x, floor_div(p.w - kTileX - u[2], kTileX) + 1);
+    ty_lo = std::max(0, ceil_div_pos(L.row0 - u[1] - L.y_begin, kTileY));
+    ty_hi = std::min(L.tiles_y, floor_div(L.row1 - kTileY - u[3] - L.y_begin, kTileY) + 1);
+    if (tx_hi <= tx_lo || ty_hi <= ty_lo) tx_lo = tx_hi = ty_lo = ty_hi = 0;
+}
+
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s) {
-    const unsigned long long wgs = static_cast<unsigned long long>(L.tiles_x) * L.tiles_y * L.nseg;
-    if (wgs == 0) return hipSuccess;
-    hipLaunchKernelGGL(stack_kernel, dim3(static_cast<unsigned>(wgs)), dim3(kThreads), 0, s, L.frames,
-                       L.frame_stride, L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.motion, L.inv_sqrt,
-                       p.min_count, p.nb, L.bps, L.tiles_x, L.y_begin, L.y_end, L.seg0, L.nseg, L.keys);
+    TileRect R{L.tiles_x, L.tiles_y, 0, 0, 0, 0};
+    interior_rect(p, L, R.tx_lo, R.tx_hi, R.ty_lo, R.ty_hi);
+    const long long all = static_cast<long long>(L.tiles_x) * L.tiles_y;
+    const long long inner = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
+    const long long ring = all - inner;
+    if (inner > 0)
+        hipLaunchKernelGGL(stack_kernel<false>, dim3(static_cast<unsigned>(inner * L.nseg)), dim3(kThreads), 0, s,
+                           L.frames, L.frame_stride, L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off,
+                           L.motion, L.inv_sqrt, p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg,
+                           L.keys);
+    if (ring > 0)
+        hipLaunchKernelGGL(stack_kernel<true>, dim3(static_cast<unsigned>(ring * L.nseg)), dim3(kThreads), 0, s,
+                           L.frames, L.frame_stride, L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off,
+                           L.motion, L.inv_sqrt, p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg,
+                           L.keys);
     return hipGetLastError();
 }
 

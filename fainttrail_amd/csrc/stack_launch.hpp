@@ -17,6 +17,7 @@ struct StackLaunch {
     const int* shift = nullptr;
     const int* box = nullptr;
     const int* ubox = nullptr;
+    const int* lds_off = nullptr;
     const int* motion = nullptr;
     const float* inv_sqrt = nullptr;
     int y_begin = 0, y_end = 0, tiles_x = 0, tiles_y = 0;

@@ -134,6 +134,9 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
         bx[1] = y0;
         bx[2] = x1 - x0;
         bx[3] = y1 - y0;
+        p.lds_off.resize(static_cast<size_t>(p.nb) * n * kVel, 0);
+        int32_t* lo = &p.lds_off[(static_cast<size_t>(b) * n + i) * kVel];
+        for (int k = 0; k < kVel; ++k) lo[k] = (sh[kVel + k] - y0) * kPitch + (sh[k] - x0);
         ux0 = std::min(ux0, x0);
         ux1 = std::max(ux1, x1);
         uy0 = std::min(uy0, y0);
@@ -174,6 +177,15 @@ StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const 
     std::vector<std::vector<int>> groups;
     kd_groups(idx, 0, idx.size(), s, fi, groups);
     for (auto& g : groups) emit_blocks(std::move(g), s, p);
+    p.all_box[0] = p.all_box[1] = std::numeric_limits<int>::max();
+    p.all_box[2] = p.all_box[3] = std::numeric_limits<int>::min();
+    for (int b = 0; b < p.nb; ++b) {
+        const int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
+        p.all_box[0] = std::min(p.all_box[0], ub[0]);
+        p.all_box[1] = std::min(p.all_box[1], ub[1]);
+        p.all_box[2] = std::max(p.all_box[2], ub[2]);
+        p.all_box[3] = std::max(p.all_box[3], ub[3]);
+    }
     p.inv_sqrt.assign(static_cast<size_t>(n) + 1, 0.0f);
     for (int c = 1; c <= n; ++c) p.inv_sqrt[static_cast<size_t>(c)] = static_cast<float>(1.0 / std::sqrt(static_cast<double>(c)));
     p.units = static_cast<double>(nv) * n * static_cast<double>(w) * h;

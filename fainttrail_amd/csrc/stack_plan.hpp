@@ -35,6 +35,8 @@ struct StackPlan {
     std::vector<int32_t> shift;      // nb x n x (kVel dx, kVel dy), clamped to +-w, +-h
     std::vector<int32_t> box;        // nb x n x 4: min dx, min dy, spread x, spread y
     std::vector<int32_t> ubox;       // nb x 4: min dx, min dy, max dx, max dy over all frames
+    std::vector<int32_t> lds_off;    // nb x n x kVel: (dy - min dy) * kPitch + (dx - min dx)
+    int all_box[4] = {0, 0, 0, 0};   // union of ubox over the blocks: min dx, min dy, max dx, max dy
     std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))
     int min_dy = 0, max_dy = 0;      // extremes of the clamped vertical shifts
     int tiles_x = 0, tiles_y = 0;
