@@ -97,6 +97,49 @@ def cpu_baseline(frames_dev, tab, min_count, target_s: float):
                       f"{n} frames): {units:.3e} units in {secs:.1f} s"}
 
 
+def search_measure(reps: int, with_plain: bool):
+    """Null-calibrated searches/s on the 512^2 configuration: 20 frames of 512 x 512 at
+    60 s cadence, sky 1000 ADU + N(0, 32^2), 150 stars (sigma_psf 1.5 px, flux
+    log-uniform 1e3-1e6 ADU), 3 movers of per-frame S/N 1.5 at 0.2-1.1 px/min, grid
+    spaced 0.5 px of total drift (grid_tol 0.125 px -> 5468 motions), 8 null runs,
+    seed 7. The device search runs everything on the GPU; the plain form is the
+    reference's own search() with the "hip" stacking backend (host preprocessing and
+    per-pass uploads, as the reference drives its CUDA backend)."""
+    import torch
+
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    n, w, h = 20, 512, 512
+    frames, mjd, _ = synth.make_sequence(n, w, h, nstars=150, nmovers=3, mover_snr=(1.5, 1.5),
+                                         rate_px_min=(0.2, 1.1), seed=7, device="cuda:0")
+    prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.125, fwhm_px=1.5 * 2.354820045030949,
+                       null_runs=8, seed=7)
+    s = Searcher(n, w, h)
+    res = s.run(frames, mjd, prm)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = s.run(frames, mjd, prm)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    out = {"config": "512x512, 20 frames, 5468 motions, 8 null runs, seed 7",
+           "device_searches_per_s": 1.0 / dt, "device_s_per_search": dt, "threshold_snr": res.threshold_snr,
+           "method": res.threshold_method, "selected": int(res.selected.x.size),
+           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items()}}
+    if with_plain:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            host = frames.cpu().numpy()
+            t0 = time.perf_counter()
+            ref = po.ref_search(host, mjd, 12.0, 66.0, 0.125, prm.fwhm_px, 8, seed=7, backend="hip")
+            dtp = time.perf_counter() - t0
+            out["plain_searches_per_s"] = 1.0 / dtp
+            out["plain_s_per_search"] = dtp
+            out["device_over_plain"] = dtp / dt
+            out["plain_threshold_equal"] = bool(ref["threshold"] == res.threshold_snr)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +148,11 @@ def main() -> int:
     ap.add_argument("--workload", default="tiled-6144x4096", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-search", action="store_true", help="skip the null-calibrated search measurement")
+    ap.add_argument("--search-reps", type=int, default=3)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, copy this many MiB with the library's plain copy kernel "
+                         "(the known-size kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this backend has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -162,11 +210,25 @@ def main() -> int:
                              "from LDS-staged frame boxes; the inner loop reads each unit's sample from LDS, "
                              "so lds_frac (against the probe-measured LDS read bandwidth) is the binding figure"},
     }
+    if args.copy_probe_mib > 0:
+        cnt = args.copy_probe_mib * 262144
+        src = torch.ones(cnt, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        ft.lib().ftk_copy_f32(src.data_ptr(), dst.data_ptr(), cnt, stream.cuda_stream)
+        torch.cuda.synchronize()
+        res["copy_probe_bytes"] = cnt * 4
+        del src, dst
     if not args.no_cpu:
         print("bench: CPU baseline (oracle, blocked)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(frames, tab, min_count, args.cpu_seconds)
     else:
         res["cpu_baseline"] = None
+    if not args.no_search:
+        del frames, plan
+        torch.cuda.empty_cache()
+        print("bench: null-calibrated search (512x512 configuration)", file=sys.stderr, flush=True)
+        res["search"] = search_measure(args.search_reps, with_plain=not args.no_cpu)
     print(json.dumps(res))
     return 0
 

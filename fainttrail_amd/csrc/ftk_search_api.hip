@@ -1,0 +1,158 @@
+// C-ABI of the device-resident search (include/ftk_hip.h, ftk_searcher_*).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ftk_hip.h"
+#include "search_device.hpp"
+
+struct ftk_searcher {
+    ftk::DeviceSearch* ds = nullptr;
+    int device = 0;
+    ftk::SearchOutput last;
+};
+
+namespace {
+void put_err2(char* err, size_t len, const std::string& msg) {
+    if (!err || len == 0) return;
+    const size_t k = std::min(len - 1, msg.size());
+    std::memcpy(err, msg.data(), k);
+    err[k] = '\0';
+}
+}  // namespace
+
+extern "C" void ftk_search_params_default(ftk_search_params* p) {
+    if (!p) return;
+    const ftk::SearchConfig c;
+    p->vmin_pxh = c.vmin_pxh;
+    p->vmax_pxh = c.vmax_pxh;
+    p->grid_tol_px = c.grid_tol_px;
+    p->fwhm_px = 3.0;
+    p->clip_sigma = c.clip_sigma;
+    p->cr_sigma = c.cr_sigma;
+    p->static_exclude = c.static_exclude;
+    p->mask_scatter = c.mask_scatter;
+    p->weight_scatter = c.weight_scatter;
+    p->min_coverage = c.min_coverage;
+    p->threshold_snr = c.threshold_snr;
+    p->min_snr_floor = c.min_snr_floor;
+    p->null_runs = c.null_runs;
+    p->target_false_per_run = c.target_false_per_run;
+    p->max_candidates = c.max_candidates;
+    p->edge_px = c.edge_px;
+    p->seed = c.seed;
+    p->fit_on_device = 0;
+}
+
+extern "C" int ftk_searcher_create(int n, int w, int h, int device, ftk_searcher** out, char* err, size_t err_len) {
+    if (!out) return FTK_ERROR;
+    *out = nullptr;
+    try {
+        if (device < 0) device = 0;
+        if (hipSetDevice(device) != hipSuccess) throw std::runtime_error("HIP device " + std::to_string(device) + " not present");
+        auto* s = new ftk_searcher();
+        s->device = device;
+        try {
+            s->ds = new ftk::DeviceSearch(n, w, h);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" void ftk_searcher_destroy(ftk_searcher* s) {
+    if (!s) return;
+    delete s->ds;
+    delete s;
+}
+
+extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
+                                const ftk_search_params* p, ftk_search_summary* sum, float* d_max_snr,
+                                int32_t* d_best_v, void* stream, char* err, size_t err_len) {
+    try {
+        if (!s || !d_raw || !mjd || !p) throw std::runtime_error("ftk_searcher_run: null argument");
+        if (stride < static_cast<size_t>(s->ds->w) * s->ds->h) throw std::runtime_error("ftk_searcher_run: frame stride smaller than a frame");
+        for (int i = 1; i < s->ds->n; ++i)
+            if (mjd[i] < mjd[i - 1]) throw std::runtime_error("ftk_searcher_run: frames must be sorted by time");
+        hipSetDevice(s->device);
+        ftk::SearchConfig c;
+        c.vmin_pxh = p->vmin_pxh;
+        c.vmax_pxh = p->vmax_pxh;
+        c.grid_tol_px = p->grid_tol_px;
+        c.fwhm_px = p->fwhm_px;
+        c.clip_sigma = p->clip_sigma;
+        c.cr_sigma = p->cr_sigma;
+        c.static_exclude = p->static_exclude;
+        c.mask_scatter = p->mask_scatter;
+        c.weight_scatter = p->weight_scatter;
+        c.min_coverage = p->min_coverage;
+        c.threshold_snr = p->threshold_snr;
+        c.min_snr_floor = p->min_snr_floor;
+        c.null_runs = p->null_runs;
+        c.target_false_per_run = p->target_false_per_run;
+        c.max_candidates = p->max_candidates;
+        c.edge_px = p->edge_px;
+        c.seed = p->seed;
+        c.fit_on_device = p->fit_on_device;
+        s->last = s->ds->run(d_raw, static_cast<long long>(stride), mjd, c, d_max_snr, d_best_v,
+                             static_cast<hipStream_t>(stream));
+        if (sum) {
+            std::memset(sum, 0, sizeof *sum);
+            const auto& o = s->last;
+            sum->threshold_snr = o.threshold;
+            sum->method = o.method;
+            sum->null_a = o.null_a;
+            sum->null_k = o.null_k;
+            sum->null_inv_s2 = o.null_inv_s2;
+            sum->null_b = o.null_b;
+            sum->dv_pxh = o.dv;
+            sum->t_ref_mjd = o.t_ref_mjd;
+            sum->span_h = o.span_h;
+            sum->n_velocities = static_cast<int>(o.grid.size());
+            sum->n_peaks = static_cast<int>(o.peaks.x.size());
+            sum->n_selected = static_cast<int>(o.selected.x.size());
+            sum->n_pooled = static_cast<int>(o.pooled.size());
+            for (const auto& t : o.timings) {
+                if (t.first == "prepare") sum->t_prepare += t.second;
+                else if (t.first == "stack (real)") sum->t_stack_real += t.second;
+                else if (t.first == "null rebuild") sum->t_null_rebuild += t.second;
+                else if (t.first == "stack (null)") sum->t_stack_null += t.second;
+                else if (t.first.rfind("peaks", 0) == 0) sum->t_peaks += t.second;
+                else sum->t_threshold += t.second;
+            }
+        }
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int32_t* x, int32_t* y, float* snr,
+                                  int32_t* vidx) {
+    if (!s) return -1;
+    const auto& o = s->last;
+    if (which == 2) {
+        const int c = static_cast<int>(o.pooled.size());
+        for (int k = 0; k < c && k < cap; ++k)
+            if (snr) snr[k] = o.pooled[static_cast<size_t>(k)];
+        return c;
+    }
+    const ftk::PeakList& p = which == 0 ? o.peaks : o.selected;
+    const int c = static_cast<int>(p.x.size());
+    for (int k = 0; k < c && k < cap; ++k) {
+        if (x) x[k] = p.x[static_cast<size_t>(k)];
+        if (y) y[k] = p.y[static_cast<size_t>(k)];
+        if (snr) snr[k] = p.snr[static_cast<size_t>(k)];
+        if (vidx) vidx[k] = p.vidx[static_cast<size_t>(k)];
+    }
+    return c;
+}

@@ -1,0 +1,89 @@
+// Device-resident blind search: the reference's search() loop (src/search.cpp:716-809)
+// with every per-pixel step on the GPU. Frames are uploaded once; the real pass and
+// each time-shuffled null run rebuild the preprocessing stage and stack on the device;
+// only peak lists (a few thousand values) come back to the host, which sorts them
+// with libstdc++'s std::sort exactly as the reference does and fits the null tail.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "prep.hpp"
+
+namespace ftk {
+
+hipError_t launch_peaks(const float* m, const int* best_v, int w, int h, float floor, int radius, int edge, int cap,
+                        int* count, int* px, int* py, float* ps, int* pv, hipStream_t s);
+hipError_t launch_tail_sums(const float* pooled_desc, int m, double runs, double* partial, int* bad, hipStream_t s);
+
+// SearchParams fields on the path (include/fainttrail/search.hpp:18-90).
+struct SearchConfig {
+    double vmin_pxh = 4.0, vmax_pxh = 40.0, grid_tol_px = 0.5;
+    double fwhm_px = 0.0;  // must be given (> 0): the PSF estimate runs once, outside this path
+    double clip_sigma = 8.0, cr_sigma = 5.0;
+    int static_exclude = -1;
+    double mask_scatter = 2.5, weight_scatter = 1.5, min_coverage = 0.7;
+    double threshold_snr = 0.0, min_snr_floor = 5.0;
+    int null_runs = 3;
+    double target_false_per_run = 0.02;
+    int max_candidates = 300;
+    int edge_px = 4;
+    uint64_t seed = 12345;
+    int fit_on_device = 0;  // 1: the FP64 tail-fit sums run on the GPU (tolerance, not bits)
+};
+
+struct PeakList {
+    std::vector<int> x, y, vidx;
+    std::vector<float> snr;
+};
+
+struct SearchOutput {
+    double threshold = 0.0;
+    int method = 0;  // 0 null-fit gauss, 1 null-fit linear, 2 null-quantile, 3 fixed, 4 gaussian
+    double null_a = 0.0, null_k = 0.0, null_inv_s2 = 0.0, null_b = 0.0;
+    double dv = 0.0, t_ref_mjd = 0.0, span_h = 0.0;
+    std::vector<double> t_h;
+    std::vector<std::pair<double, double>> grid;
+    std::vector<float> pooled;  // null peak S/N values, run after run
+    PeakList peaks;             // real-pass peaks, sorted, at most 100000
+    PeakList selected;          // peaks at or above the threshold, at most max_candidates
+    std::vector<std::pair<std::string, double>> timings;
+};
+
+class DeviceSearch {
+public:
+    DeviceSearch(int n, int w, int h);
+    ~DeviceSearch();
+    DeviceSearch(const DeviceSearch&) = delete;
+    DeviceSearch& operator=(const DeviceSearch&) = delete;
+
+    // raw: n frames (device, stride floats apart) sorted by time; mjd: their mid-exposure
+    // MJDs (host). d_max_snr / d_best_v (optional) receive the real pass's maps.
+    SearchOutput run(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg, float* d_max_snr,
+                     int* d_best_v, hipStream_t s);
+
+    int n, w, h;
+
+private:
+    PeakList peaks(const float* m, const int* b, float floor, int radius, int edge, size_t max_peaks, hipStream_t s);
+    DeviceStage stage_;
+    float* sub_ = nullptr;
+    float* filt_ = nullptr;
+    float* max_ = nullptr;
+    int* best_ = nullptr;
+    float* nmax_ = nullptr;  // null-run maps
+    int* nbest_ = nullptr;
+    int* pk_ = nullptr;  // count + 4 arrays
+    size_t pk_cap_ = 0;
+};
+
+// Host pieces of the loop, restated from the reference.
+std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv);
+std::vector<std::vector<double>> null_time_orders(const std::vector<double>& t_h, int runs, uint64_t seed);
+void set_threshold(const SearchConfig& cfg, int width, int height, double fwhm, size_t n_motions, int runs,
+                   const std::vector<float>& pooled, SearchOutput& out, const double* device_sums);
+
+}  // namespace ftk

@@ -1,0 +1,179 @@
+"""Device-resident blind search (ftk_searcher_*): prepare, the real pass, the
+time-shuffled null runs, the null-tail threshold and peak selection on the GPU.
+
+Mirrors the reference's search() (src/search.cpp:716-809) and the C API's frame
+handling (stable sort by MJD, src/c_api.cpp:178-180).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import FtkError, _err, _raise, lib
+
+_bound = False
+
+
+class _Params(C.Structure):
+    _fields_ = [("vmin_pxh", C.c_double), ("vmax_pxh", C.c_double), ("grid_tol_px", C.c_double),
+                ("fwhm_px", C.c_double), ("clip_sigma", C.c_double), ("cr_sigma", C.c_double),
+                ("static_exclude", C.c_int), ("mask_scatter", C.c_double), ("weight_scatter", C.c_double),
+                ("min_coverage", C.c_double), ("threshold_snr", C.c_double), ("min_snr_floor", C.c_double),
+                ("null_runs", C.c_int), ("target_false_per_run", C.c_double), ("max_candidates", C.c_int),
+                ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int)]
+
+
+class _Summary(C.Structure):
+    _fields_ = [("threshold_snr", C.c_double), ("method", C.c_int), ("null_a", C.c_double), ("null_k", C.c_double),
+                ("null_inv_s2", C.c_double), ("null_b", C.c_double), ("dv_pxh", C.c_double),
+                ("t_ref_mjd", C.c_double), ("span_h", C.c_double), ("n_velocities", C.c_int), ("n_peaks", C.c_int),
+                ("n_selected", C.c_int), ("n_pooled", C.c_int), ("t_prepare", C.c_double),
+                ("t_stack_real", C.c_double), ("t_null_rebuild", C.c_double), ("t_stack_null", C.c_double),
+                ("t_peaks", C.c_double), ("t_threshold", C.c_double)]
+
+
+METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"]
+
+
+def _bind():
+    global _bound
+    L = lib()
+    if _bound:
+        return L
+    vp, sz = C.c_void_p, C.c_size_t
+    L.ftk_search_params_default.argtypes = [C.POINTER(_Params)]
+    L.ftk_searcher_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.c_char_p, sz]
+    L.ftk_searcher_destroy.argtypes = [vp]
+    L.ftk_searcher_run.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.POINTER(_Summary), vp,
+                                   vp, vp, C.c_char_p, sz]
+    L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    _bound = True
+    return L
+
+
+@dataclass
+class SearchParams:
+    """The SearchParams fields on the path (include/fainttrail/search.hpp:18-90), same
+    names and defaults; fwhm_px must be given."""
+    vmin_pxh: float = 4.0
+    vmax_pxh: float = 40.0
+    grid_tol_px: float = 0.5
+    fwhm_px: float = 3.0
+    clip_sigma: float = 8.0
+    cr_sigma: float = 5.0
+    static_exclude: int = -1
+    mask_scatter: float = 2.5
+    weight_scatter: float = 1.5
+    min_coverage: float = 0.7
+    threshold_snr: float = 0.0
+    min_snr_floor: float = 5.0
+    null_runs: int = 3
+    target_false_per_run: float = 0.02
+    max_candidates: int = 300
+    edge_px: int = 4
+    seed: int = 12345
+    fit_on_device: bool = False
+
+    def c(self) -> _Params:
+        return _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
+                       self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
+                       self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
+                       self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device))
+
+
+@dataclass
+class Peaks:
+    x: np.ndarray
+    y: np.ndarray
+    snr: np.ndarray
+    vidx: np.ndarray
+
+
+@dataclass
+class SearchResult:
+    threshold_snr: float
+    threshold_method: str
+    null_a: float
+    null_k: float
+    null_inv_s2: float
+    null_b: float
+    n_velocities: int
+    dv_pxh: float
+    t_ref_mjd: float
+    peaks: Peaks
+    selected: Peaks
+    null_peaks: np.ndarray
+    timings: dict = field(default_factory=dict)
+    max_snr: object = None
+    best_v: object = None
+
+
+class Searcher:
+    """Device scratch for searches over n frames of w x h."""
+
+    def __init__(self, n: int, w: int, h: int, device: int = 0):
+        L = _bind()
+        hnd, e = C.c_void_p(), _err()
+        _raise(L.ftk_searcher_create(n, w, h, device, C.byref(hnd), e, len(e)), e)
+        self._h, self.n, self.w, self.h = hnd, n, w, h
+
+    def _peaks(self, which: int, count: int) -> Peaks:
+        L = lib()
+        x, y, v = (np.empty(count, np.int32) for _ in range(3))
+        s = np.empty(count, np.float32)
+        I32 = C.POINTER(C.c_int32)
+        L.ftk_searcher_peaks(self._h, which, count, x.ctypes.data_as(I32), y.ctypes.data_as(I32),
+                             s.ctypes.data_as(C.POINTER(C.c_float)), v.ctypes.data_as(I32))
+        return Peaks(x, y, s, v)
+
+    def run(self, frames, mjd, params: SearchParams, keep_maps: bool = True, stream=None) -> SearchResult:
+        """frames: (n, h, w) float32 device tensor sorted by time; mjd: n mid-exposure MJDs."""
+        import torch
+        if (not isinstance(frames, torch.Tensor) or not frames.is_cuda or frames.dtype != torch.float32
+                or tuple(frames.shape) != (self.n, self.h, self.w) or frames.stride(2) != 1
+                or frames.stride(1) != self.w):
+            raise FtkError("Searcher.run needs a row-contiguous (n, h, w) float32 device tensor")
+        m = np.ascontiguousarray(mjd, dtype=np.float64)
+        if m.size != self.n:
+            raise FtkError("frame and time counts differ")
+        mx = bx = None
+        if keep_maps:
+            mx = torch.empty((self.h, self.w), dtype=torch.float32, device=frames.device)
+            bx = torch.empty((self.h, self.w), dtype=torch.int32, device=frames.device)
+        s = stream if stream is not None else torch.cuda.current_stream(frames.device)
+        summ, prm, e = _Summary(), params.c(), _err()
+        _raise(lib().ftk_searcher_run(self._h, frames.data_ptr(), frames.stride(0),
+                                      m.ctypes.data_as(C.POINTER(C.c_double)), C.byref(prm), C.byref(summ),
+                                      mx.data_ptr() if mx is not None else None,
+                                      bx.data_ptr() if bx is not None else None, s.cuda_stream, e, len(e)), e)
+        pooled = self._peaks(2, summ.n_pooled).snr
+        return SearchResult(
+            summ.threshold_snr, METHODS[summ.method], summ.null_a, summ.null_k, summ.null_inv_s2, summ.null_b,
+            summ.n_velocities, summ.dv_pxh, summ.t_ref_mjd, self._peaks(0, summ.n_peaks),
+            self._peaks(1, summ.n_selected), pooled,
+            {"prepare": summ.t_prepare, "stack (real)": summ.t_stack_real, "null rebuild": summ.t_null_rebuild,
+             "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold},
+            mx, bx)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().ftk_searcher_destroy(h)
+            self._h = None
+
+
+def search(frames, mjd, params: SearchParams, device: int = 0) -> SearchResult:
+    """Search a sequence given as a host (n, h, w) array or device tensor: frames are
+    stable-sorted by MJD first, as ft_search does (src/c_api.cpp:178-180)."""
+    import torch
+    m = np.asarray(mjd, dtype=np.float64)
+    order = np.argsort(m, kind="stable")
+    if isinstance(frames, torch.Tensor):
+        dev = frames[torch.as_tensor(order, device=frames.device)].contiguous()
+    else:
+        dev = torch.from_numpy(np.ascontiguousarray(np.asarray(frames, np.float32)[order])).to(f"cuda:{device}")
+    n, h, w = dev.shape
+    return Searcher(n, w, h, device).run(dev, m[order], params)

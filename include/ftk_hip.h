@@ -136,6 +136,56 @@ int ftk_stage_build(ftk_stage* stage, const float* d_sub, size_t sub_stride, con
                     const ftk_stage_params* params, float* d_filtered, size_t filt_stride, float* d_static_sky,
                     double* sigma, float* d_mask, int32_t* cr_pixels, void* stream, char* err, size_t err_len);
 
+/* ------------------------------------------------------------------ device-resident search
+ * The reference's blind search over prepared frames, search() (src/search.cpp:716-809):
+ * prepare (background + build_stage), the real stacking pass, find_peaks, null_runs
+ * time-shuffled null runs (with_times rebuild + pass + peaks, search.cpp:761-782), the
+ * null-tail threshold (set_threshold, :573-612) and select_peaks (:614-620), with the
+ * frames resident on the device from the first step to the last. Stops before
+ * refine/finish_candidates; the null peaks are not screened for artefacts (the
+ * reference's SearchParams::null_tested = 0). */
+typedef struct ftk_search_params {
+    double vmin_pxh, vmax_pxh, grid_tol_px;
+    double fwhm_px; /* PSF FWHM, must be > 0 (estimated once, outside this path) */
+    double clip_sigma, cr_sigma;
+    int static_exclude;
+    double mask_scatter, weight_scatter, min_coverage;
+    double threshold_snr; /* 0 = calibrate with the null runs */
+    double min_snr_floor;
+    int null_runs;
+    double target_false_per_run;
+    int max_candidates, edge_px;
+    unsigned long long seed;
+    int fit_on_device; /* 1: the FP64 tail-fit sums run on the GPU (agrees to ~1e-12) */
+} ftk_search_params;
+
+typedef struct ftk_search_summary {
+    double threshold_snr;
+    int method; /* 0 null-fit gauss, 1 null-fit linear, 2 null-quantile, 3 fixed, 4 gaussian */
+    double null_a, null_k, null_inv_s2, null_b;
+    double dv_pxh, t_ref_mjd, span_h;
+    int n_velocities, n_peaks, n_selected, n_pooled;
+    double t_prepare, t_stack_real, t_null_rebuild, t_stack_null, t_peaks, t_threshold; /* seconds */
+} ftk_search_summary;
+
+typedef struct ftk_searcher ftk_searcher;
+
+/* SearchParams defaults (include/fainttrail/search.hpp:18-90), fwhm_px = 3. */
+void ftk_search_params_default(ftk_search_params* params);
+int ftk_searcher_create(int n, int w, int h, int device, ftk_searcher** out, char* err, size_t err_len);
+void ftk_searcher_destroy(ftk_searcher* searcher);
+/* d_raw: n registered frames on the device, sorted by time (as ft_search sorts them,
+ * c_api.cpp:178-180), stride floats apart; mjd: their mid-exposure MJDs (host).
+ * d_max_snr / d_best_v (optional, w x h device) receive the real pass's maps. */
+int ftk_searcher_run(ftk_searcher* searcher, const float* d_raw, size_t stride, const double* mjd,
+                     const ftk_search_params* params, ftk_search_summary* summary, float* d_max_snr,
+                     int32_t* d_best_v, void* stream, char* err, size_t err_len);
+/* Peaks of the last run: which 0 = real-pass peaks (sorted as find_peaks sorts them),
+ * 1 = selected peaks, 2 = pooled null peak S/N values (snr only). Copies up to cap
+ * entries into the non-NULL arrays and returns the total count. */
+int ftk_searcher_peaks(const ftk_searcher* searcher, int which, int cap, int32_t* x, int32_t* y, float* snr,
+                       int32_t* vidx);
+
 /* Plain device copy (n floats) on `stream`: the known-size kernel that calibrates
  * HBM traffic counters. */
 int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream);

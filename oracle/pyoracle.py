@@ -93,6 +93,8 @@ def ref() -> C.CDLL:
                                         C.c_double, F32P, I32P]
         _r.ref_check_backend_inputs.argtypes = [C.c_int, I32P, F32P, F64P, F64P, C.c_int]
         _r.ref_linear_shifts.argtypes = [C.c_int, F64P, C.c_int, F64P, I32P, I32P]
+        _r.ref_search.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, F64P, F32P,
+                                  C.c_int, F32P, C.c_char_p]
     return _r
 
 
@@ -302,3 +304,25 @@ def ref_check_backend_inputs(round_: int):
     v = np.empty((nv, 2), np.float64)
     L.ref_check_backend_inputs(round_, _p(dims, I32P), _p(fr, F32P), _p(t, F64P), _p(v, F64P), nv)
     return fr, t, v, mc
+
+
+def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, threshold=0.0, floor=5.0, target=0.02,
+               backend="cpu"):
+    """The reference's search() (null_tested = 0) on prepare(align off), with the given
+    stacking backend ("cpu", or "hip" through the linked adapter)."""
+    if backend == "hip":
+        import fainttrail_amd  # one HIP runtime: torch's, loaded before the adapter
+        fainttrail_amd.lib()
+    fr = f32(frames)
+    n, h, w = fr.shape
+    m = f64(mjd)
+    prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
+    out = np.zeros(6, np.float64)
+    cnt = ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
+                           _p(out, F64P), None, 0, None, backend.encode())
+    pooled = np.empty(max(cnt, 1), np.float32)
+    mx = np.empty((h, w), np.float32)
+    ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), _p(out, F64P),
+                     _p(pooled, F32P), cnt, _p(mx, F32P), backend.encode())
+    return {"threshold": out[0], "null_a": out[1], "null_k": out[2], "null_inv_s2": out[3], "null_b": out[4],
+            "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}

@@ -215,6 +215,49 @@ int ref_prepare(int n, int w, int h, const float* frames, const double* mjd, dou
     return 0;
 }
 
+// search() (src/search.cpp:716-809) on prepare(align off) of the given frames, with
+// null_tested = 0 and the given backend ("cpu" when NULL; "hip" is the registered
+// adapter of fainttrail_amd/integration/backend_hip.cpp, linked in). dparams: vmin, vmax, grid_tol, fwhm,
+// threshold_snr, min_snr_floor, target_false_per_run. out: threshold, null_a, null_k,
+// null_inv_s2, null_b, n_velocities. Returns the pooled null-peak count; pooled
+// (cap values) and max_snr (w x h) are written when non-NULL.
+int ref_search(int n, int w, int h, const float* frames, const double* mjd, const double* dparams, int null_runs,
+               uint64_t seed, double* out, float* pooled, int cap, float* max_snr, const char* backend) {
+    std::vector<Frame> fr(static_cast<size_t>(n));
+    const size_t plane = static_cast<size_t>(w) * h;
+    for (int i = 0; i < n; ++i) {
+        fr[static_cast<size_t>(i)].image = Image(w, h);
+        std::memcpy(fr[static_cast<size_t>(i)].image.data.data(), frames + i * plane, plane * sizeof(float));
+        fr[static_cast<size_t>(i)].mjd_mid = mjd[i];
+        fr[static_cast<size_t>(i)].exptime_s = 60.0;
+    }
+    SearchParams s;
+    s.align = "off";
+    s.backend = backend && *backend ? backend : "cpu";
+    s.vmin_pxh = dparams[0];
+    s.vmax_pxh = dparams[1];
+    s.grid_tol_px = dparams[2];
+    s.fwhm_px = dparams[3];
+    s.threshold_snr = dparams[4];
+    s.min_snr_floor = dparams[5];
+    s.target_false_per_run = dparams[6];
+    s.null_runs = null_runs;
+    s.null_tested = 0;
+    s.seed = seed;
+    const Prepared p = prepare(fr, s);
+    const SearchResult r = search(p, s);
+    out[0] = r.threshold_snr;
+    out[1] = r.null_a;
+    out[2] = r.null_k;
+    out[3] = r.null_inv_s2;
+    out[4] = r.null_b;
+    out[5] = static_cast<double>(r.n_velocities);
+    for (size_t k = 0; k < r.null_peaks.size() && static_cast<int>(k) < cap; ++k)
+        if (pooled) pooled[k] = static_cast<float>(r.null_peaks[k]);
+    if (max_snr) std::copy(r.max_snr.data.begin(), r.max_snr.data.end(), max_snr);
+    return static_cast<int>(r.null_peaks.size());
+}
+
 // The inputs of `fainttrail check-backend` (cli/main.cpp:697-742): simulate() with
 // seed 77 + round, prepare() with vmin 2 and vmax 20 / 45, velocity_grid. Returns the
 // number of motions; frames (n planes of the filtered images), t_h (n) and vxy

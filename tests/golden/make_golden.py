@@ -95,6 +95,17 @@ def main() -> None:
     st = po.ref_build_stage(prep["sub"], prep["t_h"][::-1].copy(), 3.0)
     data["stage_rev/filtered"], data["stage_rev/static_sky"] = st["filtered"], st["static_sky"]
     data["stage_rev/sigma"], data["stage_rev/mask"], data["stage_rev/cr"] = st["sigma"], st["mask"], st["cr_pixels"]
+    # a blind search with null calibration (search.cpp:716-809, null_tested = 0)
+    from fainttrail_amd import synth
+    fr, mjd, _ = synth.make_sequence(12, 96, 64, nstars=12, nmovers=2, mover_snr=(3.0, 3.0), seed=21)
+    fr = fr.numpy()
+    args = np.array([12.0, 66.0, 0.5, 3.53])
+    r = po.ref_search(fr, mjd, *args, null_runs=4, seed=77)
+    data["search/frames"], data["search/mjd"], data["search/args"] = fr, mjd, args
+    data["search/runs"] = np.int32(4)
+    data["search/threshold"] = np.float64(r["threshold"])
+    data["search/pooled"], data["search/max_snr"] = r["pooled"], r["max_snr"]
+    data["search/n_velocities"] = np.int32(r["n_velocities"])
     np.savez_compressed(OUT / "golden.npz", **data)
     print("wrote", OUT / "golden.npz", sum(v.nbytes for v in data.values()) // 1024, "KiB raw")
 
