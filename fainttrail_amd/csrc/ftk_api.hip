@@ -77,13 +77,17 @@ ftk_options default_options() {
 
 // Plan tables on the device.
 struct DevicePlan {
-    DeviceBuffer<int> shift, box, ubox, lds_off, motion;
+    DeviceBuffer<int> shift, box, ubox, lds_off, motion, cov_x, cov_y;
+    DeviceBuffer<long long> cov_off;
     DeviceBuffer<float> inv_sqrt;
     void upload(const StackPlan& p) {
         shift.upload(p.shift, "upload shift table");
         box.upload(p.box, "upload boxes");
         ubox.upload(p.ubox, "upload boxes");
         lds_off.upload(p.lds_off, "upload offsets");
+        cov_off.upload(p.cov_off, "upload coverage tables");
+        cov_x.upload(p.cov_x, "upload coverage tables");
+        cov_y.upload(p.cov_y, "upload coverage tables");
         motion.upload(p.motion, "upload motions");
         inv_sqrt.upload(p.inv_sqrt, "upload 1/sqrt table");
     }
@@ -227,6 +231,9 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                 L.box = dev.box.p;
                 L.ubox = dev.ubox.p;
                 L.lds_off = dev.lds_off.p;
+                L.cov_off = dev.cov_off.p;
+                L.cov_x = dev.cov_x.p;
+                L.cov_y = dev.cov_y.p;
                 L.motion = dev.motion.p;
                 L.inv_sqrt = dev.inv_sqrt.p;
                 L.y_begin = b.y0;
@@ -314,6 +321,9 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
         L.box = pl->dev.box.p;
         L.ubox = pl->dev.ubox.p;
         L.lds_off = pl->dev.lds_off.p;
+        L.cov_off = pl->dev.cov_off.p;
+        L.cov_x = pl->dev.cov_x.p;
+        L.cov_y = pl->dev.cov_y.p;
         L.motion = pl->dev.motion.p;
         L.inv_sqrt = pl->dev.inv_sqrt.p;
         L.y_begin = 0;

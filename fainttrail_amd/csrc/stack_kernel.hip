@@ -20,9 +20,9 @@
 // reads its 16 motions x 4 rows from it. No MFMA: this is a gather-sum.
 //
 // Two instantiations: tiles inside the plan's union shift box (every read of every
-// block lies in the frame rows present) stage their boxes with gfx950's direct
-// global->LDS loads and skip all coverage work (c = n); the ring of remaining tiles
-// stages through registers with bounds, and counts coverage per block when needed.
+// block lies in the frame rows present) skip all bounds and coverage work (c = n); the
+// ring of remaining tiles tests each block, and for blocks that reach outside the
+// frame zero-fills the box, loads only in-frame pixels and counts coverage.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -36,7 +36,7 @@ namespace {
 
 constexpr int V = kVel;
 constexpr int RPW = kRowsPerWave;
-constexpr int kPreRows = kLdsRows / kWaves;  // staged rows per wavefront, at most
+constexpr int kPreRows = kLdsRows / kWaves;  // staged rows per wavefront, at most (16 + 32 rows)
 
 // Plan tables are read at wave-uniform addresses; through the constant address space
 // the compiler issues scalar loads (s_load), which do not wait on the vector memory
@@ -90,20 +90,26 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     const float* __restrict__ frames, long long frame_stride, int row0, int row1, int n, int w, int h,
     const int* __restrict__ shift, const int* __restrict__ box, const int* __restrict__ ubox,
     const int* __restrict__ lds_off, const int* __restrict__ motion, const float* __restrict__ inv_sqrt, int min_count,
-    int nb, int bps, TileRect rect, int y_begin, int y_end, int seg0, int nseg, unsigned long long* __restrict__ keys) {
-    __shared__ float lds0[kLdsRows * kPitch];
-    __shared__ float lds1[kLdsRows * kPitch];
+    int nb, int bps, TileRect rect, int y_begin, int y_end, int seg0, int nseg, int ntiles, int lds_rows,
+    int cov_ok, const long long* __restrict__ cov_off, const int* __restrict__ cov_x,
+    const int* __restrict__ cov_y, unsigned long long* __restrict__ keys) {
+    // two frame boxes of lds_rows x kPitch floats (lds_rows = 16 + the plan's largest
+    // vertical spread), sized at launch
+    extern __shared__ float lds_dyn[];
+    float* const lds0 = lds_dyn;
+    float* const lds1 = lds_dyn + lds_rows * kPitch;
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    // Workgroups go to the 8 XCDs round-robin. Renumber so that each XCD works through
-    // a contiguous range of (tile, segment) pairs: all segments of a tile then run on
-    // one XCD and share its L2 copy of the tile's frame boxes.
-    const unsigned total = gridDim.x, b = blockIdx.x;
-    const unsigned q = total >> 3, r8 = total & 7u, xcd = b & 7u;
-    const unsigned logical = xcd * q + (xcd < r8 ? xcd : r8) + (b >> 3);
-    const int tile = static_cast<int>(logical / static_cast<unsigned>(nseg));
-    const int seg = seg0 + static_cast<int>(logical % static_cast<unsigned>(nseg));
+    // Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8). The k-th
+    // workgroup an XCD receives works on tile 8 (k / nseg) + XCD, segment k mod nseg:
+    // the segments of a tile run together on one XCD (sharing its L2 copy of the
+    // tile's frame boxes) and the eight XCDs work on neighbouring tiles (sharing frame
+    // rows in the MALL). The grid is padded to whole groups of 8 tiles.
+    const unsigned b = blockIdx.x, xcd = b & 7u, kq = b >> 3;
+    const int tile = static_cast<int>(8u * (kq / static_cast<unsigned>(nseg)) + xcd);
+    const int seg = seg0 + static_cast<int>(kq % static_cast<unsigned>(nseg));
+    if (tile >= ntiles) return;
     int tx, ty;
     tile_xy(rect, RING, tile, tx, ty);
     const int x0 = tx * kTileX;
@@ -116,6 +122,9 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     cint* const c_off = as_const(lds_off);
     cint* const c_motion = as_const(motion);
     cint* const c_shift = as_const(shift);
+    cint* const c_covy = as_const(cov_y);
+    const __attribute__((address_space(4))) long long* const c_covoff =
+        (const __attribute__((address_space(4))) long long*)(cov_off);
     const int lane_base = wave * RPW * kPitch + lane;
 
     float best[RPW];
@@ -142,55 +151,42 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
 #pragma unroll
             for (int r = 0; r < RPW; ++r) acc[k][r] = 0.0f;
 
-        // ---- staging of frame i's box into LDS buffer `dst`
-        float pre[kPreRows][2];
-        // interior: direct global -> LDS loads, one 64-float row chunk per instruction
-        auto stage_direct = [&](int i, float* dst) {
+        // ---- staging of frame i's box into LDS buffer `dst` with gfx950's direct
+        // global -> LDS loads, one 64-float row chunk per wavefront instruction (lane l
+        // writes the chunk's l-th float). A block that is not interior to this tile zeroes
+        // its box first and then loads only the lanes whose pixel lies in the frame rows
+        // present and in [0, w); the zeros stand for frames that do not cover.
+        auto stage = [&](int i, float* dst) {
             cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
             const int sx = bx[2], R = kTileY + bx[3];
-            const float* f = frames + static_cast<long long>(i) * frame_stride +
-                             static_cast<long long>(y0 + bx[1] - row0) * w + (x0 + bx[0]);
-#pragma unroll
-            for (int m = 0; m < kPreRows; ++m) {
-                const int rr = wave + kWaves * m;
-                if (rr >= R) break;
-                const float* src = f + static_cast<long long>(rr) * w;
-                __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
-                if (lane < sx) __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
-            }
-        };
-        // ring: bounds-checked loads into registers (zero outside the frame rows present
-        // and outside [0, w)), written to LDS after the current frame is consumed
-        auto gather = [&](int i) {
-            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
             const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
-            const int C = kTileX + bx[2], R = kTileY + bx[3];
+            if (RING && !interior) {
+#pragma unroll
+                for (int m = 0; m < kPreRows; ++m) {
+                    const int rr = wave + kWaves * m;
+                    if (rr >= R) break;
+                    dst[rr * kPitch + lane] = 0.0f;
+                    if (lane < kPitch - kTileX) dst[rr * kPitch + kTileX + lane] = 0.0f;
+                }
+                __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
+            }
             const float* f = frames + static_cast<long long>(i) * frame_stride;
 #pragma unroll
             for (int m = 0; m < kPreRows; ++m) {
                 const int rr = wave + kWaves * m;
-                const int gy = by0 + rr;
-                const bool row_ok = rr < R && gy >= row0 && gy < row1;
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int cc = lane + kTileX * c;
-                    const int gx = bx0 + cc;
-                    const bool ok = row_ok && cc < C && gx >= 0 && gx < w;
-                    const long long off = ok ? static_cast<long long>(gy - row0) * w + gx : 0;
-                    const float v = f[off];
-                    pre[m][c] = ok ? v : 0.0f;
-                }
-            }
-        };
-        auto deposit = [&](float* dst, int i) {
-            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
-            const int R = kTileY + bx[3];
-#pragma unroll
-            for (int m = 0; m < kPreRows; ++m) {
-                const int rr = wave + kWaves * m;
                 if (rr >= R) break;
-                dst[rr * kPitch + lane] = pre[m][0];
-                if (lane < kPitch - kTileX) dst[rr * kPitch + kTileX + lane] = pre[m][1];
+                const int gy = by0 + rr;
+                const float* src = f + static_cast<long long>(gy - row0) * w + bx0;
+                if (RING && !interior) {
+                    if (gy < row0 || gy >= row1) continue;
+                    const int gx = bx0 + lane;
+                    if (gx >= 0 && gx < w) __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
+                    if (lane < sx && gx + kTileX >= 0 && gx + kTileX < w)
+                        __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
+                } else {
+                    __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
+                    if (lane < sx) __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
+                }
             }
         };
         auto consume = [&](const float* L, int i) {
@@ -203,36 +199,19 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
             }
         };
 
-        if (!RING || interior) {
-            stage_direct(0, lds0);
+        stage(0, lds0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int i = 0; i < n; i += 2) {
+            if (i + 1 < n) stage(i + 1, lds1);
+            consume(lds0, i);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            for (int i = 0; i < n; i += 2) {
-                if (i + 1 < n) stage_direct(i + 1, lds1);
-                consume(lds0, i);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-                if (i + 1 >= n) break;
-                if (i + 2 < n) stage_direct(i + 2, lds0);
-                consume(lds1, i + 1);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-            }
-        } else {
-            gather(0);
-            deposit(lds0, 0);
+            if (i + 1 >= n) break;
+            if (i + 2 < n) stage(i + 2, lds0);
+            consume(lds1, i + 1);
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            for (int i = 0; i < n; i += 2) {
-                if (i + 1 < n) gather(i + 1);
-                consume(lds0, i);
-                if (i + 1 < n) deposit(lds1, i + 1);
-                __syncthreads();
-                if (i + 1 >= n) break;
-                if (i + 2 < n) gather(i + 2);
-                consume(lds1, i + 1);
-                if (i + 2 < n) deposit(lds0, i + 2);
-                __syncthreads();
-            }
         }
 
         // S/N of the block's motions (ascending index) and the running maximum
@@ -245,16 +224,40 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
 #pragma unroll
             for (int r = 0; r < RPW; ++r) cnt[r] = n;
             if (RING && !interior) {
-                // coverage from the full-frame bounds, frame by frame
+                // coverage from the full-frame bounds. cov_x / cov_y hold, per block and
+                // motion, the number of frames whose shift is <= d for every d of the
+                // block's range, so a pixel's count of frames landing inside [0, w) is
+                // F(w-1-x) - F(-1-x); a block that stays inside in y (or x) needs only the
+                // other table, and only corner cases walk the frames.
+                cint* ub = c_ubox + 4 * blk;
+                const bool x_full = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w;
+                const bool y_full = y0 + ub[1] >= 0 && y0 + kTileY - 1 + ub[3] < h;
+                if (cov_ok && (x_full || y_full)) {
+                    if (!x_full) {
+                        const int L = ub[2] - ub[0] + 1;
+                        const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * L;
+                        auto cdf = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= L ? n : F[d]); };
+                        const int cx = cdf(w - 1 - x) - cdf(-1 - x);
 #pragma unroll
-                for (int r = 0; r < RPW; ++r) cnt[r] = 0;
-                for (int i = 0; i < n; ++i) {
-                    cint* sh = c_shift + (static_cast<long long>(blk) * n + i) * (2 * V);
-                    const int sdx = sh[k], sdy = sh[V + k];
-                    const int xin = static_cast<unsigned>(x + sdx) < static_cast<unsigned>(w) ? 1 : 0;
+                        for (int r = 0; r < RPW; ++r) cnt[r] = cx;
+                    } else if (!y_full) {
+                        const int L = ub[3] - ub[1] + 1;
+                        cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * L;
+                        auto cdf = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= L ? n : G[d]); };
 #pragma unroll
-                    for (int r = 0; r < RPW; ++r)
-                        cnt[r] += static_cast<unsigned>(yw + r + sdy) < static_cast<unsigned>(h) ? xin : 0;
+                        for (int r = 0; r < RPW; ++r) cnt[r] = cdf(h - 1 - (yw + r)) - cdf(-1 - (yw + r));
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r) cnt[r] = 0;
+                    for (int i = 0; i < n; ++i) {
+                        cint* sh = c_shift + (static_cast<long long>(blk) * n + i) * (2 * V);
+                        const int sdx = sh[k], sdy = sh[V + k];
+                        const int xin = static_cast<unsigned>(x + sdx) < static_cast<unsigned>(w) ? 1 : 0;
+#pragma unroll
+                        for (int r = 0; r < RPW; ++r)
+                            cnt[r] += static_cast<unsigned>(yw + r + sdy) < static_cast<unsigned>(h) ? xin : 0;
+                    }
                 }
             }
 #pragma unroll
@@ -352,16 +355,19 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s)
     const long long all = static_cast<long long>(L.tiles_x) * L.tiles_y;
     const long long inner = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
     const long long ring = all - inner;
+    const int lds_rows = kTileY + p.max_sy;
+    const size_t lds_bytes = sizeof(float) * 2 * lds_rows * kPitch;
+    auto grid = [&](long long t) { return dim3(static_cast<unsigned>((t + 7) / 8 * 8 * L.nseg)); };
     if (inner > 0)
-        hipLaunchKernelGGL(stack_kernel<false>, dim3(static_cast<unsigned>(inner * L.nseg)), dim3(kThreads), 0, s,
-                           L.frames, L.frame_stride, L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off,
-                           L.motion, L.inv_sqrt, p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg,
-                           L.keys);
+        hipLaunchKernelGGL(stack_kernel<false>, grid(inner), dim3(kThreads), lds_bytes, s, L.frames, L.frame_stride,
+                           L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off, L.motion, L.inv_sqrt,
+                           p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg, static_cast<int>(inner),
+                           lds_rows, p.cov_ok ? 1 : 0, L.cov_off, L.cov_x, L.cov_y, L.keys);
     if (ring > 0)
-        hipLaunchKernelGGL(stack_kernel<true>, dim3(static_cast<unsigned>(ring * L.nseg)), dim3(kThreads), 0, s,
-                           L.frames, L.frame_stride, L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off,
-                           L.motion, L.inv_sqrt, p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg,
-                           L.keys);
+        hipLaunchKernelGGL(stack_kernel<true>, grid(ring), dim3(kThreads), lds_bytes, s, L.frames, L.frame_stride,
+                           L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off, L.motion, L.inv_sqrt,
+                           p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg, static_cast<int>(ring),
+                           lds_rows, p.cov_ok ? 1 : 0, L.cov_off, L.cov_x, L.cov_y, L.keys);
     return hipGetLastError();
 }
 

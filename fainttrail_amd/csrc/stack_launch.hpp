@@ -18,6 +18,9 @@ struct StackLaunch {
     const int* box = nullptr;
     const int* ubox = nullptr;
     const int* lds_off = nullptr;
+    const long long* cov_off = nullptr;  // per block: offsets into cov_x, cov_y
+    const int* cov_x = nullptr;
+    const int* cov_y = nullptr;
     const int* motion = nullptr;
     const float* inv_sqrt = nullptr;
     int y_begin = 0, y_end = 0, tiles_x = 0, tiles_y = 0;

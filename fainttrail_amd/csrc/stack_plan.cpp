@@ -134,6 +134,7 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
         bx[1] = y0;
         bx[2] = x1 - x0;
         bx[3] = y1 - y0;
+        p.max_sy = std::max(p.max_sy, y1 - y0);
         p.lds_off.resize(static_cast<size_t>(p.nb) * n * kVel, 0);
         int32_t* lo = &p.lds_off[(static_cast<size_t>(b) * n + i) * kVel];
         for (int k = 0; k < kVel; ++k) lo[k] = (sh[kVel + k] - y0) * kPitch + (sh[k] - x0);
@@ -147,6 +148,30 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
     ub[1] = uy0;
     ub[2] = ux1;
     ub[3] = uy1;
+    // cumulative shift counts per motion over the block's shift range (coverage of
+    // edge tiles without walking the frames); skipped when the tables grow too large
+    const long long Lx = ux1 - ux0 + 1, Ly = uy1 - uy0 + 1;
+    const long long need = static_cast<long long>(p.cov_x.size() + p.cov_y.size()) + kVel * (Lx + Ly);
+    if (!p.cov_ok || need > (1LL << 26)) {
+        p.cov_ok = false;
+        return;
+    }
+    const size_t ox = p.cov_x.size(), oy = p.cov_y.size();
+    p.cov_off.push_back(static_cast<long long>(ox));
+    p.cov_off.push_back(static_cast<long long>(oy));
+    p.cov_x.resize(ox + static_cast<size_t>(kVel * Lx), 0);
+    p.cov_y.resize(oy + static_cast<size_t>(kVel * Ly), 0);
+    for (int k = 0; k < kVel; ++k) {
+        int32_t* fx = &p.cov_x[ox + static_cast<size_t>(k * Lx)];
+        int32_t* fy = &p.cov_y[oy + static_cast<size_t>(k * Ly)];
+        for (int i = 0; i < n; ++i) {
+            const int32_t* sh = &p.shift[(static_cast<size_t>(b) * n + i) * 2 * kVel];
+            ++fx[sh[k] - ux0];
+            ++fy[sh[kVel + k] - uy0];
+        }
+        for (long long d = 1; d < Lx; ++d) fx[d] += fx[d - 1];
+        for (long long d = 1; d < Ly; ++d) fy[d] += fy[d - 1];
+    }
 }
 
 }  // namespace
@@ -197,9 +222,12 @@ void plan_tiles(StackPlan& p, int out_rows) {
     p.tiles_x = (p.w + kTileX - 1) / kTileX;
     p.tiles_y = (out_rows + kTileY - 1) / kTileY;
     const long tiles = static_cast<long>(p.tiles_x) * p.tiles_y;
-    // enough workgroups to fill 256 CUs many times over, as few key merges as possible
+    // Enough workgroups to fill 256 CUs many times over, and at least 16 segments per
+    // tile: the segments of one tile run at the same time on one XCD (the kernel's
+    // workgroup order) and read the same frame neighbourhood, so the tile's frames are
+    // fetched from HBM about once per 16 workgroups instead of once per workgroup.
     const long target = 8192;
-    long nseg = std::max<long>(1, (target + tiles - 1) / tiles);
+    long nseg = std::max<long>({1, (target + tiles - 1) / tiles, 16});
     nseg = std::min<long>(nseg, p.nb);
     p.blocks_per_seg = static_cast<int>((p.nb + nseg - 1) / nseg);
     p.nseg = (p.nb + p.blocks_per_seg - 1) / p.blocks_per_seg;

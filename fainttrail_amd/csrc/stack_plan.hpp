@@ -36,9 +36,15 @@ struct StackPlan {
     std::vector<int32_t> box;        // nb x n x 4: min dx, min dy, spread x, spread y
     std::vector<int32_t> ubox;       // nb x 4: min dx, min dy, max dx, max dy over all frames
     std::vector<int32_t> lds_off;    // nb x n x kVel: (dy - min dy) * kPitch + (dx - min dx)
+    // coverage tables: for block b and motion k, cov_x[cov_off[2b] + k * Lx + (d - min dx)]
+    // = number of frames with dx <= d (Lx = max dx - min dx + 1); cov_y likewise
+    std::vector<long long> cov_off;
+    std::vector<int32_t> cov_x, cov_y;
+    bool cov_ok = true;  // false: the tables would be too large; the kernel walks the frames
     int all_box[4] = {0, 0, 0, 0};   // union of ubox over the blocks: min dx, min dy, max dx, max dy
     std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))
     int min_dy = 0, max_dy = 0;      // extremes of the clamped vertical shifts
+    int max_sy = 0;                  // largest per-frame vertical spread of a block (<= kSpreadMax)
     int tiles_x = 0, tiles_y = 0;
     int blocks_per_seg = 1, nseg = 1;
     double units = 0.0;              // nv * n * w * h
