@@ -42,6 +42,10 @@ constexpr int kPreRows = kLdsRows / kWaves;  // staged rows per wavefront, at mo
 // the compiler issues scalar loads (s_load), which do not wait on the vector memory
 // counter that the in-flight frame prefetch holds.
 using cint = const __attribute__((address_space(4))) int;
+
+// Word 3 of a raw (untyped, stride 0) buffer descriptor on gfx9-family parts: 32-bit
+// data format, as AMD's own gfx90a/gfx94x/gfx950 kernels set it.
+constexpr int kBufferWord3 = 0x00020000;
 __device__ inline cint* as_const(const int* p) { return (cint*)(p); }
 
 __device__ inline uint32_t d_ordered(float f) {
@@ -171,21 +175,39 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
                 __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
             }
             const float* f = frames + static_cast<long long>(i) * frame_stride;
+            if (RING && !interior) {
 #pragma unroll
-            for (int m = 0; m < kPreRows; ++m) {
-                const int rr = wave + kWaves * m;
-                if (rr >= R) break;
-                const int gy = by0 + rr;
-                const float* src = f + static_cast<long long>(gy - row0) * w + bx0;
-                if (RING && !interior) {
+                for (int m = 0; m < kPreRows; ++m) {
+                    const int rr = wave + kWaves * m;
+                    if (rr >= R) break;
+                    const int gy = by0 + rr;
+                    const float* src = f + static_cast<long long>(gy - row0) * w + bx0;
                     if (gy < row0 || gy >= row1) continue;
                     const int gx = bx0 + lane;
                     if (gx >= 0 && gx < w) __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
                     if (lane < sx && gx + kTileX >= 0 && gx + kTileX < w)
                         __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
-                } else {
-                    __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
-                    if (lane < sx) __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
+                }
+            } else {
+                // uniform row pointers with a per-lane offset (the scalar-base load form);
+                // for the box's columns past 64, lanes beyond the box width re-read its
+                // last column into LDS columns that no motion reads, so every lane stays
+                // inside the box and no exec mask is needed
+                // buffer form of the LDS load: descriptor at the box's first pixel, the
+                // lane's byte offset in a VGPR (fixed for the frame), the row's byte
+                // offset as a scalar, so a row costs no vector instruction
+                const int b1 = 4 * lane;
+                const int b2 = 4 * (kTileX + min(lane, sx - 1));
+                const float* boxp = f + static_cast<long long>(by0 - row0) * w + bx0;
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(boxp), 0, 0x7fffffff, kBufferWord3);
+#pragma unroll
+                for (int m = 0; m < kPreRows; ++m) {
+                    const int rr = wave + kWaves * m;
+                    if (rr >= R) break;
+                    const int row_b = rr * 4 * w;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
+                    if (sx > 0) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
                 }
             }
         };

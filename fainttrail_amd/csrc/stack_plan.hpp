@@ -24,7 +24,7 @@ constexpr int kRowsPerWave = 4;                // tile rows per wavefront
 constexpr int kTileY = kWaves * kRowsPerWave;  // 16
 constexpr int kVel = 16;                       // motions per block
 constexpr int kSpreadMax = 32;                 // largest per-frame shift spread of a block, each axis
-constexpr int kPitch = kTileX + kSpreadMax;    // LDS row pitch (floats)
+constexpr int kPitch = 128;                     // LDS row pitch (floats): rows 0..3 of a lane are 2 x 64 dwords apart (ds_read2st64)
 constexpr int kLdsRows = kTileY + kSpreadMax;  // LDS rows per staged frame
 constexpr int kThreads = kTileX * kWaves;      // 256
 
