@@ -326,3 +326,23 @@ def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, thresh
                      _p(pooled, F32P), cnt, _p(mx, F32P), backend.encode())
     return {"threshold": out[0], "null_a": out[1], "null_k": out[2], "null_inv_s2": out[3], "null_b": out[4],
             "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}
+
+
+def ref_search_tiled(frames, mjd, vmin, vmax, tol, fwhm, null_runs, tile, seed=12345, threshold=0.0, floor=5.0,
+                     target=0.02):
+    """The reference's tiled search (search_tiled, null_tested = 0) with core tiles of `tile` px."""
+    L = ref()
+    L.ref_search_tiled.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, C.c_int, F64P,
+                                   F32P, C.c_int, F32P]
+    fr = f32(frames)
+    n, h, w = fr.shape
+    m = f64(mjd)
+    prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
+    out = np.zeros(6, np.float64)
+    cnt = L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
+                             _p(out, F64P), None, 0, None)
+    pooled = np.empty(max(cnt, 1), np.float32)
+    mx = np.empty((h, w), np.float32)
+    L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
+                       _p(out, F64P), _p(pooled, F32P), cnt, _p(mx, F32P))
+    return {"threshold": out[0], "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}

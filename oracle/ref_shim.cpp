@@ -16,6 +16,7 @@
 #include "fainttrail/backend.hpp"
 #include "fainttrail/search.hpp"
 #include "fainttrail/simulate.hpp"
+#include "fainttrail/tiled.hpp"
 #include "internal.hpp"
 
 using namespace fainttrail;
@@ -246,6 +247,48 @@ int ref_search(int n, int w, int h, const float* frames, const double* mjd, cons
     s.seed = seed;
     const Prepared p = prepare(fr, s);
     const SearchResult r = search(p, s);
+    out[0] = r.threshold_snr;
+    out[1] = r.null_a;
+    out[2] = r.null_k;
+    out[3] = r.null_inv_s2;
+    out[4] = r.null_b;
+    out[5] = static_cast<double>(r.n_velocities);
+    for (size_t k = 0; k < r.null_peaks.size() && static_cast<int>(k) < cap; ++k)
+        if (pooled) pooled[k] = static_cast<float>(r.null_peaks[k]);
+    if (max_snr) std::copy(r.max_snr.data.begin(), r.max_snr.data.end(), max_snr);
+    return static_cast<int>(r.null_peaks.size());
+}
+
+// search_tiled (src/tiled.cpp:272-645) over a memory frame source with the given core
+// tile size; same parameters and outputs as ref_search. The reference designs the
+// tiled search to equal the untiled one bit for bit (tests/test_main.cpp:1220-1276).
+int ref_search_tiled(int n, int w, int h, const float* frames, const double* mjd, const double* dparams,
+                     int null_runs, uint64_t seed, int tile, double* out, float* pooled, int cap, float* max_snr) {
+    std::vector<Frame> fr(static_cast<size_t>(n));
+    const size_t plane = static_cast<size_t>(w) * h;
+    for (int i = 0; i < n; ++i) {
+        fr[static_cast<size_t>(i)].image = Image(w, h);
+        std::memcpy(fr[static_cast<size_t>(i)].image.data.data(), frames + i * plane, plane * sizeof(float));
+        fr[static_cast<size_t>(i)].mjd_mid = mjd[i];
+        fr[static_cast<size_t>(i)].exptime_s = 60.0;
+    }
+    SearchParams s;
+    s.align = "off";
+    s.backend = "cpu";
+    s.vmin_pxh = dparams[0];
+    s.vmax_pxh = dparams[1];
+    s.grid_tol_px = dparams[2];
+    s.fwhm_px = dparams[3];
+    s.threshold_snr = dparams[4];
+    s.min_snr_floor = dparams[5];
+    s.target_false_per_run = dparams[6];
+    s.null_runs = null_runs;
+    s.null_tested = 0;
+    s.seed = seed;
+    const auto src = memory_frame_source(fr);
+    TileOptions opt;
+    opt.tile = tile;
+    const SearchResult r = search_tiled(*src, s, opt);
     out[0] = r.threshold_snr;
     out[1] = r.null_a;
     out[2] = r.null_k;

@@ -54,3 +54,24 @@ def test_search_matches_live_reference(hip, seed, runs):
     # selected peaks are the real peaks at or above the threshold, strongest first
     sel = got.selected
     assert np.all(sel.snr >= got.threshold_snr) and np.all(np.diff(sel.snr) <= 0)
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("tile", [48, 64])
+def test_search_matches_reference_tiled_search(hip, tile):
+    """search_tiled (src/tiled.cpp:272-645) with uneven tiles (150 x 110 px cut into
+    48- or 64-px cores with halos, the last ones clipped) equals the device search,
+    which keeps the whole frame resident: same max-S/N map, null peaks and threshold."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    fr, mjd, _ = synth.make_sequence(16, 150, 110, nstars=30, nmovers=2, mover_snr=(3.0, 3.0), seed=5)
+    frames = fr.numpy()
+    want = po.ref_search_tiled(frames, mjd, 12.0, 66.0, 0.5, 3.53, 3, tile, seed=9)
+    got = Searcher(16, 150, 110).run(torch.from_numpy(frames).cuda(), mjd,
+                                     SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53,
+                                                  null_runs=3, seed=9))
+    torch.cuda.synchronize()
+    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
+    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
+    assert got.threshold_snr == want["threshold"]
