@@ -115,17 +115,29 @@ def search_measure(reps: int, with_plain: bool):
     prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.125, fwhm_px=1.5 * 2.354820045030949,
                        null_runs=8, seed=7)
     s = Searcher(n, w, h)
-    res = s.run(frames, mjd, prm)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        res = s.run(frames, mjd, prm)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
+
+    def timed(p):
+        r = s.run(frames, mjd, p)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = s.run(frames, mjd, p)
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / reps
+
+    seq, dt_seq = timed(SearchParams(**{**prm.__dict__, "passes_per_launch": 1}))
+    g, seg, _ = s.tune(frames, mjd, prm, reps=1)
+    res, dt = timed(prm)
+    same = bool(seq.threshold_snr == res.threshold_snr and np.array_equal(seq.null_peaks, res.null_peaks)
+                and np.array_equal(seq.peaks.snr, res.peaks.snr)
+                and torch.equal(seq.max_snr, res.max_snr) and torch.equal(seq.best_v, res.best_v))
     out = {"config": "512x512, 20 frames, 5468 motions, 8 null runs, seed 7",
            "device_searches_per_s": 1.0 / dt, "device_s_per_search": dt, "threshold_snr": res.threshold_snr,
            "method": res.threshold_method, "selected": int(res.selected.x.size),
-           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items()}}
+           "batched": {"passes_per_launch": g, "segments": seg, "equal_to_one_pass_form": same},
+           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
+           "one_pass_form_searches_per_s": 1.0 / dt_seq,
+           "one_pass_form_stage_seconds": {k: round(v, 4) for k, v in seq.timings.items() if v > 0}}
     if with_plain:
         from oracle import pyoracle as po
         if po.ref_available():

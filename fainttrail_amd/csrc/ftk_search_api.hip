@@ -44,7 +44,46 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->edge_px = c.edge_px;
     p->seed = c.seed;
     p->fit_on_device = 0;
+    p->passes_per_launch = 0;
+    p->segments = 0;
 }
+
+namespace {
+ftk::SearchConfig config_of(const ftk_search_params* p) {
+    ftk::SearchConfig c;
+    c.vmin_pxh = p->vmin_pxh;
+    c.vmax_pxh = p->vmax_pxh;
+    c.grid_tol_px = p->grid_tol_px;
+    c.fwhm_px = p->fwhm_px;
+    c.clip_sigma = p->clip_sigma;
+    c.cr_sigma = p->cr_sigma;
+    c.static_exclude = p->static_exclude;
+    c.mask_scatter = p->mask_scatter;
+    c.weight_scatter = p->weight_scatter;
+    c.min_coverage = p->min_coverage;
+    c.threshold_snr = p->threshold_snr;
+    c.min_snr_floor = p->min_snr_floor;
+    c.null_runs = p->null_runs;
+    c.target_false_per_run = p->target_false_per_run;
+    c.max_candidates = p->max_candidates;
+    c.edge_px = p->edge_px;
+    c.seed = p->seed;
+    c.fit_on_device = p->fit_on_device;
+    c.passes_per_launch = p->passes_per_launch;
+    c.segments = p->segments;
+    return c;
+}
+
+void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
+                    const ftk_search_params* p) {
+    if (!s || !d_raw || !mjd || !p) throw std::runtime_error("ftk_searcher_run: null argument");
+    if (stride < static_cast<size_t>(s->ds->w) * s->ds->h) throw std::runtime_error("ftk_searcher_run: frame stride smaller than a frame");
+    for (int i = 1; i < s->ds->n; ++i)
+        if (mjd[i] < mjd[i - 1]) throw std::runtime_error("ftk_searcher_run: frames must be sorted by time");
+    if (p->passes_per_launch < 0 || p->segments < 0)
+        throw std::runtime_error("ftk_searcher_run: passes_per_launch and segments must be >= 0");
+}
+}  // namespace
 
 extern "C" int ftk_searcher_create(int n, int w, int h, int device, ftk_searcher** out, char* err, size_t err_len) {
     if (!out) return FTK_ERROR;
@@ -78,30 +117,9 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
                                 const ftk_search_params* p, ftk_search_summary* sum, float* d_max_snr,
                                 int32_t* d_best_v, void* stream, char* err, size_t err_len) {
     try {
-        if (!s || !d_raw || !mjd || !p) throw std::runtime_error("ftk_searcher_run: null argument");
-        if (stride < static_cast<size_t>(s->ds->w) * s->ds->h) throw std::runtime_error("ftk_searcher_run: frame stride smaller than a frame");
-        for (int i = 1; i < s->ds->n; ++i)
-            if (mjd[i] < mjd[i - 1]) throw std::runtime_error("ftk_searcher_run: frames must be sorted by time");
+        check_run_args(s, d_raw, stride, mjd, p);
         hipSetDevice(s->device);
-        ftk::SearchConfig c;
-        c.vmin_pxh = p->vmin_pxh;
-        c.vmax_pxh = p->vmax_pxh;
-        c.grid_tol_px = p->grid_tol_px;
-        c.fwhm_px = p->fwhm_px;
-        c.clip_sigma = p->clip_sigma;
-        c.cr_sigma = p->cr_sigma;
-        c.static_exclude = p->static_exclude;
-        c.mask_scatter = p->mask_scatter;
-        c.weight_scatter = p->weight_scatter;
-        c.min_coverage = p->min_coverage;
-        c.threshold_snr = p->threshold_snr;
-        c.min_snr_floor = p->min_snr_floor;
-        c.null_runs = p->null_runs;
-        c.target_false_per_run = p->target_false_per_run;
-        c.max_candidates = p->max_candidates;
-        c.edge_px = p->edge_px;
-        c.seed = p->seed;
-        c.fit_on_device = p->fit_on_device;
+        const ftk::SearchConfig c = config_of(p);
         s->last = s->ds->run(d_raw, static_cast<long long>(stride), mjd, c, d_max_snr, d_best_v,
                              static_cast<hipStream_t>(stream));
         if (sum) {
@@ -120,15 +138,37 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
             sum->n_peaks = static_cast<int>(o.peaks.x.size());
             sum->n_selected = static_cast<int>(o.selected.x.size());
             sum->n_pooled = static_cast<int>(o.pooled.size());
+            sum->passes_per_launch = o.passes_per_launch;
+            sum->segments = o.segments;
             for (const auto& t : o.timings) {
                 if (t.first == "prepare") sum->t_prepare += t.second;
                 else if (t.first == "stack (real)") sum->t_stack_real += t.second;
                 else if (t.first == "null rebuild") sum->t_null_rebuild += t.second;
                 else if (t.first == "stack (null)") sum->t_stack_null += t.second;
                 else if (t.first.rfind("peaks", 0) == 0) sum->t_peaks += t.second;
+                else if (t.first == "plan") sum->t_plan += t.second;
+                else if (t.first == "passes (batched)") sum->t_passes += t.second;
                 else sum->t_threshold += t.second;
             }
         }
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
+                                 const ftk_search_params* p, int reps, int* passes_per_launch, int* segments,
+                                 double* seconds, void* stream, char* err, size_t err_len) {
+    try {
+        check_run_args(s, d_raw, stride, mjd, p);
+        hipSetDevice(s->device);
+        const double t = s->ds->tune(d_raw, static_cast<long long>(stride), mjd, config_of(p), reps,
+                                     static_cast<hipStream_t>(stream));
+        if (passes_per_launch) *passes_per_launch = s->ds->tuned_passes;
+        if (segments) *segments = s->ds->tuned_segments;
+        if (seconds) *seconds = t;
         return FTK_OK;
     } catch (const std::exception& e) {
         put_err2(err, err_len, e.what());

@@ -231,6 +231,75 @@ void set_threshold(const SearchConfig& cfg, int width, int height, double fwhm, 
     out.method = 4;
 }
 
+std::vector<std::vector<int>> null_time_perms(int n, int runs, uint64_t seed) {
+    Xoshiro rng(seed);
+    std::vector<std::vector<int>> out;
+    for (int k = 0; k < runs; ++k) {
+        std::vector<int> p(static_cast<size_t>(n));
+        std::iota(p.begin(), p.end(), 0);
+        for (size_t i = p.size() - 1; i > 0; --i) {
+            const size_t j = static_cast<size_t>(rng.uniform() * static_cast<double>(i + 1));
+            std::swap(p[i], p[j]);
+        }
+        out.push_back(std::move(p));
+    }
+    return out;
+}
+
+PeakList order_peaks(int c, const int* x, const int* y, const float* snr, const int* vidx, size_t max_peaks) {
+    std::vector<HostPeak> pk(static_cast<size_t>(std::max(c, 0)));
+    for (int q = 0; q < c; ++q) pk[static_cast<size_t>(q)] = {x[q], y[q], snr[q], vidx[q]};
+    // the device appends in any order; the reference scans row-major
+    std::sort(pk.begin(), pk.end(), [](const HostPeak& a, const HostPeak& b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
+    std::sort(pk.begin(), pk.end(), [](const HostPeak& a, const HostPeak& b) { return a.snr > b.snr; });
+    if (pk.size() > max_peaks) pk.resize(max_peaks);
+    PeakList out;
+    out.x.reserve(pk.size());
+    out.y.reserve(pk.size());
+    out.snr.reserve(pk.size());
+    out.vidx.reserve(pk.size());
+    for (const auto& p : pk) {
+        out.x.push_back(p.x);
+        out.y.push_back(p.y);
+        out.snr.push_back(p.snr);
+        out.vidx.push_back(p.vidx);
+    }
+    return out;
+}
+
+void PassSet::release() {
+    for (void* p : {static_cast<void*>(filt), static_cast<void*>(keys), static_cast<void*>(maxm),
+                    static_cast<void*>(best), static_cast<void*>(tabs), static_cast<void*>(perm),
+                    static_cast<void*>(counts), static_cast<void*>(pk), args})
+        if (p) (void)hipFree(p);
+    if (h_counts) (void)hipHostFree(h_counts);
+    *this = PassSet();
+}
+
+void PassSet::reserve(int cap_, long long frames_, long long plane_, long long tab_ints_, long long pk_cap_,
+                      size_t args_bytes) {
+    if (cap_ <= cap && frames_ == frames && plane_ == plane && tab_ints_ <= tab_ints && pk_cap_ <= pk_cap && filt)
+        return;
+    release();
+    const size_t c = static_cast<size_t>(cap_);
+    filt = salloc<float>(c * frames_ * plane_, "allocate pass frames");
+    keys = salloc<unsigned long long>(c * plane_, "allocate pass keys");
+    maxm = salloc<float>(c * plane_, "allocate pass maps");
+    best = salloc<int>(c * plane_, "allocate pass maps");
+    tabs = salloc<int>(c * tab_ints_, "allocate pass tables");
+    perm = salloc<int>(c * frames_, "allocate pass orders");
+    counts = salloc<int>(c, "allocate peak counts");
+    pk = salloc<int>(c * 4 * pk_cap_, "allocate pass peaks");
+    args = salloc<char>(2 * c * args_bytes, "allocate pass descriptors");
+    scheck(hipHostMalloc(reinterpret_cast<void**>(&h_counts), sizeof(int) * c, hipHostMallocDefault),
+           "allocate pinned counts");
+    cap = cap_;
+    frames = frames_;
+    plane = plane_;
+    tab_ints = tab_ints_;
+    pk_cap = pk_cap_;
+}
+
 DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_(n_, w_, h_) {
     const size_t plane = static_cast<size_t>(w) * h;
     sub_ = salloc<float>(plane * n, "allocate frames");
@@ -241,6 +310,14 @@ DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_
     nbest_ = salloc<int>(plane, "allocate maps");
     pk_cap_ = std::max<size_t>(1 << 16, plane / 8);
     pk_ = salloc<int>(1 + 4 * pk_cap_, "allocate peaks");
+    scheck(hipStreamCreateWithFlags(&build_stream_, hipStreamNonBlocking), "create stream");
+    scheck(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "create stream");
+    for (int q = 0; q < 2; ++q) {
+        scheck(hipEventCreateWithFlags(&ev_built_[q], hipEventDisableTiming), "create event");
+        scheck(hipEventCreateWithFlags(&ev_stacked_[q], hipEventDisableTiming), "create event");
+        scheck(hipEventCreateWithFlags(&ev_counts_[q], hipEventDisableTiming), "create event");
+    }
+    scheck(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming), "create event");
 }
 
 DeviceSearch::~DeviceSearch() {
@@ -248,10 +325,17 @@ DeviceSearch::~DeviceSearch() {
                     static_cast<void*>(best_), static_cast<void*>(nmax_), static_cast<void*>(nbest_),
                     static_cast<void*>(pk_)})
         if (p) (void)hipFree(p);
+    for (auto& set : sets_) set.release();
+    for (int q = 0; q < 2; ++q)
+        for (hipEvent_t e : {ev_built_[q], ev_stacked_[q], ev_counts_[q]})
+            if (e) (void)hipEventDestroy(e);
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    if (build_stream_) (void)hipStreamDestroy(build_stream_);
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
 }
 
 // find_peaks (src/search.cpp:105-135): device local maxima, then the reference's
-// host order: row-major list, std::sort by descending S/N, truncation.
+// host order.
 PeakList DeviceSearch::peaks(const float* m, const int* b, float floor, int radius, int edge, size_t max_peaks,
                              hipStream_t s) {
     int* count = pk_;
@@ -274,36 +358,65 @@ PeakList DeviceSearch::peaks(const float* m, const int* b, float floor, int radi
         scheck(hipMemcpyAsync(hv.data(), pv, sizeof(int) * c, hipMemcpyDeviceToHost, s), "download peaks");
         scheck(hipStreamSynchronize(s), "peaks");
     }
-    std::vector<HostPeak> pk(static_cast<size_t>(c));
-    for (int q = 0; q < c; ++q) pk[static_cast<size_t>(q)] = {hx[q], hy[q], hs[q], hv[q]};
-    // the device appends in any order; the reference scans row-major
-    std::sort(pk.begin(), pk.end(), [](const HostPeak& a, const HostPeak& b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
-    std::sort(pk.begin(), pk.end(), [](const HostPeak& a, const HostPeak& b) { return a.snr > b.snr; });
-    if (pk.size() > max_peaks) pk.resize(max_peaks);
-    PeakList out;
-    for (const auto& p : pk) {
-        out.x.push_back(p.x);
-        out.y.push_back(p.y);
-        out.snr.push_back(p.snr);
-        out.vidx.push_back(p.vidx);
-    }
-    return out;
+    return order_peaks(c, hx.data(), hy.data(), hs.data(), hv.data(), max_peaks);
 }
+
+namespace {
+
+struct Lap {
+    SearchOutput& out;
+    hipStream_t s;
+    double t0;
+    Lap(SearchOutput& o, hipStream_t st) : out(o), s(st), t0(wall()) {}
+    void operator()(const char* name) {
+        scheck(hipStreamSynchronize(s), name);
+        const double t1 = wall();
+        out.timings.emplace_back(name, t1 - t0);
+        t0 = t1;
+    }
+};
+
+StageParams stage_params(const SearchConfig& cfg) {
+    StageParams prm;
+    prm.fwhm_px = cfg.fwhm_px;
+    prm.clip_sigma = cfg.clip_sigma;
+    prm.cr_sigma = cfg.cr_sigma;
+    prm.static_exclude = cfg.static_exclude;
+    prm.mask_scatter = cfg.mask_scatter;
+    prm.weight_scatter = cfg.weight_scatter;
+    return prm;
+}
+
+int min_count_of(const SearchConfig& cfg, int n) { return std::max(3, static_cast<int>(std::ceil(cfg.min_coverage * n))); }
+
+void shifts_for(const std::vector<std::pair<double, double>>& grid, const std::vector<double>& t, int n,
+                std::vector<int32_t>& dx, std::vector<int32_t>& dy) {
+    const size_t nv = grid.size();
+    dx.resize(nv * n);
+    dy.resize(nv * n);
+    for (size_t j = 0; j < nv; ++j)
+        for (int i = 0; i < n; ++i) {
+            dx[j * n + i] = static_cast<int32_t>(std::lround(grid[j].first * t[static_cast<size_t>(i)]));
+            dy[j * n + i] = static_cast<int32_t>(std::lround(grid[j].second * t[static_cast<size_t>(i)]));
+        }
+}
+
+void set_segments(StackPlan& p, int segments) {
+    if (segments <= 0) return;
+    const int nseg = std::max(1, std::min(segments, p.nb));
+    p.blocks_per_seg = (p.nb + nseg - 1) / nseg;
+    p.nseg = (p.nb + p.blocks_per_seg - 1) / p.blocks_per_seg;
+}
+
+}  // namespace
 
 SearchOutput DeviceSearch::run(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg,
                                float* d_max_snr, int* d_best_v, hipStream_t s) {
     if (!(cfg.fwhm_px > 0.0)) throw std::runtime_error("search: fwhm_px must be given (> 0)");
     if (n < 5) throw std::runtime_error("synthetic tracking needs at least 5 frames");
     if (w < 16 || h < 16) throw std::runtime_error("frames are too small");
+    if (cfg.passes_per_launch < 0) throw std::runtime_error("search: passes_per_launch must be >= 0");
     SearchOutput out;
-    const size_t plane = static_cast<size_t>(w) * h;
-    double t0 = wall();
-    auto lap = [&](const char* name) {
-        scheck(hipStreamSynchronize(s), name);
-        const double t1 = wall();
-        out.timings.emplace_back(name, t1 - t0);
-        t0 = t1;
-    };
     // times relative to the middle of the sequence (preprocess.cpp:541-553)
     double tmin = mjd[0], tmax = mjd[0];
     for (int i = 0; i < n; ++i) {
@@ -314,34 +427,32 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
     out.span_h = (tmax - tmin) * 24.0;
     if (out.span_h <= 0.0) throw std::runtime_error("frames must span a non-zero time interval");
     for (int i = 0; i < n; ++i) out.t_h.push_back((mjd[i] - out.t_ref_mjd) * 24.0);
+    out.grid = velocity_grid(out.span_h, cfg.vmin_pxh, cfg.vmax_pxh, cfg.grid_tol_px, &out.dv);
+    if (cfg.passes_per_launch == 1) return run_sequential(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
+    return run_batched(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
+}
 
-    StageParams prm;
-    prm.fwhm_px = cfg.fwhm_px;
-    prm.clip_sigma = cfg.clip_sigma;
-    prm.cr_sigma = cfg.cr_sigma;
-    prm.static_exclude = cfg.static_exclude;
-    prm.mask_scatter = cfg.mask_scatter;
-    prm.weight_scatter = cfg.weight_scatter;
+SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, const SearchConfig& cfg,
+                                          SearchOutput out, float* d_max_snr, int* d_best_v, hipStream_t s) {
+    const size_t plane = static_cast<size_t>(w) * h;
+    Lap lap(out, s);
+    const StageParams prm = stage_params(cfg);
     stage_.background(raw, stride, sub_, static_cast<long long>(plane), nullptr, s);
     stage_.build(sub_, static_cast<long long>(plane), out.t_h.data(), prm, filt_, static_cast<long long>(plane),
                  nullptr, nullptr, nullptr, nullptr, s);
     lap("prepare");
 
-    out.grid = velocity_grid(out.span_h, cfg.vmin_pxh, cfg.vmax_pxh, cfg.grid_tol_px, &out.dv);
     const int nv = static_cast<int>(out.grid.size());
-    const int min_count = std::max(3, static_cast<int>(std::ceil(cfg.min_coverage * n)));
+    const int min_count = min_count_of(cfg, n);
     const bool auto_thr = cfg.threshold_snr <= 0.0;
     const int runs = auto_thr ? std::max(0, cfg.null_runs) : 0;
     const int nms = std::max(2, static_cast<int>(std::lround(cfg.fwhm_px)));
     const int edge = std::max(cfg.edge_px, 1);
+    out.passes_per_launch = 1;
 
-    std::vector<int32_t> dx(static_cast<size_t>(nv) * n), dy(dx.size());
+    std::vector<int32_t> dx, dy;
     auto pass = [&](const std::vector<double>& t, float* m, int* b) {
-        for (int j = 0; j < nv; ++j)
-            for (int i = 0; i < n; ++i) {
-                dx[static_cast<size_t>(j) * n + i] = static_cast<int32_t>(std::lround(out.grid[static_cast<size_t>(j)].first * t[static_cast<size_t>(i)]));
-                dy[static_cast<size_t>(j) * n + i] = static_cast<int32_t>(std::lround(out.grid[static_cast<size_t>(j)].second * t[static_cast<size_t>(i)]));
-            }
+        shifts_for(out.grid, t, n, dx, dy);
         ftk_options opt;
         opt.device = -1;
         opt.max_mb = 0;
@@ -379,7 +490,237 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
         out.pooled.insert(out.pooled.end(), npk.snr.begin(), npk.snr.end());
         lap("peaks (null)");
     }
+    finish(cfg, runs, nv, out, s);
+    lap("threshold fit");
+    return out;
+}
 
+// The batched form. Pass 0 is the real pass, pass 1 + k the k-th null run. Passes go
+// in groups of G through two sets of device buffers: the stage builds of a group run
+// on the build stream while the previous group stacks, each group's passes stack in
+// one multi-pass launch (interior tiles) plus one for the ring tiles, and the peak
+// lists of a group come back on the copy stream while the next group stacks. One
+// plan serves every pass: the per-frame rows of its tables are permuted on the device
+// for each null run's frame order. Every pass computes exactly what the one-pass form
+// computes, in the same order per pixel and motion.
+SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
+                                       float* d_max_snr, int* d_best_v, hipStream_t s) {
+    const long long plane = static_cast<long long>(w) * h;
+    Lap lap(out, s);
+    const StageParams prm = stage_params(cfg);
+    const int nv = static_cast<int>(out.grid.size());
+    const int min_count = min_count_of(cfg, n);
+    const bool auto_thr = cfg.threshold_snr <= 0.0;
+    const int runs = auto_thr ? std::max(0, cfg.null_runs) : 0;
+    const int nms = std::max(2, static_cast<int>(std::lround(cfg.fwhm_px)));
+    const int edge = std::max(cfg.edge_px, 1);
+    const int passes = 1 + runs;
+
+    // background on the caller's stream; the build stream waits for it
+    stage_.background(raw, stride, sub_, plane, nullptr, s);
+    scheck(hipEventRecord(ev_start_, s), "record event");
+    scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
+
+    // the plan of the real pass, shared by all passes
+    std::vector<int32_t> dx, dy;
+    shifts_for(out.grid, out.t_h, n, dx, dy);
+    StackPlan plan = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
+    const int segments = cfg.segments > 0 ? cfg.segments : tuned_segments;
+    set_segments(plan, segments);
+    out.segments = plan.nseg;
+    const long long n_shift = static_cast<long long>(plan.nb) * n * 2 * kVel;
+    const long long n_box = static_cast<long long>(plan.nb) * n * 4;
+    const long long n_off = static_cast<long long>(plan.nb) * n * kVel;
+    const long long tab_ints = n_shift + n_box + n_off;
+
+    // group size: requested, tuned, or all passes, within half the free device memory
+    int G = cfg.passes_per_launch > 0 ? cfg.passes_per_launch : (tuned_passes > 0 ? tuned_passes : passes);
+    G = std::max(1, std::min(G, passes));
+    {
+        size_t free_b = 0, total_b = 0;
+        scheck(hipMemGetInfo(&free_b, &total_b), "memory info");
+        const double per_pass = 4.0 * n * plane + 24.0 * plane + 4.0 * tab_ints + 16.0 * static_cast<double>(pk_cap_) +
+                                4.0 * n + 2.0 * sizeof(StackArgs);
+        const double held = sets_[0].filt ? 2.0 * sets_[0].cap * per_pass : 0.0;  // reusable buffers count as free
+        const int fit = static_cast<int>((0.5 * (static_cast<double>(free_b) + held)) / (2.0 * per_pass));
+        if (fit < 1) throw std::runtime_error("search: not enough device memory for a pass");
+        G = std::min(G, fit);
+    }
+    out.passes_per_launch = G;
+    const int groups = (passes + G - 1) / G;
+    const int nsets = groups > 1 ? 2 : 1;
+    for (int q = 0; q < nsets; ++q)
+        sets_[q].reserve(G, n, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
+
+    // shared tables (the ones that do not depend on the frame order) and the real rows
+    std::vector<void*> owned;
+    auto upload = [&](const void* src, size_t bytes) {
+        void* d = salloc<char>(bytes, "allocate plan tables");
+        owned.push_back(d);
+        if (bytes) scheck(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s), "upload plan tables");
+        return d;
+    };
+    struct Cleanup {
+        std::vector<void*>& v;
+        ~Cleanup() {
+            for (void* p : v) (void)hipFree(p);
+        }
+    } cleanup{owned};
+    const int* d_shift0 = static_cast<const int*>(upload(plan.shift.data(), plan.shift.size() * 4));
+    const int* d_box0 = static_cast<const int*>(upload(plan.box.data(), plan.box.size() * 4));
+    const int* d_off0 = static_cast<const int*>(upload(plan.lds_off.data(), plan.lds_off.size() * 4));
+    const int* d_ubox = static_cast<const int*>(upload(plan.ubox.data(), plan.ubox.size() * 4));
+    const int* d_motion = static_cast<const int*>(upload(plan.motion.data(), plan.motion.size() * 4));
+    const float* d_inv = static_cast<const float*>(upload(plan.inv_sqrt.data(), plan.inv_sqrt.size() * 4));
+    const long long* d_covoff = static_cast<const long long*>(upload(plan.cov_off.data(), plan.cov_off.size() * 8));
+    const int* d_covx = static_cast<const int*>(upload(plan.cov_x.data(), plan.cov_x.size() * 4));
+    const int* d_covy = static_cast<const int*>(upload(plan.cov_y.data(), plan.cov_y.size() * 4));
+    scheck(hipEventRecord(ev_start_, s), "record event");
+    scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
+
+    // frame orders: identity for the real pass, the null shuffles after it
+    std::vector<std::vector<int>> perms;
+    perms.emplace_back(static_cast<size_t>(n));
+    std::iota(perms[0].begin(), perms[0].end(), 0);
+    for (auto& p : null_time_perms(n, runs, cfg.seed)) perms.push_back(std::move(p));
+    std::vector<int> all_perm;
+    for (const auto& p : perms) all_perm.insert(all_perm.end(), p.begin(), p.end());
+    int* d_perm = static_cast<int*>(upload(all_perm.data(), all_perm.size() * 4));
+
+    // descriptors of every slot of every set: fixed for the run
+    std::vector<StackArgs> hargs[2];
+    for (int q = 0; q < nsets; ++q) {
+        PassSet& S = sets_[q];
+        hargs[q].assign(static_cast<size_t>(2 * G), StackArgs{});
+        for (int g = 0; g < G; ++g) {
+            int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
+            StackLaunch L{};
+            L.frames = S.filt + static_cast<long long>(g) * n * plane;
+            L.frame_stride = plane;
+            L.row0 = 0;
+            L.row1 = h;
+            L.shift = tab;
+            L.box = tab + n_shift;
+            L.lds_off = tab + n_shift + n_box;
+            L.ubox = d_ubox;
+            L.cov_off = d_covoff;
+            L.cov_x = d_covx;
+            L.cov_y = d_covy;
+            L.motion = d_motion;
+            L.inv_sqrt = d_inv;
+            L.y_begin = 0;
+            L.y_end = h;
+            L.tiles_x = plan.tiles_x;
+            L.tiles_y = plan.tiles_y;
+            L.bps = plan.blocks_per_seg;
+            L.seg0 = 0;
+            L.nseg = plan.nseg;
+            L.keys = S.keys + static_cast<long long>(g) * plane;
+            make_stack_args(plan, L, hargs[q][static_cast<size_t>(g)], hargs[q][static_cast<size_t>(G + g)]);
+        }
+        scheck(hipMemcpyAsync(S.args, hargs[q].data(), sizeof(StackArgs) * 2 * G, hipMemcpyHostToDevice, s),
+               "upload descriptors");
+    }
+    scheck(hipEventRecord(ev_start_, s), "record event");
+    scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
+    lap("plan");
+
+    const float real_floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
+    std::vector<std::vector<double>> t_of(static_cast<size_t>(passes));
+    for (int p = 0; p < passes; ++p) {
+        t_of[static_cast<size_t>(p)].resize(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) t_of[static_cast<size_t>(p)][static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
+    }
+    std::vector<PeakList> results(static_cast<size_t>(passes));
+
+    auto enqueue = [&](int grp) {
+        const int q = grp % nsets;
+        PassSet& S = sets_[q];
+        const int p0 = grp * G, cnt = std::min(G, passes - p0);
+        for (int g = 0; g < cnt; ++g) {
+            const int p = p0 + g;
+            int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
+            const int* perm = d_perm + static_cast<long long>(p) * n;
+            scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, build_stream_), "permute tables");
+            scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, build_stream_), "permute tables");
+            scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, build_stream_),
+                   "permute tables");
+            stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, S.filt + static_cast<long long>(g) * n * plane,
+                         plane, nullptr, nullptr, nullptr, nullptr, build_stream_);
+        }
+        scheck(launch_fill_keys(S.keys, cnt * plane, build_stream_), "clear keys");
+        scheck(hipMemsetAsync(S.counts, 0, sizeof(int) * cnt, build_stream_), "clear peak counts");
+        scheck(hipEventRecord(ev_built_[q], build_stream_), "record event");
+        scheck(hipStreamWaitEvent(s, ev_built_[q], 0), "stream wait");
+        const StackArgs* dargs = static_cast<const StackArgs*>(S.args);
+        std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
+        std::vector<StackArgs> hring(hargs[q].begin() + G, hargs[q].begin() + G + cnt);
+        scheck(launch_stack_multi(hin.data(), hring.data(), dargs, dargs + G, cnt, s), "launch stack kernel");
+        scheck(launch_finalize(S.keys, cnt * plane, S.maxm, S.best, s), "finalize");
+        for (int g = 0; g < cnt; ++g) {
+            const int p = p0 + g;
+            int* px = S.pk + static_cast<long long>(g) * 4 * S.pk_cap;
+            scheck(launch_peaks(S.maxm + g * plane, S.best + g * plane, w, h, p == 0 ? real_floor : 3.0f, nms, edge,
+                                static_cast<int>(S.pk_cap), S.counts + g, px, px + S.pk_cap,
+                                reinterpret_cast<float*>(px + 2 * S.pk_cap), px + 3 * S.pk_cap, s),
+                   "peaks");
+        }
+        if (p0 == 0) {
+            if (d_max_snr) scheck(hipMemcpyAsync(d_max_snr, S.maxm, sizeof(float) * plane, hipMemcpyDeviceToDevice, s), "copy map");
+            if (d_best_v) scheck(hipMemcpyAsync(d_best_v, S.best, sizeof(int) * plane, hipMemcpyDeviceToDevice, s), "copy map");
+        }
+        scheck(hipEventRecord(ev_stacked_[q], s), "record event");
+        scheck(hipStreamWaitEvent(copy_stream_, ev_stacked_[q], 0), "stream wait");
+        scheck(hipMemcpyAsync(S.h_counts, S.counts, sizeof(int) * cnt, hipMemcpyDeviceToHost, copy_stream_),
+               "download peak counts");
+        scheck(hipEventRecord(ev_counts_[q], copy_stream_), "record event");
+    };
+    auto collect = [&](int grp) {
+        const int q = grp % nsets;
+        PassSet& S = sets_[q];
+        const int p0 = grp * G, cnt = std::min(G, passes - p0);
+        scheck(hipEventSynchronize(ev_counts_[q]), "peak counts");
+        std::vector<std::vector<int>> hv(static_cast<size_t>(cnt));
+        for (int g = 0; g < cnt; ++g) {
+            const int c = S.h_counts[g];
+            if (c < 0 || c > S.pk_cap) throw std::runtime_error("too many local maxima for the peak buffer");
+            auto& v = hv[static_cast<size_t>(g)];
+            v.resize(static_cast<size_t>(4) * c);
+            const int* px = S.pk + static_cast<long long>(g) * 4 * S.pk_cap;
+            if (c > 0)
+                for (int a = 0; a < 4; ++a)
+                    scheck(hipMemcpyAsync(v.data() + static_cast<size_t>(a) * c, px + a * S.pk_cap, sizeof(int) * c,
+                                          hipMemcpyDeviceToHost, copy_stream_),
+                           "download peaks");
+        }
+        scheck(hipStreamSynchronize(copy_stream_), "download peaks");
+        for (int g = 0; g < cnt; ++g) {
+            const int p = p0 + g, c = S.h_counts[g];
+            const auto& v = hv[static_cast<size_t>(g)];
+            results[static_cast<size_t>(p)] =
+                order_peaks(c, v.data(), v.data() + c, reinterpret_cast<const float*>(v.data() + 2 * c), v.data() + 3 * c,
+                            p == 0 ? 100000 : 20000);
+        }
+    };
+    for (int grp = 0; grp < groups; ++grp) {
+        enqueue(grp);
+        if (grp > 0) collect(grp - 1);
+    }
+    collect(groups - 1);
+    scheck(hipStreamSynchronize(build_stream_), "stage builds");
+    lap("passes (batched)");
+
+    out.peaks = std::move(results[0]);
+    for (int p = 1; p < passes; ++p)
+        out.pooled.insert(out.pooled.end(), results[static_cast<size_t>(p)].snr.begin(), results[static_cast<size_t>(p)].snr.end());
+    finish(cfg, runs, nv, out, s);
+    lap("threshold fit");
+    return out;
+}
+
+// The null-tail threshold (optionally with the FP64 sums on the device) and
+// select_peaks (search.cpp:614-620).
+void DeviceSearch::finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s) {
     double sums[10] = {0};
     const double* dsums = nullptr;
     if (cfg.fit_on_device && runs > 0) {
@@ -409,7 +750,6 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
         }
     }
     set_threshold(cfg, w, h, cfg.fwhm_px, static_cast<size_t>(nv), runs, out.pooled, out, dsums);
-    // select_peaks (search.cpp:614-620)
     for (size_t q = 0; q < out.peaks.snr.size(); ++q) {
         if (!(out.peaks.snr[q] >= out.threshold) || out.peaks.vidx[q] < 0) continue;
         if (static_cast<int>(out.selected.x.size()) >= cfg.max_candidates) break;
@@ -418,8 +758,53 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
         out.selected.snr.push_back(out.peaks.snr[q]);
         out.selected.vidx.push_back(out.peaks.vidx[q]);
     }
-    lap("threshold fit");
-    return out;
+}
+
+double DeviceSearch::tune(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg_in, int reps,
+                          hipStream_t s) {
+    SearchConfig cfg = cfg_in;
+    const int runs = cfg.threshold_snr <= 0.0 ? std::max(0, cfg.null_runs) : 0;
+    const int passes = 1 + runs;
+    // group sizes: every pass at once, about half (two groups overlap builds with
+    // stacking), and a third; segment counts around the planner's choice
+    std::vector<int> gs{passes};
+    for (int d : {2, 3})
+        if ((passes + d - 1) / d > 1) gs.push_back((passes + d - 1) / d);
+    std::sort(gs.begin(), gs.end());
+    gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
+    cfg.passes_per_launch = passes;
+    cfg.segments = 0;
+    tuned_passes = 0;
+    tuned_segments = 0;
+    const SearchOutput probe = run(raw, stride, mjd, cfg, nullptr, nullptr, s);
+    const int base = std::max(1, probe.segments);
+    std::vector<int> ss{base};
+    for (int v : {base / 4, base / 2, base * 2})
+        if (v >= 1) ss.push_back(v);
+    std::sort(ss.begin(), ss.end());
+    ss.erase(std::unique(ss.begin(), ss.end()), ss.end());
+    double best = 1e300;
+    int bg = passes, bs = base;
+    for (int g : gs)
+        for (int seg : ss) {
+            cfg.passes_per_launch = g;
+            cfg.segments = seg;
+            double t = 1e300;
+            int used = seg;
+            for (int r = 0; r < std::max(1, reps); ++r) {
+                const double t0 = wall();
+                used = run(raw, stride, mjd, cfg, nullptr, nullptr, s).segments;
+                t = std::min(t, wall() - t0);
+            }
+            if (t < best) {
+                best = t;
+                bg = g;
+                bs = used;
+            }
+        }
+    tuned_passes = bg;
+    tuned_segments = bs;
+    return best;
 }
 
 }  // namespace ftk

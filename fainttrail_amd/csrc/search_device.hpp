@@ -18,6 +18,7 @@ namespace ftk {
 hipError_t launch_peaks(const float* m, const int* best_v, int w, int h, float floor, int radius, int edge, int cap,
                         int* count, int* px, int* py, float* ps, int* pv, hipStream_t s);
 hipError_t launch_tail_sums(const float* pooled_desc, int m, double runs, double* partial, int* bad, hipStream_t s);
+hipError_t launch_permute_rows(const int* src, int* dst, int nb, int n, int row, const int* perm, hipStream_t s);
 
 // SearchParams fields on the path (include/fainttrail/search.hpp:18-90).
 struct SearchConfig {
@@ -33,6 +34,13 @@ struct SearchConfig {
     int edge_px = 4;
     uint64_t seed = 12345;
     int fit_on_device = 0;  // 1: the FP64 tail-fit sums run on the GPU (tolerance, not bits)
+    // Batched form: the real pass and the null runs go through the stacking kernel in
+    // groups of passes_per_launch passes per launch (0: the searcher's tuned value, else
+    // every pass in one group, within the memory budget; 1: one pass at a time with a
+    // plan per pass). segments: velocity-block segments per tile (0: the planner's
+    // choice). Neither changes any result bit.
+    int passes_per_launch = 1;
+    int segments = 0;
 };
 
 struct PeakList {
@@ -51,6 +59,25 @@ struct SearchOutput {
     PeakList peaks;             // real-pass peaks, sorted, at most 100000
     PeakList selected;          // peaks at or above the threshold, at most max_candidates
     std::vector<std::pair<std::string, double>> timings;
+    int passes_per_launch = 1, segments = 0;  // what the run used
+};
+
+// Device buffers of one group of stacking passes (the batched form).
+struct PassSet {
+    int cap = 0;                   // passes
+    long long plane = 0, frames = 0, tab_ints = 0, pk_cap = 0;
+    float* filt = nullptr;         // cap x frames x plane
+    unsigned long long* keys = nullptr;
+    float* maxm = nullptr;         // cap x plane
+    int* best = nullptr;
+    int* tabs = nullptr;           // cap x tab_ints: shift, box, lds_off rows of the pass
+    int* perm = nullptr;           // cap x frames
+    int* counts = nullptr;         // cap peak counts
+    int* pk = nullptr;             // cap x 4 pk_cap: x, y, snr bits, vidx
+    void* args = nullptr;          // 2 cap stacking descriptors
+    int* h_counts = nullptr;       // pinned
+    void release();
+    void reserve(int cap, long long frames, long long plane, long long tab_ints, long long pk_cap, size_t args_bytes);
 };
 
 class DeviceSearch {
@@ -66,8 +93,20 @@ public:
                      int* d_best_v, hipStream_t s);
 
     int n, w, h;
+    int tuned_passes = 0, tuned_segments = 0;  // from tune(); 0 = none
+
+    // Times the batched run over candidate group sizes and segment counts on these
+    // frames and keeps the fastest (tuned_passes, tuned_segments). Returns the seconds
+    // per search of the choice.
+    double tune(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg, int reps,
+                hipStream_t s);
 
 private:
+    SearchOutput run_sequential(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
+                                float* d_max_snr, int* d_best_v, hipStream_t s);
+    SearchOutput run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
+                             float* d_max_snr, int* d_best_v, hipStream_t s);
+    void finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s);
     PeakList peaks(const float* m, const int* b, float floor, int radius, int edge, size_t max_peaks, hipStream_t s);
     DeviceStage stage_;
     float* sub_ = nullptr;
@@ -78,11 +117,22 @@ private:
     int* nbest_ = nullptr;
     int* pk_ = nullptr;  // count + 4 arrays
     size_t pk_cap_ = 0;
+    PassSet sets_[2];
+    hipStream_t build_stream_ = nullptr, copy_stream_ = nullptr;
+    hipEvent_t ev_built_[2] = {nullptr, nullptr}, ev_stacked_[2] = {nullptr, nullptr};
+    hipEvent_t ev_counts_[2] = {nullptr, nullptr};
+    hipEvent_t ev_start_ = nullptr;
 };
 
 // Host pieces of the loop, restated from the reference.
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv);
 std::vector<std::vector<double>> null_time_orders(const std::vector<double>& t_h, int runs, uint64_t seed);
+// The same shuffles as index permutations: null_time_orders(t, runs, seed)[k][i] ==
+// t[null_time_perms(n, runs, seed)[k][i]].
+std::vector<std::vector<int>> null_time_perms(int n, int runs, uint64_t seed);
+// find_peaks' host order (search.cpp:105-135) of c device peaks: row-major, then a
+// std::sort by descending S/N, then truncation to max_peaks.
+PeakList order_peaks(int c, const int* x, const int* y, const float* snr, const int* vidx, size_t max_peaks);
 void set_threshold(const SearchConfig& cfg, int width, int height, double fwhm, size_t n_motions, int runs,
                    const std::vector<float>& pooled, SearchOutput& out, const double* device_sums);
 

@@ -3,6 +3,7 @@
 // normal equations (fit_null_tail_gauss, src/search.cpp:480-499).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "search_device.hpp"
@@ -66,6 +67,32 @@ __global__ void tail_sums_kernel(const float* __restrict__ pooled_desc, int m, d
     }
     if (threadIdx.x == 0)
         for (int q = 0; q < 9; ++q) partial[blockIdx.x * 9 + q] = red[q][0];
+}
+
+// Per-frame rows of a stacking plan's tables for a reordered frame sequence:
+// dst[(b n + i) row + e] = src[(b n + perm[i]) row + e]. A null run that assigns the
+// real pass's frame times in the order perm stacks with exactly the real pass's shifts,
+// frame i taking the shifts of frame perm[i], so its plan is the real plan with these
+// rows permuted (the blocks, coverage tables and bounds depend only on the set of
+// shifts per motion, which the permutation does not change).
+__global__ void permute_rows_kernel(const int* __restrict__ src, int* __restrict__ dst, long long total, int n,
+                                    int row, const int* __restrict__ perm) {
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < total; q += stride) {
+        const long long r = q / row;
+        const int e = static_cast<int>(q - r * row);
+        const long long b = r / n;
+        const int i = static_cast<int>(r - b * n);
+        dst[q] = src[(b * n + perm[i]) * row + e];
+    }
+}
+
+hipError_t launch_permute_rows(const int* src, int* dst, int nb, int n, int row, const int* perm, hipStream_t s) {
+    const long long total = static_cast<long long>(nb) * n * row;
+    if (total <= 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>(std::min<long long>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(permute_rows_kernel, dim3(blocks), dim3(256), 0, s, src, dst, total, n, row, perm);
+    return hipGetLastError();
 }
 
 hipError_t launch_peaks(const float* m, const int* best_v, int w, int h, float floor, int radius, int edge, int cap,

@@ -56,12 +56,6 @@ __device__ inline float d_unordered(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
-// Tile coordinates of workgroup tile index t: interior tiles form the rectangle
-// [tx_lo, tx_hi) x [ty_lo, ty_hi); the ring lists the others row by row.
-struct TileRect {
-    int tiles_x, tiles_y, tx_lo, tx_hi, ty_lo, ty_hi;
-};
-
 __device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int& ty) {
     if (!ring) {
         const int wdt = R.tx_hi - R.tx_lo;
@@ -89,14 +83,27 @@ __device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int
 
 }  // namespace
 
+// The body of one workgroup of a stacking pass: workgroup `wg` of the pass described
+// by `a` (the kernels below pick the pass and the workgroup index).
 template <bool RING>
-__global__ __launch_bounds__(kThreads) void stack_kernel(
-    const float* __restrict__ frames, long long frame_stride, int row0, int row1, int n, int w, int h,
-    const int* __restrict__ shift, const int* __restrict__ box, const int* __restrict__ ubox,
-    const int* __restrict__ lds_off, const int* __restrict__ motion, const float* __restrict__ inv_sqrt, int min_count,
-    int nb, int bps, TileRect rect, int y_begin, int y_end, int seg0, int nseg, int ntiles, int lds_rows,
-    int cov_ok, const long long* __restrict__ cov_off, const int* __restrict__ cov_x,
-    const int* __restrict__ cov_y, unsigned long long* __restrict__ keys) {
+__device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
+    const float* __restrict__ frames = a.frames;
+    const long long frame_stride = a.frame_stride;
+    const int row0 = a.row0, row1 = a.row1, n = a.n, w = a.w, h = a.h;
+    const int* __restrict__ shift = a.shift;
+    const int* __restrict__ box = a.box;
+    const int* __restrict__ ubox = a.ubox;
+    const int* __restrict__ lds_off = a.lds_off;
+    const int* __restrict__ motion = a.motion;
+    const float* __restrict__ inv_sqrt = a.inv_sqrt;
+    const int min_count = a.min_count, nb = a.nb, bps = a.bps;
+    const TileRect rect = a.rect;
+    const int y_begin = a.y_begin, y_end = a.y_end, seg0 = a.seg0, nseg = a.nseg, ntiles = a.ntiles;
+    const int lds_rows = a.lds_rows, cov_ok = a.cov_ok;
+    const long long* __restrict__ cov_off = a.cov_off;
+    const int* __restrict__ cov_x = a.cov_x;
+    const int* __restrict__ cov_y = a.cov_y;
+    unsigned long long* __restrict__ keys = a.keys;
     // two frame boxes of lds_rows x kPitch floats (lds_rows = 16 + the plan's largest
     // vertical spread), sized at launch
     extern __shared__ float lds_dyn[];
@@ -110,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     // the segments of a tile run together on one XCD (sharing its L2 copy of the
     // tile's frame boxes) and the eight XCDs work on neighbouring tiles (sharing frame
     // rows in the MALL). The grid is padded to whole groups of 8 tiles.
-    const unsigned b = blockIdx.x, xcd = b & 7u, kq = b >> 3;
+    const unsigned b = wg, xcd = b & 7u, kq = b >> 3;
     const int tile = static_cast<int>(8u * (kq / static_cast<unsigned>(nseg)) + xcd);
     const int seg = seg0 + static_cast<int>(kq % static_cast<unsigned>(nseg));
     if (tile >= ntiles) return;
@@ -304,6 +311,21 @@ __global__ __launch_bounds__(kThreads) void stack_kernel(
     }
 }
 
+template <bool RING>
+__global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
+    stack_body<RING>(a, blockIdx.x);
+}
+
+// Several passes in one launch (the real pass and null runs of a batched search, or
+// regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
+// is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
+template <bool RING>
+__global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
+    const unsigned p = blockIdx.x / per_pass;
+    const StackArgs a = passes[p];
+    stack_body<RING>(a, blockIdx.x - p * per_pass);
+}
+
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) keys[i] = 0ull;
@@ -371,25 +393,82 @@ void interior_rect(const StackPlan& p, const StackLaunch& L, int& tx_lo, int& tx
     if (tx_hi <= tx_lo || ty_hi <= ty_lo) tx_lo = tx_hi = ty_lo = ty_hi = 0;
 }
 
-hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s) {
+void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring) {
     TileRect R{L.tiles_x, L.tiles_y, 0, 0, 0, 0};
     interior_rect(p, L, R.tx_lo, R.tx_hi, R.ty_lo, R.ty_hi);
     const long long all = static_cast<long long>(L.tiles_x) * L.tiles_y;
-    const long long inner = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
-    const long long ring = all - inner;
-    const int lds_rows = kTileY + p.max_sy;
-    const size_t lds_bytes = sizeof(float) * 2 * lds_rows * kPitch;
-    auto grid = [&](long long t) { return dim3(static_cast<unsigned>((t + 7) / 8 * 8 * L.nseg)); };
-    if (inner > 0)
-        hipLaunchKernelGGL(stack_kernel<false>, grid(inner), dim3(kThreads), lds_bytes, s, L.frames, L.frame_stride,
-                           L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off, L.motion, L.inv_sqrt,
-                           p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg, static_cast<int>(inner),
-                           lds_rows, p.cov_ok ? 1 : 0, L.cov_off, L.cov_x, L.cov_y, L.keys);
-    if (ring > 0)
-        hipLaunchKernelGGL(stack_kernel<true>, grid(ring), dim3(kThreads), lds_bytes, s, L.frames, L.frame_stride,
-                           L.row0, L.row1, p.n, p.w, p.h, L.shift, L.box, L.ubox, L.lds_off, L.motion, L.inv_sqrt,
-                           p.min_count, p.nb, L.bps, R, L.y_begin, L.y_end, L.seg0, L.nseg, static_cast<int>(ring),
-                           lds_rows, p.cov_ok ? 1 : 0, L.cov_off, L.cov_x, L.cov_y, L.keys);
+    const long long in = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
+    StackArgs a{};
+    a.frames = L.frames;
+    a.frame_stride = L.frame_stride;
+    a.shift = L.shift;
+    a.box = L.box;
+    a.ubox = L.ubox;
+    a.lds_off = L.lds_off;
+    a.motion = L.motion;
+    a.inv_sqrt = L.inv_sqrt;
+    a.cov_off = L.cov_off;
+    a.cov_x = L.cov_x;
+    a.cov_y = L.cov_y;
+    a.keys = L.keys;
+    a.rect = R;
+    a.row0 = L.row0;
+    a.row1 = L.row1;
+    a.n = p.n;
+    a.w = p.w;
+    a.h = p.h;
+    a.min_count = p.min_count;
+    a.nb = p.nb;
+    a.bps = L.bps;
+    a.y_begin = L.y_begin;
+    a.y_end = L.y_end;
+    a.seg0 = L.seg0;
+    a.nseg = L.nseg;
+    a.lds_rows = kTileY + p.max_sy;
+    a.cov_ok = p.cov_ok ? 1 : 0;
+    inner = a;
+    ring = a;
+    inner.ntiles = static_cast<int>(in);
+    ring.ntiles = static_cast<int>(all - in);
+}
+
+namespace {
+// workgroups of one pass: tiles padded to whole groups of 8 (one per XCD), times the
+// segments
+unsigned pass_groups(const StackArgs& a) {
+    return static_cast<unsigned>((static_cast<long long>(a.ntiles) + 7) / 8 * 8 * a.nseg);
+}
+size_t lds_bytes_of(int lds_rows) { return sizeof(float) * 2 * lds_rows * kPitch; }
+}  // namespace
+
+hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s) {
+    StackArgs inner, ring;
+    make_stack_args(p, L, inner, ring);
+    const size_t lds = lds_bytes_of(inner.lds_rows);
+    if (inner.ntiles > 0)
+        hipLaunchKernelGGL(stack_kernel<false>, dim3(pass_groups(inner)), dim3(kThreads), lds, s, inner);
+    if (ring.ntiles > 0)
+        hipLaunchKernelGGL(stack_kernel<true>, dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
+    return hipGetLastError();
+}
+
+hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
+                              const StackArgs* dev_ring, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    unsigned per_in = 0, per_ring = 0;
+    const int lds_rows = host_inner[0].lds_rows;
+    for (int q = 0; q < count; ++q) {
+        if (host_inner[q].lds_rows != lds_rows || host_ring[q].lds_rows != lds_rows) return hipErrorInvalidValue;
+        per_in = std::max(per_in, pass_groups(host_inner[q]));
+        per_ring = std::max(per_ring, pass_groups(host_ring[q]));
+    }
+    const size_t lds = lds_bytes_of(lds_rows);
+    if (per_in > 0)
+        hipLaunchKernelGGL(stack_multi_kernel<false>, dim3(per_in * static_cast<unsigned>(count)), dim3(kThreads),
+                           lds, s, dev_inner, per_in);
+    if (per_ring > 0)
+        hipLaunchKernelGGL(stack_multi_kernel<true>, dim3(per_ring * static_cast<unsigned>(count)), dim3(kThreads),
+                           lds, s, dev_ring, per_ring);
     return hipGetLastError();
 }
 

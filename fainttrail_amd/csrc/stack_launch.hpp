@@ -28,7 +28,45 @@ struct StackLaunch {
     unsigned long long* keys = nullptr;  // (y_end - y_begin) x w packed maxima
 };
 
+// Tile coordinates of a launch: interior tiles (every block's reads inside the frame
+// rows present) form the rectangle [tx_lo, tx_hi) x [ty_lo, ty_hi); the ring lists
+// the other tiles row by row.
+struct TileRect {
+    int tiles_x, tiles_y, tx_lo, tx_hi, ty_lo, ty_hi;
+};
+
+// Everything one workgroup of a pass reads: the kernel arguments of a single-pass
+// launch, or one descriptor of a multi-pass launch (read with scalar loads).
+struct StackArgs {
+    const float* frames;
+    long long frame_stride;
+    const int* shift;
+    const int* box;
+    const int* ubox;
+    const int* lds_off;
+    const int* motion;
+    const float* inv_sqrt;
+    const long long* cov_off;
+    const int* cov_x;
+    const int* cov_y;
+    unsigned long long* keys;
+    TileRect rect;
+    int row0, row1, n, w, h, min_count, nb, bps;
+    int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
+    int pad_;
+};
+
+// Interior and ring descriptors of one launch (ntiles 0 = none of that kind).
+void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring);
+
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s);
+
+// Several passes in two launches, one over their interior tiles and one over their
+// ring tiles. host_inner / host_ring: count descriptors each, as make_stack_args
+// writes them (for the grid sizes); dev_inner / dev_ring: the same descriptors in
+// device memory. Every descriptor must have the same lds_rows.
+hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
+                              const StackArgs* dev_ring, int count, hipStream_t s);
 hipError_t launch_fill_keys(unsigned long long* keys, long long count, hipStream_t s);
 hipError_t launch_finalize(const unsigned long long* keys, long long count, float* max_snr, int* best_v,
                            hipStream_t s);

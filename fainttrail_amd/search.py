@@ -22,7 +22,8 @@ class _Params(C.Structure):
                 ("static_exclude", C.c_int), ("mask_scatter", C.c_double), ("weight_scatter", C.c_double),
                 ("min_coverage", C.c_double), ("threshold_snr", C.c_double), ("min_snr_floor", C.c_double),
                 ("null_runs", C.c_int), ("target_false_per_run", C.c_double), ("max_candidates", C.c_int),
-                ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int)]
+                ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int),
+                ("passes_per_launch", C.c_int), ("segments", C.c_int)]
 
 
 class _Summary(C.Structure):
@@ -31,7 +32,8 @@ class _Summary(C.Structure):
                 ("t_ref_mjd", C.c_double), ("span_h", C.c_double), ("n_velocities", C.c_int), ("n_peaks", C.c_int),
                 ("n_selected", C.c_int), ("n_pooled", C.c_int), ("t_prepare", C.c_double),
                 ("t_stack_real", C.c_double), ("t_null_rebuild", C.c_double), ("t_stack_null", C.c_double),
-                ("t_peaks", C.c_double), ("t_threshold", C.c_double)]
+                ("t_peaks", C.c_double), ("t_threshold", C.c_double), ("t_plan", C.c_double),
+                ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int)]
 
 
 METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"]
@@ -48,6 +50,8 @@ def _bind():
     L.ftk_searcher_destroy.argtypes = [vp]
     L.ftk_searcher_run.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.POINTER(_Summary), vp,
                                    vp, vp, C.c_char_p, sz]
+    L.ftk_searcher_tune.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), vp, C.c_char_p, sz]
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     _bound = True
@@ -76,12 +80,18 @@ class SearchParams:
     edge_px: int = 4
     seed: int = 12345
     fit_on_device: bool = False
+    # batched form (no result depends on these): passes per stacking launch (1 = one
+    # pass at a time with a plan per pass; 0 = tuned, else all passes) and
+    # velocity-block segments per tile (0 = planner / tuned)
+    passes_per_launch: int = 0
+    segments: int = 0
 
     def c(self) -> _Params:
         return _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
                        self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
                        self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
-                       self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device))
+                       self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
+                       self.passes_per_launch, self.segments)
 
 
 @dataclass
@@ -109,6 +119,8 @@ class SearchResult:
     timings: dict = field(default_factory=dict)
     max_snr: object = None
     best_v: object = None
+    passes_per_launch: int = 1
+    segments: int = 0
 
 
 class Searcher:
@@ -129,8 +141,7 @@ class Searcher:
                              s.ctypes.data_as(C.POINTER(C.c_float)), v.ctypes.data_as(I32))
         return Peaks(x, y, s, v)
 
-    def run(self, frames, mjd, params: SearchParams, keep_maps: bool = True, stream=None) -> SearchResult:
-        """frames: (n, h, w) float32 device tensor sorted by time; mjd: n mid-exposure MJDs."""
+    def _check(self, frames, mjd):
         import torch
         if (not isinstance(frames, torch.Tensor) or not frames.is_cuda or frames.dtype != torch.float32
                 or tuple(frames.shape) != (self.n, self.h, self.w) or frames.stride(2) != 1
@@ -139,6 +150,26 @@ class Searcher:
         m = np.ascontiguousarray(mjd, dtype=np.float64)
         if m.size != self.n:
             raise FtkError("frame and time counts differ")
+        return m
+
+    def tune(self, frames, mjd, params: SearchParams, reps: int = 2, stream=None):
+        """Times the batched form over group sizes and segment counts on these frames and
+        keeps the fastest for runs with passes_per_launch = 0 / segments = 0. Returns
+        (passes_per_launch, segments, seconds per search)."""
+        import torch
+        m = self._check(frames, mjd)
+        s = stream if stream is not None else torch.cuda.current_stream(frames.device)
+        prm, e = params.c(), _err()
+        g, sg, t = C.c_int(), C.c_int(), C.c_double()
+        _raise(lib().ftk_searcher_tune(self._h, frames.data_ptr(), frames.stride(0),
+                                       m.ctypes.data_as(C.POINTER(C.c_double)), C.byref(prm), reps, C.byref(g),
+                                       C.byref(sg), C.byref(t), s.cuda_stream, e, len(e)), e)
+        return g.value, sg.value, t.value
+
+    def run(self, frames, mjd, params: SearchParams, keep_maps: bool = True, stream=None) -> SearchResult:
+        """frames: (n, h, w) float32 device tensor sorted by time; mjd: n mid-exposure MJDs."""
+        import torch
+        m = self._check(frames, mjd)
         mx = bx = None
         if keep_maps:
             mx = torch.empty((self.h, self.w), dtype=torch.float32, device=frames.device)
@@ -155,8 +186,9 @@ class Searcher:
             summ.n_velocities, summ.dv_pxh, summ.t_ref_mjd, self._peaks(0, summ.n_peaks),
             self._peaks(1, summ.n_selected), pooled,
             {"prepare": summ.t_prepare, "stack (real)": summ.t_stack_real, "null rebuild": summ.t_null_rebuild,
-             "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold},
-            mx, bx)
+             "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold,
+             "plan": summ.t_plan, "passes (batched)": summ.t_passes},
+            mx, bx, summ.passes_per_launch, summ.segments)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -157,6 +157,13 @@ typedef struct ftk_search_params {
     int max_candidates, edge_px;
     unsigned long long seed;
     int fit_on_device; /* 1: the FP64 tail-fit sums run on the GPU (agrees to ~1e-12) */
+    /* Batched form (no result bit depends on these): stacking passes per launch, the
+     * real pass and the null runs in groups of this many (1: one pass at a time, a
+     * plan per pass; 0: the searcher's tuned value, else every pass in one group, within
+     * half the free device memory), and velocity-block segments per tile (0: the
+     * planner's choice, or the tuned value). */
+    int passes_per_launch;
+    int segments;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -166,6 +173,11 @@ typedef struct ftk_search_summary {
     double dv_pxh, t_ref_mjd, span_h;
     int n_velocities, n_peaks, n_selected, n_pooled;
     double t_prepare, t_stack_real, t_null_rebuild, t_stack_null, t_peaks, t_threshold; /* seconds */
+    /* batched form: host planning (with the background step) and the pipelined passes
+     * (stage builds, stacking and peaks of every pass); the group size and segment
+     * count the run used */
+    double t_plan, t_passes;
+    int passes_per_launch, segments;
 } ftk_search_summary;
 
 typedef struct ftk_searcher ftk_searcher;
@@ -180,6 +192,14 @@ void ftk_searcher_destroy(ftk_searcher* searcher);
 int ftk_searcher_run(ftk_searcher* searcher, const float* d_raw, size_t stride, const double* mjd,
                      const ftk_search_params* params, ftk_search_summary* summary, float* d_max_snr,
                      int32_t* d_best_v, void* stream, char* err, size_t err_len);
+/* Autotuner of the batched form: runs the search on these frames (params as for
+ * ftk_searcher_run) over candidate group sizes and segment counts, reps times each,
+ * and keeps the fastest for later runs with passes_per_launch = 0 / segments = 0.
+ * Writes the choice and its seconds per search (any output may be NULL). No extension
+ * of the reference: its searches run one pass at a time. */
+int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride, const double* mjd,
+                      const ftk_search_params* params, int reps, int* passes_per_launch, int* segments,
+                      double* seconds, void* stream, char* err, size_t err_len);
 /* Peaks of the last run: which 0 = real-pass peaks (sorted as find_peaks sorts them),
  * 1 = selected peaks, 2 = pooled null peak S/N values (snr only). Copies up to cap
  * entries into the non-NULL arrays and returns the total count. */

@@ -75,3 +75,64 @@ def test_search_matches_reference_tiled_search(hip, tile):
     assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
     assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
     assert got.threshold_snr == want["threshold"]
+
+
+def _same_result(a, b):
+    assert a.n_velocities == b.n_velocities
+    assert same_bits(a.max_snr.cpu().numpy(), b.max_snr.cpu().numpy())
+    assert np.array_equal(a.best_v.cpu().numpy(), b.best_v.cpu().numpy())
+    for f in ("x", "y", "vidx"):
+        assert np.array_equal(getattr(a.peaks, f), getattr(b.peaks, f))
+        assert np.array_equal(getattr(a.selected, f), getattr(b.selected, f))
+    assert same_bits(a.peaks.snr, b.peaks.snr)
+    assert same_bits(a.null_peaks, b.null_peaks)  # run after run, in order
+    assert a.threshold_snr == b.threshold_snr and a.threshold_method == b.threshold_method
+
+
+@pytest.mark.parametrize("group,segments", [(0, 0), (2, 0), (3, 5), (4, 1), (6, 64)])
+def test_batched_search_equals_one_pass_at_a_time(hip, group, segments):
+    """The batched form (one plan whose per-frame rows are permuted for each null run,
+    passes fused into multi-pass launches, stage builds on a second stream) gives the
+    one-pass-at-a-time form's results bit for bit, for any group size and segment
+    count: maps, peak lists, the pooled null peaks in run order and the threshold."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    fr, mjd, _ = synth.make_sequence(18, 200, 136, nstars=40, nmovers=3, mover_snr=(4.0, 8.0), seed=21)
+    frames = fr.cuda()
+    print(f"batched search group {group} segments {segments}", flush=True)
+    s = Searcher(18, 200, 136)
+    base = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=5, seed=4)
+    one = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 1}))
+    many = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": group, "segments": segments}))
+    torch.cuda.synchronize()
+    assert one.passes_per_launch == 1
+    assert many.passes_per_launch == (group if group else 6)
+    assert one.null_peaks.size > 0 and one.peaks.x.size > 0
+    _same_result(one, many)
+
+
+def test_batched_search_fixed_threshold_and_tuner(hip):
+    """No null runs (a fixed threshold): the batched form is the real pass alone. The
+    autotuner's choice, used by later runs, leaves every result bit unchanged."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    fr, mjd, _ = synth.make_sequence(16, 176, 128, nstars=30, nmovers=2, mover_snr=(6.0, 9.0), seed=8)
+    frames = fr.cuda()
+    s = Searcher(16, 176, 128)
+    fixed = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, threshold_snr=6.0)
+    a = s.run(frames, mjd, SearchParams(**{**fixed.__dict__, "passes_per_launch": 1}))
+    b = s.run(frames, mjd, fixed)
+    torch.cuda.synchronize()
+    assert b.threshold_method == "fixed" and b.null_peaks.size == 0
+    _same_result(a, b)
+    nulls = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=4, seed=2)
+    ref = s.run(frames, mjd, SearchParams(**{**nulls.__dict__, "passes_per_launch": 1}))
+    g, seg, secs = s.tune(frames, mjd, nulls, reps=1)
+    print(f"tuned: {g} passes per launch, {seg} segments, {secs * 1e3:.2f} ms per search", flush=True)
+    assert 1 <= g <= 5 and seg >= 1 and secs > 0
+    tuned = s.run(frames, mjd, nulls)
+    torch.cuda.synchronize()
+    assert tuned.passes_per_launch == g and tuned.segments == seg
+    _same_result(ref, tuned)
