@@ -162,6 +162,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the null-calibrated search measurement")
     ap.add_argument("--search-reps", type=int, default=3)
+    ap.add_argument("--search-only", action="store_true",
+                    help="measure only the null-calibrated search (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB with the library's plain copy kernel "
                          "(the known-size kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -177,6 +179,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.search_only:
+        print(json.dumps({"search": search_measure(args.search_reps, with_plain=not args.no_cpu)}))
+        return 0
     n, w, h = WORKLOADS[args.workload][:3]
     t, grid, dv, tab, min_count = workload_shifts(args.workload)
     dev = torch.device("cuda:0")
@@ -206,7 +211,8 @@ def main() -> int:
     value = units * args.steps / wall
     achieved = BYTES_PER_UNIT * units / kernel_s
     print(f"bench: {wall / args.steps * 1e3:.1f} ms/pass, {value:.3e} units/s", file=sys.stderr, flush=True)
-    lds_peak = ft.probe_lds(4)
+    # the inner loop reads two rows per ds_read2st64_b32 (8 B per lane per instruction)
+    lds_peak = ft.probe_lds(8)
     res = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -219,8 +225,9 @@ def main() -> int:
                      "kernel_ms": kernel_s * 1e3,
                      "lds_peak_measured": lds_peak / 1e9, "lds_frac": achieved / lds_peak,
                      "note": "algorithmic bytes = 4 B (one fp32 sample) per stacked unit; frac > 1 is reuse "
-                             "from LDS-staged frame boxes; the inner loop reads each unit's sample from LDS, "
-                             "so lds_frac (against the probe-measured LDS read bandwidth) is the binding figure"},
+                             "from LDS-staged frame boxes; the inner loop reads each unit's sample from LDS (two rows "
+                             "per ds_read2st64_b32), so lds_frac, against the LDS read bandwidth measured with "
+                             "8-byte-per-lane reads, is the binding figure"},
     }
     if args.copy_probe_mib > 0:
         cnt = args.copy_probe_mib * 262144

@@ -72,6 +72,7 @@ ftk_options default_options() {
     o.max_mb = 0.0;
     // about 10 ms of kernel time per launch at the measured stacking rate
     o.launch_adds = 1e11;
+    o.pad = 1;
     return o;
 }
 
@@ -103,7 +104,21 @@ struct ftk_plan {
     int device = 0;
     DevicePlan dev;
     DeviceBuffer<unsigned long long> keys;
+    // the frames in a zero-bordered buffer (every staged box inside it), when that
+    // costs at most twice the frames' own size
+    bool use_pad = false;
+    PadGeometry pad;
+    DeviceBuffer<float> padded;
 };
+
+namespace ftk {
+namespace {
+// Whether a padded layout is worth its memory: at most twice the frames' own size.
+bool pad_worth(const PadGeometry& g, int w, int rows) {
+    return static_cast<double>(g.pitch) * g.rows <= 2.0 * static_cast<double>(w) * rows;
+}
+}  // namespace
+}  // namespace ftk
 
 extern "C" int ftk_parse_options(const char* spec, ftk_options* out, char* err, size_t err_len) {
     if (!out) return FTK_ERROR;
@@ -122,7 +137,9 @@ extern "C" int ftk_parse_options(const char* spec, ftk_options* out, char* err, 
             if (key == "device") o.device = static_cast<int>(d);
             else if (key == "max_mb") o.max_mb = d;
             else if (key == "launch_adds") o.launch_adds = d;
-            else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds)");
+            else if (key == "pad" && (d == 0.0 || d == 1.0)) o.pad = static_cast<int>(d);
+            else if (key == "pad") throw std::runtime_error("hip backend option 'pad' must be 0 or 1");
+            else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds, pad)");
         }
         if (o.max_mb < 0 || o.launch_adds <= 0) throw std::runtime_error("hip backend options must be positive");
     } catch (const std::exception& e) {
@@ -186,7 +203,9 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
         if (opt.max_mb > 0) budget = std::min(budget, opt.max_mb * 1048576.0);
         // per output row a band also holds keys (8 w B) and the two output maps (8 w B)
         const double share = static_cast<double>(n) / (n + 4.0);
-        const std::vector<Band> bands = plan_row_bands(w, h, n, p.min_dy, p.max_dy, std::max(1.0, budget * share));
+        // rows are budgeted at the padded width (a zero border for the shifts, see below)
+        const int w_pad = std::max(0, -p.all_box[0]) + std::max(w, kTileX * p.tiles_x + std::max(0, p.all_box[2]));
+        const std::vector<Band> bands = plan_row_bands(w_pad, h, n, p.min_dy, p.max_dy, std::max(1.0, budget * share));
 
         DevicePlan dev;
         dev.upload(p);
@@ -200,19 +219,26 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             const int rows = b.r1 - b.r0, out_rows = b.y1 - b.y0;
             const size_t np = static_cast<size_t>(out_rows) * w;
             auto t0 = clk::now();
-            d_frames.reserve(static_cast<size_t>(n) * rows * w, "allocate frames");
+            plan_tiles(p, out_rows);
+            // the band's frame rows inside a zero border that holds every staged box, when
+            // that costs at most twice the rows' own size; else plain rows
+            const PadGeometry g = pad_geometry(p, b.r0, b.r1, b.y0, p.tiles_x, p.tiles_y);
+            const bool use_pad = opt.pad != 0 && pad_worth(g, w, rows);
+            const long long stride = use_pad ? static_cast<long long>(g.rows) * g.pitch : static_cast<long long>(rows) * w;
+            const long long origin = use_pad ? static_cast<long long>(g.top) * g.pitch + g.left : 0;
+            d_frames.reserve(static_cast<size_t>(n) * stride, "allocate frames");
             d_keys.reserve(np, "allocate keys");
             d_max.reserve(np, "allocate output");
             d_best.reserve(np, "allocate output");
+            if (use_pad) check(hipMemset(d_frames.p, 0, sizeof(float) * n * stride), "clear frames");
             for (int i = 0; i < n; ++i)
-                check(hipMemcpy(d_frames.p + static_cast<size_t>(i) * rows * w,
-                                pass->frames[i] + static_cast<size_t>(b.r0) * w, static_cast<size_t>(rows) * w * sizeof(float),
-                                hipMemcpyHostToDevice),
+                check(hipMemcpy2D(d_frames.p + i * stride + origin, sizeof(float) * (use_pad ? g.pitch : w),
+                                  pass->frames[i] + static_cast<size_t>(b.r0) * w, sizeof(float) * w, sizeof(float) * w,
+                                  static_cast<size_t>(rows), hipMemcpyHostToDevice),
                       "upload frames");
             auto t1 = clk::now();
             t_up += std::chrono::duration<double>(t1 - t0).count();
 
-            plan_tiles(p, out_rows);
             // launches of about launch_adds units: whole segments of blocks
             const double units_per_block = static_cast<double>(out_rows) * w * n * kVel;
             const int blocks_per_launch =
@@ -223,8 +249,10 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             check(launch_fill_keys(d_keys.p, static_cast<long long>(np), nullptr), "clear keys");
             for (int s0 = 0; s0 < nseg_total; s0 += segs_per_launch) {
                 StackLaunch L{};
-                L.frames = d_frames.p;
-                L.frame_stride = static_cast<long long>(rows) * w;
+                L.frames = d_frames.p + origin;
+                L.frame_stride = stride;
+                L.pitch = use_pad ? g.pitch : w;
+                L.padded = use_pad;
                 L.row0 = b.r0;
                 L.row1 = b.r1;
                 L.shift = dev.shift.p;
@@ -290,6 +318,14 @@ extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, c
             check(hipSetDevice(pl->device), "hipSetDevice");
             pl->dev.upload(pl->plan);
             pl->keys.reserve(static_cast<size_t>(w) * h, "allocate keys");
+            const StackPlan& p = pl->plan;
+            pl->pad = pad_geometry(p, 0, h, 0, p.tiles_x, p.tiles_y);
+            pl->use_pad = opt.pad != 0 && pad_worth(pl->pad, w, h);
+            if (pl->use_pad) {
+                const size_t count = static_cast<size_t>(n) * pl->pad.rows * pl->pad.pitch;
+                pl->padded.reserve(count, "allocate padded frames");
+                check(hipMemset(pl->padded.p, 0, count * sizeof(float)), "clear padded frames");
+            }
         } catch (...) {
             delete pl;
             throw;
@@ -315,6 +351,19 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
         StackLaunch L{};
         L.frames = d_frames;
         L.frame_stride = static_cast<long long>(frame_stride);
+        if (pl->use_pad) {
+            // the frames into the interior of the zero-bordered buffer (its border stays 0)
+            const PadGeometry& g = pl->pad;
+            const long long stride = static_cast<long long>(g.rows) * g.pitch;
+            float* origin = pl->padded.p + static_cast<long long>(g.top) * g.pitch + g.left;
+            check(launch_copy_rows(d_frames, static_cast<long long>(frame_stride), origin, stride, g.pitch, p.w, p.h,
+                                   p.n, s),
+                  "pad frames");
+            L.frames = origin;
+            L.frame_stride = stride;
+            L.pitch = g.pitch;
+            L.padded = true;
+        }
         L.row0 = 0;
         L.row1 = p.h;
         L.shift = pl->dev.shift.p;

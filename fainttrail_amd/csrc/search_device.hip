@@ -276,13 +276,14 @@ void PassSet::release() {
     *this = PassSet();
 }
 
-void PassSet::reserve(int cap_, long long frames_, long long plane_, long long tab_ints_, long long pk_cap_,
-                      size_t args_bytes) {
-    if (cap_ <= cap && frames_ == frames && plane_ == plane && tab_ints_ <= tab_ints && pk_cap_ <= pk_cap && filt)
+void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long long plane_, long long tab_ints_,
+                      long long pk_cap_, size_t args_bytes) {
+    if (cap_ <= cap && frames_ == frames && frame_floats_ <= frame_floats && plane_ == plane && tab_ints_ <= tab_ints &&
+        pk_cap_ <= pk_cap && filt)
         return;
     release();
     const size_t c = static_cast<size_t>(cap_);
-    filt = salloc<float>(c * frames_ * plane_, "allocate pass frames");
+    filt = salloc<float>(c * frames_ * frame_floats_, "allocate pass frames");
     keys = salloc<unsigned long long>(c * plane_, "allocate pass keys");
     maxm = salloc<float>(c * plane_, "allocate pass maps");
     best = salloc<int>(c * plane_, "allocate pass maps");
@@ -295,7 +296,9 @@ void PassSet::reserve(int cap_, long long frames_, long long plane_, long long t
            "allocate pinned counts");
     cap = cap_;
     frames = frames_;
+    frame_floats = frame_floats_;
     plane = plane_;
+    layout = -1;
     tab_ints = tab_ints_;
     pk_cap = pk_cap_;
 }
@@ -457,6 +460,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         opt.device = -1;
         opt.max_mb = 0;
         opt.launch_adds = 1e11;
+        opt.pad = 1;
         ftk_plan* plan = nullptr;
         char err[512] = {0};
         if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
@@ -533,13 +537,23 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const long long n_off = static_cast<long long>(plan.nb) * n * kVel;
     const long long tab_ints = n_shift + n_box + n_off;
 
+    // frames of a pass in a zero-bordered buffer that holds every staged box (the ring
+    // tiles then stage like interior ones), when that costs at most twice their size
+    const PadGeometry pad = pad_geometry(plan, 0, h, 0, plan.tiles_x, plan.tiles_y);
+    const bool use_pad = static_cast<double>(pad.pitch) * pad.rows <= 2.0 * static_cast<double>(plane);
+    const long long fstride = use_pad ? static_cast<long long>(pad.rows) * pad.pitch : plane;
+    const long long forigin = use_pad ? static_cast<long long>(pad.top) * pad.pitch + pad.left : 0;
+    const long long layout = use_pad ? ((static_cast<long long>(pad.pitch) << 40) ^ (static_cast<long long>(pad.rows) << 20) ^
+                                        (static_cast<long long>(pad.top) << 10) ^ pad.left)
+                                     : 0;
+
     // group size: requested, tuned, or all passes, within half the free device memory
     int G = cfg.passes_per_launch > 0 ? cfg.passes_per_launch : (tuned_passes > 0 ? tuned_passes : passes);
     G = std::max(1, std::min(G, passes));
     {
         size_t free_b = 0, total_b = 0;
         scheck(hipMemGetInfo(&free_b, &total_b), "memory info");
-        const double per_pass = 4.0 * n * plane + 24.0 * plane + 4.0 * tab_ints + 16.0 * static_cast<double>(pk_cap_) +
+        const double per_pass = 4.0 * n * fstride + 24.0 * plane + 4.0 * tab_ints + 16.0 * static_cast<double>(pk_cap_) +
                                 4.0 * n + 2.0 * sizeof(StackArgs);
         const double held = sets_[0].filt ? 2.0 * sets_[0].cap * per_pass : 0.0;  // reusable buffers count as free
         const int fit = static_cast<int>((0.5 * (static_cast<double>(free_b) + held)) / (2.0 * per_pass));
@@ -549,8 +563,15 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     out.passes_per_launch = G;
     const int groups = (passes + G - 1) / G;
     const int nsets = groups > 1 ? 2 : 1;
-    for (int q = 0; q < nsets; ++q)
-        sets_[q].reserve(G, n, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
+    for (int q = 0; q < nsets; ++q) {
+        PassSet& S = sets_[q];
+        S.reserve(G, n, fstride, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
+        if (use_pad && S.layout != layout) {
+            // the border must read as zeros; the interiors are rewritten by every pass
+            scheck(hipMemsetAsync(S.filt, 0, sizeof(float) * S.cap * S.frames * S.frame_floats, s), "clear frames");
+            S.layout = layout;
+        }
+    }
 
     // shared tables (the ones that do not depend on the frame order) and the real rows
     std::vector<void*> owned;
@@ -595,8 +616,10 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         for (int g = 0; g < G; ++g) {
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             StackLaunch L{};
-            L.frames = S.filt + static_cast<long long>(g) * n * plane;
-            L.frame_stride = plane;
+            L.frames = S.filt + static_cast<long long>(g) * n * fstride + forigin;
+            L.frame_stride = fstride;
+            L.pitch = use_pad ? pad.pitch : w;
+            L.padded = use_pad;
             L.row0 = 0;
             L.row1 = h;
             L.shift = tab;
@@ -645,8 +668,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, build_stream_), "permute tables");
             scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, build_stream_),
                    "permute tables");
-            stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, S.filt + static_cast<long long>(g) * n * plane,
-                         plane, nullptr, nullptr, nullptr, nullptr, build_stream_);
+            float* slot = S.filt + static_cast<long long>(g) * n * fstride;
+            if (use_pad) {
+                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, filt_, plane, nullptr, nullptr,
+                             nullptr, nullptr, build_stream_);
+                scheck(launch_copy_rows(filt_, plane, slot + forigin, fstride, pad.pitch, w, h, n, build_stream_),
+                       "pad frames");
+            } else {
+                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, slot, plane, nullptr, nullptr,
+                             nullptr, nullptr, build_stream_);
+            }
         }
         scheck(launch_fill_keys(S.keys, cnt * plane, build_stream_), "clear keys");
         scheck(hipMemsetAsync(S.counts, 0, sizeof(int) * cnt, build_stream_), "clear peak counts");

@@ -65,8 +65,9 @@ struct SearchOutput {
 // Device buffers of one group of stacking passes (the batched form).
 struct PassSet {
     int cap = 0;                   // passes
-    long long plane = 0, frames = 0, tab_ints = 0, pk_cap = 0;
-    float* filt = nullptr;         // cap x frames x plane
+    long long plane = 0, frames = 0, frame_floats = 0, tab_ints = 0, pk_cap = 0;
+    long long layout = -1;         // the frame layout the border zeros were written for
+    float* filt = nullptr;         // cap x frames x frame_floats (a padded frame each)
     unsigned long long* keys = nullptr;
     float* maxm = nullptr;         // cap x plane
     int* best = nullptr;
@@ -77,7 +78,8 @@ struct PassSet {
     void* args = nullptr;          // 2 cap stacking descriptors
     int* h_counts = nullptr;       // pinned
     void release();
-    void reserve(int cap, long long frames, long long plane, long long tab_ints, long long pk_cap, size_t args_bytes);
+    void reserve(int cap, long long frames, long long frame_floats, long long plane, long long tab_ints,
+                 long long pk_cap, size_t args_bytes);
 };
 
 class DeviceSearch {

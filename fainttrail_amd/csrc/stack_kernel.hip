@@ -85,11 +85,11 @@ __device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int
 
 // The body of one workgroup of a stacking pass: workgroup `wg` of the pass described
 // by `a` (the kernels below pick the pass and the workgroup index).
-template <bool RING>
+template <bool RING, bool PADDED>
 __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     const float* __restrict__ frames = a.frames;
     const long long frame_stride = a.frame_stride;
-    const int row0 = a.row0, row1 = a.row1, n = a.n, w = a.w, h = a.h;
+    const int row0 = a.row0, row1 = a.row1, n = a.n, w = a.w, h = a.h, pitch = a.pitch;
     const int* __restrict__ shift = a.shift;
     const int* __restrict__ box = a.box;
     const int* __restrict__ ubox = a.ubox;
@@ -164,14 +164,16 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
 
         // ---- staging of frame i's box into LDS buffer `dst` with gfx950's direct
         // global -> LDS loads, one 64-float row chunk per wavefront instruction (lane l
-        // writes the chunk's l-th float). A block that is not interior to this tile zeroes
-        // its box first and then loads only the lanes whose pixel lies in the frame rows
-        // present and in [0, w); the zeros stand for frames that do not cover.
+        // writes the chunk's l-th float). In a padded frame buffer (a zero border wide
+        // enough for every block's box) every box is loaded whole. Otherwise a block that
+        // is not interior to this tile zeroes its box first and then loads only the lanes
+        // whose pixel lies in the frame rows present and in [0, w). Either way the zeros
+        // stand for frames that do not cover.
         auto stage = [&](int i, float* dst) {
             cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
             const int sx = bx[2], R = kTileY + bx[3];
             const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
-            if (RING && !interior) {
+            if (RING && !PADDED && !interior) {
 #pragma unroll
                 for (int m = 0; m < kPreRows; ++m) {
                     const int rr = wave + kWaves * m;
@@ -182,13 +184,13 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
             }
             const float* f = frames + static_cast<long long>(i) * frame_stride;
-            if (RING && !interior) {
+            if (RING && !PADDED && !interior) {
 #pragma unroll
                 for (int m = 0; m < kPreRows; ++m) {
                     const int rr = wave + kWaves * m;
                     if (rr >= R) break;
                     const int gy = by0 + rr;
-                    const float* src = f + static_cast<long long>(gy - row0) * w + bx0;
+                    const float* src = f + static_cast<long long>(gy - row0) * pitch + bx0;
                     if (gy < row0 || gy >= row1) continue;
                     const int gx = bx0 + lane;
                     if (gx >= 0 && gx < w) __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
@@ -205,14 +207,14 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 // offset as a scalar, so a row costs no vector instruction
                 const int b1 = 4 * lane;
                 const int b2 = 4 * (kTileX + min(lane, sx - 1));
-                const float* boxp = f + static_cast<long long>(by0 - row0) * w + bx0;
+                const float* boxp = f + static_cast<long long>(by0 - row0) * pitch + bx0;
                 const __amdgpu_buffer_rsrc_t rs =
                     __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(boxp), 0, 0x7fffffff, kBufferWord3);
 #pragma unroll
                 for (int m = 0; m < kPreRows; ++m) {
                     const int rr = wave + kWaves * m;
                     if (rr >= R) break;
-                    const int row_b = rr * 4 * w;
+                    const int row_b = rr * 4 * pitch;
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
                     if (sx > 0) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
                 }
@@ -311,19 +313,19 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     }
 }
 
-template <bool RING>
+template <bool RING, bool PADDED>
 __global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
-    stack_body<RING>(a, blockIdx.x);
+    stack_body<RING, PADDED>(a, blockIdx.x);
 }
 
 // Several passes in one launch (the real pass and null runs of a batched search, or
 // regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
 // is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
-template <bool RING>
+template <bool RING, bool PADDED>
 __global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
     const unsigned p = blockIdx.x / per_pass;
     const StackArgs a = passes[p];
-    stack_body<RING>(a, blockIdx.x - p * per_pass);
+    stack_body<RING, PADDED>(a, blockIdx.x - p * per_pass);
 }
 
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
@@ -417,6 +419,8 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.n = p.n;
     a.w = p.w;
     a.h = p.h;
+    a.pitch = L.pitch > 0 ? L.pitch : p.w;
+    a.padded = L.padded ? 1 : 0;
     a.min_count = p.min_count;
     a.nb = p.nb;
     a.bps = L.bps;
@@ -446,9 +450,13 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s)
     make_stack_args(p, L, inner, ring);
     const size_t lds = lds_bytes_of(inner.lds_rows);
     if (inner.ntiles > 0)
-        hipLaunchKernelGGL(stack_kernel<false>, dim3(pass_groups(inner)), dim3(kThreads), lds, s, inner);
-    if (ring.ntiles > 0)
-        hipLaunchKernelGGL(stack_kernel<true>, dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
+        hipLaunchKernelGGL((stack_kernel<false, false>), dim3(pass_groups(inner)), dim3(kThreads), lds, s, inner);
+    if (ring.ntiles > 0) {
+        if (ring.padded)
+            hipLaunchKernelGGL((stack_kernel<true, true>), dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
+        else
+            hipLaunchKernelGGL((stack_kernel<true, false>), dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
+    }
     return hipGetLastError();
 }
 
@@ -457,18 +465,47 @@ hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host
     if (count <= 0) return hipSuccess;
     unsigned per_in = 0, per_ring = 0;
     const int lds_rows = host_inner[0].lds_rows;
+    const int padded = host_ring[0].padded;
     for (int q = 0; q < count; ++q) {
-        if (host_inner[q].lds_rows != lds_rows || host_ring[q].lds_rows != lds_rows) return hipErrorInvalidValue;
+        if (host_inner[q].lds_rows != lds_rows || host_ring[q].lds_rows != lds_rows || host_ring[q].padded != padded)
+            return hipErrorInvalidValue;
         per_in = std::max(per_in, pass_groups(host_inner[q]));
         per_ring = std::max(per_ring, pass_groups(host_ring[q]));
     }
     const size_t lds = lds_bytes_of(lds_rows);
+    const dim3 block(kThreads);
     if (per_in > 0)
-        hipLaunchKernelGGL(stack_multi_kernel<false>, dim3(per_in * static_cast<unsigned>(count)), dim3(kThreads),
-                           lds, s, dev_inner, per_in);
-    if (per_ring > 0)
-        hipLaunchKernelGGL(stack_multi_kernel<true>, dim3(per_ring * static_cast<unsigned>(count)), dim3(kThreads),
-                           lds, s, dev_ring, per_ring);
+        hipLaunchKernelGGL((stack_multi_kernel<false, false>), dim3(per_in * static_cast<unsigned>(count)), block, lds, s,
+                           dev_inner, per_in);
+    if (per_ring > 0) {
+        if (padded)
+            hipLaunchKernelGGL((stack_multi_kernel<true, true>), dim3(per_ring * static_cast<unsigned>(count)), block,
+                               lds, s, dev_ring, per_ring);
+        else
+            hipLaunchKernelGGL((stack_multi_kernel<true, false>), dim3(per_ring * static_cast<unsigned>(count)), block,
+                               lds, s, dev_ring, per_ring);
+    }
+    return hipGetLastError();
+}
+
+// Frame rows into a padded buffer: frame i's rows [0, rows) of w floats from
+// src + i * src_stride (pitch w) to dst + i * dst_stride (pitch dst_pitch).
+__global__ void copy_rows_kernel(const float* __restrict__ src, long long src_stride, float* __restrict__ dst,
+                                 long long dst_stride, int dst_pitch, int w) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    dst[i * dst_stride + static_cast<long long>(y) * dst_pitch + x] = src[i * src_stride + static_cast<long long>(y) * w + x];
+}
+
+hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
+                            int w, int rows, int n, hipStream_t s) {
+    if (w <= 0 || rows <= 0 || n <= 0) return hipSuccess;
+    for (int y0 = 0; y0 < rows; y0 += 32768) {
+        const int ry = std::min(32768, rows - y0);
+        hipLaunchKernelGGL(copy_rows_kernel, dim3((w + 255) / 256, ry, n), dim3(256), 0, s,
+                           src + static_cast<long long>(y0) * w, src_stride, dst + static_cast<long long>(y0) * dst_pitch,
+                           dst_stride, dst_pitch, w);
+    }
     return hipGetLastError();
 }
 

@@ -10,10 +10,16 @@ namespace ftk {
 // One launch of the stacking kernel over all tiles of output rows [y_begin, y_end)
 // and the block segments [seg0, seg0 + nseg) (bps blocks each). The frames buffer
 // holds frame rows [row0, row1) of every frame, frame i at frames + i * frame_stride.
+//
+// Frame layout: pixel (x, y) of frame i is frames[i * frame_stride + (y - row0) * pitch
+// + x] (pitch 0 = w). A padded buffer (padded = true) holds zeros around the frame
+// rows, laid out as pad_geometry says, so that every box a tile stages lies inside it.
 struct StackLaunch {
     const float* frames = nullptr;
     long long frame_stride = 0;
     int row0 = 0, row1 = 0;
+    int pitch = 0;
+    bool padded = false;
     const int* shift = nullptr;
     const int* box = nullptr;
     const int* ubox = nullptr;
@@ -51,10 +57,12 @@ struct StackArgs {
     const int* cov_y;
     unsigned long long* keys;
     TileRect rect;
-    int row0, row1, n, w, h, min_count, nb, bps;
+    int row0, row1, n, w, h, pitch, padded, min_count, nb, bps;
     int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
-    int pad_;
 };
+
+hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
+                            int w, int rows, int n, hipStream_t s);
 
 // Interior and ring descriptors of one launch (ntiles 0 = none of that kind).
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring);

@@ -251,4 +251,18 @@ std::vector<Band> plan_row_bands(int w, int h, int n, int min_dy, int max_dy, do
     return bands;
 }
 
+PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, int tiles_x, int tiles_y) {
+    const int* u = p.all_box;
+    PadGeometry g;
+    // columns read: [min dx, 64 tiles_x - 1 + max dx] (a tile's staged box ends 63 + its
+    // largest dx past the tile start); rows: [y_begin + min dy, y_begin + 16 tiles_y - 1 + max dy]
+    g.left = std::max(0, -u[0]);
+    g.pitch = g.left + std::max(p.w, kTileX * tiles_x + std::max(0, u[2]));
+    const int first = std::min(row0, y_begin + std::min(0, u[1]));
+    const int last = std::max(row1, y_begin + kTileY * tiles_y + std::max(0, u[3]));
+    g.top = row0 - first;
+    g.rows = last - first;
+    return g;
+}
+
 }  // namespace ftk

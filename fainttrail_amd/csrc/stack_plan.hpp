@@ -83,4 +83,13 @@ struct Band {
 };
 std::vector<Band> plan_row_bands(int w, int h, int n, int min_dy, int max_dy, double budget_bytes);
 
+// Zero border of a padded frame buffer for frame rows [row0, row1) and the tiles of
+// output rows from y_begin: `left` zero columns, rows of `pitch` floats, `top` zero
+// rows above row0, `rows` rows in all; frames points at (x 0, row0), i.e. top * pitch +
+// left floats into a frame's buffer.
+struct PadGeometry {
+    int left = 0, pitch = 0, top = 0, rows = 0;
+};
+PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, int tiles_x, int tiles_y);
+
 }  // namespace ftk

@@ -33,16 +33,19 @@ extern "C" {
  * Same shape as ft_progress_fn (fainttrail.h:71) and ProgressFn (types.hpp:76). */
 typedef int (*ftk_progress_fn)(double fraction, const char* stage, void* user);
 
-/* Backend options, "hip:device=N,max_mb=M,launch_adds=A".
+/* Backend options, "hip:device=N,max_mb=M,launch_adds=A,pad=P".
  * Replaces CudaOptions / parse_cuda_options (src/cuda_plan.hpp:27-32,
  * src/cuda_plan.cpp:14-33): device -1 = FAINTTRAIL_HIP_DEVICE or 0; max_mb caps the
  * device memory for frame rows (0 = 80 % of free memory; small values force row
  * bands); launch_adds = stacked units per kernel launch (progress and cancel
- * granularity). */
+ * granularity); pad = 1 (default) keeps the frames in HBM inside a zero border that holds
+ * every shifted read (when that costs at most twice their size), 0 = plain rows. The
+ * results do not depend on pad. */
 typedef struct ftk_options {
     int device;
     double max_mb;
     double launch_adds;
+    int pad;
 } ftk_options;
 
 /* Parses the part after "hip:" (NULL or "" = defaults). Unknown keys or

@@ -38,8 +38,9 @@ def test_options_parse_like_the_reference():
     o = ft.parse_options("device=1,max_mb=64")
     assert o.device == 1 and o.max_mb == 64
     o = ft.parse_options("")
-    assert o.device == -1 and o.max_mb == 0 and o.launch_adds > 0
-    for bad in ("speed=11", "device=", "max_mb=abc", "max_mb=-1", "launch_adds=0"):
+    assert o.device == -1 and o.max_mb == 0 and o.launch_adds > 0 and o.pad == 1
+    assert ft.parse_options("pad=0").pad == 0
+    for bad in ("speed=11", "device=", "max_mb=abc", "max_mb=-1", "launch_adds=0", "pad=2"):
         with pytest.raises(ft.FtkError):
             ft.parse_options(bad)
 

@@ -25,12 +25,14 @@ def run_hip(ft, frames, dx, dy, min_count, options=""):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("options", ["", "max_mb=0.01,launch_adds=1", "max_mb=0.003,launch_adds=3000"])
+@pytest.mark.parametrize("options", ["", "pad=0", "max_mb=0.01,launch_adds=1", "max_mb=0.003,launch_adds=3000",
+                                     "max_mb=0.01,launch_adds=1,pad=0"])
 def test_reference_test_inputs_bit_identical(hip, golden, name, options):
     """test_backend_contract / test_shift_tables / test_cuda_algorithm inputs (NaN,
     shifts beyond the frame, 171 motions, curved tables), unbanded and forced into
     row bands with tiny launches (test_main.cpp:602-609 runs "cuda:max_mb=0.01,
-    launch_adds=1")."""
+    launch_adds=1"); pad=0 keeps plain frame rows (masked edge staging) instead of
+    the zero-bordered layout."""
     g = case(golden, name)
     print(f"case {name} options '{options}'", flush=True)
     m, b = run_hip(hip, g["frames"], g["dx"], g["dy"], g["min_count"], options)
@@ -77,12 +79,13 @@ def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
     mc = max(3, int(np.ceil(0.7 * n)))
     print("cfg1 shape: oracle", flush=True)
     om, ob = po.stack_blocked(frames, tab.dx, tab.dy, mc)
-    print("cfg1 shape: device plan", flush=True)
-    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
     d = torch.from_numpy(frames).cuda()
-    m, b = plan.run(d)
-    torch.cuda.synchronize()
-    assert same_bits(m.cpu().numpy(), om) and np.array_equal(b.cpu().numpy(), ob)
+    for opts in ("", "pad=0"):
+        print(f"cfg1 shape: device plan '{opts}'", flush=True)
+        plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc, opts)
+        m, b = plan.run(d)
+        torch.cuda.synchronize()
+        assert same_bits(m.cpu().numpy(), om) and np.array_equal(b.cpu().numpy(), ob), opts
     print("cfg1 shape: host pass", flush=True)
     m2, b2 = run_hip(hip, frames, tab.dx, tab.dy, mc)
     assert same_bits(m2, om) and np.array_equal(b2, ob)
@@ -94,12 +97,14 @@ def test_plan_reuse_and_stride(hip, golden):
     g = case(golden, "cuda-linear")
     fr = g["frames"]
     n, h, w = fr.shape
-    plan = hip.Plan(n, w, h, g["dx"], g["dy"], int(g["min_count"]))
     big = torch.zeros((n, h * w + 77), dtype=torch.float32, device="cuda")
     big[:, : h * w] = torch.from_numpy(fr.reshape(n, -1)).cuda()
     m = torch.empty((h, w), dtype=torch.float32, device="cuda")
     b = torch.empty((h, w), dtype=torch.int32, device="cuda")
-    for _ in range(3):
+    plan = hip.Plan(n, w, h, g["dx"], g["dy"], int(g["min_count"]))
+    for k in range(4):
+        if k == 3:
+            plan = hip.Plan(n, w, h, g["dx"], g["dy"], int(g["min_count"]), "pad=0")
         plan.run_ptr(big.data_ptr(), h * w + 77, m.data_ptr(), b.data_ptr(), torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         assert same_bits(m.cpu().numpy(), g["max_snr"]) and np.array_equal(b.cpu().numpy(), g["best_v"])
