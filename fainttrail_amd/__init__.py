@@ -85,6 +85,12 @@ def lib() -> C.CDLL:
     L.ftk_plan_units.restype = C.c_double
     L.ftk_plan_blocks.argtypes = [C.c_void_p]
     L.ftk_plan_destroy.argtypes = [C.c_void_p]
+    L.ftk_plan_store.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_char_p, C.c_size_t]
+    L.ftk_plan_run_stored.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+    L.ftk_plan_scales.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.ftk_stack_fp64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_char_p, C.c_size_t]
     L.ftk_copy_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
     L.ftk_probe_lds.argtypes = [C.c_int, C.POINTER(C.c_double)]
     L.ftk_emulate_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32),
@@ -337,11 +343,79 @@ class Plan:
         self.run_ptr(frames.data_ptr(), frames.stride(0), max_snr.data_ptr(), best_v.data_ptr(), s.cuda_stream)
         return max_snr, best_v
 
+    def store(self, frames, storage: str = "f16", stream=None) -> None:
+        """Encodes (n, h, w) fp32 device frames into the plan's 16-bit storage ("f16" or
+        "bf16"): q = RNE(g / s_i) with s_i = 2^(e - 14), max|g_i| = m 2^e."""
+        import torch
+        self._check_frames(frames)
+        code = STORAGES[storage]
+        if code == 0:
+            raise FtkError("Plan.store: storage must be 'f16' or 'bf16'")
+        s = stream if stream is not None else torch.cuda.current_stream(frames.device)
+        e = _err()
+        _raise(lib().ftk_plan_store(self._h, C.c_void_p(frames.data_ptr()), frames.stride(0), code, C.c_void_p(s.cuda_stream),
+                                    e, len(e)), e)
+        self._stored_device = frames.device
+
+    def run_stored(self, max_snr=None, best_v=None, stream=None):
+        """A pass over the stored 16-bit frames. Returns (max_snr, best_v) tensors."""
+        import torch
+        dev = getattr(self, "_stored_device", None)
+        if dev is None:
+            raise FtkError("Plan.run_stored: no frames stored")
+        if max_snr is None:
+            max_snr = torch.empty((self.h, self.w), dtype=torch.float32, device=dev)
+        if best_v is None:
+            best_v = torch.empty((self.h, self.w), dtype=torch.int32, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        e = _err()
+        _raise(lib().ftk_plan_run_stored(self._h, C.c_void_p(max_snr.data_ptr()), C.c_void_p(best_v.data_ptr()),
+                                         C.c_void_p(s.cuda_stream), e, len(e)), e)
+        return max_snr, best_v
+
+    def scales(self) -> np.ndarray:
+        """The per-frame scales of the stored frames."""
+        out = np.empty(self.n, np.float32)
+        if lib().ftk_plan_scales(self._h, out.ctypes.data_as(C.POINTER(C.c_float))) != 0:
+            raise FtkError("Plan.scales: no frames stored")
+        return out
+
+    def _check_frames(self, frames):
+        import torch
+        if frames.dtype != torch.float32 or not frames.is_cuda or frames.dim() != 3:
+            raise FtkError("Plan needs a (n, h, w) float32 device tensor")
+        if tuple(frames.shape) != (self.n, self.h, self.w) or frames.stride(2) != 1 or frames.stride(1) != self.w:
+            raise FtkError("Plan: frames must be row-contiguous (n, h, w) of the plan's shape")
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value and _lib is not None:
             _lib.ftk_plan_destroy(h)
             self._h = None
+
+
+STORAGES = {"f32": 0, "f16": 1, "bf16": 2}
+
+
+def stack_fp64(frames, dx, dy, min_count: int, y0: int = 0, y1=None):
+    """The stacking contract in FP64 on the GPU (ftk_stack_fp64) for output rows
+    [y0, y1) of (n, h, w) fp32 device frames: returns (max_snr float64, best_v int32)
+    device tensors of (y1 - y0, w)."""
+    import torch
+    n, h, w = frames.shape
+    y1 = h if y1 is None else y1
+    dx, dy = _i32(dx), _i32(dy)
+    if frames.dtype != torch.float32 or not frames.is_cuda or frames.stride(2) != 1 or frames.stride(1) != w:
+        raise FtkError("stack_fp64 needs row-contiguous (n, h, w) float32 device frames")
+    out = torch.empty((max(0, y1 - y0), w), dtype=torch.float64, device=frames.device)
+    best = torch.empty((max(0, y1 - y0), w), dtype=torch.int32, device=frames.device)
+    s = torch.cuda.current_stream(frames.device)
+    e = _err()
+    _raise(lib().ftk_stack_fp64(n, w, h, C.c_void_p(frames.data_ptr()), frames.stride(0), dx.shape[0],
+                                _ptr(dx, C.c_int32), _ptr(dy, C.c_int32), int(min_count), int(y0), int(y1),
+                                C.c_void_p(out.data_ptr()), C.c_void_p(best.data_ptr()), C.c_void_p(s.cuda_stream),
+                                e, len(e)), e)
+    return out, best
 
 
 def velocity_grid(span_h: float, vmin_pxh: float, vmax_pxh: float, grid_tol_px: float):

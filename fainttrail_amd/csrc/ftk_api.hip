@@ -17,6 +17,7 @@
 
 #include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
+#include "store.hpp"
 #include "stack_plan.hpp"
 
 namespace ftk {
@@ -109,6 +110,13 @@ struct ftk_plan {
     bool use_pad = false;
     PadGeometry pad;
     DeviceBuffer<float> padded;
+    // 16-bit storage (ftk_plan_store): frames in a zero-bordered 16-bit buffer with a
+    // power-of-two scale per frame
+    int storage = 0;
+    PadGeometry pad16;
+    DeviceBuffer<uint16_t> stored;
+    DeviceBuffer<float> scales;
+    DeviceBuffer<unsigned> maxbits;
 };
 
 namespace ftk {
@@ -338,6 +346,43 @@ extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, c
     }
 }
 
+namespace ftk {
+namespace {
+// The launch of a whole-frame pass over the plan's tables; the caller sets the frames.
+StackLaunch plan_launch(ftk_plan* pl) {
+    const StackPlan& p = pl->plan;
+    StackLaunch L{};
+    L.row0 = 0;
+    L.row1 = p.h;
+    L.shift = pl->dev.shift.p;
+    L.box = pl->dev.box.p;
+    L.ubox = pl->dev.ubox.p;
+    L.lds_off = pl->dev.lds_off.p;
+    L.cov_off = pl->dev.cov_off.p;
+    L.cov_x = pl->dev.cov_x.p;
+    L.cov_y = pl->dev.cov_y.p;
+    L.motion = pl->dev.motion.p;
+    L.inv_sqrt = pl->dev.inv_sqrt.p;
+    L.y_begin = 0;
+    L.y_end = p.h;
+    L.tiles_x = p.tiles_x;
+    L.tiles_y = p.tiles_y;
+    L.bps = p.blocks_per_seg;
+    L.seg0 = 0;
+    L.nseg = p.nseg;
+    L.keys = pl->keys.p;
+    return L;
+}
+
+void run_launch(ftk_plan* pl, const StackLaunch& L, float* d_max_snr, int32_t* d_best_v, hipStream_t s) {
+    const long long np = static_cast<long long>(pl->plan.w) * pl->plan.h;
+    check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
+    check(launch_stack(pl->plan, L, s), "launch stack kernel");
+    check(launch_finalize(pl->keys.p, np, d_max_snr, d_best_v, s), "finalize");
+}
+}  // namespace
+}  // namespace ftk
+
 extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_stride, float* d_max_snr,
                             int32_t* d_best_v, void* stream, char* err, size_t err_len) {
     try {
@@ -346,9 +391,7 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
         if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_run: frame stride smaller than a frame");
         check(hipSetDevice(pl->device), "hipSetDevice");
         hipStream_t s = static_cast<hipStream_t>(stream);
-        const long long np = static_cast<long long>(p.w) * p.h;
-        check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
-        StackLaunch L{};
+        StackLaunch L = plan_launch(pl);
         L.frames = d_frames;
         L.frame_stride = static_cast<long long>(frame_stride);
         if (pl->use_pad) {
@@ -364,27 +407,95 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
             L.pitch = g.pitch;
             L.padded = true;
         }
-        L.row0 = 0;
-        L.row1 = p.h;
-        L.shift = pl->dev.shift.p;
-        L.box = pl->dev.box.p;
-        L.ubox = pl->dev.ubox.p;
-        L.lds_off = pl->dev.lds_off.p;
-        L.cov_off = pl->dev.cov_off.p;
-        L.cov_x = pl->dev.cov_x.p;
-        L.cov_y = pl->dev.cov_y.p;
-        L.motion = pl->dev.motion.p;
-        L.inv_sqrt = pl->dev.inv_sqrt.p;
-        L.y_begin = 0;
-        L.y_end = p.h;
-        L.tiles_x = p.tiles_x;
-        L.tiles_y = p.tiles_y;
-        L.bps = p.blocks_per_seg;
-        L.seg0 = 0;
-        L.nseg = p.nseg;
-        L.keys = pl->keys.p;
-        check(launch_stack(p, L, s), "launch stack kernel");
-        check(launch_finalize(pl->keys.p, np, d_max_snr, d_best_v, s), "finalize");
+        run_launch(pl, L, d_max_snr, d_best_v, s);
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_store(ftk_plan* pl, const float* d_frames, size_t frame_stride, int storage, void* stream,
+                              char* err, size_t err_len) {
+    try {
+        if (!pl || !d_frames) throw std::runtime_error("ftk_plan_store: null argument");
+        if (storage != FTK_STORE_F16 && storage != FTK_STORE_BF16)
+            throw std::runtime_error("ftk_plan_store: storage must be FTK_STORE_F16 or FTK_STORE_BF16");
+        const StackPlan& p = pl->plan;
+        if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_store: frame stride smaller than a frame");
+        check(hipSetDevice(pl->device), "hipSetDevice");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (!pl->stored.p) {
+            // 16-bit frames are always padded (their fetch reads whole pixel pairs)
+            pl->pad16 = pad_geometry16(pad_geometry(p, 0, p.h, 0, p.tiles_x, p.tiles_y));
+            const size_t count = static_cast<size_t>(p.n) * pl->pad16.rows * pl->pad16.pitch;
+            pl->stored.reserve(count, "allocate 16-bit frames");
+            check(hipMemsetAsync(pl->stored.p, 0, count * sizeof(uint16_t), s), "clear 16-bit frames");
+            pl->scales.reserve(static_cast<size_t>(p.n), "allocate scales");
+            pl->maxbits.reserve(static_cast<size_t>(p.n), "allocate scratch");
+        }
+        const PadGeometry& g = pl->pad16;
+        const long long stride = static_cast<long long>(g.rows) * g.pitch;
+        uint16_t* origin = pl->stored.p + static_cast<long long>(g.top) * g.pitch + g.left;
+        check(launch_store(d_frames, static_cast<long long>(frame_stride), p.w, p.n, p.w, p.h, storage, origin, stride,
+                           g.pitch, pl->maxbits.p, pl->scales.p, s),
+              "store frames");
+        pl->storage = storage;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_run_stored(ftk_plan* pl, float* d_max_snr, int32_t* d_best_v, void* stream, char* err,
+                                   size_t err_len) {
+    try {
+        if (!pl || !d_max_snr || !d_best_v) throw std::runtime_error("ftk_plan_run_stored: null argument");
+        if (!pl->storage) throw std::runtime_error("ftk_plan_run_stored: no frames stored (ftk_plan_store)");
+        check(hipSetDevice(pl->device), "hipSetDevice");
+        const PadGeometry& g = pl->pad16;
+        StackLaunch L = plan_launch(pl);
+        L.frames = pl->stored.p + static_cast<long long>(g.top) * g.pitch + g.left;
+        L.frame_stride = static_cast<long long>(g.rows) * g.pitch;
+        L.pitch = g.pitch;
+        L.padded = true;
+        L.storage = pl->storage;
+        L.scale = pl->scales.p;
+        run_launch(pl, L, d_max_snr, d_best_v, static_cast<hipStream_t>(stream));
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_scales(const ftk_plan* pl, float* scales) {
+    if (!pl || !scales || !pl->storage) return FTK_ERROR;
+    if (hipSetDevice(pl->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return FTK_ERROR;
+    return hipMemcpy(scales, pl->scales.p, sizeof(float) * pl->plan.n, hipMemcpyDeviceToHost) == hipSuccess ? FTK_OK
+                                                                                                            : FTK_ERROR;
+}
+
+extern "C" int ftk_stack_fp64(int n, int w, int h, const float* d_frames, size_t frame_stride, int nv,
+                              const int32_t* dx, const int32_t* dy, int min_count, int y0, int y1, double* d_max_snr,
+                              int32_t* d_best_v, void* stream, char* err, size_t err_len) {
+    try {
+        if (!d_frames || !dx || !dy || !d_max_snr || !d_best_v) throw std::runtime_error("ftk_stack_fp64: null argument");
+        if (n < 1 || w < 1 || h < 1 || nv < 1 || min_count < 1) throw std::runtime_error("ftk_stack_fp64: bad sizes");
+        if (y0 < 0 || y1 > h || y1 < y0) throw std::runtime_error("ftk_stack_fp64: row range outside the frame");
+        if (frame_stride < static_cast<size_t>(w) * h) throw std::runtime_error("ftk_stack_fp64: frame stride smaller than a frame");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t cnt = static_cast<size_t>(nv) * n;
+        DeviceBuffer<int> ddx, ddy;
+        ddx.reserve(cnt, "allocate shifts");
+        ddy.reserve(cnt, "allocate shifts");
+        check(hipMemcpyAsync(ddx.p, dx, cnt * sizeof(int), hipMemcpyHostToDevice, s), "upload shifts");
+        check(hipMemcpyAsync(ddy.p, dy, cnt * sizeof(int), hipMemcpyHostToDevice, s), "upload shifts");
+        check(launch_fp64(d_frames, static_cast<long long>(frame_stride), n, w, h, nv, ddx.p, ddy.p, min_count, y0, y1,
+                          d_max_snr, d_best_v, s),
+              "fp64 kernel");
+        check(hipStreamSynchronize(s), "fp64 kernel");  // the tables are freed on return
         return FTK_OK;
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());

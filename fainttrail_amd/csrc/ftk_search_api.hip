@@ -46,6 +46,7 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->fit_on_device = 0;
     p->passes_per_launch = 0;
     p->segments = 0;
+    p->storage = FTK_STORE_F32;
 }
 
 namespace {
@@ -71,6 +72,7 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.fit_on_device = p->fit_on_device;
     c.passes_per_launch = p->passes_per_launch;
     c.segments = p->segments;
+    c.storage = p->storage;
     return c;
 }
 
@@ -82,6 +84,8 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
         if (mjd[i] < mjd[i - 1]) throw std::runtime_error("ftk_searcher_run: frames must be sorted by time");
     if (p->passes_per_launch < 0 || p->segments < 0)
         throw std::runtime_error("ftk_searcher_run: passes_per_launch and segments must be >= 0");
+    if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
+        throw std::runtime_error("ftk_searcher_run: unknown frame storage");
 }
 }  // namespace
 

@@ -11,6 +11,7 @@
 #include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
 #include "stack_plan.hpp"
+#include "store.hpp"
 
 namespace ftk {
 
@@ -270,7 +271,8 @@ PeakList order_peaks(int c, const int* x, const int* y, const float* snr, const 
 void PassSet::release() {
     for (void* p : {static_cast<void*>(filt), static_cast<void*>(keys), static_cast<void*>(maxm),
                     static_cast<void*>(best), static_cast<void*>(tabs), static_cast<void*>(perm),
-                    static_cast<void*>(counts), static_cast<void*>(pk), args})
+                    static_cast<void*>(counts), static_cast<void*>(pk), static_cast<void*>(scl),
+                    static_cast<void*>(maxb), args})
         if (p) (void)hipFree(p);
     if (h_counts) (void)hipHostFree(h_counts);
     *this = PassSet();
@@ -291,6 +293,8 @@ void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long
     perm = salloc<int>(c * frames_, "allocate pass orders");
     counts = salloc<int>(c, "allocate peak counts");
     pk = salloc<int>(c * 4 * pk_cap_, "allocate pass peaks");
+    scl = salloc<float>(c * frames_, "allocate frame scales");
+    maxb = salloc<unsigned>(c * frames_, "allocate frame scratch");
     args = salloc<char>(2 * c * args_bytes, "allocate pass descriptors");
     scheck(hipHostMalloc(reinterpret_cast<void**>(&h_counts), sizeof(int) * c, hipHostMallocDefault),
            "allocate pinned counts");
@@ -465,7 +469,13 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         char err[512] = {0};
         if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
             throw std::runtime_error(err);
-        const int st = ftk_plan_run(plan, filt_, plane, m, b, s, err, sizeof err);
+        int st = FTK_OK;
+        if (cfg.storage == kStoreF32) {
+            st = ftk_plan_run(plan, filt_, plane, m, b, s, err, sizeof err);
+        } else {
+            st = ftk_plan_store(plan, filt_, plane, cfg.storage, s, err, sizeof err);
+            if (st == FTK_OK) st = ftk_plan_run_stored(plan, m, b, s, err, sizeof err);
+        }
         // the plan's device tables must outlive the launch
         scheck(hipStreamSynchronize(s), "stack pass");
         ftk_plan_destroy(plan);
@@ -538,13 +548,19 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const long long tab_ints = n_shift + n_box + n_off;
 
     // frames of a pass in a zero-bordered buffer that holds every staged box (the ring
-    // tiles then stage like interior ones), when that costs at most twice their size
-    const PadGeometry pad = pad_geometry(plan, 0, h, 0, plan.tiles_x, plan.tiles_y);
-    const bool use_pad = static_cast<double>(pad.pitch) * pad.rows <= 2.0 * static_cast<double>(plane);
-    const long long fstride = use_pad ? static_cast<long long>(pad.rows) * pad.pitch : plane;
+    // tiles then stage like interior ones), when that costs at most twice their size;
+    // 16-bit frames (with a scale per frame) always use it, with its 16-bit geometry
+    const int st = cfg.storage;
+    if (st != kStoreF32 && st != kStoreF16 && st != kStoreBF16) throw std::runtime_error("search: unknown frame storage");
+    const PadGeometry pad32 = pad_geometry(plan, 0, h, 0, plan.tiles_x, plan.tiles_y);
+    const PadGeometry pad = st == kStoreF32 ? pad32 : pad_geometry16(pad32);
+    const bool use_pad = st != kStoreF32 || static_cast<double>(pad.pitch) * pad.rows <= 2.0 * static_cast<double>(plane);
+    const long long fstride = use_pad ? static_cast<long long>(pad.rows) * pad.pitch : plane;  // elements
     const long long forigin = use_pad ? static_cast<long long>(pad.top) * pad.pitch + pad.left : 0;
+    const int ebytes = st == kStoreF32 ? 4 : 2;
+    const long long ffloats = (fstride * ebytes + 3) / 4;  // a frame's buffer in floats
     const long long layout = use_pad ? ((static_cast<long long>(pad.pitch) << 40) ^ (static_cast<long long>(pad.rows) << 20) ^
-                                        (static_cast<long long>(pad.top) << 10) ^ pad.left)
+                                        (static_cast<long long>(pad.top) << 10) ^ (static_cast<long long>(st) << 8) ^ pad.left)
                                      : 0;
 
     // group size: requested, tuned, or all passes, within half the free device memory
@@ -553,8 +569,8 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     {
         size_t free_b = 0, total_b = 0;
         scheck(hipMemGetInfo(&free_b, &total_b), "memory info");
-        const double per_pass = 4.0 * n * fstride + 24.0 * plane + 4.0 * tab_ints + 16.0 * static_cast<double>(pk_cap_) +
-                                4.0 * n + 2.0 * sizeof(StackArgs);
+        const double per_pass = 4.0 * n * ffloats + 24.0 * plane + 4.0 * tab_ints + 16.0 * static_cast<double>(pk_cap_) +
+                                12.0 * n + 2.0 * sizeof(StackArgs);
         const double held = sets_[0].filt ? 2.0 * sets_[0].cap * per_pass : 0.0;  // reusable buffers count as free
         const int fit = static_cast<int>((0.5 * (static_cast<double>(free_b) + held)) / (2.0 * per_pass));
         if (fit < 1) throw std::runtime_error("search: not enough device memory for a pass");
@@ -565,7 +581,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const int nsets = groups > 1 ? 2 : 1;
     for (int q = 0; q < nsets; ++q) {
         PassSet& S = sets_[q];
-        S.reserve(G, n, fstride, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
+        S.reserve(G, n, ffloats, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
         if (use_pad && S.layout != layout) {
             // the border must read as zeros; the interiors are rewritten by every pass
             scheck(hipMemsetAsync(S.filt, 0, sizeof(float) * S.cap * S.frames * S.frame_floats, s), "clear frames");
@@ -616,10 +632,15 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         for (int g = 0; g < G; ++g) {
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             StackLaunch L{};
-            L.frames = S.filt + static_cast<long long>(g) * n * fstride + forigin;
-            L.frame_stride = fstride;
+            if (st == kStoreF32)
+                L.frames = S.filt + static_cast<long long>(g) * n * ffloats + forigin;
+            else
+                L.frames = reinterpret_cast<uint16_t*>(S.filt + static_cast<long long>(g) * n * ffloats) + forigin;
+            L.frame_stride = ffloats * 4 / ebytes;
             L.pitch = use_pad ? pad.pitch : w;
             L.padded = use_pad;
+            L.storage = st;
+            L.scale = st == kStoreF32 ? nullptr : S.scl + static_cast<long long>(g) * n;
             L.row0 = 0;
             L.row1 = h;
             L.shift = tab;
@@ -668,11 +689,19 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, build_stream_), "permute tables");
             scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, build_stream_),
                    "permute tables");
-            float* slot = S.filt + static_cast<long long>(g) * n * fstride;
-            if (use_pad) {
+            float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
+            const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
+            if (st != kStoreF32) {
                 stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, filt_, plane, nullptr, nullptr,
                              nullptr, nullptr, build_stream_);
-                scheck(launch_copy_rows(filt_, plane, slot + forigin, fstride, pad.pitch, w, h, n, build_stream_),
+                scheck(launch_store(filt_, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
+                                    S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n,
+                                    build_stream_),
+                       "store frames");
+            } else if (use_pad) {
+                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, filt_, plane, nullptr, nullptr,
+                             nullptr, nullptr, build_stream_);
+                scheck(launch_copy_rows(filt_, plane, slot + forigin, fs, pad.pitch, w, h, n, build_stream_),
                        "pad frames");
             } else {
                 stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, slot, plane, nullptr, nullptr,

@@ -41,6 +41,9 @@ struct SearchConfig {
     // choice). Neither changes any result bit.
     int passes_per_launch = 1;
     int segments = 0;
+    // Frame storage of the stacking passes: 0 fp32 (the exact form), 1 fp16, 2 bf16
+    // (a power-of-two scale per frame, fp32 accumulation).
+    int storage = 0;
 };
 
 struct PeakList {
@@ -75,6 +78,8 @@ struct PassSet {
     int* perm = nullptr;           // cap x frames
     int* counts = nullptr;         // cap peak counts
     int* pk = nullptr;             // cap x 4 pk_cap: x, y, snr bits, vidx
+    float* scl = nullptr;          // cap x frames: scales of 16-bit frames
+    unsigned* maxb = nullptr;      // cap x frames: scratch of the 16-bit encoder
     void* args = nullptr;          // 2 cap stacking descriptors
     int* h_counts = nullptr;       // pinned
     void release();

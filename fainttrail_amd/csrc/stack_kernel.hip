@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "stack_launch.hpp"
 #include "stack_plan.hpp"
@@ -85,9 +86,13 @@ __device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int
 
 // The body of one workgroup of a stacking pass: workgroup `wg` of the pass described
 // by `a` (the kernels below pick the pass and the workgroup index).
-template <bool RING, bool PADDED>
+template <bool RING, bool PADDED, int ST>
 __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
-    const float* __restrict__ frames = a.frames;
+    static_assert(ST == kStoreF32 || (ST == kStoreF16 || ST == kStoreBF16), "frame storage");
+    static_assert(ST == kStoreF32 || !RING || PADDED, "16-bit frames are always padded");
+    using elem_t = typename std::conditional<ST == kStoreF32, float, uint16_t>::type;
+    const elem_t* __restrict__ frames = static_cast<const elem_t*>(a.frames);
+    const float* __restrict__ scale = a.scale;
     const long long frame_stride = a.frame_stride;
     const int row0 = a.row0, row1 = a.row1, n = a.n, w = a.w, h = a.h, pitch = a.pitch;
     const int* __restrict__ shift = a.shift;
@@ -183,7 +188,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 }
                 __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
             }
-            const float* f = frames + static_cast<long long>(i) * frame_stride;
+            const float* f = static_cast<const float*>(a.frames) + static_cast<long long>(i) * frame_stride;
             if (RING && !PADDED && !interior) {
 #pragma unroll
                 for (int m = 0; m < kPreRows; ++m) {
@@ -230,19 +235,96 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
             }
         };
 
-        stage(0, lds0);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        for (int i = 0; i < n; i += 2) {
-            if (i + 1 < n) stage(i + 1, lds1);
-            consume(lds0, i);
+        if constexpr (ST == kStoreF32) {
+            stage(0, lds0);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            if (i + 1 >= n) break;
-            if (i + 2 < n) stage(i + 2, lds0);
-            consume(lds1, i + 1);
-            __builtin_amdgcn_s_waitcnt(0);
+            for (int i = 0; i < n; i += 2) {
+                if (i + 1 < n) stage(i + 1, lds1);
+                consume(lds0, i);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (i + 1 >= n) break;
+                if (i + 2 < n) stage(i + 2, lds0);
+                consume(lds1, i + 1);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+        } else {
+            // 16-bit frames: the next frame's box rows are fetched into registers while
+            // the current frame is consumed, one dword (two pixels) per lane and row from
+            // the even column at or left of the box start, then widened, multiplied by the
+            // frame's power-of-two scale (exact) and written to the other LDS buffer as
+            // fp32, box column c at LDS column c + 1 (so every lane's pair lands at a
+            // non-negative, 8-byte or 4-byte aligned pair of columns). The sum is the
+            // fp32 sum of the decoded frames. Lanes past the box repeat its last pair.
+            uint32_t pre[kPreRows];
+            int pre_rows = 0, pre_col = 0;
+            auto fetch = [&](int i) {
+                cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
+                const int sx = bx[2], R = kTileY + bx[3];
+                const int bxs = x0 + bx[0];
+                const int off = bxs & 1;  // 1: the pair starts one column left of the box
+                const int lmax = (kTileX - 1 + sx + off) >> 1;
+                const int l = min(lane, lmax);
+                const elem_t* rowp =
+                    frames + static_cast<long long>(i) * frame_stride + static_cast<long long>(y0 + bx[1] - row0) * pitch + (bxs - off);
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc(const_cast<elem_t*>(rowp), 0, 0x7fffffff, kBufferWord3);
+                pre_rows = R;
+                pre_col = 2 * l - off + 1;  // LDS column of the pair's first pixel
+#pragma unroll
+                for (int m = 0; m < kPreRows; ++m) {
+                    const int rr = min(wave + kWaves * m, R - 1);  // rows past the box repeat its last
+                    pre[m] = __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * l, rr * 2 * pitch, 0);
+                }
+            };
+            auto commit = [&](int i, float* dst) {
+                const float sc = reinterpret_cast<const __attribute__((address_space(4))) float*>(
+                    (const __attribute__((address_space(4))) char*)(scale))[i];
+                float* d = dst + pre_col;
+#pragma unroll
+                for (int m = 0; m < kPreRows; ++m) {
+                    const int rr = min(wave + kWaves * m, pre_rows - 1);
+                    const uint32_t v = pre[m];
+                    float lo, hi;
+                    if constexpr (ST == kStoreF16) {
+                        lo = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(v & 0xffffu)));
+                        hi = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(v >> 16)));
+                    } else {
+                        lo = __uint_as_float(v << 16);
+                        hi = __uint_as_float(v & 0xffff0000u);
+                    }
+                    d[rr * kPitch] = lo * sc;
+                    d[rr * kPitch + 1] = hi * sc;
+                }
+            };
+            const int lane_base16 = lane_base + 1;
+            auto consume16 = [&](const float* L, int i) {
+                cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const float* p = L + lane_base16 + lo[k];
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
+                }
+            };
+            fetch(0);
+            commit(0, lds0);
             __syncthreads();
+            // (straight-line code: past the last frame the last one is staged again into
+            // the buffer that is not read any more)
+            for (int i = 0; i < n; i += 2) {
+                fetch(min(i + 1, n - 1));
+                consume16(lds0, i);
+                commit(min(i + 1, n - 1), lds1);
+                __syncthreads();
+                if (i + 1 >= n) break;
+                fetch(min(i + 2, n - 1));
+                consume16(lds1, i + 1);
+                commit(min(i + 2, n - 1), lds0);
+                __syncthreads();
+            }
         }
 
         // S/N of the block's motions (ascending index) and the running maximum
@@ -313,19 +395,19 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     }
 }
 
-template <bool RING, bool PADDED>
+template <bool RING, bool PADDED, int ST>
 __global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
-    stack_body<RING, PADDED>(a, blockIdx.x);
+    stack_body<RING, PADDED, ST>(a, blockIdx.x);
 }
 
 // Several passes in one launch (the real pass and null runs of a batched search, or
 // regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
 // is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
-template <bool RING, bool PADDED>
+template <bool RING, bool PADDED, int ST>
 __global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
     const unsigned p = blockIdx.x / per_pass;
     const StackArgs a = passes[p];
-    stack_body<RING, PADDED>(a, blockIdx.x - p * per_pass);
+    stack_body<RING, PADDED, ST>(a, blockIdx.x - p * per_pass);
 }
 
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
@@ -402,6 +484,8 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     const long long in = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
     StackArgs a{};
     a.frames = L.frames;
+    a.scale = L.scale;
+    a.storage = L.storage;
     a.frame_stride = L.frame_stride;
     a.shift = L.shift;
     a.box = L.box;
@@ -445,18 +529,41 @@ unsigned pass_groups(const StackArgs& a) {
 size_t lds_bytes_of(int lds_rows) { return sizeof(float) * 2 * lds_rows * kPitch; }
 }  // namespace
 
+namespace {
+// kernel for (ring tiles, padded frames, storage)
+using SingleFn = void (*)(StackArgs);
+using MultiFn = void (*)(const StackArgs*, unsigned);
+SingleFn single_kernel(bool ring, bool padded, int st) {
+    if (!ring) return st == kStoreF16 ? stack_kernel<false, false, kStoreF16>
+                    : st == kStoreBF16 ? stack_kernel<false, false, kStoreBF16> : stack_kernel<false, false, kStoreF32>;
+    if (st == kStoreF16) return stack_kernel<true, true, kStoreF16>;
+    if (st == kStoreBF16) return stack_kernel<true, true, kStoreBF16>;
+    return padded ? stack_kernel<true, true, kStoreF32> : stack_kernel<true, false, kStoreF32>;
+}
+MultiFn multi_kernel(bool ring, bool padded, int st) {
+    if (!ring) return st == kStoreF16 ? stack_multi_kernel<false, false, kStoreF16>
+                    : st == kStoreBF16 ? stack_multi_kernel<false, false, kStoreBF16> : stack_multi_kernel<false, false, kStoreF32>;
+    if (st == kStoreF16) return stack_multi_kernel<true, true, kStoreF16>;
+    if (st == kStoreBF16) return stack_multi_kernel<true, true, kStoreBF16>;
+    return padded ? stack_multi_kernel<true, true, kStoreF32> : stack_multi_kernel<true, false, kStoreF32>;
+}
+bool storage_ok(const StackArgs& a) {
+    if (a.storage == kStoreF32) return true;
+    return (a.storage == kStoreF16 || a.storage == kStoreBF16) && a.padded && a.scale != nullptr;
+}
+}  // namespace
+
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s) {
     StackArgs inner, ring;
     make_stack_args(p, L, inner, ring);
+    if (!storage_ok(ring)) return hipErrorInvalidValue;
     const size_t lds = lds_bytes_of(inner.lds_rows);
     if (inner.ntiles > 0)
-        hipLaunchKernelGGL((stack_kernel<false, false>), dim3(pass_groups(inner)), dim3(kThreads), lds, s, inner);
-    if (ring.ntiles > 0) {
-        if (ring.padded)
-            hipLaunchKernelGGL((stack_kernel<true, true>), dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
-        else
-            hipLaunchKernelGGL((stack_kernel<true, false>), dim3(pass_groups(ring)), dim3(kThreads), lds, s, ring);
-    }
+        hipLaunchKernelGGL(single_kernel(false, false, inner.storage), dim3(pass_groups(inner)), dim3(kThreads), lds, s,
+                           inner);
+    if (ring.ntiles > 0)
+        hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage), dim3(pass_groups(ring)), dim3(kThreads),
+                           lds, s, ring);
     return hipGetLastError();
 }
 
@@ -465,26 +572,23 @@ hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host
     if (count <= 0) return hipSuccess;
     unsigned per_in = 0, per_ring = 0;
     const int lds_rows = host_inner[0].lds_rows;
-    const int padded = host_ring[0].padded;
+    const int padded = host_ring[0].padded, storage = host_ring[0].storage;
     for (int q = 0; q < count; ++q) {
-        if (host_inner[q].lds_rows != lds_rows || host_ring[q].lds_rows != lds_rows || host_ring[q].padded != padded)
+        const StackArgs &a = host_inner[q], &r = host_ring[q];
+        if (a.lds_rows != lds_rows || r.lds_rows != lds_rows || r.padded != padded || a.storage != storage ||
+            r.storage != storage || !storage_ok(r))
             return hipErrorInvalidValue;
-        per_in = std::max(per_in, pass_groups(host_inner[q]));
-        per_ring = std::max(per_ring, pass_groups(host_ring[q]));
+        per_in = std::max(per_in, pass_groups(a));
+        per_ring = std::max(per_ring, pass_groups(r));
     }
     const size_t lds = lds_bytes_of(lds_rows);
     const dim3 block(kThreads);
     if (per_in > 0)
-        hipLaunchKernelGGL((stack_multi_kernel<false, false>), dim3(per_in * static_cast<unsigned>(count)), block, lds, s,
-                           dev_inner, per_in);
-    if (per_ring > 0) {
-        if (padded)
-            hipLaunchKernelGGL((stack_multi_kernel<true, true>), dim3(per_ring * static_cast<unsigned>(count)), block,
-                               lds, s, dev_ring, per_ring);
-        else
-            hipLaunchKernelGGL((stack_multi_kernel<true, false>), dim3(per_ring * static_cast<unsigned>(count)), block,
-                               lds, s, dev_ring, per_ring);
-    }
+        hipLaunchKernelGGL(multi_kernel(false, false, storage), dim3(per_in * static_cast<unsigned>(count)), block, lds,
+                           s, dev_inner, per_in);
+    if (per_ring > 0)
+        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage), dim3(per_ring * static_cast<unsigned>(count)), block,
+                           lds, s, dev_ring, per_ring);
     return hipGetLastError();
 }
 

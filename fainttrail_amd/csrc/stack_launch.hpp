@@ -7,6 +7,12 @@
 
 namespace ftk {
 
+// Frame storage of a stacking pass: fp32, or 16-bit (fp16 / bf16) values q with a
+// power-of-two scale per frame, decoded exactly as q * scale[i] before the fp32 sum.
+constexpr int kStoreF32 = 0;
+constexpr int kStoreF16 = 1;
+constexpr int kStoreBF16 = 2;
+
 // One launch of the stacking kernel over all tiles of output rows [y_begin, y_end)
 // and the block segments [seg0, seg0 + nseg) (bps blocks each). The frames buffer
 // holds frame rows [row0, row1) of every frame, frame i at frames + i * frame_stride.
@@ -15,7 +21,9 @@ namespace ftk {
 // + x] (pitch 0 = w). A padded buffer (padded = true) holds zeros around the frame
 // rows, laid out as pad_geometry says, so that every box a tile stages lies inside it.
 struct StackLaunch {
-    const float* frames = nullptr;
+    const void* frames = nullptr;  // float, or uint16_t for 16-bit storage (always padded)
+    int storage = kStoreF32;
+    const float* scale = nullptr;  // per frame, 16-bit storage only
     long long frame_stride = 0;
     int row0 = 0, row1 = 0;
     int pitch = 0;
@@ -44,7 +52,8 @@ struct TileRect {
 // Everything one workgroup of a pass reads: the kernel arguments of a single-pass
 // launch, or one descriptor of a multi-pass launch (read with scalar loads).
 struct StackArgs {
-    const float* frames;
+    const void* frames;
+    const float* scale;
     long long frame_stride;
     const int* shift;
     const int* box;
@@ -57,7 +66,7 @@ struct StackArgs {
     const int* cov_y;
     unsigned long long* keys;
     TileRect rect;
-    int row0, row1, n, w, h, pitch, padded, min_count, nb, bps;
+    int row0, row1, n, w, h, pitch, padded, storage, min_count, nb, bps;
     int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
 };
 

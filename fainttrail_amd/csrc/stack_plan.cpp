@@ -265,4 +265,12 @@ PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, in
     return g;
 }
 
+PadGeometry pad_geometry16(const PadGeometry& g32) {
+    PadGeometry g = g32;
+    g.left = (g32.left + 1 + 1) & ~1;                 // >= left + 1, even
+    const int right = g32.pitch - g32.left + 1;       // columns from x = 0, one more
+    g.pitch = (g.left + right + 1) & ~1;              // even
+    return g;
+}
+
 }  // namespace ftk

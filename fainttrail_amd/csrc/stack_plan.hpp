@@ -91,5 +91,9 @@ struct PadGeometry {
     int left = 0, pitch = 0, top = 0, rows = 0;
 };
 PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, int tiles_x, int tiles_y);
+// The same for 16-bit frames, which are fetched as pixel pairs from the even column at
+// or left of each box start: one more zero column each side, and an even left border,
+// pitch and frame size so every pair is 4-byte aligned.
+PadGeometry pad_geometry16(const PadGeometry& g32);
 
 }  // namespace ftk

@@ -1,0 +1,137 @@
+// 16-bit frame storage (fp16 / bf16 with a power-of-two scale per frame) and the
+// FP64 evaluation of the stacking contract, for the precision form of the stacking
+// pass.
+//
+// Storage: for frame i, s_i = 2^(e_i - 14) where max|g_i| = m 2^e_i (m in [0.5, 1));
+// q = RNE16(g / s_i) (the division by a power of two is exact), so |q| < 2^14 and
+// the decoded value q * s_i is exact in fp32. The stacking kernel sums the decoded
+// values in fp32 in the contract's order, so a 16-bit pass equals the fp32 pass over
+// the decoded frames bit for bit; the storage rounding (2^-11 relative for fp16, 2^-8
+// for bf16) is the only error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "stack_launch.hpp"
+#include "store.hpp"
+
+namespace ftk {
+
+namespace {
+
+// maximum of |v| over the finite samples of frame blockIdx.y, as ordered bits (a
+// non-negative float orders like its bit pattern)
+__global__ void absmax_kernel(const float* __restrict__ src, long long stride, int pitch, int w, int h,
+                              unsigned* __restrict__ maxbits) {
+    const int i = blockIdx.y;
+    const float* f = src + i * stride;
+    const long long count = static_cast<long long>(w) * h;
+    float m = 0.0f;
+    for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < count;
+         q += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long y = q / w;
+        const float v = fabsf(f[y * pitch + (q - y * w)]);
+        if (v <= 3.402823466e38f) m = fmaxf(m, v);  // NaN and inf do not set the range
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (static_cast<int>(threadIdx.x) < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&maxbits[i], __float_as_uint(red[0]));
+}
+
+__global__ void scale_kernel(const unsigned* __restrict__ maxbits, int n, float* __restrict__ scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float m = __uint_as_float(maxbits[i]);
+    int e = 0;
+    if (m > 0.0f) (void)frexpf(m, &e);
+    scale[i] = m > 0.0f ? ldexpf(1.0f, max(e - 14, -126)) : 1.0f;  // a normal number, so 1/scale is finite
+}
+
+__device__ inline uint16_t to_f16(float x) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x)); }
+
+__device__ inline uint16_t to_bf16(float x) {
+    const uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x0040u);  // quiet NaN
+    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);                  // nearest, ties even
+}
+
+__global__ void encode_kernel(const float* __restrict__ src, long long sstride, int spitch, uint16_t* __restrict__ dst,
+                              long long dstride, int dpitch, const float* __restrict__ scale, int w, int storage) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const float inv = 1.0f / scale[i];  // a power of two: exact
+    const float v = src[i * sstride + static_cast<long long>(y) * spitch + x] * inv;
+    dst[i * dstride + static_cast<long long>(y) * dpitch + x] = storage == kStoreBF16 ? to_bf16(v) : to_f16(v);
+}
+
+// One thread per output pixel of rows [y0, y1): for every motion j (ascending) the sum
+// over covering frames i (ascending) in double, times 1/sqrt(c) in double; strict >
+// over j (ties to the lowest index); undefined pixels 0 / -1.
+__global__ void fp64_kernel(const float* __restrict__ frames, long long stride, int n, int w, int h, int nv,
+                            const int* __restrict__ dx, const int* __restrict__ dy, int min_count, int y0, int y1,
+                            double* __restrict__ out, int* __restrict__ best) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = y0 + blockIdx.y;
+    if (x >= w || y >= y1) return;
+    const __attribute__((address_space(4))) int* cdx = (const __attribute__((address_space(4))) int*)(dx);
+    const __attribute__((address_space(4))) int* cdy = (const __attribute__((address_space(4))) int*)(dy);
+    double bv = -HUGE_VAL;  // a NaN sum is never selected
+    int bj = -1;
+    for (int j = 0; j < nv; ++j) {
+        double s = 0.0;
+        int c = 0;
+        for (int i = 0; i < n; ++i) {
+            const int sx = x + cdx[static_cast<long long>(j) * n + i], sy = y + cdy[static_cast<long long>(j) * n + i];
+            if (static_cast<unsigned>(sx) < static_cast<unsigned>(w) && static_cast<unsigned>(sy) < static_cast<unsigned>(h)) {
+                s += static_cast<double>(frames[i * stride + static_cast<long long>(sy) * w + sx]);
+                ++c;
+            }
+        }
+        if (c < min_count) continue;
+        const double v = s * (1.0 / sqrt(static_cast<double>(c)));
+        if (v > bv) {
+            bv = v;
+            bj = j;
+        }
+    }
+    const long long o = static_cast<long long>(y - y0) * w + x;
+    out[o] = bj < 0 ? 0.0 : bv;
+    best[o] = bj;
+}
+
+}  // namespace
+
+hipError_t launch_store(const float* src, long long sstride, int spitch, int n, int w, int h, int storage,
+                        uint16_t* dst, long long dstride, int dpitch, unsigned* maxbits, float* scale, hipStream_t s) {
+    if (n <= 0 || w <= 0 || h <= 0) return hipSuccess;
+    if (storage != kStoreF16 && storage != kStoreBF16) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(maxbits, 0, sizeof(unsigned) * n, s);
+    if (e != hipSuccess) return e;
+    const long long count = static_cast<long long>(w) * h;
+    const unsigned bx = static_cast<unsigned>(std::min<long long>(1024, (count + 255) / 256));
+    hipLaunchKernelGGL(absmax_kernel, dim3(bx, n), dim3(256), 0, s, src, sstride, spitch, w, h, maxbits);
+    hipLaunchKernelGGL(scale_kernel, dim3((n + 63) / 64), dim3(64), 0, s, maxbits, n, scale);
+    for (int y = 0; y < h; y += 32768) {
+        const int ry = std::min(32768, h - y);
+        hipLaunchKernelGGL(encode_kernel, dim3((w + 255) / 256, ry, n), dim3(256), 0, s,
+                           src + static_cast<long long>(y) * spitch, sstride, spitch,
+                           dst + static_cast<long long>(y) * dpitch, dstride, dpitch, scale, w, storage);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fp64(const float* frames, long long stride, int n, int w, int h, int nv, const int* d_dx,
+                       const int* d_dy, int min_count, int y0, int y1, double* out, int* best, hipStream_t s) {
+    if (y1 <= y0) return hipSuccess;
+    hipLaunchKernelGGL(fp64_kernel, dim3((w + 63) / 64, y1 - y0), dim3(64), 0, s, frames, stride, n, w, h, nv, d_dx,
+                       d_dy, min_count, y0, y1, out, best);
+    return hipGetLastError();
+}
+
+}  // namespace ftk

@@ -23,7 +23,7 @@ class _Params(C.Structure):
                 ("min_coverage", C.c_double), ("threshold_snr", C.c_double), ("min_snr_floor", C.c_double),
                 ("null_runs", C.c_int), ("target_false_per_run", C.c_double), ("max_candidates", C.c_int),
                 ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int),
-                ("passes_per_launch", C.c_int), ("segments", C.c_int)]
+                ("passes_per_launch", C.c_int), ("segments", C.c_int), ("storage", C.c_int)]
 
 
 class _Summary(C.Structure):
@@ -85,13 +85,19 @@ class SearchParams:
     # velocity-block segments per tile (0 = planner / tuned)
     passes_per_launch: int = 0
     segments: int = 0
+    # frame storage of the stacking passes: "f32" (exact), "f16" or "bf16" (a
+    # power-of-two scale per frame, fp32 accumulation)
+    storage: str = "f32"
 
     def c(self) -> _Params:
+        from . import STORAGES
+        if self.storage not in STORAGES:
+            raise FtkError(f"unknown frame storage '{self.storage}' (f32, f16, bf16)")
         return _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
                        self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
                        self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
                        self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
-                       self.passes_per_launch, self.segments)
+                       self.passes_per_launch, self.segments, STORAGES[self.storage])
 
 
 @dataclass

@@ -104,6 +104,32 @@ double ftk_plan_units(const ftk_plan* plan);
 int ftk_plan_blocks(const ftk_plan* plan);
 void ftk_plan_destroy(ftk_plan* plan);
 
+/* ------------------------------------------------------------------ precision form
+ * No reference counterpart (the reference stacks fp32 frames only). Frames kept in HBM
+ * as fp16 or bf16 values q with a power-of-two scale s_i per frame, s_i = 2^(e - 14)
+ * for max|g_i| = m 2^e (m in [0.5, 1)), q = round-to-nearest-even(g / s_i); the
+ * stacking kernel widens q * s_i exactly and accumulates in fp32 in the contract's
+ * order, so a stored pass equals the fp32 pass over the decoded frames bit for bit. */
+#define FTK_STORE_F32 0
+#define FTK_STORE_F16 1
+#define FTK_STORE_BF16 2
+/* Encodes n fp32 frames (device, frame_stride floats apart) into the plan's 16-bit
+ * storage (the fp32 frames may be freed afterwards). Asynchronous. */
+int ftk_plan_store(ftk_plan* plan, const float* d_frames, size_t frame_stride, int storage, void* stream, char* err,
+                   size_t err_len);
+/* A stacking pass over the stored 16-bit frames. Asynchronous. */
+int ftk_plan_run_stored(ftk_plan* plan, float* d_max_snr, int32_t* d_best_v, void* stream, char* err, size_t err_len);
+/* The n per-frame scales of the stored frames (host copy; synchronises the device). */
+int ftk_plan_scales(const ftk_plan* plan, float* scales);
+/* The stacking contract evaluated in FP64 on the GPU, the precision reference for
+ * stored and fp32 passes: output rows [y0, y1) of n fp32 frames (plain rows of w),
+ * sums of the covering frames in double, times 1/sqrt(c) in double, strict maximum
+ * over the motions (ties to the lowest index). dx, dy: host tables, motion-major.
+ * d_max_snr (double) / d_best_v: (y1 - y0) x w device outputs. Synchronous. */
+int ftk_stack_fp64(int n, int w, int h, const float* d_frames, size_t frame_stride, int nv, const int32_t* dx,
+                   const int32_t* dy, int min_count, int y0, int y1, double* d_max_snr, int32_t* d_best_v, void* stream,
+                   char* err, size_t err_len);
+
 /* ------------------------------------------------------------------ preprocessing stage
  * The time-order-dependent preprocessing of a search, build_stage
  * (src/preprocess.cpp:286-486), and the per-frame background of prepare_core
@@ -167,6 +193,9 @@ typedef struct ftk_search_params {
      * planner's choice, or the tuned value). */
     int passes_per_launch;
     int segments;
+    /* Frame storage of the stacking passes: FTK_STORE_F32 (the exact form), or
+     * FTK_STORE_F16 / FTK_STORE_BF16 (see the precision form below). */
+    int storage;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {

@@ -1,0 +1,155 @@
+"""Precision form: frames stored in HBM as fp16 / bf16 with a power-of-two scale per
+frame, accumulated in fp32 (ftk_plan_store / ftk_plan_run_stored), and the FP64
+evaluation of the stacking contract (ftk_stack_fp64).
+
+* The stored pass must equal the exact fp32 pass over the decoded frames bit for bit:
+  the decoding q * s is exact and the kernel keeps the contract's summation order. The
+  decoded frames are computed here with numpy (fp16: numpy's round-to-nearest-even
+  cast; bf16: round-to-nearest-even on the fp32 bits), from the scales the device
+  chose, so the device encoder is checked too.
+* The FP64 kernel is pinned by the known answer (constant frames give sqrt(c) at the
+  lowest motion of largest coverage) and by a plain numpy FP64 restatement of the
+  contract on small inputs.
+* Error bounds: |dS/N| / max(1, |S/N_fp64|) <= 1e-3 for fp16 storage and <= 1e-5 for
+  the fp32 pass, on unit-variance frames with bright sources (the error metric of the
+  reference's check tolerance, backend.hpp:37-40)."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from refcases import same_bits
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["contract-random", "shift-linear", "shift-curved", "cuda-linear", "cuda-table"]
+
+
+def case(golden, name):
+    return {k.split("/", 1)[1]: v for k, v in golden.items() if k.startswith(name + "/")}
+
+
+def decode(frames, scales, storage):
+    """numpy restatement of the 16-bit storage: q = RNE(g / s), decoded q * s."""
+    s = scales.astype(np.float32)[:, None, None]
+    g = (frames.astype(np.float32) / s).astype(np.float32)
+    if storage == "f16":
+        q = g.astype(np.float16).astype(np.float32)
+    else:
+        u = g.view(np.uint32).astype(np.uint64)
+        r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+        q = r.view(np.float32).copy()
+        q[np.isnan(g)] = np.nan
+    return (q * s).astype(np.float32)
+
+
+def contract_fp64(frames, dx, dy, min_count):
+    """The stacking contract (backend.hpp:10-40) in numpy float64, pixel by pixel."""
+    n, h, w = frames.shape
+    nv = dx.shape[0]
+    best = np.full((h, w), -np.inf)
+    arg = np.full((h, w), -1, np.int32)
+    ys, xs = np.mgrid[0:h, 0:w]
+    for j in range(nv):
+        s = np.zeros((h, w))
+        c = np.zeros((h, w), np.int64)
+        for i in range(n):
+            sx, sy = xs + dx[j, i], ys + dy[j, i]
+            ok = (sx >= 0) & (sx < w) & (sy >= 0) & (sy < h)
+            s[ok] += frames[i][sy[ok], sx[ok]].astype(np.float64)
+            c[ok] += 1
+        with np.errstate(invalid="ignore", divide="ignore"):
+            v = s * (1.0 / np.sqrt(c))
+        upd = (c >= min_count) & (v > best)
+        best[upd] = v[upd]
+        arg[upd] = j
+    return np.where(arg >= 0, best, 0.0), arg
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("name", CASES)
+def test_stored_pass_is_fp32_pass_of_decoded_frames(hip, golden, name, storage):
+    import torch
+    g = case(golden, name)
+    fr = g["frames"]
+    n, h, w = fr.shape
+    print(f"stored pass {name} {storage}", flush=True)
+    plan = hip.Plan(n, w, h, g["dx"], g["dy"], int(g["min_count"]))
+    plan.store(torch.from_numpy(fr).cuda(), storage)
+    m16, b16 = plan.run_stored()
+    sc = plan.scales()
+    # power-of-two scales with max|g| / s in [2^13, 2^14)
+    finite = np.where(np.isfinite(fr), np.abs(fr), 0.0).reshape(n, -1).max(axis=1)
+    for i in range(n):
+        if finite[i] > 0:
+            mant, e = np.frexp(np.float32(finite[i]))
+            assert sc[i] == np.float32(2.0 ** max(int(e) - 14, -126)), (i, sc[i], finite[i])
+        else:
+            assert sc[i] == 1.0
+    dec = decode(fr, sc, storage)
+    m32, b32 = plan.run(torch.from_numpy(dec).cuda())
+    torch.cuda.synchronize()
+    assert same_bits(m16.cpu().numpy(), m32.cpu().numpy()), name
+    assert np.array_equal(b16.cpu().numpy(), b32.cpu().numpy()), name
+    om, ob = po.stack_blocked(dec, g["dx"], g["dy"], int(g["min_count"]))
+    assert same_bits(m16.cpu().numpy(), om) and np.array_equal(b16.cpu().numpy(), ob), name
+
+
+def test_fp64_known_answer_and_numpy_restatement(hip, golden):
+    import torch
+    # constant frames: S = c / sqrt(c) = sqrt(c), maximal at full coverage, lowest index
+    n, h, w = 7, 20, 24
+    fr = np.ones((n, h, w), np.float32)
+    rng = np.random.default_rng(3)
+    dx = rng.integers(-3, 4, size=(9, n)).astype(np.int32)
+    dy = rng.integers(-3, 4, size=(9, n)).astype(np.int32)
+    dx[4], dy[4] = 0, 0
+    m, b = hip.stack_fp64(torch.from_numpy(fr).cuda(), dx, dy, 3)
+    m, b = m.cpu().numpy(), b.cpu().numpy()
+    want_m, want_b = contract_fp64(fr, dx, dy, 3)
+    assert np.array_equal(m, want_m) and np.array_equal(b, want_b)
+    assert abs(m[10, 12] - np.sqrt(7.0)) <= 1e-15 * np.sqrt(7.0) and b[10, 12] == 0
+    # reference test inputs (NaN frames, far shifts, curved tables) and a row range
+    for name in ("contract-random", "shift-curved", "cuda-table"):
+        g = case(golden, name)
+        fr, dx, dy, mc = g["frames"], g["dx"], g["dy"], int(g["min_count"])
+        want_m, want_b = contract_fp64(fr, dx, dy, mc)
+        h = fr.shape[1]
+        m, b = hip.stack_fp64(torch.from_numpy(fr).cuda(), dx, dy, mc, 2, h - 1)
+        assert np.array_equal(m.cpu().numpy(), want_m[2:h - 1]), name
+        assert np.array_equal(b.cpu().numpy(), want_b[2:h - 1]), name
+
+
+def test_storage_error_against_fp64(hip):
+    """Unit-variance frames with bright sources (a few hundred sigma), 24 frames of
+    192 x 160, 400 motions: fp32 within 1e-5, fp16 within 1e-3, bf16 within 1e-2 of the
+    FP64 map (relative to max(1, |S/N|)); the same maxima above S/N 6 in all forms."""
+    import torch
+    rng = np.random.default_rng(11)
+    n, h, w = 24, 160, 192
+    fr = rng.standard_normal((n, h, w), dtype=np.float32)
+    for _ in range(12):
+        y, x = rng.integers(8, h - 8), rng.integers(8, w - 8)
+        fr[:, y, x] += rng.uniform(20, 300)
+    t = np.linspace(-0.2, 0.2, n)
+    grid, _ = hip.velocity_grid(t[-1] - t[0], 10.0, 60.0, 0.35)
+    tab = hip.linear_shift_table(t, grid[:400])
+    mc = 17
+    d = torch.from_numpy(fr).cuda()
+    m64, b64 = hip.stack_fp64(d, tab.dx, tab.dy, mc)
+    m64 = m64.cpu().numpy()
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    m32, _ = plan.run(d)
+    errs = {}
+    maps = {"f32": m32.cpu().numpy()}
+    for st in ("f16", "bf16"):
+        plan.store(d, st)
+        m, _ = plan.run_stored()
+        maps[st] = m.cpu().numpy()
+    torch.cuda.synchronize()
+    for k, m in maps.items():
+        errs[k] = float(np.max(np.abs(m.astype(np.float64) - m64) / np.maximum(1.0, np.abs(m64))))
+    print("max relative S/N error vs FP64:", errs, flush=True)
+    assert errs["f32"] <= 1e-5 and errs["f16"] <= 1e-3 and errs["bf16"] <= 1e-2
+    strong = m64 > 6.0
+    for k, m in maps.items():
+        assert np.array_equal(m > 6.0, strong) or np.sum((m > 6.0) != strong) <= 2, k

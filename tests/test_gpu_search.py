@@ -89,8 +89,9 @@ def _same_result(a, b):
     assert a.threshold_snr == b.threshold_snr and a.threshold_method == b.threshold_method
 
 
-@pytest.mark.parametrize("group,segments", [(0, 0), (2, 0), (3, 5), (4, 1), (6, 64)])
-def test_batched_search_equals_one_pass_at_a_time(hip, group, segments):
+@pytest.mark.parametrize("group,segments,storage", [(0, 0, "f32"), (2, 0, "f32"), (3, 5, "f32"), (4, 1, "f32"),
+                                                    (6, 64, "f32"), (0, 0, "f16"), (4, 0, "bf16")])
+def test_batched_search_equals_one_pass_at_a_time(hip, group, segments, storage):
     """The batched form (one plan whose per-frame rows are permuted for each null run,
     passes fused into multi-pass launches, stage builds on a second stream) gives the
     one-pass-at-a-time form's results bit for bit, for any group size and segment
@@ -100,9 +101,10 @@ def test_batched_search_equals_one_pass_at_a_time(hip, group, segments):
     from fainttrail_amd.search import SearchParams, Searcher
     fr, mjd, _ = synth.make_sequence(18, 200, 136, nstars=40, nmovers=3, mover_snr=(4.0, 8.0), seed=21)
     frames = fr.cuda()
-    print(f"batched search group {group} segments {segments}", flush=True)
+    print(f"batched search group {group} segments {segments} storage {storage}", flush=True)
     s = Searcher(18, 200, 136)
-    base = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=5, seed=4)
+    base = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=5, seed=4,
+                        storage=storage)
     one = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 1}))
     many = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": group, "segments": segments}))
     torch.cuda.synchronize()
@@ -136,3 +138,33 @@ def test_batched_search_fixed_threshold_and_tuner(hip):
     torch.cuda.synchronize()
     assert tuned.passes_per_launch == g and tuned.segments == seg
     _same_result(ref, tuned)
+
+
+def test_sixteen_bit_search_agrees_with_exact_search(hip):
+    """Stored-frame searches (fp16, bf16) against the exact fp32 search on a sequence
+    with bright movers: the same selected candidates (position and motion), S/N of each
+    within 1e-3 (fp16) / 1e-2 (bf16) of max(1, S/N), thresholds within 1 %."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    fr, mjd, _ = synth.make_sequence(20, 224, 160, nstars=40, nmovers=4, mover_snr=(6.0, 9.0), seed=31)
+    frames = fr.cuda()
+    s = Searcher(20, 224, 160)
+    prm = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=6, seed=8)
+    ref = s.run(frames, mjd, prm)
+    assert ref.selected.x.size >= 2
+    for st, tol in (("f16", 1e-3), ("bf16", 1e-2)):
+        got = s.run(frames, mjd, SearchParams(**{**prm.__dict__, "storage": st}))
+        torch.cuda.synchronize()
+        print(f"{st}: threshold {got.threshold_snr:.6f} vs {ref.threshold_snr:.6f}, "
+              f"{got.selected.x.size} vs {ref.selected.x.size} selected", flush=True)
+        assert abs(got.threshold_snr - ref.threshold_snr) <= 0.01 * ref.threshold_snr, st
+        a = {(int(x), int(y), int(v)): float(q) for x, y, v, q in zip(ref.selected.x, ref.selected.y, ref.selected.vidx,
+                                                                        ref.selected.snr)}
+        b = {(int(x), int(y), int(v)): float(q) for x, y, v, q in zip(got.selected.x, got.selected.y, got.selected.vidx,
+                                                                        got.selected.snr)}
+        assert a.keys() == b.keys(), st
+        for k in a:
+            assert abs(a[k] - b[k]) <= tol * max(1.0, abs(a[k])), (st, k)
+        err = (got.max_snr - ref.max_snr).abs() / ref.max_snr.abs().clamp(min=1.0)
+        assert float(err.max()) <= 2 * tol, st
