@@ -131,13 +131,15 @@ def search_measure(reps: int, with_plain: bool):
     same = bool(seq.threshold_snr == res.threshold_snr and np.array_equal(seq.null_peaks, res.null_peaks)
                 and np.array_equal(seq.peaks.snr, res.peaks.snr)
                 and torch.equal(seq.max_snr, res.max_snr) and torch.equal(seq.best_v, res.best_v))
+    res16, dt16 = timed(SearchParams(**{**prm.__dict__, "storage": "f16"}))
     out = {"config": "512x512, 20 frames, 5468 motions, 8 null runs, seed 7",
            "device_searches_per_s": 1.0 / dt, "device_s_per_search": dt, "threshold_snr": res.threshold_snr,
            "method": res.threshold_method, "selected": int(res.selected.x.size),
            "batched": {"passes_per_launch": g, "segments": seg, "equal_to_one_pass_form": same},
            "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
            "one_pass_form_searches_per_s": 1.0 / dt_seq,
-           "one_pass_form_stage_seconds": {k: round(v, 4) for k, v in seq.timings.items() if v > 0}}
+           "one_pass_form_stage_seconds": {k: round(v, 4) for k, v in seq.timings.items() if v > 0},
+           "f16_storage": {"searches_per_s": 1.0 / dt16, **candidate_agreement(res, res16)}}
     if with_plain:
         from oracle import pyoracle as po
         if po.ref_available():
@@ -149,6 +151,163 @@ def search_measure(reps: int, with_plain: bool):
             out["plain_s_per_search"] = dtp
             out["device_over_plain"] = dtp / dt
             out["plain_threshold_equal"] = bool(ref["threshold"] == res.threshold_snr)
+            out["f16_over_plain"] = dtp / dt16
+    return out
+
+
+def candidate_agreement(exact, other):
+    """Selected candidates (position, motion) and S/N of a search against the exact one."""
+    a = {(int(x), int(y), int(v)): float(q) for x, y, v, q in
+         zip(exact.selected.x, exact.selected.y, exact.selected.vidx, exact.selected.snr)}
+    b = {(int(x), int(y), int(v)): float(q) for x, y, v, q in
+         zip(other.selected.x, other.selected.y, other.selected.vidx, other.selected.snr)}
+    err = (other.max_snr.double() - exact.max_snr.double()).abs() / exact.max_snr.double().abs().clamp(min=1.0)
+    common = set(a) & set(b)
+    return {"selected": len(b), "selected_exact": len(a), "same_candidates": a.keys() == b.keys(),
+            "threshold": other.threshold_snr, "threshold_exact": exact.threshold_snr,
+            "max_rel_snr_error_vs_exact_map": float(err.max()),
+            "max_rel_snr_error_candidates": max((abs(a[k] - b[k]) / max(1.0, abs(a[k])) for k in common), default=0.0)}
+
+
+def local_maxima(m, floor: float):
+    """Pixels at or above floor that are the maximum of their 5 x 5 window (torch)."""
+    import torch
+    mx = torch.nn.functional.max_pool2d(m[None, None].float(), 5, stride=1, padding=2)[0, 0]
+    ys, xs = torch.nonzero((m >= floor) & (m == mx), as_tuple=True)
+    return set(zip(ys.tolist(), xs.tolist()))
+
+
+def precision_measure(fp64_rows: int):
+    """Precision form on the 100-frame 6144 x 4096 configuration: sky 2000 ADU +
+    N(0, 45^2), 3000 stars (flux log-uniform 1e3-1.6e6 ADU, so the brightest saturate;
+    clipped at 65535 ADU), one frame with 50 cosmic-ray hits, 30 movers of per-frame
+    S/N 0.4-1.5 at 0.1-2 px/min; the frames are prepared on the device (background,
+    build_stage) and stacked over 29 996 motions (grid_tol 0.509). One pass each with
+    fp32, fp16 and bf16 frame storage; the FP64 evaluation on fp64_rows output rows
+    (half at the top edge, half in the middle) is the reference for the S/N error
+    |dS/N| / max(1, |S/N_fp64|); candidates are 5 x 5 local maxima at S/N >= 6."""
+    import torch
+
+    import fainttrail_amd as ft
+    from fainttrail_amd import synth
+    from fainttrail_amd.stage import Stage, StageParams
+    n, w, h = 100, 6144, 4096
+    fwhm = 1.5 * 2.354820045030949
+    t0 = time.perf_counter()
+    raw, mjd, _ = synth.make_sequence(n, w, h, sky=2000.0, noise=45.0, nstars=3000, star_flux=(1e3, 1.6e6),
+                                      nmovers=30, mover_snr=(0.4, 1.5), rate_px_min=(0.1, 2.0), seed=41,
+                                      device="cuda:0", saturate=65535.0, cr_frames=1, cr_hits=50)
+    t_h = (mjd - 0.5 * (mjd[0] + mjd[-1])) * 24.0
+    stage = Stage(n, w, h)
+    sub = stage.background(raw)
+    del raw
+    filt = stage.build(sub, t_h, StageParams(fwhm_px=fwhm))
+    if isinstance(filt, tuple):
+        filt = filt[0]
+    del sub, stage
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    t_gen = time.perf_counter() - t0
+    grid, _ = ft.velocity_grid(float(t_h[-1] - t_h[0]), 6.0, 120.0, 0.509)
+    tab = ft.linear_shift_table(t_h, grid)
+    mc = max(3, int(np.ceil(0.7 * n)))
+    plan = ft.Plan(n, w, h, tab.dx, tab.dy, mc)
+    stream = torch.cuda.current_stream()
+    print(f"bench: precision: {n} x {w}x{h}, {tab.motions} motions, {plan.units:.3e} units/pass "
+          f"(frames prepared in {t_gen:.1f} s)", file=sys.stderr, flush=True)
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        r = fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return r, a.elapsed_time(b) / 1e3
+
+    maps, secs = {}, {}
+    (m, _), secs["f32"] = timed(lambda: plan.run(filt))
+    maps["f32"] = m
+    store_s = {}
+    for st in ("f16", "bf16"):
+        _, store_s[st] = timed(lambda: plan.store(filt, st))
+        (m, _), secs[st] = timed(lambda: plan.run_stored())
+        maps[st] = m
+    half = fp64_rows // 2
+    bands = [(0, half), (h // 2 - half // 2, h // 2 - half // 2 + (fp64_rows - half))]
+    t0 = time.perf_counter()
+    ref = {b: ft.stack_fp64(filt, tab.dx, tab.dy, mc, b[0], b[1])[0] for b in bands}
+    t_fp64 = time.perf_counter() - t0
+    out = {"config": f"{n} frames of {w}x{h}, sky 2000 + N(0, 45^2), 3000 stars (saturating), 1 frame with 50 "
+                     f"cosmic rays, 30 movers S/N 0.4-1.5, {tab.motions} motions, prepared on the device",
+           "units_per_pass": plan.units, "fp64_rows": [list(b) for b in bands], "fp64_seconds": t_fp64}
+    for st in ("f32", "f16", "bf16"):
+        errs = []
+        for (y0, y1), r in ref.items():
+            d = (maps[st][y0:y1].double() - r).abs() / r.abs().clamp(min=1.0)
+            errs.append(float(d.max()))
+        cand_ref = set()
+        cand = set()
+        for (y0, y1), r in ref.items():
+            cand_ref |= {(y + y0, x) for y, x in local_maxima(r, 6.0) if 2 <= y < y1 - y0 - 2}
+            cand |= {(y + y0, x) for y, x in local_maxima(maps[st][y0:y1], 6.0) if 2 <= y < y1 - y0 - 2}
+        out[st] = {"units_per_s": plan.units / secs[st], "pass_seconds": secs[st],
+                   "max_rel_snr_error_vs_fp64": max(errs),
+                   "candidates_vs_fp64_rows": {"fp64": len(cand_ref), "this": len(cand), "common": len(cand & cand_ref)}}
+        if st != "f32":
+            out[st]["store_seconds"] = store_s[st]
+            full_ref, full = local_maxima(maps["f32"], 6.0), local_maxima(maps[st], 6.0)
+            out[st]["candidates_vs_fp32_full_frame"] = {"fp32": len(full_ref), "this": len(full),
+                                                        "common": len(full & full_ref)}
+    return out
+
+
+def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
+    """Batched null calibration on the 2048^2 configuration: 40 frames of 2048 x 2048
+    (60 s cadence), 300 stars, 10 movers of per-frame S/N 0.7 at 0.2-1.1 px/min, grid
+    tol 0.212 px (7996 motions), 256 shuffled-time null runs plus the real pass, false
+    alarms 0.02 per search, seed 7, the batched group size and segment count autotuned
+    on a 16-null search first. The plain form (the reference's own search() with the
+    "hip" stacking backend) is timed with 0 and plain_nulls null runs and its per-null
+    cost extrapolated to 256."""
+    import torch
+
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    n, w, h = 40, 2048, 2048
+    frames, mjd, _ = synth.make_sequence(n, w, h, nstars=300, nmovers=10, mover_snr=(0.7, 0.7),
+                                         rate_px_min=(0.2, 1.1), seed=7, device="cuda:0")
+    fwhm = 1.5 * 2.354820045030949
+    prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.212, fwhm_px=fwhm, null_runs=256, seed=7)
+    s = Searcher(n, w, h)
+    g, seg, t_tune = s.tune(frames, mjd, SearchParams(**{**prm.__dict__, "null_runs": 16}), reps=1)
+    print(f"bench: batched nulls: tuned {g} passes per launch, {seg} segments", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = s.run(frames, mjd, prm)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    units = float(res.n_velocities) * n * w * h * 257
+    out = {"config": "2048x2048, 40 frames, 7996 motions, 256 null runs + real, seed 7",
+           "n_velocities": res.n_velocities, "device_s_per_search": dt, "device_searches_per_s": 1.0 / dt,
+           "stacked_units_per_s": units / dt, "threshold_snr": res.threshold_snr, "method": res.threshold_method,
+           "selected": int(res.selected.x.size), "tuned_on_16_nulls": {"passes_per_launch": g, "segments": seg},
+           "passes_per_launch": res.passes_per_launch, "segments": res.segments,
+           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0}}
+    if with_plain:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            host = frames.cpu().numpy()
+            t0 = time.perf_counter()
+            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, 0, seed=7, backend="hip")
+            t_real = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, plain_nulls, seed=7, backend="hip")
+            t_k = time.perf_counter() - t0
+            per_null = max(0.0, (t_k - t_real) / plain_nulls)
+            est = t_real + 256 * per_null
+            out["plain"] = {"seconds_without_nulls": t_real, f"seconds_with_{plain_nulls}_nulls": t_k,
+                            "seconds_per_null_run": per_null, "extrapolated_s_per_search_256_nulls": est,
+                            "note": "extrapolated from the measured runs: real pass + 256 x the per-null cost"}
+            out["device_over_plain_extrapolated"] = est / dt
     return out
 
 
@@ -162,6 +321,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the null-calibrated search measurement")
     ap.add_argument("--search-reps", type=int, default=3)
+    ap.add_argument("--no-precision", action="store_true", help="skip the fp16/bf16/FP64 precision measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-null batched calibration measurement")
+    ap.add_argument("--fp64-rows", type=int, default=16)
     ap.add_argument("--search-only", action="store_true",
                     help="measure only the null-calibrated search (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -243,11 +405,19 @@ def main() -> int:
         res["cpu_baseline"] = cpu_baseline(frames, tab, min_count, args.cpu_seconds)
     else:
         res["cpu_baseline"] = None
+    del frames, plan
+    torch.cuda.empty_cache()
     if not args.no_search:
-        del frames, plan
-        torch.cuda.empty_cache()
         print("bench: null-calibrated search (512x512 configuration)", file=sys.stderr, flush=True)
         res["search"] = search_measure(args.search_reps, with_plain=not args.no_cpu)
+        torch.cuda.empty_cache()
+    if not args.no_batched:
+        print("bench: batched null calibration (2048x2048, 256 nulls)", file=sys.stderr, flush=True)
+        res["batched_nulls"] = batched_nulls_measure(with_plain=not args.no_cpu)
+        torch.cuda.empty_cache()
+    if not args.no_precision:
+        print("bench: precision form (100 x 6144x4096)", file=sys.stderr, flush=True)
+        res["precision"] = precision_measure(args.fp64_rows)
     print(json.dumps(res))
     return 0
 

@@ -7,6 +7,7 @@
 
 #include "../../include/ftk_hip.h"
 #include "search_device.hpp"
+#include "serve.hpp"
 
 struct ftk_searcher {
     ftk::DeviceSearch* ds = nullptr;
@@ -76,6 +77,35 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     return c;
 }
 
+void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
+    std::memset(sum, 0, sizeof *sum);
+    sum->threshold_snr = o.threshold;
+    sum->method = o.method;
+    sum->null_a = o.null_a;
+    sum->null_k = o.null_k;
+    sum->null_inv_s2 = o.null_inv_s2;
+    sum->null_b = o.null_b;
+    sum->dv_pxh = o.dv;
+    sum->t_ref_mjd = o.t_ref_mjd;
+    sum->span_h = o.span_h;
+    sum->n_velocities = static_cast<int>(o.grid.size());
+    sum->n_peaks = static_cast<int>(o.peaks.x.size());
+    sum->n_selected = static_cast<int>(o.selected.x.size());
+    sum->n_pooled = static_cast<int>(o.pooled.size());
+    sum->passes_per_launch = o.passes_per_launch;
+    sum->segments = o.segments;
+    for (const auto& t : o.timings) {
+        if (t.first == "prepare") sum->t_prepare += t.second;
+        else if (t.first == "stack (real)") sum->t_stack_real += t.second;
+        else if (t.first == "null rebuild") sum->t_null_rebuild += t.second;
+        else if (t.first == "stack (null)") sum->t_stack_null += t.second;
+        else if (t.first.rfind("peaks", 0) == 0) sum->t_peaks += t.second;
+        else if (t.first == "plan") sum->t_plan += t.second;
+        else if (t.first == "passes (batched)") sum->t_passes += t.second;
+        else sum->t_threshold += t.second;
+    }
+}
+
 void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
                     const ftk_search_params* p) {
     if (!s || !d_raw || !mjd || !p) throw std::runtime_error("ftk_searcher_run: null argument");
@@ -126,35 +156,7 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
         const ftk::SearchConfig c = config_of(p);
         s->last = s->ds->run(d_raw, static_cast<long long>(stride), mjd, c, d_max_snr, d_best_v,
                              static_cast<hipStream_t>(stream));
-        if (sum) {
-            std::memset(sum, 0, sizeof *sum);
-            const auto& o = s->last;
-            sum->threshold_snr = o.threshold;
-            sum->method = o.method;
-            sum->null_a = o.null_a;
-            sum->null_k = o.null_k;
-            sum->null_inv_s2 = o.null_inv_s2;
-            sum->null_b = o.null_b;
-            sum->dv_pxh = o.dv;
-            sum->t_ref_mjd = o.t_ref_mjd;
-            sum->span_h = o.span_h;
-            sum->n_velocities = static_cast<int>(o.grid.size());
-            sum->n_peaks = static_cast<int>(o.peaks.x.size());
-            sum->n_selected = static_cast<int>(o.selected.x.size());
-            sum->n_pooled = static_cast<int>(o.pooled.size());
-            sum->passes_per_launch = o.passes_per_launch;
-            sum->segments = o.segments;
-            for (const auto& t : o.timings) {
-                if (t.first == "prepare") sum->t_prepare += t.second;
-                else if (t.first == "stack (real)") sum->t_stack_real += t.second;
-                else if (t.first == "null rebuild") sum->t_null_rebuild += t.second;
-                else if (t.first == "stack (null)") sum->t_stack_null += t.second;
-                else if (t.first.rfind("peaks", 0) == 0) sum->t_peaks += t.second;
-                else if (t.first == "plan") sum->t_plan += t.second;
-                else if (t.first == "passes (batched)") sum->t_passes += t.second;
-                else sum->t_threshold += t.second;
-            }
-        }
+        if (sum) fill_summary(s->last, sum);
         return FTK_OK;
     } catch (const std::exception& e) {
         put_err2(err, err_len, e.what());
@@ -180,10 +182,8 @@ extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t str
     }
 }
 
-extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int32_t* x, int32_t* y, float* snr,
-                                  int32_t* vidx) {
-    if (!s) return -1;
-    const auto& o = s->last;
+namespace {
+int copy_peaks(const ftk::SearchOutput& o, int which, int cap, int32_t* x, int32_t* y, float* snr, int32_t* vidx) {
     if (which == 2) {
         const int c = static_cast<int>(o.pooled.size());
         for (int k = 0; k < c && k < cap; ++k)
@@ -199,4 +199,105 @@ extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int
         if (vidx) vidx[k] = p.vidx[static_cast<size_t>(k)];
     }
     return c;
+}
+}  // namespace
+
+extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int32_t* x, int32_t* y, float* snr,
+                                  int32_t* vidx) {
+    if (!s) return -1;
+    return copy_peaks(s->last, which, cap, x, y, snr, vidx);
+}
+
+// ------------------------------------------------------------------ server
+
+struct ftk_server {
+    ftk::SearchServer* srv = nullptr;
+};
+
+extern "C" int ftk_server_create(int device, double budget_mb, int slots, ftk_server** out, char* err,
+                                 size_t err_len) {
+    if (!out) return FTK_ERROR;
+    *out = nullptr;
+    try {
+        if (device < 0) device = 0;
+        auto* s = new ftk_server();
+        try {
+            s->srv = new ftk::SearchServer(device, budget_mb > 0 ? budget_mb * 1048576.0 : 0.0, slots);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" void ftk_server_destroy(ftk_server* s) {
+    if (!s) return;
+    delete s->srv;
+    delete s;
+}
+
+extern "C" int ftk_server_submit(ftk_server* s, const float* d_raw, size_t stride, int n, int w, int h,
+                                 const double* mjd, const ftk_search_params* p, long long* job, char* err,
+                                 size_t err_len) {
+    try {
+        if (!s || !p || !job) throw std::runtime_error("ftk_server_submit: null argument");
+        if (p->passes_per_launch < 0 || p->segments < 0) throw std::runtime_error("ftk_server_submit: negative batching");
+        if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
+            throw std::runtime_error("ftk_server_submit: unknown frame storage");
+        if (!(p->fwhm_px > 0.0)) throw std::runtime_error("search: fwhm_px must be given (> 0)");
+        *job = s->srv->submit(d_raw, static_cast<long long>(stride), n, w, h, mjd, config_of(p));
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_server_wait(ftk_server* s, long long job, double timeout_s, ftk_search_summary* sum,
+                               double* times, char* err, size_t err_len) {
+    try {
+        if (!s) throw std::runtime_error("ftk_server_wait: null server");
+        if (!s->srv->wait(job, timeout_s)) return FTK_TIMEOUT;
+        const ftk::ServerJob* j = s->srv->job(job);
+        if (times) {
+            times[0] = j->t_submit;
+            times[1] = j->t_admit;
+            times[2] = j->t_end;
+            times[3] = j->bytes;
+        }
+        if (j->status != 2) throw std::runtime_error(j->error);
+        if (sum) fill_summary(j->out, sum);
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_server_peaks(ftk_server* s, long long job, int which, int cap, int32_t* x, int32_t* y, float* snr,
+                                int32_t* vidx) {
+    if (!s) return -1;
+    const ftk::ServerJob* j = s->srv->job(job);
+    if (!j || j->status != 2) return -1;
+    return copy_peaks(j->out, which, cap, x, y, snr, vidx);
+}
+
+extern "C" void ftk_server_release(ftk_server* s, long long job) {
+    if (s) s->srv->release(job);
+}
+
+extern "C" int ftk_server_stats(ftk_server* s, double* stats) {
+    if (!s || !stats) return FTK_ERROR;
+    const auto st = s->srv->stats();
+    const double v[10] = {st.used, st.peak, st.budget, st.idle_bytes, static_cast<double>(st.running),
+                          static_cast<double>(st.queued), static_cast<double>(st.completed),
+                          static_cast<double>(st.failed), static_cast<double>(st.evicted),
+                          static_cast<double>(st.max_running)};
+    for (int k = 0; k < 10; ++k) stats[k] = v[k];
+    return FTK_OK;
 }

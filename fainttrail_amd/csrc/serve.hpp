@@ -1,0 +1,97 @@
+// Device-resident search server: a continuous-batching request loop over
+// DeviceSearch (search_device.hpp). No reference counterpart (the reference runs one
+// search per process, src/search.cpp:716-809).
+//
+// Jobs (frames already in HBM, times, search parameters) queue in arrival order. A job
+// is admitted when one of `slots` worker slots is free and its HBM footprint fits the
+// memory budget next to the jobs already running and the idle searchers kept for
+// reuse (idle searchers are evicted, oldest first, to make room). Admission is FIFO: a
+// job that does not fit waits, and the jobs behind it wait too (no job overtakes an
+// earlier one, so none starves). Each slot runs its job's batched search on its own
+// streams, so the stage builds, stacking launches and peak transfers of different jobs
+// overlap on the device. A job's result depends on its inputs only: the same alone
+// and in any mix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "search_device.hpp"
+
+namespace ftk {
+
+// Device bytes of a DeviceSearch over n frames of w x h and of its batched run with
+// nv motions in groups of G passes (two sets of pass buffers when the run has more
+// than one group), for the given frame storage: what admission reserves.
+double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage);
+
+struct ServerJob {
+    long long id = 0;
+    int n = 0, w = 0, h = 0;
+    const float* raw = nullptr;
+    long long stride = 0;
+    std::vector<double> mjd;
+    SearchConfig cfg;
+    long long nv = 0;
+    int passes = 1;
+    double bytes = 0.0;                  // reserved footprint
+    double t_submit = 0, t_admit = 0, t_end = 0;  // seconds on the server's clock
+    int status = 0;                      // 0 queued, 1 running, 2 done, 3 failed
+    std::string error;
+    SearchOutput out;
+};
+
+class SearchServer {
+public:
+    SearchServer(int device, double budget_bytes, int slots);
+    ~SearchServer();  // finishes the running jobs, fails the queued ones
+    SearchServer(const SearchServer&) = delete;
+    SearchServer& operator=(const SearchServer&) = delete;
+
+    long long submit(const float* raw, long long stride, int n, int w, int h, const double* mjd, const SearchConfig& cfg);
+    // Waits up to timeout_s (< 0: forever) for the job to end; false on time-out.
+    bool wait(long long id, double timeout_s);
+    // The job (nullptr if unknown); valid until release(id).
+    const ServerJob* job(long long id);
+    void release(long long id);
+    struct Stats {
+        double used = 0, peak = 0, budget = 0, idle_bytes = 0;
+        int running = 0, queued = 0, completed = 0, failed = 0, evicted = 0, max_running = 0;
+    };
+    Stats stats();
+    double now() const;
+
+private:
+    struct Idle {
+        int n, w, h;
+        std::unique_ptr<DeviceSearch> ds;
+        double bytes;
+    };
+    void worker();
+    bool head_admissible();  // with mu_ held
+
+    int device_;
+    double budget_;
+    int slots_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<ServerJob*> queue_;
+    std::map<long long, std::unique_ptr<ServerJob>> jobs_;
+    std::vector<Idle> idle_;
+    double used_ = 0.0, peak_ = 0.0;
+    int running_ = 0, max_running_ = 0, completed_ = 0, failed_ = 0, evicted_ = 0;
+    long long next_id_ = 1;
+    bool stop_ = false;
+    double t0_ = 0.0;
+    std::vector<std::thread> workers_;
+};
+
+}  // namespace ftk

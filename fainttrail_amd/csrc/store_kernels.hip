@@ -71,19 +71,25 @@ __global__ void encode_kernel(const float* __restrict__ src, long long sstride, 
     dst[i * dstride + static_cast<long long>(y) * dpitch + x] = storage == kStoreBF16 ? to_bf16(v) : to_f16(v);
 }
 
-// One thread per output pixel of rows [y0, y1): for every motion j (ascending) the sum
-// over covering frames i (ascending) in double, times 1/sqrt(c) in double; strict >
-// over j (ties to the lowest index); undefined pixels 0 / -1.
-__global__ void fp64_kernel(const float* __restrict__ frames, long long stride, int n, int w, int h, int nv,
-                            const int* __restrict__ dx, const int* __restrict__ dy, int min_count, int y0, int y1,
-                            double* __restrict__ out, int* __restrict__ best) {
+// FP64 evaluation, phase 1: one thread per output pixel of rows [y0, y1) and one chunk
+// of motions [j0, j0 + kFp64Chunk): for each motion j (ascending) the sum over covering
+// frames i (ascending) in double, times 1/sqrt(c) in double; the strict maximum over
+// the chunk (ties to the lowest index; a NaN sum is never selected) goes to the chunk's
+// slot. Phase 2 takes the first maximum over the chunks in order, which is the strict
+// maximum over all motions with ties to the lowest index.
+constexpr int kFp64Chunk = 128;
+
+__global__ void fp64_chunk_kernel(const float* __restrict__ frames, long long stride, int n, int w, int h, int nv,
+                                  const int* __restrict__ dx, const int* __restrict__ dy, int min_count, int y0, int y1,
+                                  int ybase, long long npx, double* __restrict__ cval, int* __restrict__ cidx) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = y0 + blockIdx.y;
     if (x >= w || y >= y1) return;
+    const int j0 = blockIdx.z * kFp64Chunk, j1 = min(nv, j0 + kFp64Chunk);
     const __attribute__((address_space(4))) int* cdx = (const __attribute__((address_space(4))) int*)(dx);
     const __attribute__((address_space(4))) int* cdy = (const __attribute__((address_space(4))) int*)(dy);
-    double bv = -HUGE_VAL;  // a NaN sum is never selected
+    double bv = -HUGE_VAL;
     int bj = -1;
-    for (int j = 0; j < nv; ++j) {
+    for (int j = j0; j < j1; ++j) {
         double s = 0.0;
         int c = 0;
         for (int i = 0; i < n; ++i) {
@@ -100,9 +106,27 @@ __global__ void fp64_kernel(const float* __restrict__ frames, long long stride, 
             bj = j;
         }
     }
-    const long long o = static_cast<long long>(y - y0) * w + x;
-    out[o] = bj < 0 ? 0.0 : bv;
-    best[o] = bj;
+    const long long o = static_cast<long long>(blockIdx.z) * npx + static_cast<long long>(y - ybase) * w + x;
+    cval[o] = bv;
+    cidx[o] = bj;
+}
+
+__global__ void fp64_reduce_kernel(const double* __restrict__ cval, const int* __restrict__ cidx, int chunks,
+                                   long long npx, double* __restrict__ out, int* __restrict__ best) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= npx) return;
+    double bv = -HUGE_VAL;
+    int bj = -1;
+    for (int c = 0; c < chunks; ++c) {
+        const int j = cidx[c * npx + q];
+        const double v = cval[c * npx + q];
+        if (j >= 0 && v > bv) {
+            bv = v;
+            bj = j;
+        }
+    }
+    out[q] = bj < 0 ? 0.0 : bv;
+    best[q] = bj;
 }
 
 }  // namespace
@@ -129,9 +153,29 @@ hipError_t launch_store(const float* src, long long sstride, int spitch, int n, 
 hipError_t launch_fp64(const float* frames, long long stride, int n, int w, int h, int nv, const int* d_dx,
                        const int* d_dy, int min_count, int y0, int y1, double* out, int* best, hipStream_t s) {
     if (y1 <= y0) return hipSuccess;
-    hipLaunchKernelGGL(fp64_kernel, dim3((w + 63) / 64, y1 - y0), dim3(64), 0, s, frames, stride, n, w, h, nv, d_dx,
-                       d_dy, min_count, y0, y1, out, best);
-    return hipGetLastError();
+    const int chunks = (nv + kFp64Chunk - 1) / kFp64Chunk;
+    const long long npx = static_cast<long long>(y1 - y0) * w;
+    double* cval = nullptr;
+    int* cidx = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&cval), sizeof(double) * chunks * npx, s);
+    if (e != hipSuccess) return e;
+    e = hipMallocAsync(reinterpret_cast<void**>(&cidx), sizeof(int) * chunks * npx, s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(cval, s);
+        return e;
+    }
+    for (int r = y0; r < y1; r += 32768) {
+        const int r1 = std::min(y1, r + 32768);
+        // rows [r, r1) write their slots of the [y0, y1) layout
+        hipLaunchKernelGGL(fp64_chunk_kernel, dim3((w + 63) / 64, r1 - r, chunks), dim3(64), 0, s, frames, stride, n, w, h,
+                           nv, d_dx, d_dy, min_count, r, r1, y0, npx, cval, cidx);
+    }
+    hipLaunchKernelGGL(fp64_reduce_kernel, dim3(static_cast<unsigned>((npx + 255) / 256)), dim3(256), 0, s, cval, cidx,
+                       chunks, npx, out, best);
+    e = hipGetLastError();
+    (void)hipFreeAsync(cval, s);
+    (void)hipFreeAsync(cidx, s);
+    return e;
 }
 
 }  // namespace ftk

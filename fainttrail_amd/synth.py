@@ -47,8 +47,13 @@ def _add_source(img: np.ndarray, x0: float, y0: float, flux: float, sigma: float
 def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float = 32.0, nstars: int = 150,
                   star_flux=(1e3, 1e6), psf_sigma: float = 1.5, nmovers: int = 3, mover_snr=(1.5, 1.5),
                   rate_px_min=(0.2, 1.1), cadence_s: float = 60.0, exptime_s: float = 60.0, seed: int = 7,
-                  device: str = "cpu", mjd0: float = 61000.0):
-    """Returns (frames (n, h, w) float32 torch tensor on `device`, mjd (n,), movers)."""
+                  device: str = "cpu", mjd0: float = 61000.0, saturate: float = 0.0, cr_frames: int = 0,
+                  cr_hits: int = 50):
+    """Returns (frames (n, h, w) float32 torch tensor on `device`, mjd (n,), movers).
+
+    saturate > 0 clips every pixel at that level (a detector's full well, e.g. 65535
+    ADU); cr_frames frames (chosen by the seed) get cr_hits cosmic-ray hits of 1-4
+    pixels and 800-15000 ADU each."""
     import torch
     rng = np.random.default_rng(seed)
     static = np.zeros((h, w), np.float32)
@@ -79,4 +84,14 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
                 tt = t_h[i] + ((k + 0.5) / steps - 0.5) * exptime_s / 3600.0
                 _add_source(mv, m.x0 + m.vx_pxh * tt, m.y0 + m.vy_pxh * tt, m.flux / steps, psf_sigma)
         frames[i] = base + torch.from_numpy(mv).to(device) + noise * torch.randn((h, w), generator=gen, device=device)
+    for i in rng.choice(n, size=min(cr_frames, n), replace=False) if cr_frames > 0 else []:
+        for _ in range(cr_hits):
+            x, y = int(rng.integers(0, w)), int(rng.integers(0, h))
+            amp = float(rng.uniform(800.0, 15000.0))
+            for _ in range(int(rng.integers(1, 5))):
+                frames[i, y, x] += amp
+                x = min(w - 1, max(0, x + int(rng.integers(-1, 2))))
+                y = min(h - 1, max(0, y + int(rng.integers(-1, 2))))
+    if saturate > 0.0:
+        frames.clamp_(max=saturate)
     return frames, mjd, movers

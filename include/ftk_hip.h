@@ -28,6 +28,7 @@ extern "C" {
 #define FTK_OK 0
 #define FTK_ERROR 1
 #define FTK_CANCELLED 2
+#define FTK_TIMEOUT 3 /* ftk_server_wait: the job has not ended yet */
 
 /* Progress in [0,1] and a stage name; return 0 to cancel.
  * Same shape as ft_progress_fn (fainttrail.h:71) and ProgressFn (types.hpp:76). */
@@ -237,6 +238,38 @@ int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride,
  * entries into the non-NULL arrays and returns the total count. */
 int ftk_searcher_peaks(const ftk_searcher* searcher, int which, int cap, int32_t* x, int32_t* y, float* snr,
                        int32_t* vidx);
+
+/* ------------------------------------------------------------------ serving form
+ * No reference counterpart (one search per process there). A device-resident request
+ * loop: submitted searches queue in arrival order and are admitted, first come first
+ * served, when one of `slots` worker slots is free and the job's HBM footprint (an
+ * estimate of its searcher and batched pass buffers) fits the budget beside the running
+ * jobs and the idle searchers kept for reuse (evicted, oldest first, when room is
+ * needed). Each slot runs its job's batched search on its own streams, so jobs overlap
+ * on the device. A job's results depend on its inputs only. */
+typedef struct ftk_server ftk_server;
+/* budget_mb <= 0: 85 % of the free device memory. */
+int ftk_server_create(int device, double budget_mb, int slots, ftk_server** out, char* err, size_t err_len);
+/* Finishes the running jobs; queued jobs fail. */
+void ftk_server_destroy(ftk_server* server);
+/* d_raw: n frames (device, stride floats apart) sorted by time, kept valid and
+ * unchanged until the job ends; mjd: n times (copied). params as for
+ * ftk_searcher_run (passes_per_launch 0: up to 8, within the budget). */
+int ftk_server_submit(ftk_server* server, const float* d_raw, size_t stride, int n, int w, int h, const double* mjd,
+                      const ftk_search_params* params, long long* job, char* err, size_t err_len);
+/* Waits for a job (timeout_s < 0: no limit): FTK_OK with its summary, FTK_ERROR with
+ * the job's error, FTK_TIMEOUT. times (may be NULL): submitted, admitted, ended (s on
+ * the server's clock), reserved bytes. */
+int ftk_server_wait(ftk_server* server, long long job, double timeout_s, ftk_search_summary* summary, double* times,
+                    char* err, size_t err_len);
+/* Peaks of an ended job, as ftk_searcher_peaks; -1 for an unknown or failed job. */
+int ftk_server_peaks(ftk_server* server, long long job, int which, int cap, int32_t* x, int32_t* y, float* snr,
+                     int32_t* vidx);
+/* Forgets an ended job. */
+void ftk_server_release(ftk_server* server, long long job);
+/* stats[10]: reserved bytes, peak reserved bytes, budget, idle-searcher bytes, running,
+ * queued, completed, failed, evicted searchers, most jobs running at once. */
+int ftk_server_stats(ftk_server* server, double* stats);
 
 /* Plain device copy (n floats) on `stream`: the known-size kernel that calibrates
  * HBM traffic counters. */

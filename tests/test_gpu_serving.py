@@ -1,0 +1,101 @@
+"""Serving form (ftk_server_*): jobs of different frame sizes, frame counts and velocity
+grids submitted together run concurrently, and each job's result (peaks, selected
+candidates, pooled null peaks in run order, threshold) is bit-identical to the same
+search run alone; a tight memory budget serialises admission without changing any
+result; a job larger than the budget is refused at submission."""
+import numpy as np
+import pytest
+
+from refcases import same_bits
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [  # (side_x, side_y, frames, nulls, grid_tol, seed)
+    (160, 120, 12, 3, 0.6, 1),
+    (224, 176, 16, 4, 0.5, 2),
+    (96, 96, 10, 2, 0.4, 3),
+    (200, 136, 18, 5, 0.5, 4),
+    (128, 200, 14, 3, 0.7, 5),
+]
+
+
+def _inputs():
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams
+    jobs = []
+    for w, h, n, nulls, tol, seed in SPECS:
+        fr, mjd, _ = synth.make_sequence(n, w, h, nstars=30, nmovers=2, mover_snr=(4.0, 8.0), seed=seed,
+                                         device="cuda:0")
+        prm = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=tol, fwhm_px=3.0, null_runs=nulls, seed=seed)
+        jobs.append((fr.contiguous(), mjd, prm))
+    return jobs
+
+
+def _same(alone, mixed):
+    assert alone.n_velocities == mixed.n_velocities
+    assert alone.threshold_snr == mixed.threshold_snr and alone.threshold_method == mixed.threshold_method
+    for a, b in ((alone.peaks, mixed.peaks), (alone.selected, mixed.selected)):
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y) and np.array_equal(a.vidx, b.vidx)
+        assert same_bits(a.snr, b.snr)
+    assert same_bits(alone.null_peaks, mixed.null_peaks)
+
+
+def test_jobs_alone_and_in_the_mix(hip):
+    import torch
+    from fainttrail_amd.search import Searcher
+    from fainttrail_amd.serve import Server
+    jobs = _inputs()
+    alone = []
+    for fr, mjd, prm in jobs:
+        n, h, w = fr.shape
+        alone.append(Searcher(n, w, h).run(fr, mjd, prm))
+    torch.cuda.synchronize()
+    srv = Server(slots=3)
+    ids = [srv.submit(fr, mjd, prm) for fr, mjd, prm in jobs]
+    got = [srv.wait(j) for j in ids]
+    st = srv.stats()
+    print("server stats:", st, flush=True)
+    srv.close()
+    for a, b in zip(alone, got):
+        _same(a, b)
+        assert b.t_submit <= b.t_admit <= b.t_end
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert 1 <= st["max_running"] <= 3
+    assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+
+
+def test_tight_budget_serialises_without_changing_results(hip):
+    from fainttrail_amd import FtkError
+    from fainttrail_amd.search import Searcher
+    from fainttrail_amd.serve import Server
+    jobs = _inputs()[:3]
+    alone = []
+    for fr, mjd, prm in jobs:
+        n, h, w = fr.shape
+        alone.append(Searcher(n, w, h).run(fr, mjd, prm))
+    # a budget that holds the largest job but not two of them at once
+    probe = Server(slots=3)
+    sizes = []
+    for fr, mjd, prm in jobs:
+        j = probe.submit(fr, mjd, prm)
+        sizes.append(probe.wait(j).reserved_bytes)
+    probe.close()
+    budget_mb = 1.2 * max(sizes) / 1048576.0
+    srv = Server(budget_mb=budget_mb, slots=3)
+    ids = [srv.submit(fr, mjd, prm) for fr, mjd, prm in jobs]
+    got = [srv.wait(j) for j in ids]
+    st = srv.stats()
+    print("tight budget stats:", st, flush=True)
+    for a, b in zip(alone, got):
+        _same(a, b)
+    assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+    assert sum(sizes) > st["budget_bytes"] and st["max_running"] < len(jobs)
+    # admission is first come, first served
+    assert all(got[k].t_admit <= got[k + 1].t_admit for k in range(len(got) - 1))
+    # a job larger than the whole budget is refused when submitted
+    tiny = Server(budget_mb=1.0, slots=1)
+    fr, mjd, prm = jobs[0]
+    with pytest.raises(FtkError):
+        tiny.submit(fr, mjd, prm)
+    tiny.close()
+    srv.close()
