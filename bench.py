@@ -172,8 +172,9 @@ def candidate_agreement(exact, other):
 def local_maxima(m, floor: float):
     """Pixels at or above floor that are the maximum of their 5 x 5 window (torch)."""
     import torch
-    mx = torch.nn.functional.max_pool2d(m[None, None].float(), 5, stride=1, padding=2)[0, 0]
-    ys, xs = torch.nonzero((m >= floor) & (m == mx), as_tuple=True)
+    md = m.double()
+    mx = torch.nn.functional.max_pool2d(md[None, None], 5, stride=1, padding=2)[0, 0]
+    ys, xs = torch.nonzero((md >= floor) & (md == mx), as_tuple=True)
     return set(zip(ys.tolist(), xs.tolist()))
 
 
@@ -199,7 +200,7 @@ def precision_measure(fp64_rows: int):
                                       device="cuda:0", saturate=65535.0, cr_frames=1, cr_hits=50)
     t_h = (mjd - 0.5 * (mjd[0] + mjd[-1])) * 24.0
     stage = Stage(n, w, h)
-    sub = stage.background(raw)
+    sub, _ = stage.background(raw)
     del raw
     filt = stage.build(sub, t_h, StageParams(fwhm_px=fwhm))
     if isinstance(filt, tuple):
@@ -259,6 +260,67 @@ def precision_measure(fp64_rows: int):
             out[st]["candidates_vs_fp32_full_frame"] = {"fp32": len(full_ref), "this": len(full),
                                                         "common": len(full & full_ref)}
     return out
+
+
+def serving_measure(count: int, rate_hz: float, seed: int, slots: int, sustain_count: int):
+    """Serving form: a seeded Poisson stream of search jobs (mean rate_hz arrivals/s;
+    per job, log-uniform: side 256-4096 px, 10-120 frames, 500-50 000 motions, 4-64
+    null runs; 0-15 movers) fed to the device-resident server with `slots` slots and
+    85 % of the free HBM as its budget. Inputs are generated on the device before the
+    clock starts; a job's latency runs from its submission (at its arrival time) to the
+    end of its search. The stream at the stated rate overloads one GPU (offered load
+    rho = arrival rate x mean service time >> 1), so its latency is mostly queueing;
+    the first sustain_count jobs are then replayed at 70 % of the measured capacity."""
+    import torch
+
+    import fainttrail_amd as ft
+    from fainttrail_amd.serve import Server, job_inputs, poisson_jobs
+
+    def run(specs):
+        inputs = [job_inputs(sp) for sp in specs]
+        units = []
+        for sp, (fr, mjd, prm) in zip(specs, inputs):
+            span_h = (mjd[-1] - mjd[0]) * 24.0
+            nv = ft.velocity_grid(span_h, prm.vmin_pxh, prm.vmax_pxh, prm.grid_tol_px)[0].shape[0]
+            units.append(float(nv) * sp.frames * sp.side * sp.side * (1 + sp.nulls))
+        torch.cuda.synchronize()
+        srv = Server(slots=slots)
+        t0 = time.perf_counter()
+        ids = []
+        for sp, (fr, mjd, prm) in zip(specs, inputs):
+            delay = t0 + sp.arrival_s - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(fr, mjd, prm))
+        res = [srv.wait(j) for j in ids]
+        st = srv.stats()
+        srv.close()
+        del inputs
+        torch.cuda.empty_cache()
+        lat = np.array([r.latency for r in res])
+        wait = np.array([r.t_admit - r.t_submit for r in res])
+        span = max(r.t_end for r in res) - min(r.t_submit for r in res)
+        busy = sum(r.t_end - r.t_admit for r in res)
+        return {"jobs": len(res), "p50_latency_s": float(np.percentile(lat, 50)),
+                "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
+                "max_latency_s": float(lat.max()), "p50_queue_wait_s": float(np.percentile(wait, 50)),
+                "jobs_per_s": len(res) / span, "units_per_s": sum(units) / span, "units": sum(units),
+                "wall_s": span, "mean_job_s_in_slot": busy / len(res), "max_running": st["max_running"],
+                "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9,
+                "evicted_searchers": st["evicted"]}
+
+    specs = poisson_jobs(count, rate_hz, seed)
+    over = run(specs)
+    service = over["wall_s"] / over["jobs"]  # the GPU's mean time per job, all slots together
+    over["offered_load_rho"] = rate_hz * service
+    over["arrival_rate_hz"] = rate_hz
+    sustain_rate = 0.7 * over["jobs_per_s"]
+    sus = run(poisson_jobs(sustain_count, sustain_rate, seed))
+    sus["arrival_rate_hz"] = sustain_rate
+    sus["offered_load_rho"] = sustain_rate * service
+    return {"config": f"Poisson stream, seed {seed}: side 256-4096, 10-120 frames, 500-50000 motions, 4-64 nulls, "
+                      f"0-15 movers; {slots} slots, budget 85 % of free HBM",
+            "at_stated_rate": over, "at_sustainable_rate": sus}
 
 
 def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
@@ -324,6 +386,9 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true", help="skip the fp16/bf16/FP64 precision measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-null batched calibration measurement")
     ap.add_argument("--fp64-rows", type=int, default=16)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson stream) measurement")
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-seed", type=int, default=3)
     ap.add_argument("--search-only", action="store_true",
                     help="measure only the null-calibrated search (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -410,14 +475,22 @@ def main() -> int:
     if not args.no_search:
         print("bench: null-calibrated search (512x512 configuration)", file=sys.stderr, flush=True)
         res["search"] = search_measure(args.search_reps, with_plain=not args.no_cpu)
+        print("bench: search: " + json.dumps(res["search"]), file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_batched:
         print("bench: batched null calibration (2048x2048, 256 nulls)", file=sys.stderr, flush=True)
         res["batched_nulls"] = batched_nulls_measure(with_plain=not args.no_cpu)
+        print("bench: batched_nulls: " + json.dumps(res["batched_nulls"]), file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_precision:
         print("bench: precision form (100 x 6144x4096)", file=sys.stderr, flush=True)
         res["precision"] = precision_measure(args.fp64_rows)
+        print("bench: precision: " + json.dumps(res["precision"]), file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        print("bench: serving (Poisson stream of search jobs)", file=sys.stderr, flush=True)
+        res["serving"] = serving_measure(args.serve_jobs, 20.0, args.serve_seed, 4, 12)
+        print("bench: serving: " + json.dumps(res["serving"]), file=sys.stderr, flush=True)
     print(json.dumps(res))
     return 0
 
