@@ -144,9 +144,11 @@ def search_measure(reps: int, with_plain: bool):
         from oracle import pyoracle as po
         if po.ref_available():
             host = frames.cpu().numpy()
-            t0 = time.perf_counter()
-            ref = po.ref_search(host, mjd, 12.0, 66.0, 0.125, prm.fwhm_px, 8, seed=7, backend="hip")
-            dtp = time.perf_counter() - t0
+            dtp = float("inf")
+            for _ in range(2):  # the better of two runs (the first also pays one-time set-up)
+                t0 = time.perf_counter()
+                ref = po.ref_search(host, mjd, 12.0, 66.0, 0.125, prm.fwhm_px, 8, seed=7, backend="hip")
+                dtp = min(dtp, time.perf_counter() - t0)
             out["plain_searches_per_s"] = 1.0 / dtp
             out["plain_s_per_search"] = dtp
             out["device_over_plain"] = dtp / dt
@@ -383,6 +385,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the null-calibrated search measurement")
     ap.add_argument("--search-reps", type=int, default=3)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="frame storage of the timed stacking passes (16-bit: a power-of-two scale per frame)")
     ap.add_argument("--no-precision", action="store_true", help="skip the fp16/bf16/FP64 precision measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-null batched calibration measurement")
     ap.add_argument("--fp64-rows", type=int, default=16)
@@ -421,29 +425,39 @@ def main() -> int:
     stream = torch.cuda.current_stream(dev)
     print(f"bench: {args.workload}: {n} x {w}x{h}, {tab.motions} motions, {plan.blocks} blocks, "
           f"{plan.units:.3e} units/pass", file=sys.stderr, flush=True)
+    if args.storage == "f32":
+        def one_pass():
+            plan.run(frames, out_m, out_b, stream)
+    else:
+        plan.store(frames, args.storage, stream)
+
+        def one_pass():
+            plan.run_stored(out_m, out_b, stream)
     for _ in range(args.warmup):
-        plan.run(frames, out_m, out_b, stream)
+        one_pass()
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for k in range(args.steps):
         starts[k].record(stream)
-        plan.run(frames, out_m, out_b, stream)
+        one_pass()
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_s = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / 1e3 / args.steps
     units = plan.units
     value = units * args.steps / wall
-    achieved = BYTES_PER_UNIT * units / kernel_s
+    bytes_per_unit = BYTES_PER_UNIT if args.storage == "f32" else 2  # one stored sample per unit
+    achieved = bytes_per_unit * units / kernel_s
     print(f"bench: {wall / args.steps * 1e3:.1f} ms/pass, {value:.3e} units/s", file=sys.stderr, flush=True)
     # the inner loop reads two rows per ds_read2st64_b32 (8 B per lane per instruction)
     lds_peak = ft.probe_lds(8)
     res = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (unit-variance fp32 frames, seed 7)",
+        "dtype": "f32" if args.storage == "f32" else f"{args.storage} storage, f32 accumulation",
+        "data": "synthetic (unit-variance fp32 frames, seed 7)",
         "config": {"workload": args.workload, "frames": n, "width": w, "height": h, "motions": int(tab.motions),
                    "grid_tol_px": WORKLOADS[args.workload][5], "dv_pxh": dv, "min_count": min_count,
                    "units_per_step": units},

@@ -63,7 +63,7 @@ def lib() -> C.CDLL:
     if _lib is not None:
         return _lib
     if not _LIB_PATH.exists():
-        raise FtkError(f"{_LIB_PATH} is not built; run python -c 'import __graft_entry__ as g; g.build()'")
+        raise FtkError(f"{_LIB_PATH} is not built; run python -m fainttrail_amd.build")
     # One HIP runtime per process: torch ships its own libamdhip64.so.7. Loading torch
     # first makes the dynamic loader bind this library to that same copy (equal
     # SONAME); loading /opt/rocm's copy first leaves torch without a usable device.

@@ -86,7 +86,26 @@ __device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int
 
 // The body of one workgroup of a stacking pass: workgroup `wg` of the pass described
 // by `a` (the kernels below pick the pass and the workgroup index).
-template <bool RING, bool PADDED, int ST>
+// acc + q s for the 16-bit value q in the low or high half of w and a power-of-two
+// scale s: the product is exact, so the single rounding of the fused operation equals
+// adding the decoded value q s. fp16: one v_fma_mix_f32 (widening inside the FMA);
+// bf16: the value's fp32 bits are its 16 bits shifted up.
+template <int ST>
+__device__ __forceinline__ float fma_q16(uint32_t w, bool hi, float s, float acc) {
+    if constexpr (ST == kStoreF16) {
+        if (hi)
+            asm("v_fma_mix_f32 %0, %1, %2, %0 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(w), "s"(s));
+        else
+            asm("v_fma_mix_f32 %0, %1, %2, %0 op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(w), "s"(s));
+        return acc;
+    } else {
+        return __builtin_fmaf(__uint_as_float(hi ? (w & 0xffff0000u) : (w << 16)), s, acc);
+    }
+}
+
+// PRE: for 16-bit frames, box rows staged per wavefront (the kernel is instantiated
+// for the few values the planner picks from its largest vertical spread); 0 for fp32.
+template <bool RING, bool PADDED, int ST, int PRE>
 __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     static_assert(ST == kStoreF32 || (ST == kStoreF16 || ST == kStoreBF16), "frame storage");
     static_assert(ST == kStoreF32 || !RING || PADDED, "16-bit frames are always padded");
@@ -251,78 +270,91 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 __syncthreads();
             }
         } else {
-            // 16-bit frames: the next frame's box rows are fetched into registers while
-            // the current frame is consumed, one dword (two pixels) per lane and row from
-            // the even column at or left of the box start, then widened, multiplied by the
-            // frame's power-of-two scale (exact) and written to the other LDS buffer as
-            // fp32, box column c at LDS column c + 1 (so every lane's pair lands at a
-            // non-negative, 8-byte or 4-byte aligned pair of columns). The sum is the
-            // fp32 sum of the decoded frames. Lanes past the box repeat its last pair.
-            uint32_t pre[kPreRows];
-            int pre_rows = 0, pre_col = 0;
+            // 16-bit frames, kept 16-bit in LDS as row pairs: LDS word (r, c) holds the
+            // values of box rows r (low half) and r + 1 (high half) at box column c. Each
+            // buffer starts one word into the LDS area, so a pair fetched from one column
+            // left of the box (odd start) writes its first word to the word before the
+            // buffer (the area's first word, or the previous buffer's last word, which no
+            // motion reads). A lane's four rows of a motion are then two words, read
+            // with one ds_read2st64, half the LDS traffic of fp32 rows; each value is
+            // widened and multiplied by the frame's power-of-two scale inside a fused
+            // multiply-add whose product is exact (q times a power of two), so every sum
+            // is the fp32 sum of the decoded frames (q * s), added in frame order.
+            //
+            // Staging: wave w fetches box rows w PRE .. w PRE + PRE (clamped to the box)
+            // as dwords of two pixels from the even column at or left of the box start,
+            // while the current frame is consumed, then packs the pairs of rows into the
+            // word rows w PRE .. w PRE + PRE - 1 of the other buffer (word rows past the
+            // box are written but never read; the buffer has at least 4 PRE rows).
+            uint32_t* const lw0 = reinterpret_cast<uint32_t*>(lds0) + 1;
+            uint32_t* const lw1 = reinterpret_cast<uint32_t*>(lds1) + 1;
+            uint32_t pre[PRE + 1];
+            int pre_col = 0;
+            float sc_next = 1.0f;
             auto fetch = [&](int i) {
                 cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
                 const int sx = bx[2], R = kTileY + bx[3];
                 const int bxs = x0 + bx[0];
                 const int off = bxs & 1;  // 1: the pair starts one column left of the box
                 const int lmax = (kTileX - 1 + sx + off) >> 1;
-                const int l = min(lane, lmax);
+                const int l = min(lane, lmax);  // lanes past the box repeat its last pair
                 const elem_t* rowp =
                     frames + static_cast<long long>(i) * frame_stride + static_cast<long long>(y0 + bx[1] - row0) * pitch + (bxs - off);
                 const __amdgpu_buffer_rsrc_t rs =
                     __builtin_amdgcn_make_buffer_rsrc(const_cast<elem_t*>(rowp), 0, 0x7fffffff, kBufferWord3);
-                pre_rows = R;
-                pre_col = 2 * l - off + 1;  // LDS column of the pair's first pixel
+                pre_col = 2 * l - off;  // word column of the pair's first pixel (-1 for the guard word)
+                sc_next = reinterpret_cast<const __attribute__((address_space(4))) float*>(
+                    (const __attribute__((address_space(4))) char*)(scale))[i];
 #pragma unroll
-                for (int m = 0; m < kPreRows; ++m) {
-                    const int rr = min(wave + kWaves * m, R - 1);  // rows past the box repeat its last
+                for (int m = 0; m <= PRE; ++m) {
+                    const int rr = min(wave * PRE + m, R - 1);
                     pre[m] = __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * l, rr * 2 * pitch, 0);
                 }
             };
-            auto commit = [&](int i, float* dst) {
-                const float sc = reinterpret_cast<const __attribute__((address_space(4))) float*>(
-                    (const __attribute__((address_space(4))) char*)(scale))[i];
-                float* d = dst + pre_col;
+            auto commit = [&](uint32_t* dst) {
+                uint32_t* d = dst + wave * PRE * kPitch + pre_col;
 #pragma unroll
-                for (int m = 0; m < kPreRows; ++m) {
-                    const int rr = min(wave + kWaves * m, pre_rows - 1);
-                    const uint32_t v = pre[m];
-                    float lo, hi;
-                    if constexpr (ST == kStoreF16) {
-                        lo = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(v & 0xffffu)));
-                        hi = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(v >> 16)));
-                    } else {
-                        lo = __uint_as_float(v << 16);
-                        hi = __uint_as_float(v & 0xffff0000u);
-                    }
-                    d[rr * kPitch] = lo * sc;
-                    d[rr * kPitch + 1] = hi * sc;
+                for (int m = 0; m < PRE; ++m) {
+                    d[m * kPitch] = (pre[m] & 0xffffu) | (pre[m + 1] << 16);
+                    d[m * kPitch + 1] = (pre[m] >> 16) | (pre[m + 1] & 0xffff0000u);
                 }
             };
-            const int lane_base16 = lane_base + 1;
-            auto consume16 = [&](const float* L, int i) {
+            // per-lane read bases of the two buffers, formed once (the empty asm keeps the
+            // compiler from splitting the guard-word offset back out of them, which would
+            // cost one more add per motion)
+            int lb = lane_base + 1;
+            asm volatile("" : "+v"(lb));
+            const uint32_t* const rb0 = reinterpret_cast<const uint32_t*>(lds0) + lb;
+            const uint32_t* const rb1 = reinterpret_cast<const uint32_t*>(lds1) + lb;
+            auto consume16 = [&](const uint32_t* L, int i, float sc) {
                 cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
 #pragma unroll
                 for (int k = 0; k < V; ++k) {
-                    const float* p = L + lane_base16 + lo[k];
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
+                    const uint32_t* p = L + lo[k];
+                    const uint32_t w01 = p[0], w23 = p[2 * kPitch];
+                    acc[k][0] = fma_q16<ST>(w01, false, sc, acc[k][0]);
+                    acc[k][1] = fma_q16<ST>(w01, true, sc, acc[k][1]);
+                    acc[k][2] = fma_q16<ST>(w23, false, sc, acc[k][2]);
+                    acc[k][3] = fma_q16<ST>(w23, true, sc, acc[k][3]);
                 }
             };
             fetch(0);
-            commit(0, lds0);
+            commit(lw0);
+            float sc = sc_next;
             __syncthreads();
             // (straight-line code: past the last frame the last one is staged again into
             // the buffer that is not read any more)
             for (int i = 0; i < n; i += 2) {
                 fetch(min(i + 1, n - 1));
-                consume16(lds0, i);
-                commit(min(i + 1, n - 1), lds1);
+                consume16(rb0, i, sc);
+                commit(lw1);
+                sc = sc_next;
                 __syncthreads();
                 if (i + 1 >= n) break;
                 fetch(min(i + 2, n - 1));
-                consume16(lds1, i + 1);
-                commit(min(i + 2, n - 1), lds0);
+                consume16(rb1, i + 1, sc);
+                commit(lw0);
+                sc = sc_next;
                 __syncthreads();
             }
         }
@@ -395,19 +427,19 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     }
 }
 
-template <bool RING, bool PADDED, int ST>
+template <bool RING, bool PADDED, int ST, int PRE>
 __global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
-    stack_body<RING, PADDED, ST>(a, blockIdx.x);
+    stack_body<RING, PADDED, ST, PRE>(a, blockIdx.x);
 }
 
 // Several passes in one launch (the real pass and null runs of a batched search, or
 // regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
 // is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
-template <bool RING, bool PADDED, int ST>
+template <bool RING, bool PADDED, int ST, int PRE>
 __global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
     const unsigned p = blockIdx.x / per_pass;
     const StackArgs a = passes[p];
-    stack_body<RING, PADDED, ST>(a, blockIdx.x - p * per_pass);
+    stack_body<RING, PADDED, ST, PRE>(a, blockIdx.x - p * per_pass);
 }
 
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
@@ -513,6 +545,12 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.seg0 = L.seg0;
     a.nseg = L.nseg;
     a.lds_rows = kTileY + p.max_sy;
+    if (L.storage != kStoreF32) {
+        // word rows: 15 + the largest vertical spread, PRE per wave (5, 8 or 12)
+        const int need = (kTileY - 1 + p.max_sy + kWaves - 1) / kWaves;
+        a.pre = need <= 5 ? 5 : need <= 8 ? 8 : 12;
+        a.lds_rows = std::max(a.lds_rows, kWaves * a.pre);
+    }
     a.cov_ok = p.cov_ok ? 1 : 0;
     inner = a;
     ring = a;
@@ -526,26 +564,34 @@ namespace {
 unsigned pass_groups(const StackArgs& a) {
     return static_cast<unsigned>((static_cast<long long>(a.ntiles) + 7) / 8 * 8 * a.nseg);
 }
-size_t lds_bytes_of(int lds_rows) { return sizeof(float) * 2 * lds_rows * kPitch; }
+// two buffers and the guard word of the 16-bit layout
+size_t lds_bytes_of(int lds_rows) { return sizeof(float) * (2 * lds_rows * kPitch + 1); }
 }  // namespace
 
 namespace {
-// kernel for (ring tiles, padded frames, storage)
+// kernel for (ring tiles, padded frames, storage, 16-bit rows per wave)
 using SingleFn = void (*)(StackArgs);
 using MultiFn = void (*)(const StackArgs*, unsigned);
-SingleFn single_kernel(bool ring, bool padded, int st) {
-    if (!ring) return st == kStoreF16 ? stack_kernel<false, false, kStoreF16>
-                    : st == kStoreBF16 ? stack_kernel<false, false, kStoreBF16> : stack_kernel<false, false, kStoreF32>;
-    if (st == kStoreF16) return stack_kernel<true, true, kStoreF16>;
-    if (st == kStoreBF16) return stack_kernel<true, true, kStoreBF16>;
-    return padded ? stack_kernel<true, true, kStoreF32> : stack_kernel<true, false, kStoreF32>;
+template <bool RING, int ST>
+SingleFn single16(int pre) {
+    return pre <= 5 ? stack_kernel<RING, true, ST, 5> : pre <= 8 ? stack_kernel<RING, true, ST, 8> : stack_kernel<RING, true, ST, 12>;
 }
-MultiFn multi_kernel(bool ring, bool padded, int st) {
-    if (!ring) return st == kStoreF16 ? stack_multi_kernel<false, false, kStoreF16>
-                    : st == kStoreBF16 ? stack_multi_kernel<false, false, kStoreBF16> : stack_multi_kernel<false, false, kStoreF32>;
-    if (st == kStoreF16) return stack_multi_kernel<true, true, kStoreF16>;
-    if (st == kStoreBF16) return stack_multi_kernel<true, true, kStoreBF16>;
-    return padded ? stack_multi_kernel<true, true, kStoreF32> : stack_multi_kernel<true, false, kStoreF32>;
+template <bool RING, int ST>
+MultiFn multi16(int pre) {
+    return pre <= 5 ? stack_multi_kernel<RING, true, ST, 5>
+                    : pre <= 8 ? stack_multi_kernel<RING, true, ST, 8> : stack_multi_kernel<RING, true, ST, 12>;
+}
+SingleFn single_kernel(bool ring, bool padded, int st, int pre) {
+    if (st == kStoreF16) return ring ? single16<true, kStoreF16>(pre) : single16<false, kStoreF16>(pre);
+    if (st == kStoreBF16) return ring ? single16<true, kStoreBF16>(pre) : single16<false, kStoreBF16>(pre);
+    if (!ring) return stack_kernel<false, false, kStoreF32, 0>;
+    return padded ? stack_kernel<true, true, kStoreF32, 0> : stack_kernel<true, false, kStoreF32, 0>;
+}
+MultiFn multi_kernel(bool ring, bool padded, int st, int pre) {
+    if (st == kStoreF16) return ring ? multi16<true, kStoreF16>(pre) : multi16<false, kStoreF16>(pre);
+    if (st == kStoreBF16) return ring ? multi16<true, kStoreBF16>(pre) : multi16<false, kStoreBF16>(pre);
+    if (!ring) return stack_multi_kernel<false, false, kStoreF32, 0>;
+    return padded ? stack_multi_kernel<true, true, kStoreF32, 0> : stack_multi_kernel<true, false, kStoreF32, 0>;
 }
 bool storage_ok(const StackArgs& a) {
     if (a.storage == kStoreF32) return true;
@@ -559,11 +605,11 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s)
     if (!storage_ok(ring)) return hipErrorInvalidValue;
     const size_t lds = lds_bytes_of(inner.lds_rows);
     if (inner.ntiles > 0)
-        hipLaunchKernelGGL(single_kernel(false, false, inner.storage), dim3(pass_groups(inner)), dim3(kThreads), lds, s,
-                           inner);
+        hipLaunchKernelGGL(single_kernel(false, false, inner.storage, inner.pre), dim3(pass_groups(inner)), dim3(kThreads),
+                           lds, s, inner);
     if (ring.ntiles > 0)
-        hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage), dim3(pass_groups(ring)), dim3(kThreads),
-                           lds, s, ring);
+        hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage, ring.pre), dim3(pass_groups(ring)),
+                           dim3(kThreads), lds, s, ring);
     return hipGetLastError();
 }
 
@@ -572,11 +618,11 @@ hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host
     if (count <= 0) return hipSuccess;
     unsigned per_in = 0, per_ring = 0;
     const int lds_rows = host_inner[0].lds_rows;
-    const int padded = host_ring[0].padded, storage = host_ring[0].storage;
+    const int padded = host_ring[0].padded, storage = host_ring[0].storage, pre = host_ring[0].pre;
     for (int q = 0; q < count; ++q) {
         const StackArgs &a = host_inner[q], &r = host_ring[q];
         if (a.lds_rows != lds_rows || r.lds_rows != lds_rows || r.padded != padded || a.storage != storage ||
-            r.storage != storage || !storage_ok(r))
+            r.storage != storage || a.pre != pre || r.pre != pre || !storage_ok(r))
             return hipErrorInvalidValue;
         per_in = std::max(per_in, pass_groups(a));
         per_ring = std::max(per_ring, pass_groups(r));
@@ -584,11 +630,11 @@ hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host
     const size_t lds = lds_bytes_of(lds_rows);
     const dim3 block(kThreads);
     if (per_in > 0)
-        hipLaunchKernelGGL(multi_kernel(false, false, storage), dim3(per_in * static_cast<unsigned>(count)), block, lds,
-                           s, dev_inner, per_in);
+        hipLaunchKernelGGL(multi_kernel(false, false, storage, pre), dim3(per_in * static_cast<unsigned>(count)), block,
+                           lds, s, dev_inner, per_in);
     if (per_ring > 0)
-        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage), dim3(per_ring * static_cast<unsigned>(count)), block,
-                           lds, s, dev_ring, per_ring);
+        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage, pre), dim3(per_ring * static_cast<unsigned>(count)),
+                           block, lds, s, dev_ring, per_ring);
     return hipGetLastError();
 }
 

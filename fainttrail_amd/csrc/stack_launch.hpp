@@ -68,6 +68,8 @@ struct StackArgs {
     TileRect rect;
     int row0, row1, n, w, h, pitch, padded, storage, min_count, nb, bps;
     int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
+    int pre;  // 16-bit frames: box rows staged per wavefront (5, 8 or 12)
+    int pad_;
 };
 
 hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,

@@ -13,7 +13,7 @@
 // run unchanged. Options come from FAINTTRAIL_HIP_OPTIONS ("device=0,max_mb=2,...",
 // the same keys as "cuda:...", src/cuda_plan.cpp:14-33), since registered names are
 // matched whole. Making "auto" prefer it is a two-line change to make_backend and
-// backend_names, shown in INTEGRATION.md.
+// backend_names (src/backend.cpp:46-87).
 #include <cstdlib>
 #include <memory>
 #include <stdexcept>
