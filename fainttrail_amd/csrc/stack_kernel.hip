@@ -315,8 +315,9 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 uint32_t* d = dst + wave * PRE * kPitch + pre_col;
 #pragma unroll
                 for (int m = 0; m < PRE; ++m) {
-                    d[m * kPitch] = (pre[m] & 0xffffu) | (pre[m + 1] << 16);
-                    d[m * kPitch + 1] = (pre[m] >> 16) | (pre[m + 1] & 0xffff0000u);
+                    // v_perm_b32 picks the bytes: low halves of rows m, m + 1, then high halves
+                    d[m * kPitch] = __builtin_amdgcn_perm(pre[m + 1], pre[m], 0x05040100u);
+                    d[m * kPitch + 1] = __builtin_amdgcn_perm(pre[m + 1], pre[m], 0x07060302u);
                 }
             };
             // per-lane read bases of the two buffers, formed once (the empty asm keeps the
