@@ -338,6 +338,10 @@ DeviceSearch::~DeviceSearch() {
             if (e) (void)hipEventDestroy(e);
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (build_stream_) (void)hipStreamDestroy(build_stream_);
+    if (filt2_) (void)hipFree(filt2_);
+    for (hipEvent_t e : ev_built2_)
+        if (e) (void)hipEventDestroy(e);
+    if (build_stream2_) (void)hipStreamDestroy(build_stream2_);
     if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
 }
 
@@ -665,8 +669,18 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         scheck(hipMemcpyAsync(S.args, hargs[q].data(), sizeof(StackArgs) * 2 * G, hipMemcpyHostToDevice, s),
                "upload descriptors");
     }
+    // build lanes: a second one for small frames, where a stage build's many small
+    // kernels leave most of the GPU idle (up to 64 M floats of frames per pass)
+    const int lanes = (passes > 1 && static_cast<double>(n) * plane <= 67108864.0) ? 2 : 1;
+    if (lanes == 2 && !stage2_) {
+        stage2_ = std::make_unique<DeviceStage>(n, w, h);
+        filt2_ = salloc<float>(static_cast<size_t>(plane) * n, "allocate frames");
+        scheck(hipStreamCreateWithFlags(&build_stream2_, hipStreamNonBlocking), "create stream");
+        for (auto& e : ev_built2_) scheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+    }
     scheck(hipEventRecord(ev_start_, s), "record event");
     scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
+    if (lanes == 2) scheck(hipStreamWaitEvent(build_stream2_, ev_start_, 0), "stream wait");
     lap("plan");
 
     const float real_floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
@@ -683,35 +697,41 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         const int p0 = grp * G, cnt = std::min(G, passes - p0);
         for (int g = 0; g < cnt; ++g) {
             const int p = p0 + g;
+            // pass g of the group on build lane g mod lanes
+            const bool l2 = lanes == 2 && (g & 1);
+            hipStream_t bs = l2 ? build_stream2_ : build_stream_;
+            DeviceStage& stg = l2 ? *stage2_ : stage_;
+            float* scratch = l2 ? filt2_ : filt_;
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             const int* perm = d_perm + static_cast<long long>(p) * n;
-            scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, build_stream_), "permute tables");
-            scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, build_stream_), "permute tables");
-            scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, build_stream_),
-                   "permute tables");
+            scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, bs), "permute tables");
+            scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, bs), "permute tables");
+            scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, bs), "permute tables");
             float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
             if (st != kStoreF32) {
-                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, filt_, plane, nullptr, nullptr,
-                             nullptr, nullptr, build_stream_);
-                scheck(launch_store(filt_, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
-                                    S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n,
-                                    build_stream_),
+                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, scratch, plane, nullptr, nullptr, nullptr,
+                          nullptr, bs);
+                scheck(launch_store(scratch, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
+                                    S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n, bs),
                        "store frames");
             } else if (use_pad) {
-                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, filt_, plane, nullptr, nullptr,
-                             nullptr, nullptr, build_stream_);
-                scheck(launch_copy_rows(filt_, plane, slot + forigin, fs, pad.pitch, w, h, n, build_stream_),
-                       "pad frames");
+                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, scratch, plane, nullptr, nullptr, nullptr,
+                          nullptr, bs);
+                scheck(launch_copy_rows(scratch, plane, slot + forigin, fs, pad.pitch, w, h, n, bs), "pad frames");
             } else {
-                stage_.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, slot, plane, nullptr, nullptr,
-                             nullptr, nullptr, build_stream_);
+                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, slot, plane, nullptr, nullptr, nullptr,
+                          nullptr, bs);
             }
         }
         scheck(launch_fill_keys(S.keys, cnt * plane, build_stream_), "clear keys");
         scheck(hipMemsetAsync(S.counts, 0, sizeof(int) * cnt, build_stream_), "clear peak counts");
         scheck(hipEventRecord(ev_built_[q], build_stream_), "record event");
         scheck(hipStreamWaitEvent(s, ev_built_[q], 0), "stream wait");
+        if (lanes == 2) {
+            scheck(hipEventRecord(ev_built2_[q], build_stream2_), "record event");
+            scheck(hipStreamWaitEvent(s, ev_built2_[q], 0), "stream wait");
+        }
         const StackArgs* dargs = static_cast<const StackArgs*>(S.args);
         std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
         std::vector<StackArgs> hring(hargs[q].begin() + G, hargs[q].begin() + G + cnt);
@@ -768,6 +788,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     }
     collect(groups - 1);
     scheck(hipStreamSynchronize(build_stream_), "stage builds");
+    if (lanes == 2) scheck(hipStreamSynchronize(build_stream2_), "stage builds");
     lap("passes (batched)");
 
     out.peaks = std::move(results[0]);

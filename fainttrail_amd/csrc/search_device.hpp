@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -126,6 +127,12 @@ private:
     size_t pk_cap_ = 0;
     PassSet sets_[2];
     hipStream_t build_stream_ = nullptr, copy_stream_ = nullptr;
+    // a second build lane for small frames (its own stage scratch, build target and
+    // stream), created on first use: two passes of a group build at the same time
+    std::unique_ptr<DeviceStage> stage2_;
+    float* filt2_ = nullptr;
+    hipStream_t build_stream2_ = nullptr;
+    hipEvent_t ev_built2_[2] = {nullptr, nullptr};
     hipEvent_t ev_built_[2] = {nullptr, nullptr}, ev_stacked_[2] = {nullptr, nullptr};
     hipEvent_t ev_counts_[2] = {nullptr, nullptr};
     hipEvent_t ev_start_ = nullptr;

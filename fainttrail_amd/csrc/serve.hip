@@ -21,7 +21,9 @@ double search_footprint(int n, int w, int h, long long nv, int G, int passes, in
     const double pk_cap = std::max(65536.0, P / 8.0);
     // DeviceSearch: background-subtracted and filtered frames, four maps, peak buffer;
     // DeviceStage: residuals, flags, sky / scatter / mask planes, small tables
-    const double base = 8.0 * n * P + 16.0 * P + 16.0 * pk_cap + 5.0 * n * P + 12.0 * P + 1048576.0 + 65536.0 * n;
+    double base = 8.0 * n * P + 16.0 * P + 16.0 * pk_cap + 5.0 * n * P + 12.0 * P + 1048576.0 + 65536.0 * n;
+    // small frames get a second build lane: stage scratch and a build target
+    if (passes > 1 && static_cast<double>(n) * P <= 67108864.0) base += 9.0 * n * P + 12.0 * P + 65536.0 * n;
     // one pass of a group: frames (a padded layout is at most twice a frame for fp32;
     // 16-bit frames take half of that), keys and maps, plan rows (shift, box, LDS
     // offsets: 52 ints per block and frame, blocks of 16 motions, some split), peaks

@@ -362,37 +362,45 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
 
         // S/N of the block's motions (ascending index) and the running maximum
         cint* mo = c_motion + V * blk;
+        const __attribute__((address_space(4))) float* const c_inv =
+            (const __attribute__((address_space(4))) float*)(inv_sqrt);
+        // coverage from the full-frame bounds. cov_x / cov_y hold, per block and motion,
+        // the number of frames whose shift is <= d for every d of the block's range, so a
+        // pixel's count of frames landing inside [0, w) is F(w-1-x) - F(-1-x). A block that
+        // stays inside in y needs only the x table (one count per lane, the same for the
+        // lane's rows); one that stays inside in x only the y table (one count per row,
+        // the same for the wave: scalar loads); only corner cases walk the frames.
+        int cover = 0;  // 0: c = n; 1: x table; 2: y table; 3: frame walk
+        if (RING && !interior) {
+            cint* ub = c_ubox + 4 * blk;
+            const bool x_full = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w;
+            const bool y_full = y0 + ub[1] >= 0 && y0 + kTileY - 1 + ub[3] < h;
+            cover = (x_full && y_full) ? 0 : (cov_ok && y_full) ? 1 : (cov_ok && x_full) ? 2 : 3;
+        }
 #pragma unroll
         for (int k = 0; k < V; ++k) {
             const int j = mo[k];
             if (j < 0) continue;
             int cnt[RPW];
+            float ix = inv_n;  // the factor of every row when the count is the same for them
 #pragma unroll
             for (int r = 0; r < RPW; ++r) cnt[r] = n;
-            if (RING && !interior) {
-                // coverage from the full-frame bounds. cov_x / cov_y hold, per block and
-                // motion, the number of frames whose shift is <= d for every d of the
-                // block's range, so a pixel's count of frames landing inside [0, w) is
-                // F(w-1-x) - F(-1-x); a block that stays inside in y (or x) needs only the
-                // other table, and only corner cases walk the frames.
+            if (RING && cover != 0) {
                 cint* ub = c_ubox + 4 * blk;
-                const bool x_full = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w;
-                const bool y_full = y0 + ub[1] >= 0 && y0 + kTileY - 1 + ub[3] < h;
-                if (cov_ok && (x_full || y_full)) {
-                    if (!x_full) {
-                        const int L = ub[2] - ub[0] + 1;
-                        const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * L;
-                        auto cdf = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= L ? n : F[d]); };
-                        const int cx = cdf(w - 1 - x) - cdf(-1 - x);
+                if (cover == 1) {
+                    const int L = ub[2] - ub[0] + 1;
+                    const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * L;
+                    auto cdf = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= L ? n : F[d]); };
+                    const int cx = cdf(w - 1 - x) - cdf(-1 - x);
+                    ix = inv_sqrt[cx];
 #pragma unroll
-                        for (int r = 0; r < RPW; ++r) cnt[r] = cx;
-                    } else if (!y_full) {
-                        const int L = ub[3] - ub[1] + 1;
-                        cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * L;
-                        auto cdf = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= L ? n : G[d]); };
+                    for (int r = 0; r < RPW; ++r) cnt[r] = cx;
+                } else if (cover == 2) {
+                    const int L = ub[3] - ub[1] + 1;
+                    cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * L;
+                    auto cdf = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= L ? n : G[d]); };
 #pragma unroll
-                        for (int r = 0; r < RPW; ++r) cnt[r] = cdf(h - 1 - (yw + r)) - cdf(-1 - (yw + r));
-                    }
+                    for (int r = 0; r < RPW; ++r) cnt[r] = cdf(h - 1 - (yw + r)) - cdf(-1 - (yw + r));
                 } else {
 #pragma unroll
                     for (int r = 0; r < RPW; ++r) cnt[r] = 0;
@@ -409,7 +417,10 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 if (cnt[r] < min_count) continue;
-                const float snr = acc[k][r] * ((RING && !interior) ? inv_sqrt[cnt[r]] : inv_n);
+                // x table or c = n: one factor; y table: the count is the wave's (scalar
+                // load); frame walk: a per-lane count
+                const float f = (!RING || cover <= 1) ? ix : cover == 2 ? c_inv[cnt[r]] : inv_sqrt[cnt[r]];
+                const float snr = acc[k][r] * f;
                 if (snr > best[r] || (snr == best[r] && j < arg[r])) {
                     best[r] = snr;
                     arg[r] = j;
