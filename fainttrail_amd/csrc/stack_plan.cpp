@@ -11,29 +11,45 @@ namespace ftk {
 
 namespace {
 
+// Clamped shifts, motion-major like the input: x(j, i) = clamp(dx[j n + i], -w, w) at
+// X[j n + i], so the n frames of one motion are contiguous (a block of 16 motions
+// reads 16 short rows).
 struct Shifts {
-    const int32_t* dx;
-    const int32_t* dy;
-    int n, w, h;
-    int x(int j, int i) const { return std::clamp(dx[static_cast<size_t>(j) * n + i], -w, w); }
-    int y(int j, int i) const { return std::clamp(dy[static_cast<size_t>(j) * n + i], -h, h); }
+    std::vector<int32_t> X, Y;
+    int n, nv;
+    Shifts(const int32_t* dx, const int32_t* dy, int n_, int nv_, int w, int h) : n(n_), nv(nv_) {
+        const size_t m = static_cast<size_t>(n) * nv;
+        X.resize(m);
+        Y.resize(m);
+        for (size_t q = 0; q < m; ++q) {
+            X[q] = std::clamp(dx[q], -w, w);
+            Y[q] = std::clamp(dy[q], -h, h);
+        }
+    }
+    const int32_t* xm(int j) const { return &X[static_cast<size_t>(j) * n]; }
+    const int32_t* ym(int j) const { return &Y[static_cast<size_t>(j) * n]; }
 };
 
 // The frame whose shifts spread most over all motions (for a straight-line grid the
 // frame furthest from the reference epoch): its shifts order the motions.
-int widest_frame(const Shifts& s, int nv) {
+int widest_frame(const Shifts& s) {
+    std::vector<int> x0(static_cast<size_t>(s.n), std::numeric_limits<int>::max()), y0 = x0;
+    std::vector<int> x1(static_cast<size_t>(s.n), std::numeric_limits<int>::min()), y1 = x1;
+    for (int j = 0; j < s.nv; ++j) {
+        const int32_t* xr = s.xm(j);
+        const int32_t* yr = s.ym(j);
+        for (int i = 0; i < s.n; ++i) {
+            x0[static_cast<size_t>(i)] = std::min(x0[static_cast<size_t>(i)], xr[i]);
+            x1[static_cast<size_t>(i)] = std::max(x1[static_cast<size_t>(i)], xr[i]);
+            y0[static_cast<size_t>(i)] = std::min(y0[static_cast<size_t>(i)], yr[i]);
+            y1[static_cast<size_t>(i)] = std::max(y1[static_cast<size_t>(i)], yr[i]);
+        }
+    }
     int best = 0;
     long best_spread = -1;
     for (int i = 0; i < s.n; ++i) {
-        int x0 = std::numeric_limits<int>::max(), x1 = std::numeric_limits<int>::min();
-        int y0 = x0, y1 = x1;
-        for (int j = 0; j < nv; ++j) {
-            x0 = std::min(x0, s.x(j, i));
-            x1 = std::max(x1, s.x(j, i));
-            y0 = std::min(y0, s.y(j, i));
-            y1 = std::max(y1, s.y(j, i));
-        }
-        const long spread = static_cast<long>(x1 - x0) + (y1 - y0);
+        const size_t u = static_cast<size_t>(i);
+        const long spread = static_cast<long>(x1[u] - x0[u]) + (y1[u] - y0[u]);
         if (spread > best_spread) {
             best_spread = spread;
             best = i;
@@ -43,8 +59,9 @@ int widest_frame(const Shifts& s, int nv) {
 }
 
 // Split the motions into groups of kVel that are compact in the shift space of one
-// frame (a k-d split along the wider axis, cut at a multiple of kVel).
-void kd_groups(std::vector<int>& idx, size_t lo, size_t hi, const Shifts& s, int fi,
+// frame (a k-d split along the wider axis, cut at a multiple of kVel; ties ordered by
+// the other axis, then the motion index).
+void kd_groups(std::vector<int>& idx, size_t lo, size_t hi, const int32_t* fx, const int32_t* fy,
                std::vector<std::vector<int>>& out) {
     const size_t cnt = hi - lo;
     if (cnt <= static_cast<size_t>(kVel)) {
@@ -53,53 +70,68 @@ void kd_groups(std::vector<int>& idx, size_t lo, size_t hi, const Shifts& s, int
     }
     int x0 = std::numeric_limits<int>::max(), x1 = std::numeric_limits<int>::min(), y0 = x0, y1 = x1;
     for (size_t q = lo; q < hi; ++q) {
-        x0 = std::min(x0, s.x(idx[q], fi));
-        x1 = std::max(x1, s.x(idx[q], fi));
-        y0 = std::min(y0, s.y(idx[q], fi));
-        y1 = std::max(y1, s.y(idx[q], fi));
+        x0 = std::min(x0, fx[idx[q]]);
+        x1 = std::max(x1, fx[idx[q]]);
+        y0 = std::min(y0, fy[idx[q]]);
+        y1 = std::max(y1, fy[idx[q]]);
     }
     const bool by_x = (x1 - x0) >= (y1 - y0);
+    const int32_t* a1 = by_x ? fx : fy;
+    const int32_t* a2 = by_x ? fy : fx;
     const size_t groups = (cnt + kVel - 1) / kVel;
     const size_t mid = lo + ((groups + 1) / 2) * kVel;
-    auto key = [&](int j) { return by_x ? std::make_pair(s.x(j, fi), s.y(j, fi)) : std::make_pair(s.y(j, fi), s.x(j, fi)); };
     std::nth_element(idx.begin() + static_cast<ptrdiff_t>(lo), idx.begin() + static_cast<ptrdiff_t>(mid),
                      idx.begin() + static_cast<ptrdiff_t>(hi), [&](int a, int b) {
-                         const auto ka = key(a), kb = key(b);
-                         return ka != kb ? ka < kb : a < b;
+                         if (a1[a] != a1[b]) return a1[a] < a1[b];
+                         if (a2[a] != a2[b]) return a2[a] < a2[b];
+                         return a < b;
                      });
-    kd_groups(idx, lo, mid, s, fi, out);
-    kd_groups(idx, mid, hi, s, fi, out);
+    kd_groups(idx, lo, mid, fx, fy, out);
+    kd_groups(idx, mid, hi, fx, fy, out);
 }
 
 // Emit a group as blocks whose per-frame spread fits the LDS box; groups that do not
 // fit are halved along the worst frame and axis (a single motion always fits).
 void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
-    int worst_frame = -1, worst_axis = 0, worst = 0;
-    for (int i = 0; i < s.n; ++i) {
-        int x0 = std::numeric_limits<int>::max(), x1 = std::numeric_limits<int>::min(), y0 = x0, y1 = x1;
-        for (int j : g) {
-            x0 = std::min(x0, s.x(j, i));
-            x1 = std::max(x1, s.x(j, i));
-            y0 = std::min(y0, s.y(j, i));
-            y1 = std::max(y1, s.y(j, i));
+    const int n = s.n;
+    // per-frame extremes over the group (frames in the inner loop: contiguous rows)
+    thread_local std::vector<int> ex;
+    ex.assign(static_cast<size_t>(4) * n, 0);
+    int* x0 = ex.data();
+    int* x1 = x0 + n;
+    int* y0 = x1 + n;
+    int* y1 = y0 + n;
+    std::fill(x0, x0 + n, std::numeric_limits<int>::max());
+    std::fill(x1, x1 + n, std::numeric_limits<int>::min());
+    std::fill(y0, y0 + n, std::numeric_limits<int>::max());
+    std::fill(y1, y1 + n, std::numeric_limits<int>::min());
+    for (int j : g) {
+        const int32_t* xr = s.xm(j);
+        const int32_t* yr = s.ym(j);
+        for (int i = 0; i < n; ++i) {
+            x0[i] = std::min(x0[i], xr[i]);
+            x1[i] = std::max(x1[i], xr[i]);
+            y0[i] = std::min(y0[i], yr[i]);
+            y1[i] = std::max(y1[i], yr[i]);
         }
-        if (x1 - x0 > kSpreadMax && x1 - x0 - kSpreadMax > worst) {
-            worst = x1 - x0 - kSpreadMax;
+    }
+    int worst_frame = -1, worst_axis = 0, worst = 0;
+    for (int i = 0; i < n; ++i) {
+        if (x1[i] - x0[i] > kSpreadMax && x1[i] - x0[i] - kSpreadMax > worst) {
+            worst = x1[i] - x0[i] - kSpreadMax;
             worst_frame = i;
             worst_axis = 0;
         }
-        if (y1 - y0 > kSpreadMax && y1 - y0 - kSpreadMax > worst) {
-            worst = y1 - y0 - kSpreadMax;
+        if (y1[i] - y0[i] > kSpreadMax && y1[i] - y0[i] - kSpreadMax > worst) {
+            worst = y1[i] - y0[i] - kSpreadMax;
             worst_frame = i;
             worst_axis = 1;
         }
     }
     if (worst_frame >= 0) {
-        std::sort(g.begin(), g.end(), [&](int a, int b) {
-            const int ka = worst_axis ? s.y(a, worst_frame) : s.x(a, worst_frame);
-            const int kb = worst_axis ? s.y(b, worst_frame) : s.x(b, worst_frame);
-            return ka != kb ? ka < kb : a < b;
-        });
+        const int32_t* tab = worst_axis ? s.Y.data() : s.X.data();
+        auto key = [&](int j) { return tab[static_cast<size_t>(j) * n + worst_frame]; };
+        std::sort(g.begin(), g.end(), [&](int a, int b) { return key(a) != key(b) ? key(a) < key(b) : a < b; });
         const size_t half = g.size() / 2;
         emit_blocks(std::vector<int>(g.begin(), g.begin() + static_cast<ptrdiff_t>(half)), s, p);
         emit_blocks(std::vector<int>(g.begin() + static_cast<ptrdiff_t>(half), g.end()), s, p);
@@ -109,68 +141,91 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
     // lowest index among equal S/N values of the block
     std::sort(g.begin(), g.end());
     const int b = p.nb++;
-    const int n = s.n;
     p.motion.resize(static_cast<size_t>(p.nb) * kVel, -1);
     p.shift.resize(static_cast<size_t>(p.nb) * n * 2 * kVel, 0);
     p.box.resize(static_cast<size_t>(p.nb) * n * 4, 0);
     p.ubox.resize(static_cast<size_t>(p.nb) * 4, 0);
-    for (int k = 0; k < kVel; ++k) p.motion[static_cast<size_t>(b) * kVel + k] = k < static_cast<int>(g.size()) ? g[static_cast<size_t>(k)] : -1;
+    p.lds_off.resize(static_cast<size_t>(p.nb) * n * kVel, 0);
+    int gj[kVel];
+    for (int k = 0; k < kVel; ++k) {
+        const bool real = k < static_cast<int>(g.size());
+        p.motion[static_cast<size_t>(b) * kVel + k] = real ? g[static_cast<size_t>(k)] : -1;
+        gj[k] = g[static_cast<size_t>(real ? k : 0)];  // padding lanes repeat the first motion: inside the box
+    }
     int ux0 = std::numeric_limits<int>::max(), uy0 = ux0, ux1 = std::numeric_limits<int>::min(), uy1 = ux1;
-    for (int i = 0; i < n; ++i) {
-        int32_t* sh = &p.shift[(static_cast<size_t>(b) * n + i) * 2 * kVel];
-        int x0 = std::numeric_limits<int>::max(), x1 = std::numeric_limits<int>::min(), y0 = x0, y1 = x1;
-        for (int k = 0; k < kVel; ++k) {
-            // padding lanes repeat the block's first motion: always inside the box
-            const int j = g[static_cast<size_t>(k < static_cast<int>(g.size()) ? k : 0)];
-            sh[k] = s.x(j, i);
-            sh[kVel + k] = s.y(j, i);
-            x0 = std::min(x0, sh[k]);
-            x1 = std::max(x1, sh[k]);
-            y0 = std::min(y0, sh[kVel + k]);
-            y1 = std::max(y1, sh[kVel + k]);
+    int32_t* shb = &p.shift[static_cast<size_t>(b) * n * 2 * kVel];
+    for (int k = 0; k < kVel; ++k) {
+        const int32_t* xr = s.xm(gj[k]);
+        const int32_t* yr = s.ym(gj[k]);
+        for (int i = 0; i < n; ++i) {
+            shb[static_cast<size_t>(i) * 2 * kVel + k] = xr[i];
+            shb[static_cast<size_t>(i) * 2 * kVel + kVel + k] = yr[i];
         }
+    }
+    for (int i = 0; i < n; ++i) {
+        // the padding lanes repeat a motion of the group, so the group's extremes are
+        // the block's
+        const int32_t* sh = shb + static_cast<size_t>(i) * 2 * kVel;
         int32_t* bx = &p.box[(static_cast<size_t>(b) * n + i) * 4];
-        bx[0] = x0;
-        bx[1] = y0;
-        bx[2] = x1 - x0;
-        bx[3] = y1 - y0;
-        p.max_sy = std::max(p.max_sy, y1 - y0);
-        p.lds_off.resize(static_cast<size_t>(p.nb) * n * kVel, 0);
+        bx[0] = x0[i];
+        bx[1] = y0[i];
+        bx[2] = x1[i] - x0[i];
+        bx[3] = y1[i] - y0[i];
+        p.max_sy = std::max(p.max_sy, y1[i] - y0[i]);
         int32_t* lo = &p.lds_off[(static_cast<size_t>(b) * n + i) * kVel];
-        for (int k = 0; k < kVel; ++k) lo[k] = (sh[kVel + k] - y0) * kPitch + (sh[k] - x0);
-        ux0 = std::min(ux0, x0);
-        ux1 = std::max(ux1, x1);
-        uy0 = std::min(uy0, y0);
-        uy1 = std::max(uy1, y1);
+        for (int k = 0; k < kVel; ++k) lo[k] = (sh[kVel + k] - y0[i]) * kPitch + (sh[k] - x0[i]);
+        ux0 = std::min(ux0, x0[i]);
+        ux1 = std::max(ux1, x1[i]);
+        uy0 = std::min(uy0, y0[i]);
+        uy1 = std::max(uy1, y1[i]);
     }
     int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
     ub[0] = ux0;
     ub[1] = uy0;
     ub[2] = ux1;
     ub[3] = uy1;
-    // cumulative shift counts per motion over the block's shift range (coverage of
-    // edge tiles without walking the frames); skipped when the tables grow too large
-    const long long Lx = ux1 - ux0 + 1, Ly = uy1 - uy0 + 1;
-    const long long need = static_cast<long long>(p.cov_x.size() + p.cov_y.size()) + kVel * (Lx + Ly);
-    if (!p.cov_ok || need > (1LL << 26)) {
+}
+
+// Cumulative shift counts per block and motion over the block's shift range: coverage
+// of edge tiles without walking the frames. One allocation for all blocks; skipped
+// (cov_ok = false) when the tables would exceed 2^26 entries.
+void build_coverage(StackPlan& p) {
+    const int n = p.n;
+    long long tx = 0, ty = 0;
+    for (int b = 0; b < p.nb; ++b) {
+        const int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
+        tx += kVel * static_cast<long long>(ub[2] - ub[0] + 1);
+        ty += kVel * static_cast<long long>(ub[3] - ub[1] + 1);
+    }
+    if (tx + ty > (1LL << 26)) {
         p.cov_ok = false;
         return;
     }
-    const size_t ox = p.cov_x.size(), oy = p.cov_y.size();
-    p.cov_off.push_back(static_cast<long long>(ox));
-    p.cov_off.push_back(static_cast<long long>(oy));
-    p.cov_x.resize(ox + static_cast<size_t>(kVel * Lx), 0);
-    p.cov_y.resize(oy + static_cast<size_t>(kVel * Ly), 0);
-    for (int k = 0; k < kVel; ++k) {
-        int32_t* fx = &p.cov_x[ox + static_cast<size_t>(k * Lx)];
-        int32_t* fy = &p.cov_y[oy + static_cast<size_t>(k * Ly)];
+    p.cov_off.resize(static_cast<size_t>(p.nb) * 2);
+    p.cov_x.assign(static_cast<size_t>(tx), 0);
+    p.cov_y.assign(static_cast<size_t>(ty), 0);
+    long long ox = 0, oy = 0;
+    for (int b = 0; b < p.nb; ++b) {
+        const int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
+        const int ux0 = ub[0], uy0 = ub[1];
+        const long long Lx = ub[2] - ux0 + 1, Ly = ub[3] - uy0 + 1;
+        p.cov_off[2 * static_cast<size_t>(b)] = ox;
+        p.cov_off[2 * static_cast<size_t>(b) + 1] = oy;
+        int32_t* cx = &p.cov_x[static_cast<size_t>(ox)];
+        int32_t* cy = &p.cov_y[static_cast<size_t>(oy)];
         for (int i = 0; i < n; ++i) {
             const int32_t* sh = &p.shift[(static_cast<size_t>(b) * n + i) * 2 * kVel];
-            ++fx[sh[k] - ux0];
-            ++fy[sh[kVel + k] - uy0];
+            for (int k = 0; k < kVel; ++k) {
+                ++cx[k * Lx + sh[k] - ux0];
+                ++cy[k * Ly + sh[kVel + k] - uy0];
+            }
         }
-        for (long long d = 1; d < Lx; ++d) fx[d] += fx[d - 1];
-        for (long long d = 1; d < Ly; ++d) fy[d] += fy[d - 1];
+        for (int k = 0; k < kVel; ++k) {
+            for (long long d = 1; d < Lx; ++d) cx[k * Lx + d] += cx[k * Lx + d - 1];
+            for (long long d = 1; d < Ly; ++d) cy[k * Ly + d] += cy[k * Ly + d - 1];
+        }
+        ox += kVel * Lx;
+        oy += kVel * Ly;
     }
 }
 
@@ -188,20 +243,29 @@ StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const 
     p.h = h;
     p.nv = nv;
     p.min_count = min_count;
-    const Shifts s{dx, dy, n, w, h};
-    p.min_dy = std::numeric_limits<int>::max();
-    p.max_dy = std::numeric_limits<int>::min();
-    for (int j = 0; j < nv; ++j)
-        for (int i = 0; i < n; ++i) {
-            p.min_dy = std::min(p.min_dy, s.y(j, i));
-            p.max_dy = std::max(p.max_dy, s.y(j, i));
-        }
-    const int fi = widest_frame(s, nv);
+    const Shifts s(dx, dy, n, nv, w, h);
+    const auto [ylo, yhi] = std::minmax_element(s.Y.begin(), s.Y.end());
+    p.min_dy = *ylo;
+    p.max_dy = *yhi;
+    const int fi = widest_frame(s);
+    std::vector<int32_t> fx(static_cast<size_t>(nv)), fy(static_cast<size_t>(nv));
+    for (int j = 0; j < nv; ++j) {
+        fx[static_cast<size_t>(j)] = s.xm(j)[fi];
+        fy[static_cast<size_t>(j)] = s.ym(j)[fi];
+    }
     std::vector<int> idx(static_cast<size_t>(nv));
     std::iota(idx.begin(), idx.end(), 0);
     std::vector<std::vector<int>> groups;
-    kd_groups(idx, 0, idx.size(), s, fi, groups);
+    groups.reserve(static_cast<size_t>(nv) / kVel + 1);
+    kd_groups(idx, 0, idx.size(), fx.data(), fy.data(), groups);
+    const size_t est = groups.size() + groups.size() / 4 + 1;  // blocks (some groups split)
+    p.motion.reserve(est * kVel);
+    p.shift.reserve(est * n * 2 * kVel);
+    p.box.reserve(est * n * 4);
+    p.ubox.reserve(est * 4);
+    p.lds_off.reserve(est * n * kVel);
     for (auto& g : groups) emit_blocks(std::move(g), s, p);
+    build_coverage(p);
     p.all_box[0] = p.all_box[1] = std::numeric_limits<int>::max();
     p.all_box[2] = p.all_box[3] = std::numeric_limits<int>::min();
     for (int b = 0; b < p.nb; ++b) {
