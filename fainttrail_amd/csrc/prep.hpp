@@ -49,25 +49,41 @@ public:
     void build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm, float* filtered,
                long long filt_stride, float* static_sky, double* sigma_host, float* mask_out, int* cr_host,
                hipStream_t s);
+    // The same with the time ranks already on the device (d_rank: n ints, time_ranks of
+    // the frame times). Enqueues work only (no host copy) once the filter taps for
+    // prm.fwhm_px are in place, so several builds can be queued on several streams.
+    void build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
+                      float* filtered, long long filt_stride, float* static_sky, double* sigma_host, float* mask_out,
+                      int* cr_host, hipStream_t s);
+    // Matched-filter taps for prm.fwhm_px (uploaded when they change; synchronises s then).
+    void set_taps(const StageParams& prm, hipStream_t s);
 
     int n, w, h;
     long long plane;
 
 private:
-    void select(std::vector<SelectSeg>& segs, const float* mask, bool deviation, int unit_floor, hipStream_t s);
+    // Order-statistic selection over the sample grids (border) of the n images base + i
+    // stride: median (deviation false) or robust sigma around the median of the
+    // previous call (deviation true).
+    void select(const float* base, long long stride, int border, const float* mask, bool deviation, int unit_floor,
+                hipStream_t s);
     float* resid_ = nullptr;     // n planes: residuals, then z, then the row-filtered frames
     unsigned char* flags_ = nullptr;  // n planes of cosmic-ray hit flags / blank flags
     float* sky_ = nullptr;
     float* scatter_ = nullptr;
     float* mask_ = nullptr;
     SelectSeg* seg_ = nullptr;
-    unsigned int* hist_ = nullptr;
+    unsigned int* hist_ = nullptr;  // n x 2 x 256 digit counts, zero between passes
     float* inv_ = nullptr;
     double* sigma_ = nullptr;
     int* cr_ = nullptr;
     int* rank_ = nullptr;
     int* any_ = nullptr;
     float* taps_ = nullptr;
+    double taps_sigma_ = -1.0;  // PSF sigma of the taps on the device
 };
+
+// Time ranks of n frame times: rank[i] = position of frame i in a stable sort by time.
+void time_ranks(const double* t_h, int n, int* rank);
 
 }  // namespace ftk

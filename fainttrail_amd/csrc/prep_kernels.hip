@@ -322,7 +322,19 @@ __device__ inline bool sample_key(const SelectSeg& s, long long q, const float* 
     return true;
 }
 
-__global__ void select_count_kernel(SelectSeg* __restrict__ seg, const float* __restrict__ mask) {
+// Segment i of a uniform frame stack: the sample grid proto on image base + i stride,
+// with the selection state cleared (the host computes the grid once for all frames).
+__global__ void seg_setup_kernel(SelectSeg* __restrict__ seg, const float* base, long long stride, SelectSeg proto,
+                                 int nseg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseg) return;
+    SelectSeg s = proto;
+    s.img = base + i * stride;
+    seg[i] = s;
+}
+
+// Counts the valid samples of each segment.
+__global__ __launch_bounds__(256) void select_count_kernel(SelectSeg* __restrict__ seg, const float* __restrict__ mask) {
     SelectSeg& s = seg[blockIdx.y];
     const long long G = static_cast<long long>(s.nx) * s.ny;
     int c = 0;
@@ -336,18 +348,9 @@ __global__ void select_count_kernel(SelectSeg* __restrict__ seg, const float* __
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s.count, c);
 }
 
-// Targets of median_inplace: element m/2 and, for even m, element m/2 - 1.
-__global__ void select_init_kernel(SelectSeg* __restrict__ seg, int nseg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nseg) return;
-    SelectSeg& s = seg[i];
-    const int m = s.count;
-    s.k[1] = m / 2;
-    s.k[0] = (m % 2 == 0) ? m / 2 - 1 : m / 2;
-    if (s.k[0] < 0) s.k[0] = 0;
-    s.prefix[0] = s.prefix[1] = 0u;
-}
-
+// One 8-bit digit pass of the radix select, counting for both targets of every segment
+// (grid x = workgroups per segment, y = segment) the keys whose higher digits equal
+// the target's prefix, per digit.
 __global__ __launch_bounds__(256) void select_hist_kernel(SelectSeg* __restrict__ seg, const float* __restrict__ mask,
                                                           int mode, int shift, unsigned int* __restrict__ hist) {
     __shared__ unsigned int lh[2][256];
@@ -373,52 +376,87 @@ __global__ __launch_bounds__(256) void select_hist_kernel(SelectSeg* __restrict_
     }
 }
 
-__global__ void select_digit_kernel(SelectSeg* __restrict__ seg, int nseg, int shift, unsigned int* __restrict__ hist) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 2 * nseg) return;
-    const int sidx = i >> 1, t = i & 1;
-    SelectSeg& s = seg[sidx];
-    unsigned int* h = hist + static_cast<long long>(sidx) * 512 + t * 256;
-    unsigned int cum = 0;
-    const unsigned int k = static_cast<unsigned int>(s.k[t]);
-    int digit = 255;
-    for (int d = 0; d < 256; ++d) {
-        if (cum + h[d] > k) {
-            digit = d;
-            break;
-        }
-        cum += h[d];
-    }
-    s.prefix[t] |= static_cast<uint32_t>(digit) << shift;
-    s.k[t] = static_cast<int>(k - cum);
-    for (int d = 0; d < 256; ++d) h[d] = 0u;
-}
-
-// median_inplace (common.hpp:64-74): element m/2, averaged in double with element
-// m/2 - 1 for even m; 0 for no samples. stage 0: store as the centre (the median);
-// stage 1: result = 1.4826 * median of the deviations (robust_sigma, common.hpp:77-82),
-// 0 when fewer than 3 samples; with `unit_floor`, results <= 0 become 1
-// (sigma_from_samples, preprocess.cpp:265-268).
-__global__ void select_finish_kernel(SelectSeg* __restrict__ seg, int nseg, int stage, int unit_floor) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nseg) return;
-    SelectSeg& s = seg[i];
+// The digit of a pass, one workgroup of 256 per segment: inclusive prefix sums of the
+// segment's two histograms in LDS, then for each target the digit whose range
+// [exclusive, inclusive) holds its rank k (the first digit whose inclusive sum exceeds
+// k); when the counts end at or below k, digit 255 and the whole count, as a sequential
+// scan would leave them. The histograms are cleared for the next pass.
+// mode 0 starts (shift 24) by setting the targets of median_inplace from the count:
+// element m/2 and, for even m, element m/2 - 1. After the lowest digit (shift 0) the
+// selection finishes: median_inplace (common.hpp:64-74), element m/2 averaged in double
+// with element m/2 - 1 for even m, 0 for no samples. mode 0 stores it as the centre (the
+// median) and sets the targets again for the deviation pass; mode 1 gives result =
+// 1.4826 * median of the deviations (robust_sigma, common.hpp:77-82), 0 when fewer than
+// 3 samples, and with `unit_floor` results <= 0 become 1 (sigma_from_samples,
+// preprocess.cpp:265-268).
+__global__ __launch_bounds__(256) void select_digit_kernel(SelectSeg* __restrict__ seg, int mode, int shift,
+                                                           unsigned int* __restrict__ hist, int unit_floor) {
+    __shared__ unsigned int lh[2][256];
+    __shared__ uint32_t pick[2];
+    __shared__ int rem[2];
+    SelectSeg& s = seg[blockIdx.x];
     const int m = s.count;
+    int kk[2] = {s.k[0], s.k[1]};
+    if (mode == 0 && shift == 24) {
+        kk[1] = m / 2;
+        kk[0] = (m % 2 == 0) ? m / 2 - 1 : m / 2;
+        if (kk[0] < 0) kk[0] = 0;
+    }
+    const int d = threadIdx.x;
+    unsigned int* gh = hist + static_cast<long long>(blockIdx.x) * 512;
+    const unsigned int h0 = gh[d], h1 = gh[256 + d];
+    gh[d] = 0u;
+    gh[256 + d] = 0u;
+    lh[0][d] = h0;
+    lh[1][d] = h1;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned int a0 = d >= off ? lh[0][d - off] : 0u, a1 = d >= off ? lh[1][d - off] : 0u;
+        __syncthreads();
+        lh[0][d] += a0;
+        lh[1][d] += a1;
+        __syncthreads();
+    }
+    for (int t = 0; t < 2; ++t) {
+        const unsigned int k = static_cast<unsigned int>(kk[t]);
+        const unsigned int incl = lh[t][d], excl = incl - (t ? h1 : h0);
+        if (excl <= k && k < incl) {
+            pick[t] = static_cast<uint32_t>(d);
+            rem[t] = static_cast<int>(k - excl);
+        } else if (d == 255 && incl <= k) {
+            pick[t] = 255u;
+            rem[t] = static_cast<int>(k - incl);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint32_t pre0 = (shift == 24 ? 0u : s.prefix[0]) | (pick[0] << shift);
+    const uint32_t pre1 = (shift == 24 ? 0u : s.prefix[1]) | (pick[1] << shift);
+    if (shift != 0) {
+        s.prefix[0] = pre0;
+        s.prefix[1] = pre1;
+        s.k[0] = rem[0];
+        s.k[1] = rem[1];
+        return;
+    }
     double med = 0.0;
     if (m > 0) {
-        med = static_cast<double>(unord_bits(s.prefix[1]));
-        if (m % 2 == 0) med = 0.5 * (med + static_cast<double>(unord_bits(s.prefix[0])));
+        med = static_cast<double>(unord_bits(pre1));
+        if (m % 2 == 0) med = 0.5 * (med + static_cast<double>(unord_bits(pre0)));
     }
-    if (stage == 0) {
+    s.prefix[0] = s.prefix[1] = 0u;
+    if (mode == 0) {
         s.center = med;
         s.result = med;
+        s.k[1] = m / 2;
+        s.k[0] = (m % 2 == 0) ? m / 2 - 1 : m / 2;
+        if (s.k[0] < 0) s.k[0] = 0;
     } else {
         double r = m < 3 ? 0.0 : 1.4826 * med;
         if (unit_floor && !(r > 0.0)) r = 1.0;
         s.result = r;
+        s.k[0] = s.k[1] = 0;
     }
-    s.k[0] = s.k[1] = 0;
-    s.prefix[0] = s.prefix[1] = 0u;
 }
 
 // 1/sigma as float for the z kernel (preprocess.cpp:416)
@@ -503,37 +541,27 @@ DeviceStage::~DeviceStage() {
         if (p) (void)hipFree(p);
 }
 
-void DeviceStage::select(std::vector<SelectSeg>& segs, const float* mask, bool deviation, int unit_floor,
-                         hipStream_t s) {
-    const int ns = static_cast<int>(segs.size());
-    int maxg = 1;
-    for (const auto& g : segs) maxg = std::max(maxg, g.nx * g.ny);
+void DeviceStage::select(const float* base, long long stride, int border, const float* mask, bool deviation,
+                         int unit_floor, hipStream_t s) {
+    const int ns = n;
+    const SelectSeg proto = grid_seg(base, w, h, border);
+    const int maxg = std::max(1, proto.nx * proto.ny);
     const unsigned gx = std::max(1u, std::min(64u, blocks_for(maxg, 256)));
+    const int mode = deviation ? 1 : 0;
     if (!deviation) {
-        hcheck(hipMemcpyAsync(seg_, segs.data(), sizeof(SelectSeg) * ns, hipMemcpyHostToDevice, s), "upload segments");
+        hipLaunchKernelGGL(seg_setup_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, base, stride, proto, ns);
         hipLaunchKernelGGL(select_count_kernel, dim3(gx, ns), dim3(256), 0, s, seg_, mask);
-        hipLaunchKernelGGL(select_init_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns);
     }
-    for (int pass = 0; pass < 2; ++pass) {
-        const int mode = pass;
-        if (pass == 1 && !deviation) break;
-        if (pass == 0 && deviation) continue;
-        for (int shift = 24; shift >= 0; shift -= 8) {
-            hipLaunchKernelGGL(select_hist_kernel, dim3(gx, ns), dim3(256), 0, s, seg_, mask, mode, shift, hist_);
-            hipLaunchKernelGGL(select_digit_kernel, dim3(blocks_for(2 * ns, 64)), dim3(64), 0, s, seg_, ns, shift,
-                               hist_);
-        }
-        hipLaunchKernelGGL(select_finish_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns, mode, unit_floor);
-        if (mode == 0) hipLaunchKernelGGL(select_init_kernel, dim3(blocks_for(ns, 64)), dim3(64), 0, s, seg_, ns);
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(select_hist_kernel, dim3(gx, ns), dim3(256), 0, s, seg_, mask, mode, shift, hist_);
+        hipLaunchKernelGGL(select_digit_kernel, dim3(ns), dim3(256), 0, s, seg_, mode, shift, hist_, unit_floor);
     }
     hcheck(hipGetLastError(), "selection kernels");
 }
 
 void DeviceStage::background(const float* raw, long long raw_stride, float* sub, long long sub_stride,
                              double* bg_host, hipStream_t s) {
-    std::vector<SelectSeg> segs;
-    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(raw + i * raw_stride, w, h, 0));
-    select(segs, nullptr, false, 0, s);
+    select(raw, raw_stride, 0, nullptr, false, 0, s);
     hcheck(hipMemsetAsync(any_, 0, sizeof(int) * n, s), "clear flags");
     hipLaunchKernelGGL(subtract_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, raw, raw_stride, plane, seg_,
                        sub, sub_stride, flags_, any_);
@@ -548,47 +576,26 @@ void DeviceStage::background(const float* raw, long long raw_stride, float* sub,
     }
 }
 
+void time_ranks(const double* t_h, int n, int* rank) {
+    // stable sort of the frame times (preprocess.cpp:319-331)
+    std::vector<int> order(static_cast<size_t>(n));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t_h[a] < t_h[b]; });
+    for (int r = 0; r < n; ++r) rank[order[static_cast<size_t>(r)]] = r;
+}
+
 void DeviceStage::build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm,
                         float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
                         float* mask_out, int* cr_host, hipStream_t s) {
-    // time ranks: stable sort of the frame times (preprocess.cpp:319-331)
-    std::vector<int> order(static_cast<size_t>(n)), rank(static_cast<size_t>(n));
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t_h[a] < t_h[b]; });
-    for (int r = 0; r < n; ++r) rank[static_cast<size_t>(order[static_cast<size_t>(r)])] = r;
-    const int exclude = prm.static_exclude >= 0 ? prm.static_exclude : std::max(1, static_cast<int>(std::lround(n / 8.0)));
-    const int min_finite = static_cast<int>(std::max<size_t>(5, static_cast<size_t>(0.6 * n)));
+    std::vector<int> rank(static_cast<size_t>(n));
+    time_ranks(t_h, n, rank.data());
     hcheck(hipMemcpyAsync(rank_, rank.data(), sizeof(int) * n, hipMemcpyHostToDevice, s), "upload ranks");
+    build_ranked(sub, sub_stride, rank_, prm, filtered, filt_stride, static_sky, sigma_host, mask_out, cr_host, s);
+}
 
-    // static sky + leave-out residuals
-    float* sky = static_sky ? static_sky : sky_;
-    const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
-    hipLaunchKernelGGL(sky_kernel, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, rank_,
-                       exclude, min_finite, sky, resid_, plane);
-    // per-frame residual noise: robust sigma of the grid samples, border 2
-    std::vector<SelectSeg> segs;
-    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(resid_ + i * plane, w, h, 2));
-    select(segs, nullptr, false, 1, s);
-    select(segs, nullptr, true, 1, s);
-    hipLaunchKernelGGL(inv_sigma_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, seg_, n, inv_, sigma_);
-    // scatter map and mask
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(plane, 64)), dim3(64), static_cast<size_t>(n) * 64 * 4, s,
-                       resid_, plane, n, plane, sigma_, scatter_);
-    const int dil = std::max(1, static_cast<int>(std::lround(0.5 * prm.fwhm_px)));
-    float* mask = mask_out ? mask_out : mask_;
-    hipLaunchKernelGGL(mask_kernel, dim3(blocks_for(w, 128), h), dim3(128), 0, s, scatter_, w, h, dil,
-                       prm.mask_scatter, mask);
-    // z, cosmic rays, clip
+void DeviceStage::set_taps(const StageParams& prm, hipStream_t s) {
     const double psf_sigma = prm.fwhm_px * kFwhmToSigmaD;
-    hipLaunchKernelGGL(z_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, resid_, plane, plane, mask,
-                       scatter_, inv_, static_cast<float>(prm.weight_scatter));
-    hcheck(hipMemsetAsync(cr_, 0, sizeof(int) * n, s), "clear counts");
-    const float ratio = static_cast<float>(0.5 * std::exp(-0.5 / (psf_sigma * psf_sigma)));
-    hipLaunchKernelGGL(hit_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
-                       static_cast<float>(prm.cr_sigma), ratio, flags_, cr_);
-    hipLaunchKernelGGL(kill_clip_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, flags_,
-                       static_cast<float>(prm.clip_sigma), filtered, filt_stride);
-    // matched filter: taps from the host (glibc exp), truncated at the edges
+    if (taps_sigma_ == psf_sigma) return;
     const int R = std::max(1, static_cast<int>(std::ceil(3.0 * psf_sigma)));
     if (R > 64) throw std::runtime_error("build_stage: PSF too wide for the device filter (R > 64)");
     std::vector<float> taps(static_cast<size_t>(2 * R + 1)), taps_norm(taps.size());
@@ -602,17 +609,56 @@ void DeviceStage::build(const float* sub, long long sub_stride, const double* t_
     for (size_t q = 0; q < taps.size(); ++q) taps_norm[q] = taps[q] * norm;
     std::vector<float> both(taps);
     both.insert(both.end(), taps_norm.begin(), taps_norm.end());
-    hcheck(hipMemcpyAsync(taps_, both.data(), sizeof(float) * both.size(), hipMemcpyHostToDevice, s), "upload taps");
+    // earlier builds on this stream may still read the old taps; the copy is complete
+    // when it returns, so builds on any stream see the new ones
+    hcheck(hipStreamSynchronize(s), "build_stage");
+    hcheck(hipMemcpy(taps_, both.data(), sizeof(float) * both.size(), hipMemcpyHostToDevice), "upload taps");
+    taps_sigma_ = psf_sigma;
+}
+
+void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
+                               float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
+                               float* mask_out, int* cr_host, hipStream_t s) {
+    const int exclude = prm.static_exclude >= 0 ? prm.static_exclude : std::max(1, static_cast<int>(std::lround(n / 8.0)));
+    const int min_finite = static_cast<int>(std::max<size_t>(5, static_cast<size_t>(0.6 * n)));
+    const double psf_sigma = prm.fwhm_px * kFwhmToSigmaD;
+    set_taps(prm, s);
+
+    // static sky + leave-out residuals
+    float* sky = static_sky ? static_sky : sky_;
+    const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
+    hipLaunchKernelGGL(sky_kernel, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
+                       exclude, min_finite, sky, resid_, plane);
+    // per-frame residual noise: robust sigma of the grid samples, border 2
+    select(resid_, plane, 2, nullptr, false, 1, s);
+    select(resid_, plane, 2, nullptr, true, 1, s);
+    hipLaunchKernelGGL(inv_sigma_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, seg_, n, inv_, sigma_);
+    // scatter map and mask
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(plane, 64)), dim3(64), static_cast<size_t>(n) * 64 * 4, s,
+                       resid_, plane, n, plane, sigma_, scatter_);
+    const int dil = std::max(1, static_cast<int>(std::lround(0.5 * prm.fwhm_px)));
+    float* mask = mask_out ? mask_out : mask_;
+    hipLaunchKernelGGL(mask_kernel, dim3(blocks_for(w, 128), h), dim3(128), 0, s, scatter_, w, h, dil,
+                       prm.mask_scatter, mask);
+    // z, cosmic rays, clip
+    hipLaunchKernelGGL(z_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, resid_, plane, plane, mask,
+                       scatter_, inv_, static_cast<float>(prm.weight_scatter));
+    hcheck(hipMemsetAsync(cr_, 0, sizeof(int) * n, s), "clear counts");
+    const float ratio = static_cast<float>(0.5 * std::exp(-0.5 / (psf_sigma * psf_sigma)));
+    hipLaunchKernelGGL(hit_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
+                       static_cast<float>(prm.cr_sigma), ratio, flags_, cr_);
+    hipLaunchKernelGGL(kill_clip_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, flags_,
+                       static_cast<float>(prm.clip_sigma), filtered, filt_stride);
+    // matched filter: taps from the host (glibc exp, set_taps), truncated at the edges
+    const int R = std::max(1, static_cast<int>(std::ceil(3.0 * psf_sigma)));
     hipLaunchKernelGGL(mf_rows_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, filtered, filt_stride, w, R,
                        taps_, resid_, plane);
     hipLaunchKernelGGL(mf_cols_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, R,
                        taps_ + (2 * R + 1), filtered, filt_stride);
     // normalisation: robust sigma of the masked grid samples, border ceil(3 sigma) + 1
     const int fborder = static_cast<int>(std::ceil(3 * psf_sigma)) + 1;
-    segs.clear();
-    for (int i = 0; i < n; ++i) segs.push_back(grid_seg(filtered + i * filt_stride, w, h, fborder));
-    select(segs, mask, false, 0, s);
-    select(segs, mask, true, 0, s);
+    select(filtered, filt_stride, fborder, mask, false, 0, s);
+    select(filtered, filt_stride, fborder, mask, true, 0, s);
     hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, filtered, filt_stride, plane, seg_);
     hcheck(hipGetLastError(), "build_stage kernels");
     if (sigma_host || cr_host) {

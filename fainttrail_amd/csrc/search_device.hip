@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <numeric>
 #include <stdexcept>
@@ -272,14 +273,14 @@ void PassSet::release() {
     for (void* p : {static_cast<void*>(filt), static_cast<void*>(keys), static_cast<void*>(maxm),
                     static_cast<void*>(best), static_cast<void*>(tabs), static_cast<void*>(perm),
                     static_cast<void*>(counts), static_cast<void*>(pk), static_cast<void*>(scl),
-                    static_cast<void*>(maxb), args})
+                    static_cast<void*>(maxb)})
         if (p) (void)hipFree(p);
     if (h_counts) (void)hipHostFree(h_counts);
     *this = PassSet();
 }
 
 void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long long plane_, long long tab_ints_,
-                      long long pk_cap_, size_t args_bytes) {
+                      long long pk_cap_) {
     if (cap_ <= cap && frames_ == frames && frame_floats_ <= frame_floats && plane_ == plane && tab_ints_ <= tab_ints &&
         pk_cap_ <= pk_cap && filt)
         return;
@@ -295,7 +296,6 @@ void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long
     pk = salloc<int>(c * 4 * pk_cap_, "allocate pass peaks");
     scl = salloc<float>(c * frames_, "allocate frame scales");
     maxb = salloc<unsigned>(c * frames_, "allocate frame scratch");
-    args = salloc<char>(2 * c * args_bytes, "allocate pass descriptors");
     scheck(hipHostMalloc(reinterpret_cast<void**>(&h_counts), sizeof(int) * c, hipHostMallocDefault),
            "allocate pinned counts");
     cap = cap_;
@@ -325,6 +325,8 @@ DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_
         scheck(hipEventCreateWithFlags(&ev_counts_[q], hipEventDisableTiming), "create event");
     }
     scheck(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming), "create event");
+    scheck(hipEventCreateWithFlags(&ev_tables_, hipEventDisableTiming), "create event");
+    scheck(hipEventCreateWithFlags(&ev_upload_, hipEventDisableTiming), "create event");
 }
 
 DeviceSearch::~DeviceSearch() {
@@ -336,7 +338,10 @@ DeviceSearch::~DeviceSearch() {
     for (int q = 0; q < 2; ++q)
         for (hipEvent_t e : {ev_built_[q], ev_stacked_[q], ev_counts_[q]})
             if (e) (void)hipEventDestroy(e);
-    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    for (hipEvent_t e : {ev_start_, ev_tables_, ev_upload_})
+        if (e) (void)hipEventDestroy(e);
+    if (tab_dev_) (void)hipFree(tab_dev_);
+    if (tab_host_) (void)hipHostFree(tab_host_);
     if (build_stream_) (void)hipStreamDestroy(build_stream_);
     if (filt2_) (void)hipFree(filt2_);
     for (hipEvent_t e : ev_built2_)
@@ -534,10 +539,8 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const int edge = std::max(cfg.edge_px, 1);
     const int passes = 1 + runs;
 
-    // background on the caller's stream; the build stream waits for it
+    // background on the caller's stream (the build lanes wait for it below)
     stage_.background(raw, stride, sub_, plane, nullptr, s);
-    scheck(hipEventRecord(ev_start_, s), "record event");
-    scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
 
     // the plan of the real pass, shared by all passes
     std::vector<int32_t> dx, dy;
@@ -585,54 +588,94 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const int nsets = groups > 1 ? 2 : 1;
     for (int q = 0; q < nsets; ++q) {
         PassSet& S = sets_[q];
-        S.reserve(G, n, ffloats, plane, tab_ints, static_cast<long long>(pk_cap_), sizeof(StackArgs));
+        S.reserve(G, n, ffloats, plane, tab_ints, static_cast<long long>(pk_cap_));
         if (use_pad && S.layout != layout) {
             // the border must read as zeros; the interiors are rewritten by every pass
             scheck(hipMemsetAsync(S.filt, 0, sizeof(float) * S.cap * S.frames * S.frame_floats, s), "clear frames");
             S.layout = layout;
         }
     }
-
-    // shared tables (the ones that do not depend on the frame order) and the real rows
-    std::vector<void*> owned;
-    auto upload = [&](const void* src, size_t bytes) {
-        void* d = salloc<char>(bytes, "allocate plan tables");
-        owned.push_back(d);
-        if (bytes) scheck(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, s), "upload plan tables");
-        return d;
-    };
-    struct Cleanup {
-        std::vector<void*>& v;
-        ~Cleanup() {
-            for (void* p : v) (void)hipFree(p);
-        }
-    } cleanup{owned};
-    const int* d_shift0 = static_cast<const int*>(upload(plan.shift.data(), plan.shift.size() * 4));
-    const int* d_box0 = static_cast<const int*>(upload(plan.box.data(), plan.box.size() * 4));
-    const int* d_off0 = static_cast<const int*>(upload(plan.lds_off.data(), plan.lds_off.size() * 4));
-    const int* d_ubox = static_cast<const int*>(upload(plan.ubox.data(), plan.ubox.size() * 4));
-    const int* d_motion = static_cast<const int*>(upload(plan.motion.data(), plan.motion.size() * 4));
-    const float* d_inv = static_cast<const float*>(upload(plan.inv_sqrt.data(), plan.inv_sqrt.size() * 4));
-    const long long* d_covoff = static_cast<const long long*>(upload(plan.cov_off.data(), plan.cov_off.size() * 8));
-    const int* d_covx = static_cast<const int*>(upload(plan.cov_x.data(), plan.cov_x.size() * 4));
-    const int* d_covy = static_cast<const int*>(upload(plan.cov_y.data(), plan.cov_y.size() * 4));
+    // the build lanes start after the background and the border clears
     scheck(hipEventRecord(ev_start_, s), "record event");
     scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
 
-    // frame orders: identity for the real pass, the null shuffles after it
+    // frame orders: identity for the real pass, the null shuffles after it; the time
+    // ranks of each pass's frame times (build_stage's stable sort)
     std::vector<std::vector<int>> perms;
     perms.emplace_back(static_cast<size_t>(n));
     std::iota(perms[0].begin(), perms[0].end(), 0);
     for (auto& p : null_time_perms(n, runs, cfg.seed)) perms.push_back(std::move(p));
-    std::vector<int> all_perm;
+    std::vector<int> all_perm, all_rank(static_cast<size_t>(passes) * n);
     for (const auto& p : perms) all_perm.insert(all_perm.end(), p.begin(), p.end());
-    int* d_perm = static_cast<int*>(upload(all_perm.data(), all_perm.size() * 4));
+    {
+        std::vector<double> t(static_cast<size_t>(n));
+        for (int p = 0; p < passes; ++p) {
+            for (int i = 0; i < n; ++i)
+                t[static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
+            time_ranks(t.data(), n, all_rank.data() + static_cast<size_t>(p) * n);
+        }
+    }
+
+    // per-run tables in one pinned block and one copy: the shared tables (the ones that
+    // do not depend on the frame order), the real rows, orders, ranks and descriptors
+    struct Part {
+        const void* src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<Part> parts;
+    size_t total = 0;
+    auto part = [&](const void* src, size_t bytes) {
+        parts.push_back(Part{src, bytes, total});
+        total += (bytes + 255) & ~static_cast<size_t>(255);
+        return parts.size() - 1;
+    };
+    const size_t i_shift = part(plan.shift.data(), plan.shift.size() * 4);
+    const size_t i_box = part(plan.box.data(), plan.box.size() * 4);
+    const size_t i_off = part(plan.lds_off.data(), plan.lds_off.size() * 4);
+    const size_t i_ubox = part(plan.ubox.data(), plan.ubox.size() * 4);
+    const size_t i_motion = part(plan.motion.data(), plan.motion.size() * 4);
+    const size_t i_inv = part(plan.inv_sqrt.data(), plan.inv_sqrt.size() * 4);
+    const size_t i_covoff = part(plan.cov_off.data(), plan.cov_off.size() * 8);
+    const size_t i_covx = part(plan.cov_x.data(), plan.cov_x.size() * 4);
+    const size_t i_covy = part(plan.cov_y.data(), plan.cov_y.size() * 4);
+    const size_t i_perm = part(all_perm.data(), all_perm.size() * 4);
+    const size_t i_rank = part(all_rank.data(), all_rank.size() * 4);
+    std::vector<StackArgs> hargs[2];
+    size_t i_args[2] = {0, 0};
+    for (int q = 0; q < nsets; ++q) {
+        hargs[q].assign(static_cast<size_t>(2 * G), StackArgs{});
+        i_args[q] = part(hargs[q].data(), hargs[q].size() * sizeof(StackArgs));
+    }
+    // the previous run's copy and kernels are done with both buffers before they change
+    if (tables_used_) scheck(hipEventSynchronize(ev_tables_), "previous run");
+    if (total > tab_bytes_) {
+        if (tab_dev_) scheck(hipFree(tab_dev_), "free tables");
+        if (tab_host_) scheck(hipHostFree(tab_host_), "free tables");
+        tab_dev_ = tab_host_ = nullptr;
+        tab_bytes_ = 0;
+        const size_t bytes = total + total / 4;
+        scheck(hipMalloc(&tab_dev_, bytes), "allocate plan tables");
+        scheck(hipHostMalloc(&tab_host_, bytes, hipHostMallocDefault), "allocate pinned tables");
+        tab_bytes_ = bytes;
+    }
+    char* const dbase = static_cast<char*>(tab_dev_);
+    auto dptr = [&](size_t k) { return static_cast<void*>(dbase + parts[k].off); };
+    const int* d_shift0 = static_cast<const int*>(dptr(i_shift));
+    const int* d_box0 = static_cast<const int*>(dptr(i_box));
+    const int* d_off0 = static_cast<const int*>(dptr(i_off));
+    const int* d_ubox = static_cast<const int*>(dptr(i_ubox));
+    const int* d_motion = static_cast<const int*>(dptr(i_motion));
+    const float* d_inv = static_cast<const float*>(dptr(i_inv));
+    const long long* d_covoff = static_cast<const long long*>(dptr(i_covoff));
+    const int* d_covx = static_cast<const int*>(dptr(i_covx));
+    const int* d_covy = static_cast<const int*>(dptr(i_covy));
+    const int* d_perm = static_cast<const int*>(dptr(i_perm));
+    const int* d_rank = static_cast<const int*>(dptr(i_rank));
 
     // descriptors of every slot of every set: fixed for the run
-    std::vector<StackArgs> hargs[2];
     for (int q = 0; q < nsets; ++q) {
         PassSet& S = sets_[q];
-        hargs[q].assign(static_cast<size_t>(2 * G), StackArgs{});
         for (int g = 0; g < G; ++g) {
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             StackLaunch L{};
@@ -666,9 +709,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             L.keys = S.keys + static_cast<long long>(g) * plane;
             make_stack_args(plan, L, hargs[q][static_cast<size_t>(g)], hargs[q][static_cast<size_t>(G + g)]);
         }
-        scheck(hipMemcpyAsync(S.args, hargs[q].data(), sizeof(StackArgs) * 2 * G, hipMemcpyHostToDevice, s),
-               "upload descriptors");
     }
+    char* const hbase = static_cast<char*>(tab_host_);
+    for (const Part& pt : parts)
+        if (pt.bytes) std::memcpy(hbase + pt.off, pt.src, pt.bytes);
+    // the copy stream is idle here (the previous run collected everything on it)
+    scheck(hipMemcpyAsync(tab_dev_, tab_host_, total, hipMemcpyHostToDevice, copy_stream_), "upload plan tables");
+    scheck(hipEventRecord(ev_upload_, copy_stream_), "record event");
+    scheck(hipStreamWaitEvent(s, ev_upload_, 0), "stream wait");
+    scheck(hipStreamWaitEvent(build_stream_, ev_upload_, 0), "stream wait");
+
     // build lanes: a second one for small frames, where a stage build's many small
     // kernels leave most of the GPU idle (up to 64 M floats of frames per pass)
     const int lanes = (passes > 1 && static_cast<double>(n) * plane <= 67108864.0) ? 2 : 1;
@@ -678,17 +728,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         scheck(hipStreamCreateWithFlags(&build_stream2_, hipStreamNonBlocking), "create stream");
         for (auto& e : ev_built2_) scheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
     }
-    scheck(hipEventRecord(ev_start_, s), "record event");
-    scheck(hipStreamWaitEvent(build_stream_, ev_start_, 0), "stream wait");
-    if (lanes == 2) scheck(hipStreamWaitEvent(build_stream2_, ev_start_, 0), "stream wait");
+    // filter taps in place before the builds are queued (a copy only when they change)
+    stage_.set_taps(prm, build_stream_);
+    if (lanes == 2) {
+        stage2_->set_taps(prm, build_stream2_);
+        scheck(hipStreamWaitEvent(build_stream2_, ev_start_, 0), "stream wait");
+        scheck(hipStreamWaitEvent(build_stream2_, ev_upload_, 0), "stream wait");
+    }
     lap("plan");
 
     const float real_floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
-    std::vector<std::vector<double>> t_of(static_cast<size_t>(passes));
-    for (int p = 0; p < passes; ++p) {
-        t_of[static_cast<size_t>(p)].resize(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i) t_of[static_cast<size_t>(p)][static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
-    }
     std::vector<PeakList> results(static_cast<size_t>(passes));
 
     auto enqueue = [&](int grp) {
@@ -704,24 +753,25 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             float* scratch = l2 ? filt2_ : filt_;
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             const int* perm = d_perm + static_cast<long long>(p) * n;
+            const int* rk = d_rank + static_cast<long long>(p) * n;
             scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, bs), "permute tables");
             scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, bs), "permute tables");
             scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, bs), "permute tables");
             float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
             if (st != kStoreF32) {
-                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, scratch, plane, nullptr, nullptr, nullptr,
-                          nullptr, bs);
+                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, nullptr, nullptr, nullptr,
+                                 nullptr, bs);
                 scheck(launch_store(scratch, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
                                     S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n, bs),
                        "store frames");
             } else if (use_pad) {
-                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, scratch, plane, nullptr, nullptr, nullptr,
-                          nullptr, bs);
+                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, nullptr, nullptr, nullptr,
+                                 nullptr, bs);
                 scheck(launch_copy_rows(scratch, plane, slot + forigin, fs, pad.pitch, w, h, n, bs), "pad frames");
             } else {
-                stg.build(sub_, plane, t_of[static_cast<size_t>(p)].data(), prm, slot, plane, nullptr, nullptr, nullptr,
-                          nullptr, bs);
+                stg.build_ranked(sub_, plane, rk, prm, slot, plane, nullptr, nullptr, nullptr,
+                                 nullptr, bs);
             }
         }
         scheck(launch_fill_keys(S.keys, cnt * plane, build_stream_), "clear keys");
@@ -732,7 +782,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(hipEventRecord(ev_built2_[q], build_stream2_), "record event");
             scheck(hipStreamWaitEvent(s, ev_built2_[q], 0), "stream wait");
         }
-        const StackArgs* dargs = static_cast<const StackArgs*>(S.args);
+        const StackArgs* dargs = static_cast<const StackArgs*>(dptr(i_args[q]));
         std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
         std::vector<StackArgs> hring(hargs[q].begin() + G, hargs[q].begin() + G + cnt);
         scheck(launch_stack_multi(hin.data(), hring.data(), dargs, dargs + G, cnt, s), "launch stack kernel");
@@ -789,6 +839,8 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     collect(groups - 1);
     scheck(hipStreamSynchronize(build_stream_), "stage builds");
     if (lanes == 2) scheck(hipStreamSynchronize(build_stream2_), "stage builds");
+    scheck(hipEventRecord(ev_tables_, s), "record event");
+    tables_used_ = true;
     lap("passes (batched)");
 
     out.peaks = std::move(results[0]);

@@ -81,11 +81,10 @@ struct PassSet {
     int* pk = nullptr;             // cap x 4 pk_cap: x, y, snr bits, vidx
     float* scl = nullptr;          // cap x frames: scales of 16-bit frames
     unsigned* maxb = nullptr;      // cap x frames: scratch of the 16-bit encoder
-    void* args = nullptr;          // 2 cap stacking descriptors
     int* h_counts = nullptr;       // pinned
     void release();
     void reserve(int cap, long long frames, long long frame_floats, long long plane, long long tab_ints,
-                 long long pk_cap, size_t args_bytes);
+                 long long pk_cap);
 };
 
 class DeviceSearch {
@@ -136,6 +135,14 @@ private:
     hipEvent_t ev_built_[2] = {nullptr, nullptr}, ev_stacked_[2] = {nullptr, nullptr};
     hipEvent_t ev_counts_[2] = {nullptr, nullptr};
     hipEvent_t ev_start_ = nullptr;
+    // per-run tables of the batched form (plan rows, frame orders, time ranks, stacking
+    // descriptors): packed in pinned host memory and sent in one copy to a device buffer,
+    // both grown on demand and reused; ev_tables_ marks the previous run's last use
+    void* tab_host_ = nullptr;
+    void* tab_dev_ = nullptr;
+    size_t tab_bytes_ = 0;
+    hipEvent_t ev_tables_ = nullptr, ev_upload_ = nullptr;
+    bool tables_used_ = false;
 };
 
 // Host pieces of the loop, restated from the reference.
