@@ -451,8 +451,10 @@ def main() -> int:
     bytes_per_unit = BYTES_PER_UNIT if args.storage == "f32" else 2  # one stored sample per unit
     achieved = bytes_per_unit * units / kernel_s
     print(f"bench: {wall / args.steps * 1e3:.1f} ms/pass, {value:.3e} units/s", file=sys.stderr, flush=True)
-    # the inner loop reads two rows per ds_read2st64_b32 (8 B per lane per instruction)
+    # the inner loop reads two rows per ds_read2st64_b32 (8 B per lane per instruction):
+    # the probe of that form is the kernel's LDS ceiling; 16-byte reads go faster
     lds_peak = ft.probe_lds(8)
+    lds_peak16 = ft.probe_lds(16)
     res = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -465,10 +467,12 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles(args.workload),
                      "kernel_ms": kernel_s * 1e3,
                      "lds_peak_measured": lds_peak / 1e9, "lds_frac": achieved / lds_peak,
+                     "lds_peak_b128_measured": lds_peak16 / 1e9,
                      "note": "algorithmic bytes = 4 B (one fp32 sample) per stacked unit; frac > 1 is reuse "
                              "from LDS-staged frame boxes; the inner loop reads each unit's sample from LDS (two rows "
-                             "per ds_read2st64_b32), so lds_frac, against the LDS read bandwidth measured with "
-                             "8-byte-per-lane reads, is the binding figure"},
+                             "per ds_read2st64_b32), so lds_frac, against the LDS read bandwidth measured with that "
+                             "instruction form, is the binding figure (ds_read_b128 reads reach "
+                             "lds_peak_b128_measured)"},
     }
     if args.copy_probe_mib > 0:
         cnt = args.copy_probe_mib * 262144

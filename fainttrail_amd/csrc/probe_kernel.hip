@@ -1,7 +1,10 @@
 // LDS read-bandwidth probe: the ceiling of the stacking kernel, whose inner loop is
 // one LDS read and one fp32 add per stacked unit. Each workgroup fills 32 KiB of LDS
-// and streams wave-contiguous reads of `width` bytes per lane (4, 8 or 16) through
-// it, summing what it reads so no read is dead. Returns achieved bytes per second.
+// and streams wave-contiguous reads of `width` bytes per lane through it, summing what
+// it reads so no read is dead. Returns achieved bytes per second. Widths: 4
+// (ds_read_b32), 8 (two dwords 128 floats apart in one ds_read2st64_b32, the form of
+// the fp32 stacking loop, which reads a lane's rows r and r + 1 of a 128-float pitch)
+// and 16 (ds_read_b128).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +25,8 @@ __global__ __launch_bounds__(256) void lds_probe_kernel(float* __restrict__ out,
     __syncthreads();
     float acc[4 * W] = {};
     int base = threadIdx.x * W;
+    const int lane = threadIdx.x & 63;
+    int wbase = static_cast<int>(threadIdx.x >> 6) * 1024;
     for (int it = 0; it < iters; ++it) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -29,9 +34,10 @@ __global__ __launch_bounds__(256) void lds_probe_kernel(float* __restrict__ out,
             if constexpr (W == 1) {
                 acc[k] += p[0];
             } else if constexpr (W == 2) {
-                const float2 v = *reinterpret_cast<const float2*>(p);
-                acc[2 * k] += v.x;
-                acc[2 * k + 1] += v.y;
+                // lane-consecutive dwords at q and q + 128 (q < 4096 + 64, in bounds)
+                const float* r = &buf[((wbase + k * 256) & (kProbeFloats / 2 - 1)) + lane];
+                acc[2 * k] += r[0];
+                acc[2 * k + 1] += r[128];
             } else {
                 const float4 v = *reinterpret_cast<const float4*>(p);
                 acc[4 * k] += v.x;
@@ -41,6 +47,7 @@ __global__ __launch_bounds__(256) void lds_probe_kernel(float* __restrict__ out,
             }
         }
         base = (base + 64 * W) & (kProbeFloats - 1);
+        wbase = (wbase + 64) & (kProbeFloats / 2 - 1);
     }
     float s = 0.0f;
 #pragma unroll

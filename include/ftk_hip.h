@@ -276,8 +276,9 @@ int ftk_server_stats(ftk_server* server, double* stats);
 int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream);
 
 /* Measured LDS read bandwidth of the current device (bytes/s) for reads of
- * width_bytes (4, 8 or 16) per lane: the roofline of the stacking kernel's inner
- * loop, which reads one fp32 sample from LDS per stacked unit. */
+ * width_bytes per lane: 4 (ds_read_b32), 8 (two dwords in one ds_read2st64_b32, the
+ * form of the fp32 stacking loop) or 16 (ds_read_b128). The roofline of the stacking
+ * kernel's inner loop, which reads one fp32 sample from LDS per stacked unit. */
 int ftk_probe_lds(int width_bytes, double* bytes_per_s);
 
 /* Host emulation of the HIP algorithm (tiles, velocity blocks, -0.0 padding, edge
