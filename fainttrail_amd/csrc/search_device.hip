@@ -157,6 +157,12 @@ struct HostPeak {
 
 }  // namespace
 
+int build_lanes(int n, long long plane, int passes) {
+    if (passes < 2) return 1;
+    const double floats = static_cast<double>(n) * static_cast<double>(plane);
+    return floats <= 16777216.0 ? 4 : floats <= 67108864.0 ? 2 : 1;
+}
+
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv_out) {
     if (!(span_h > 0.0)) throw std::runtime_error("invalid time span");
     if (vmax <= 0.0 || vmax < vmin) throw std::runtime_error("invalid velocity range");
@@ -343,10 +349,13 @@ DeviceSearch::~DeviceSearch() {
     if (tab_dev_) (void)hipFree(tab_dev_);
     if (tab_host_) (void)hipHostFree(tab_host_);
     if (build_stream_) (void)hipStreamDestroy(build_stream_);
-    if (filt2_) (void)hipFree(filt2_);
-    for (hipEvent_t e : ev_built2_)
-        if (e) (void)hipEventDestroy(e);
-    if (build_stream2_) (void)hipStreamDestroy(build_stream2_);
+    for (BuildLane& l : lanes_) {
+        if (l.filt) (void)hipFree(l.filt);
+        for (hipEvent_t e : l.built)
+            if (e) (void)hipEventDestroy(e);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+        l.stage.reset();
+    }
     if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
 }
 
@@ -518,8 +527,9 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     return out;
 }
 
-// The batched form. Pass 0 is the real pass, pass 1 + k the k-th null run. Passes go
-// in groups of G through two sets of device buffers: the stage builds of a group run
+// The batched form. Pass 0 is the real pass, pass 1 + k the k-th null run. The real
+// pass goes first on its own, then the null runs in groups of G, through two sets of
+// device buffers: the stage builds of a group run
 // on the build stream while the previous group stacks, each group's passes stack in
 // one multi-pass launch (interior tiles) plus one for the ring tiles, and the peak
 // lists of a group come back on the copy stream while the next group stacks. One
@@ -584,7 +594,12 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         G = std::min(G, fit);
     }
     out.passes_per_launch = G;
-    const int groups = (passes + G - 1) / G;
+    // group k holds passes [gstart[k], gstart[k + 1]): the real pass alone first (the
+    // only stage build the GPU waits for before it stacks), then groups of G
+    std::vector<int> gstart{0};
+    if (passes > 1) gstart.push_back(1);
+    while (gstart.back() < passes) gstart.push_back(std::min(passes, gstart.back() + G));
+    const int groups = static_cast<int>(gstart.size()) - 1;
     const int nsets = groups > 1 ? 2 : 1;
     for (int q = 0; q < nsets; ++q) {
         PassSet& S = sets_[q];
@@ -719,21 +734,24 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     scheck(hipStreamWaitEvent(s, ev_upload_, 0), "stream wait");
     scheck(hipStreamWaitEvent(build_stream_, ev_upload_, 0), "stream wait");
 
-    // build lanes: a second one for small frames, where a stage build's many small
-    // kernels leave most of the GPU idle (up to 64 M floats of frames per pass)
-    const int lanes = (passes > 1 && static_cast<double>(n) * plane <= 67108864.0) ? 2 : 1;
-    if (lanes == 2 && !stage2_) {
-        stage2_ = std::make_unique<DeviceStage>(n, w, h);
-        filt2_ = salloc<float>(static_cast<size_t>(plane) * n, "allocate frames");
-        scheck(hipStreamCreateWithFlags(&build_stream2_, hipStreamNonBlocking), "create stream");
-        for (auto& e : ev_built2_) scheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+    // build lanes: more for small frames, where a stage build's many small kernels
+    // leave most of the GPU idle
+    const int lanes = build_lanes(n, plane, passes);
+    while (static_cast<int>(lanes_.size()) < lanes - 1) {
+        BuildLane l;
+        l.stage = std::make_unique<DeviceStage>(n, w, h);
+        l.filt = salloc<float>(static_cast<size_t>(plane) * n, "allocate frames");
+        scheck(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking), "create stream");
+        for (auto& e : l.built) scheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+        lanes_.push_back(std::move(l));
     }
     // filter taps in place before the builds are queued (a copy only when they change)
     stage_.set_taps(prm, build_stream_);
-    if (lanes == 2) {
-        stage2_->set_taps(prm, build_stream2_);
-        scheck(hipStreamWaitEvent(build_stream2_, ev_start_, 0), "stream wait");
-        scheck(hipStreamWaitEvent(build_stream2_, ev_upload_, 0), "stream wait");
+    for (int l = 1; l < lanes; ++l) {
+        BuildLane& L = lanes_[static_cast<size_t>(l) - 1];
+        L.stage->set_taps(prm, L.stream);
+        scheck(hipStreamWaitEvent(L.stream, ev_start_, 0), "stream wait");
+        scheck(hipStreamWaitEvent(L.stream, ev_upload_, 0), "stream wait");
     }
     lap("plan");
 
@@ -743,14 +761,15 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     auto enqueue = [&](int grp) {
         const int q = grp % nsets;
         PassSet& S = sets_[q];
-        const int p0 = grp * G, cnt = std::min(G, passes - p0);
+        const int p0 = gstart[static_cast<size_t>(grp)], cnt = gstart[static_cast<size_t>(grp) + 1] - p0;
         for (int g = 0; g < cnt; ++g) {
             const int p = p0 + g;
             // pass g of the group on build lane g mod lanes
-            const bool l2 = lanes == 2 && (g & 1);
-            hipStream_t bs = l2 ? build_stream2_ : build_stream_;
-            DeviceStage& stg = l2 ? *stage2_ : stage_;
-            float* scratch = l2 ? filt2_ : filt_;
+            const int lane = g % lanes;
+            BuildLane* const L = lane > 0 ? &lanes_[static_cast<size_t>(lane) - 1] : nullptr;
+            hipStream_t bs = L ? L->stream : build_stream_;
+            DeviceStage& stg = L ? *L->stage : stage_;
+            float* scratch = L ? L->filt : filt_;
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             const int* perm = d_perm + static_cast<long long>(p) * n;
             const int* rk = d_rank + static_cast<long long>(p) * n;
@@ -778,9 +797,10 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         scheck(hipMemsetAsync(S.counts, 0, sizeof(int) * cnt, build_stream_), "clear peak counts");
         scheck(hipEventRecord(ev_built_[q], build_stream_), "record event");
         scheck(hipStreamWaitEvent(s, ev_built_[q], 0), "stream wait");
-        if (lanes == 2) {
-            scheck(hipEventRecord(ev_built2_[q], build_stream2_), "record event");
-            scheck(hipStreamWaitEvent(s, ev_built2_[q], 0), "stream wait");
+        for (int l = 1; l < std::min(lanes, cnt); ++l) {
+            BuildLane& L = lanes_[static_cast<size_t>(l) - 1];
+            scheck(hipEventRecord(L.built[q], L.stream), "record event");
+            scheck(hipStreamWaitEvent(s, L.built[q], 0), "stream wait");
         }
         const StackArgs* dargs = static_cast<const StackArgs*>(dptr(i_args[q]));
         std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
@@ -808,7 +828,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     auto collect = [&](int grp) {
         const int q = grp % nsets;
         PassSet& S = sets_[q];
-        const int p0 = grp * G, cnt = std::min(G, passes - p0);
+        const int p0 = gstart[static_cast<size_t>(grp)], cnt = gstart[static_cast<size_t>(grp) + 1] - p0;
         scheck(hipEventSynchronize(ev_counts_[q]), "peak counts");
         std::vector<std::vector<int>> hv(static_cast<size_t>(cnt));
         for (int g = 0; g < cnt; ++g) {
@@ -838,7 +858,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     }
     collect(groups - 1);
     scheck(hipStreamSynchronize(build_stream_), "stage builds");
-    if (lanes == 2) scheck(hipStreamSynchronize(build_stream2_), "stage builds");
+    for (int l = 1; l < lanes; ++l) scheck(hipStreamSynchronize(lanes_[static_cast<size_t>(l) - 1].stream), "stage builds");
     scheck(hipEventRecord(ev_tables_, s), "record event");
     tables_used_ = true;
     lap("passes (batched)");

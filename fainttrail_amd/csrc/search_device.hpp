@@ -126,12 +126,15 @@ private:
     size_t pk_cap_ = 0;
     PassSet sets_[2];
     hipStream_t build_stream_ = nullptr, copy_stream_ = nullptr;
-    // a second build lane for small frames (its own stage scratch, build target and
-    // stream), created on first use: two passes of a group build at the same time
-    std::unique_ptr<DeviceStage> stage2_;
-    float* filt2_ = nullptr;
-    hipStream_t build_stream2_ = nullptr;
-    hipEvent_t ev_built2_[2] = {nullptr, nullptr};
+    // more build lanes for small frames (each its own stage scratch, build target and
+    // stream), created on first use: several passes of a group build at the same time
+    struct BuildLane {
+        std::unique_ptr<DeviceStage> stage;
+        float* filt = nullptr;
+        hipStream_t stream = nullptr;
+        hipEvent_t built[2] = {nullptr, nullptr};
+    };
+    std::vector<BuildLane> lanes_;  // lanes 1, 2, ... (lane 0: stage_, filt_, build_stream_)
     hipEvent_t ev_built_[2] = {nullptr, nullptr}, ev_stacked_[2] = {nullptr, nullptr};
     hipEvent_t ev_counts_[2] = {nullptr, nullptr};
     hipEvent_t ev_start_ = nullptr;
@@ -144,6 +147,9 @@ private:
     hipEvent_t ev_tables_ = nullptr, ev_upload_ = nullptr;
     bool tables_used_ = false;
 };
+
+// Build lanes of the batched form for n frames of `plane` pixels and `passes` passes.
+int build_lanes(int n, long long plane, int passes);
 
 // Host pieces of the loop, restated from the reference.
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv);

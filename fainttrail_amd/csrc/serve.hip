@@ -22,16 +22,16 @@ double search_footprint(int n, int w, int h, long long nv, int G, int passes, in
     // DeviceSearch: background-subtracted and filtered frames, four maps, peak buffer;
     // DeviceStage: residuals, flags, sky / scatter / mask planes, small tables
     double base = 8.0 * n * P + 16.0 * P + 16.0 * pk_cap + 5.0 * n * P + 12.0 * P + 1048576.0 + 65536.0 * n;
-    // small frames get a second build lane: stage scratch and a build target
-    if (passes > 1 && static_cast<double>(n) * P <= 67108864.0) base += 9.0 * n * P + 12.0 * P + 65536.0 * n;
+    // small frames get more build lanes: stage scratch and a build target each
+    base += (build_lanes(n, static_cast<long long>(w) * h, passes) - 1) * (9.0 * n * P + 12.0 * P + 65536.0 * n);
     // one pass of a group: frames (a padded layout is at most twice a frame for fp32;
     // 16-bit frames take half of that), keys and maps, plan rows (shift, box, LDS
     // offsets: 52 ints per block and frame, blocks of 16 motions, some split), peaks
     const double frame = storage == 0 ? 8.0 * P : 4.0 * P;
     const double tab = 52.0 * 4.0 * std::ceil(1.25 * static_cast<double>(nv) / kVel) * n;
     const double per_pass = n * frame + 24.0 * P + tab + 16.0 * pk_cap + 12.0 * n + 1024.0;
-    const int groups = (passes + G - 1) / std::max(1, G);
-    const double sets = (groups > 1 ? 2.0 : 1.0) * G * per_pass;
+    // the real pass goes alone first, so any search with null runs uses both sets
+    const double sets = (passes > 1 ? 2.0 : 1.0) * std::max(1, G) * per_pass;
     // per-run tables: the real rows, coverage tables (at most 2^26 entries each), orders
     const double run = tab + 2.0 * 4.0 * std::min(67108864.0, 16.0 * std::ceil(1.25 * nv / kVel) * (w + h)) +
                        4.0 * n * passes;

@@ -234,13 +234,28 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 const float* boxp = f + static_cast<long long>(by0 - row0) * pitch + bx0;
                 const __amdgpu_buffer_rsrc_t rs =
                     __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(boxp), 0, 0x7fffffff, kBufferWord3);
+                if constexpr (RING) {
+                    // a rolled loop with running offsets: the edge kernel has no scalar
+                    // registers to spare for the unrolled loop's per-row offsets
+                    int row_b = wave * 4 * pitch;
+                    float* d = dst + wave * kPitch;
+#pragma unroll 1
+                    for (int rr = wave; rr < R; rr += kWaves) {
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, b1, row_b, 0, 0);
+                        if (sx > 0) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d + kTileX, 4, b2, row_b, 0, 0);
+                        row_b += kWaves * 4 * pitch;
+                        d += kWaves * kPitch;
+                    }
+                } else {
 #pragma unroll
-                for (int m = 0; m < kPreRows; ++m) {
-                    const int rr = wave + kWaves * m;
-                    if (rr >= R) break;
-                    const int row_b = rr * 4 * pitch;
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
-                    if (sx > 0) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
+                    for (int m = 0; m < kPreRows; ++m) {
+                        const int rr = wave + kWaves * m;
+                        if (rr >= R) break;
+                        const int row_b = rr * 4 * pitch;
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
+                        if (sx > 0)
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
+                    }
                 }
             }
         };
