@@ -573,9 +573,9 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.nseg = L.nseg;
     a.lds_rows = kTileY + p.max_sy;
     if (L.storage != kStoreF32) {
-        // word rows: 15 + the largest vertical spread, PRE per wave (5, 8 or 12)
+        // word rows: 15 + the largest vertical spread, PRE per wave (5, 6, 8 or 12)
         const int need = (kTileY - 1 + p.max_sy + kWaves - 1) / kWaves;
-        a.pre = need <= 5 ? 5 : need <= 8 ? 8 : 12;
+        a.pre = need <= 5 ? 5 : need <= 6 ? 6 : need <= 8 ? 8 : 12;
         a.lds_rows = std::max(a.lds_rows, kWaves * a.pre);
     }
     a.cov_ok = p.cov_ok ? 1 : 0;
@@ -601,12 +601,17 @@ using SingleFn = void (*)(StackArgs);
 using MultiFn = void (*)(const StackArgs*, unsigned);
 template <bool RING, int ST>
 SingleFn single16(int pre) {
-    return pre <= 5 ? stack_kernel<RING, true, ST, 5> : pre <= 8 ? stack_kernel<RING, true, ST, 8> : stack_kernel<RING, true, ST, 12>;
+    return pre <= 5   ? stack_kernel<RING, true, ST, 5>
+           : pre <= 6 ? stack_kernel<RING, true, ST, 6>
+           : pre <= 8 ? stack_kernel<RING, true, ST, 8>
+                      : stack_kernel<RING, true, ST, 12>;
 }
 template <bool RING, int ST>
 MultiFn multi16(int pre) {
-    return pre <= 5 ? stack_multi_kernel<RING, true, ST, 5>
-                    : pre <= 8 ? stack_multi_kernel<RING, true, ST, 8> : stack_multi_kernel<RING, true, ST, 12>;
+    return pre <= 5   ? stack_multi_kernel<RING, true, ST, 5>
+           : pre <= 6 ? stack_multi_kernel<RING, true, ST, 6>
+           : pre <= 8 ? stack_multi_kernel<RING, true, ST, 8>
+                      : stack_multi_kernel<RING, true, ST, 12>;
 }
 SingleFn single_kernel(bool ring, bool padded, int st, int pre) {
     if (st == kStoreF16) return ring ? single16<true, kStoreF16>(pre) : single16<false, kStoreF16>(pre);
