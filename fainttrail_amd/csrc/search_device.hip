@@ -453,8 +453,18 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
     if (out.span_h <= 0.0) throw std::runtime_error("frames must span a non-zero time interval");
     for (int i = 0; i < n; ++i) out.t_h.push_back((mjd[i] - out.t_ref_mjd) * 24.0);
     out.grid = velocity_grid(out.span_h, cfg.vmin_pxh, cfg.vmax_pxh, cfg.grid_tol_px, &out.dv);
-    if (cfg.passes_per_launch == 1) return run_sequential(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
-    return run_batched(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
+    try {
+        if (cfg.passes_per_launch == 1) return run_sequential(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
+        return run_batched(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
+    } catch (...) {
+        // a run that stops early may leave work queued that reads this searcher's
+        // buffers and tables: let it finish before the buffers can be reused or freed
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(build_stream_);
+        (void)hipStreamSynchronize(copy_stream_);
+        for (BuildLane& l : lanes_) (void)hipStreamSynchronize(l.stream);
+        throw;
+    }
 }
 
 SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, const SearchConfig& cfg,

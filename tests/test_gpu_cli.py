@@ -1,4 +1,4 @@
-"""The reference's own command line, unchanged, searching with the "hip" backend.
+"""The reference's own command line and C API, unchanged, searching with the "hip" backend.
 
 `oracle/_ref/fainttrail` is the reference CLI (cli/main.cpp and src/*.cpp compiled where
 they lie by `make -C oracle cli`) linked with the "hip" SearchBackend adapter
@@ -10,6 +10,7 @@ bytes. That is the whole host pipeline of the reference (FITS reading, registrat
 check, PSF estimate, build_stage, null calibration, refine, candidate formatting,
 src/search.cpp:716-809 and io.cpp:82-115) over this backend's stacking passes.
 """
+import ctypes as C
 import os
 import subprocess
 from pathlib import Path
@@ -67,3 +68,72 @@ def test_search_csv_and_map_identical(hip, sequence, name, extra):
     assert cpu[0].count(b"\n") >= 3, "the search wrote no candidates CSV"
     assert gpu[0] == cpu[0], f"{name}: candidates CSV differs between --backend hip and --backend cpu"
     assert gpu[1] == cpu[1], f"{name}: max-S/N map differs between --backend hip and --backend cpu"
+
+
+# --------------------------------------------------------------------- the C API (ft_*)
+# include/fainttrail/fainttrail.h of the reference: ft_params and ft_candidate as declared
+# there; oracle/_ref/libftref.so is the reference library (src/*.cpp, c_api.cpp among
+# them) with the "hip" adapter registered.
+
+class _FtParams(C.Structure):
+    _fields_ = [(name, C.c_double) for name in (
+        "vmin_pxh", "vmax_pxh", "grid_tol_px", "fwhm_px", "clip_sigma", "cr_sigma", "mask_scatter",
+        "weight_scatter", "min_coverage", "threshold_snr", "min_snr_floor", "target_false_per_run",
+        "max_frame_fraction", "half_tolerance")] + [
+        ("static_exclude", C.c_int), ("null_runs", C.c_int), ("max_candidates", C.c_int), ("edge_px", C.c_int),
+        ("threads", C.c_int), ("seed", C.c_ulonglong), ("backend", C.c_char * 16)]
+
+
+class _FtCandidate(C.Structure):
+    _fields_ = [("id", C.c_int), ("ok", C.c_int)] + [(name, C.c_double) for name in (
+        "x", "y", "vx_pxh", "vy_pxh", "pos_err_px", "vel_err_pxh", "snr", "snr_coarse", "snr_half1", "snr_half2",
+        "max_frame_fraction", "false_alarms_per_run", "flux", "mag_inst", "ra_deg", "dec_deg", "rate_arcsec_min",
+        "pa_deg")] + [("flags", C.c_char * 64)]
+
+
+def _c_api_search(lib, fits_files, backend, null_runs):
+    ctx = lib.ft_create()
+    assert ctx
+    try:
+        for f in fits_files:
+            assert lib.ft_add_fits(ctx, str(f).encode()) == 0, lib.ft_last_error(ctx)
+        prm = _FtParams()
+        lib.ft_params_default(C.byref(prm))
+        prm.backend = backend.encode()
+        prm.null_runs = null_runs
+        st = lib.ft_search(ctx, C.byref(prm), None, None)
+        assert st == 0, lib.ft_last_error(ctx)
+        cands = []
+        for i in range(lib.ft_candidate_count(ctx)):
+            c = _FtCandidate()
+            assert lib.ft_get_candidate(ctx, i, C.byref(c)) == 0
+            cands.append(bytes(c))
+        return lib.ft_threshold(ctx), lib.ft_reference_mjd(ctx), cands
+    finally:
+        lib.ft_destroy(ctx)
+
+
+@pytest.mark.skipif(not (ROOT / "oracle" / "_ref" / "libftref.so").exists(), reason="reference library not built")
+@pytest.mark.parametrize("null_runs", [3, 8])
+def test_c_api_search_identical(hip, sequence, null_runs):
+    lib = C.CDLL(str(ROOT / "oracle" / "_ref" / "libftref.so"))
+    lib.ft_create.restype = C.c_void_p
+    lib.ft_destroy.argtypes = [C.c_void_p]
+    lib.ft_add_fits.argtypes = [C.c_void_p, C.c_char_p]
+    lib.ft_params_default.argtypes = [C.POINTER(_FtParams)]
+    lib.ft_search.argtypes = [C.c_void_p, C.POINTER(_FtParams), C.c_void_p, C.c_void_p]
+    lib.ft_candidate_count.argtypes = [C.c_void_p]
+    lib.ft_get_candidate.argtypes = [C.c_void_p, C.c_int, C.POINTER(_FtCandidate)]
+    lib.ft_threshold.argtypes = [C.c_void_p]
+    lib.ft_threshold.restype = C.c_double
+    lib.ft_reference_mjd.argtypes = [C.c_void_p]
+    lib.ft_reference_mjd.restype = C.c_double
+    lib.ft_last_error.argtypes = [C.c_void_p]
+    lib.ft_last_error.restype = C.c_char_p
+    files = sorted((sequence / "seq").glob("*.fits"))
+    assert len(files) == 20
+    thr_c, mjd_c, cands_c = _c_api_search(lib, files, "cpu", null_runs)
+    thr_h, mjd_h, cands_h = _c_api_search(lib, files, "hip", null_runs)
+    assert len(cands_c) >= 1, "the search found no candidate"
+    assert thr_h == thr_c and mjd_h == mjd_c
+    assert cands_h == cands_c, "ft_search candidates differ between backend hip and backend cpu"
