@@ -91,6 +91,43 @@ def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
     assert same_bits(m2, om) and np.array_equal(b2, ob)
 
 
+def test_full_size_bands_bit_identical_to_blocked_oracle(hip):
+    """The full tiled-6144x4096 shape: 60 frames of 6144 x 4096, the 19952-motion grid
+    (grid_tol 0.372, 6-120 px/h, 60 s cadence), min_count 42. One device pass over the
+    whole frame; the oracle's blocked CPU pass on three bands of output rows (the top
+    edge, the middle and the bottom edge, every column: edge tiles on all four sides)
+    must give the same bits, and the whole map must keep the contract's form (indices
+    in range, undefined pixels 0 / -1)."""
+    import os
+
+    import torch
+    n, h, w = 60, 4096, 6144
+    t = (np.arange(n) - (n - 1) / 2.0) / 60.0
+    grid, _ = hip.velocity_grid(t[-1] - t[0], 6.0, 120.0, 0.372)
+    tab = hip.linear_shift_table(t, grid)
+    assert tab.motions == 19952
+    mc = max(3, int(np.ceil(0.7 * n)))
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(11)
+    d = torch.randn((n, h, w), generator=gen, device="cuda:0", dtype=torch.float32)
+    print("full size: device pass", flush=True)
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    m, b = plan.run(d)
+    torch.cuda.synchronize()
+    mh, bh = m.cpu().numpy(), b.cpu().numpy()
+    assert bh.min() >= -1 and bh.max() < tab.motions
+    assert np.all(mh[bh < 0] == 0.0) and np.all(np.isfinite(mh))
+    threads = len(os.sched_getaffinity(0))
+    lo_dy, hi_dy = int(tab.dy.min()), int(tab.dy.max())
+    for y0, y1 in ((0, 6), (2045, 2051), (h - 6, h)):
+        print(f"full size: oracle rows {y0}-{y1 - 1}", flush=True)
+        r0, r1 = max(0, y0 + lo_dy), min(h, y1 + hi_dy)
+        om, ob = po.stack_blocked_rows(d[:, r0:r1].cpu().numpy(), h, r0, tab.dx, tab.dy, mc, y0, y1, threads)
+        assert same_bits(mh[y0:y1], om) and np.array_equal(bh[y0:y1], ob), (y0, y1)
+    del d, m, b
+    torch.cuda.empty_cache()
+
+
 def test_plan_reuse_and_stride(hip, golden):
     """A plan runs repeatedly on resident frames; a padded frame stride gives the same maps."""
     import torch
