@@ -94,6 +94,40 @@ def test_stored_pass_is_fp32_pass_of_decoded_frames(hip, golden, name, storage):
     assert same_bits(m16.cpu().numpy(), om) and np.array_equal(b16.cpu().numpy(), ob), name
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stored_pass_full_size(hip, storage):
+    """The full tiled-6144x4096 shape (60 frames, 19952 motions) with frames of
+    unit-variance noise plus bright sources: the stored pass equals the fp32 pass over
+    the decoded frames on every pixel. The decoding is done on the device with torch's
+    round-to-nearest-even casts (fp16 / bf16) from the scales the device chose."""
+    import torch
+    n, h, w = 60, 4096, 6144
+    t = (np.arange(n) - (n - 1) / 2.0) / 60.0
+    grid, _ = hip.velocity_grid(t[-1] - t[0], 6.0, 120.0, 0.372)
+    tab = hip.linear_shift_table(t, grid)
+    mc = max(3, int(np.ceil(0.7 * n)))
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(5)
+    d = torch.randn((n, h, w), generator=gen, device="cuda:0", dtype=torch.float32)
+    d[:, 1000:1004, 3000:3004] += 300.0  # bright sources set the scales of every frame
+    d[7, 50, 60] = 4000.0
+    print(f"full size stored pass {storage}", flush=True)
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    plan.store(d, storage)
+    m16, b16 = plan.run_stored()
+    sc = torch.from_numpy(np.asarray(plan.scales(), np.float32)).cuda().view(n, 1, 1)
+    q = (d / sc).half() if storage == "f16" else (d / sc).bfloat16()
+    dec = q.float() * sc
+    del q, d
+    print(f"full size fp32 pass of decoded frames {storage}", flush=True)
+    m32, b32 = plan.run(dec)
+    torch.cuda.synchronize()
+    assert torch.equal(m16.view(torch.int32), m32.view(torch.int32))
+    assert torch.equal(b16, b32)
+    del dec, m16, b16, m32, b32
+    torch.cuda.empty_cache()
+
+
 def test_fp64_known_answer_and_numpy_restatement(hip, golden):
     import torch
     # constant frames: S = c / sqrt(c) = sqrt(c), maximal at full coverage, lowest index
