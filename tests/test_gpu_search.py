@@ -114,6 +114,27 @@ def test_batched_search_equals_one_pass_at_a_time(hip, group, segments, storage)
     _same_result(one, many)
 
 
+def test_batched_search_equals_one_pass_full_size(hip):
+    """The 2048^2 null-calibration shape: 40 frames of 2048 x 2048, the 7980-motion grid
+    (grid_tol 0.212, 12-66 px/h), 16 null runs. The batched form (real pass first, null
+    runs in fused groups) gives the one-pass-at-a-time form's results bit for bit."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    n, w, h = 40, 2048, 2048
+    fr, mjd, _ = synth.make_sequence(n, w, h, nstars=300, nmovers=10, mover_snr=(0.7, 0.7), seed=7,
+                                     device="cuda:0")
+    print("full-size batched search", flush=True)
+    s = Searcher(n, w, h)
+    base = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.212, fwhm_px=3.0, null_runs=16, seed=7)
+    one = s.run(fr, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 1}))
+    many = s.run(fr, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 6}))
+    torch.cuda.synchronize()
+    assert one.passes_per_launch == 1 and many.passes_per_launch == 6
+    assert one.null_peaks.size > 0 and one.peaks.x.size > 0
+    _same_result(one, many)
+
+
 def test_batched_search_fixed_threshold_and_tuner(hip):
     """No null runs (a fixed threshold): the batched form is the real pass alone. The
     autotuner's choice, used by later runs, leaves every result bit unchanged."""
