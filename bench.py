@@ -97,6 +97,28 @@ def cpu_baseline(frames_dev, tab, min_count, target_s: float):
                       f"{n} frames): {units:.3e} units in {secs:.1f} s"}
 
 
+def host_pass_measure(frames_dev, tab, min_count, dev_max, dev_best):
+    """The same pass through the reference-shaped drop-in, ftk_stack_pass (CudaBackend::run,
+    src/backend_cuda.cu:99-195): frames in host memory, maps returned to host memory, so
+    the PCIe upload and download are inside the timed call. Its maps must equal the
+    device-resident pass's."""
+    import fainttrail_amd as ft
+    host = frames_dev.cpu().numpy()
+    n, h, w = host.shape
+    be = ft.make_backend("hip")
+    t0 = time.perf_counter()
+    r = be.run(ft.StackPass(frames=host, shifts=tab, min_count=int(min_count)))
+    wall = time.perf_counter() - t0
+    same = bool(np.array_equal(r.max_snr.view(np.uint32), dev_max.cpu().numpy().view(np.uint32)) and
+                np.array_equal(r.best_v, dev_best.cpu().numpy()))
+    units = float(tab.motions) * n * w * h
+    tim = dict(r.timings)
+    return {"units_per_s": units / wall, "wall_s": wall, "upload_s": tim["upload"], "kernel_s": tim["kernel"],
+            "download_s": tim["download"], "setup_s": tim["setup"],
+            "upload_gb_per_s": host.nbytes / max(tim["upload"], 1e-9) / 1e9, "maps_equal_device_pass": same,
+            "note": "PCIe-inclusive drop-in rate (host frames in, host maps out); never the headline value"}
+
+
 def search_measure(reps: int, with_plain: bool):
     """Null-calibrated searches/s on the 512^2 configuration: 20 frames of 512 x 512 at
     60 s cadence, sky 1000 ADU + N(0, 32^2), 150 stars (sigma_psf 1.5 px, flux
@@ -391,6 +413,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-null batched calibration measurement")
     ap.add_argument("--fp64-rows", type=int, default=16)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (Poisson stream) measurement")
+    ap.add_argument("--no-host-pass", action="store_true",
+                    help="skip the host-buffer pass (the drop-in with PCIe upload and download)")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-seed", type=int, default=3)
     ap.add_argument("--search-only", action="store_true",
@@ -488,6 +512,10 @@ def main() -> int:
         res["cpu_baseline"] = cpu_baseline(frames, tab, min_count, args.cpu_seconds)
     else:
         res["cpu_baseline"] = None
+    if not args.no_host_pass and args.storage == "f32":
+        print("bench: host-buffer pass (ftk_stack_pass)", file=sys.stderr, flush=True)
+        res["host_buffers"] = host_pass_measure(frames, tab, min_count, out_m, out_b)
+        print("bench: host_buffers: " + json.dumps(res["host_buffers"]), file=sys.stderr, flush=True)
     del frames, plan
     torch.cuda.empty_cache()
     if not args.no_search:

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <sstream>
 #include <stdexcept>
@@ -36,6 +37,50 @@ void check(hipError_t e, const char* what) {
 }
 
 struct Cancel {};
+
+// Launches of a host-buffer pass, alternating between two streams so that one launch's
+// last workgroups overlap the next launch's first (launches write disjoint output rows,
+// or merge into the keys by atomic maximum), with an event per launch and up to three
+// launches queued: progress is reported as each launch completes, in order. Destruction (normal or on cancel or
+// error) waits for everything launched.
+struct LaunchQueue {
+    hipStream_t st[2] = {nullptr, nullptr};
+    std::deque<std::pair<hipEvent_t, double>> pending;  // launch done, fraction of the pass after it
+    int count = 0;
+    LaunchQueue() {
+        for (auto& s : st) check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "create stream");
+    }
+    ~LaunchQueue() {
+        for (auto& e : pending) {
+            (void)hipEventSynchronize(e.first);
+            (void)hipEventDestroy(e.first);
+        }
+        for (auto& s : st)
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+    }
+    hipStream_t next() { return st[count++ & 1]; }
+    void launched(hipStream_t s, double done) {
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create event");
+        pending.emplace_back(e, done);
+        check(hipEventRecord(e, s), "record event");
+    }
+    // waits for launches until at most `keep` are in flight; false: the progress callback cancelled
+    bool settle(size_t keep, ftk_progress_fn progress, void* user) {
+        while (pending.size() > keep) {
+            auto [e, done] = pending.front();
+            pending.pop_front();
+            const hipError_t r = hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+            check(r, "stack kernel");
+            if (progress && !progress(done, "stack", user)) return false;
+        }
+        return true;
+    }
+};
 
 template <typename T>
 struct DeviceBuffer {
@@ -71,8 +116,10 @@ ftk_options default_options() {
     ftk_options o;
     o.device = -1;
     o.max_mb = 0.0;
-    // about 10 ms of kernel time per launch at the measured stacking rate
-    o.launch_adds = 1e11;
+    // about 0.25 s of kernel time per launch at the measured stacking rate: progress and
+    // cancel stay responsive, and launch boundaries cost a few % (1e11, 10 ms launches,
+    // cost ~20 % on the 6144 x 4096 x 60 x 19952 pass when each launch was waited for)
+    o.launch_adds = 4e12;
     o.pad = 1;
     return o;
 }
@@ -217,12 +264,11 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
 
         DevicePlan dev;
         dev.upload(p);
-        DeviceBuffer<float> d_frames, d_max;
+        DeviceBuffer<float> d_frames, d_max, d_stage;
         DeviceBuffer<int> d_best;
         DeviceBuffer<unsigned long long> d_keys;
         double t_up = 0, t_kernel = 0, t_down = 0;
-        const double total_work = static_cast<double>(bands.size()) * p.nb;
-        double done = 0;
+        double done = 0;  // fraction of the output rows stacked
         for (const Band& b : bands) {
             const int rows = b.r1 - b.r0, out_rows = b.y1 - b.y0;
             const size_t np = static_cast<size_t>(out_rows) * w;
@@ -238,53 +284,79 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             d_keys.reserve(np, "allocate keys");
             d_max.reserve(np, "allocate output");
             d_best.reserve(np, "allocate output");
-            if (use_pad) check(hipMemset(d_frames.p, 0, sizeof(float) * n * stride), "clear frames");
-            for (int i = 0; i < n; ++i)
-                check(hipMemcpy2D(d_frames.p + i * stride + origin, sizeof(float) * (use_pad ? g.pitch : w),
-                                  pass->frames[i] + static_cast<size_t>(b.r0) * w, sizeof(float) * w, sizeof(float) * w,
-                                  static_cast<size_t>(rows), hipMemcpyHostToDevice),
-                      "upload frames");
+            // frame rows as one contiguous copy each (pitched copies from pageable memory
+            // go row by row); into the zero border through a device-side row copy
+            const size_t frame_floats = static_cast<size_t>(rows) * w;
+            if (use_pad) {
+                check(hipMemset(d_frames.p, 0, sizeof(float) * n * stride), "clear frames");
+                d_stage.reserve(frame_floats, "allocate upload buffer");
+            }
+            for (int i = 0; i < n; ++i) {
+                const float* src = pass->frames[i] + static_cast<size_t>(b.r0) * w;
+                float* dst = use_pad ? d_stage.p : d_frames.p + i * stride;
+                check(hipMemcpy(dst, src, sizeof(float) * frame_floats, hipMemcpyHostToDevice), "upload frames");
+                if (use_pad)
+                    check(launch_copy_rows(d_stage.p, static_cast<long long>(frame_floats), d_frames.p + i * stride + origin,
+                                           stride, g.pitch, w, rows, 1, nullptr),
+                          "pad frames");
+            }
+            if (use_pad) check(hipDeviceSynchronize(), "pad frames");
             auto t1 = clk::now();
             t_up += std::chrono::duration<double>(t1 - t0).count();
 
-            // launches of about launch_adds units: whole segments of blocks
-            const double units_per_block = static_cast<double>(out_rows) * w * n * kVel;
-            const int blocks_per_launch =
-                static_cast<int>(std::clamp(opt.launch_adds / std::max(1.0, units_per_block), 1.0, static_cast<double>(p.nb)));
-            const int bps = std::min(p.blocks_per_seg, blocks_per_launch);
-            const int nseg_total = (p.nb + bps - 1) / bps;
-            const int segs_per_launch = std::max(1, blocks_per_launch / bps);
+            // launches of about launch_adds units: whole tile rows of the band with every
+            // velocity block (all segments of a tile in one launch share its frame boxes
+            // in L2); when one tile row is already more, segments of its blocks
+            const double units_per_row = static_cast<double>(w) * n * p.nv;
+            int rows_per_launch = static_cast<int>(std::clamp(opt.launch_adds / std::max(1.0, units_per_row), 1.0,
+                                                              static_cast<double>(out_rows)));
+            rows_per_launch = std::max(kTileY, rows_per_launch / kTileY * kTileY);
             check(launch_fill_keys(d_keys.p, static_cast<long long>(np), nullptr), "clear keys");
-            for (int s0 = 0; s0 < nseg_total; s0 += segs_per_launch) {
-                StackLaunch L{};
-                L.frames = d_frames.p + origin;
-                L.frame_stride = stride;
-                L.pitch = use_pad ? g.pitch : w;
-                L.padded = use_pad;
-                L.row0 = b.r0;
-                L.row1 = b.r1;
-                L.shift = dev.shift.p;
-                L.box = dev.box.p;
-                L.ubox = dev.ubox.p;
-                L.lds_off = dev.lds_off.p;
-                L.cov_off = dev.cov_off.p;
-                L.cov_x = dev.cov_x.p;
-                L.cov_y = dev.cov_y.p;
-                L.motion = dev.motion.p;
-                L.inv_sqrt = dev.inv_sqrt.p;
-                L.y_begin = b.y0;
-                L.y_end = b.y1;
-                L.tiles_x = p.tiles_x;
-                L.tiles_y = p.tiles_y;
-                L.bps = bps;
-                L.seg0 = s0;
-                L.nseg = std::min(segs_per_launch, nseg_total - s0);
-                L.keys = d_keys.p;
-                check(launch_stack(p, L, nullptr), "launch stack kernel");
-                check(hipDeviceSynchronize(), "stack kernel");
-                done += std::min(p.nb, (s0 + L.nseg) * bps) - std::min(p.nb, s0 * bps);
-                if (progress && !progress(done / total_work, "stack", user)) throw Cancel{};
+            check(hipDeviceSynchronize(), "clear keys");
+            LaunchQueue lq;
+            for (int y = b.y0; y < b.y1; y += rows_per_launch) {
+                const int sub_rows = std::min(rows_per_launch, b.y1 - y);
+                plan_tiles(p, sub_rows);
+                const double units_per_block = static_cast<double>(sub_rows) * w * n * kVel;
+                const int blocks_per_launch = static_cast<int>(
+                    std::clamp(opt.launch_adds / std::max(1.0, units_per_block), 1.0, static_cast<double>(p.nb)));
+                const int bps = std::min(p.blocks_per_seg, blocks_per_launch);
+                const int nseg_total = (p.nb + bps - 1) / bps;
+                const int segs_per_launch = std::max(1, blocks_per_launch / bps);
+                for (int s0 = 0; s0 < nseg_total; s0 += segs_per_launch) {
+                    StackLaunch L{};
+                    L.frames = d_frames.p + origin;
+                    L.frame_stride = stride;
+                    L.pitch = use_pad ? g.pitch : w;
+                    L.padded = use_pad;
+                    L.row0 = b.r0;
+                    L.row1 = b.r1;
+                    L.shift = dev.shift.p;
+                    L.box = dev.box.p;
+                    L.ubox = dev.ubox.p;
+                    L.lds_off = dev.lds_off.p;
+                    L.cov_off = dev.cov_off.p;
+                    L.cov_x = dev.cov_x.p;
+                    L.cov_y = dev.cov_y.p;
+                    L.motion = dev.motion.p;
+                    L.inv_sqrt = dev.inv_sqrt.p;
+                    L.y_begin = y;
+                    L.y_end = y + sub_rows;
+                    L.tiles_x = p.tiles_x;
+                    L.tiles_y = p.tiles_y;
+                    L.bps = bps;
+                    L.seg0 = s0;
+                    L.nseg = std::min(segs_per_launch, nseg_total - s0);
+                    L.keys = d_keys.p + static_cast<size_t>(y - b.y0) * w;
+                    hipStream_t st = lq.next();
+                    check(launch_stack(p, L, st), "launch stack kernel");
+                    const int blocks_done = std::min(p.nb, (s0 + L.nseg) * bps) - std::min(p.nb, s0 * bps);
+                    done += static_cast<double>(sub_rows) / h * blocks_done / p.nb;
+                    lq.launched(st, done);
+                    if (!lq.settle(2, progress, user)) throw Cancel{};
+                }
             }
+            if (!lq.settle(0, progress, user)) throw Cancel{};
             check(launch_finalize(d_keys.p, static_cast<long long>(np), d_max.p, d_best.p, nullptr), "finalize");
             check(hipDeviceSynchronize(), "finalize");
             auto t2 = clk::now();

@@ -491,7 +491,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         ftk_options opt;
         opt.device = -1;
         opt.max_mb = 0;
-        opt.launch_adds = 1e11;
+        opt.launch_adds = 1e12;
         opt.pad = 1;
         ftk_plan* plan = nullptr;
         char err[512] = {0};
