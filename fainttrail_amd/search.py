@@ -199,7 +199,10 @@ class Searcher:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().ftk_searcher_destroy(h)
+            try:
+                lib().ftk_searcher_destroy(h)
+            except Exception:  # interpreter shutdown: module globals may already be cleared
+                pass
             self._h = None
 
 

@@ -128,7 +128,10 @@ class Server:
         self._frames = {}
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: module globals may already be cleared
+            pass
 
 
 @dataclass

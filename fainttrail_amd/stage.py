@@ -121,5 +121,8 @@ class Stage:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().ftk_stage_destroy(h)
+            try:
+                lib().ftk_stage_destroy(h)
+            except Exception:  # interpreter shutdown: module globals may already be cleared
+                pass
             self._h = None
