@@ -344,14 +344,39 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
             const uint32_t* const rb1 = reinterpret_cast<const uint32_t*>(lds1) + lb;
             auto consume16 = [&](const uint32_t* L, int i, float sc) {
                 cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
+                // the words of motions k + 2, k + 3 are read while motions k, k + 1 are
+                // added, so a wavefront keeps four LDS reads in flight
+                uint32_t cur[2][2], nxt[2][2];
 #pragma unroll
-                for (int k = 0; k < V; ++k) {
-                    const uint32_t* p = L + lo[k];
-                    const uint32_t w01 = p[0], w23 = p[2 * kPitch];
-                    acc[k][0] = fma_q16<ST>(w01, false, sc, acc[k][0]);
-                    acc[k][1] = fma_q16<ST>(w01, true, sc, acc[k][1]);
-                    acc[k][2] = fma_q16<ST>(w23, false, sc, acc[k][2]);
-                    acc[k][3] = fma_q16<ST>(w23, true, sc, acc[k][3]);
+                for (int t = 0; t < 2; ++t) {
+                    const uint32_t* p = L + lo[t];
+                    cur[t][0] = p[0];
+                    cur[t][1] = p[2 * kPitch];
+                }
+#pragma unroll
+                for (int k = 0; k < V; k += 2) {
+                    if (k + 2 < V) {
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) {
+                            const uint32_t* p = L + lo[k + 2 + t];
+                            nxt[t][0] = p[0];
+                            nxt[t][1] = p[2 * kPitch];
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        acc[k + t][0] = fma_q16<ST>(cur[t][0], false, sc, acc[k + t][0]);
+                        acc[k + t][1] = fma_q16<ST>(cur[t][0], true, sc, acc[k + t][1]);
+                        acc[k + t][2] = fma_q16<ST>(cur[t][1], false, sc, acc[k + t][2]);
+                        acc[k + t][3] = fma_q16<ST>(cur[t][1], true, sc, acc[k + t][3]);
+                    }
+                    if (k + 2 < V) {
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) {
+                            cur[t][0] = nxt[t][0];
+                            cur[t][1] = nxt[t][1];
+                        }
+                    }
                 }
             };
             fetch(0);
