@@ -54,6 +54,9 @@ def test_backends_lists_hip(hip, sequence):
     ("fixed_threshold", ["--threshold", "5.5", "--null-runs", "0"]),
     # the reference's tiled search (uneven tiles: 512 = 2 x 192 + 128)
     ("tiled_192", ["--tile", "192"]),
+    # near-Earth search: curved tracks from the diurnal parallax at 6 trial distances,
+    # explicit shift tables through the backend (parallax.cpp, search.cpp:73-97)
+    ("neo_curved", ["--neo", "0.0002", "--site", "50.8,4.4,100"]),
 ])
 def test_search_csv_and_map_identical(hip, sequence, name, extra):
     tmp = sequence
