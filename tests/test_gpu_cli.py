@@ -73,6 +73,20 @@ def test_search_csv_and_map_identical(hip, sequence, name, extra):
     assert gpu[1] == cpu[1], f"{name}: max-S/N map differs between --backend hip and --backend cpu"
 
 
+def test_reference_regression_cases_strict(hip, tmp_path):
+    """The reference's real-data regression suite (three registered sequences: Ceres,
+    the Leo triplet with (21996) 1999 XP31, (12631) Mariekebaan; fixtures copied by
+    tests/golden/make_regression_fixtures.py) through `fainttrail regress --backend hip
+    --strict`: every known asteroid found, nothing else passing, thresholds 6.419 /
+    6.399 / 7.599 and the candidate lists equal to the reference's baselines
+    (tests/regression/README.md: "same cases, another backend")."""
+    cases = sorted((ROOT / "tests" / "golden" / "regression").glob("*.case"))
+    assert len(cases) == 3
+    out = _run(["regress"] + cases + ["--backend", "hip", "--strict"], tmp_path)
+    assert "backend hip" in out
+    assert "regression: 3 passed, 0 changed, 0 failed, 0 skipped" in out, out[-3000:]
+
+
 # --------------------------------------------------------------------- the C API (ft_*)
 # include/fainttrail/fainttrail.h of the reference: ft_params and ft_candidate as declared
 # there; oracle/_ref/libftref.so is the reference library (src/*.cpp, c_api.cpp among
