@@ -542,6 +542,21 @@ extern "C" int ftk_plan_run_stored(ftk_plan* pl, float* d_max_snr, int32_t* d_be
     }
 }
 
+namespace ftk {
+// The 16-bit frames of a plan after ftk_plan_store: frame i element (x, y) at
+// origin + i stride + y pitch + x, times scale[i] (device). False when none are stored.
+bool plan_stored_frames(const ftk_plan* pl, const uint16_t** origin, long long* stride, int* pitch,
+                        const float** scale) {
+    if (!pl || !pl->storage || !pl->stored.p) return false;
+    const PadGeometry& g = pl->pad16;
+    *origin = pl->stored.p + static_cast<long long>(g.top) * g.pitch + g.left;
+    *stride = static_cast<long long>(g.rows) * g.pitch;
+    *pitch = g.pitch;
+    *scale = pl->scales.p;
+    return true;
+}
+}  // namespace ftk
+
 extern "C" int ftk_plan_scales(const ftk_plan* pl, float* scales) {
     if (!pl || !scales || !pl->storage) return FTK_ERROR;
     if (hipSetDevice(pl->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return FTK_ERROR;

@@ -48,6 +48,10 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->passes_per_launch = 0;
     p->segments = 0;
     p->storage = FTK_STORE_F32;
+    p->null_tested = c.null_tested;
+    p->max_frame_fraction = c.max_frame_fraction;
+    p->reg_dx = nullptr;
+    p->reg_dy = nullptr;
 }
 
 namespace {
@@ -74,7 +78,18 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.passes_per_launch = p->passes_per_launch;
     c.segments = p->segments;
     c.storage = p->storage;
+    c.null_tested = p->null_tested;
+    c.max_frame_fraction = p->max_frame_fraction;
     return c;
+}
+
+// The registration offsets belong to the frames: n of each, copied (a queued job may
+// outlive the caller's arrays).
+void set_offsets(ftk::SearchConfig& c, const ftk_search_params* p, int n) {
+    if (!p->reg_dx && !p->reg_dy) return;
+    if (!p->reg_dx || !p->reg_dy) throw std::runtime_error("search: give both registration offsets (reg_dx, reg_dy) or none");
+    c.reg_dx.assign(p->reg_dx, p->reg_dx + n);
+    c.reg_dy.assign(p->reg_dy, p->reg_dy + n);
 }
 
 void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
@@ -94,6 +109,8 @@ void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
     sum->n_pooled = static_cast<int>(o.pooled.size());
     sum->passes_per_launch = o.passes_per_launch;
     sum->segments = o.segments;
+    sum->n_screened = o.screened;
+    sum->n_artefacts = o.artefacts;
     for (const auto& t : o.timings) {
         if (t.first == "prepare") sum->t_prepare += t.second;
         else if (t.first == "stack (real)") sum->t_stack_real += t.second;
@@ -116,6 +133,7 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
         throw std::runtime_error("ftk_searcher_run: passes_per_launch and segments must be >= 0");
     if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
         throw std::runtime_error("ftk_searcher_run: unknown frame storage");
+    if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
 }
 }  // namespace
 
@@ -153,7 +171,8 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
     try {
         check_run_args(s, d_raw, stride, mjd, p);
         hipSetDevice(s->device);
-        const ftk::SearchConfig c = config_of(p);
+        ftk::SearchConfig c = config_of(p);
+        set_offsets(c, p, s->ds->n);
         s->last = s->ds->run(d_raw, static_cast<long long>(stride), mjd, c, d_max_snr, d_best_v,
                              static_cast<hipStream_t>(stream));
         if (sum) fill_summary(s->last, sum);
@@ -170,7 +189,9 @@ extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t str
     try {
         check_run_args(s, d_raw, stride, mjd, p);
         hipSetDevice(s->device);
-        const double t = s->ds->tune(d_raw, static_cast<long long>(stride), mjd, config_of(p), reps,
+        ftk::SearchConfig c = config_of(p);
+        set_offsets(c, p, s->ds->n);
+        const double t = s->ds->tune(d_raw, static_cast<long long>(stride), mjd, c, reps,
                                      static_cast<hipStream_t>(stream));
         if (passes_per_launch) *passes_per_launch = s->ds->tuned_passes;
         if (segments) *segments = s->ds->tuned_segments;
@@ -250,7 +271,10 @@ extern "C" int ftk_server_submit(ftk_server* s, const float* d_raw, size_t strid
         if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
             throw std::runtime_error("ftk_server_submit: unknown frame storage");
         if (!(p->fwhm_px > 0.0)) throw std::runtime_error("search: fwhm_px must be given (> 0)");
-        *job = s->srv->submit(d_raw, static_cast<long long>(stride), n, w, h, mjd, config_of(p));
+        if (p->null_tested < 0) throw std::runtime_error("ftk_server_submit: null_tested must be >= 0");
+        ftk::SearchConfig c = config_of(p);
+        set_offsets(c, p, n);
+        *job = s->srv->submit(d_raw, static_cast<long long>(stride), n, w, h, mjd, std::move(c));
         return FTK_OK;
     } catch (const std::exception& e) {
         put_err2(err, err_len, e.what());

@@ -12,6 +12,7 @@
 #include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
 #include "stack_plan.hpp"
+#include "screen.hpp"
 #include "store.hpp"
 
 namespace ftk {
@@ -323,6 +324,7 @@ DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_
     nbest_ = salloc<int>(plane, "allocate maps");
     pk_cap_ = std::max<size_t>(1 << 16, plane / 8);
     pk_ = salloc<int>(1 + 4 * pk_cap_, "allocate peaks");
+    sky_ = salloc<float>(plane, "allocate maps");
     scheck(hipStreamCreateWithFlags(&build_stream_, hipStreamNonBlocking), "create stream");
     scheck(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "create stream");
     for (int q = 0; q < 2; ++q) {
@@ -338,8 +340,12 @@ DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_
 DeviceSearch::~DeviceSearch() {
     for (void* p : {static_cast<void*>(sub_), static_cast<void*>(filt_), static_cast<void*>(max_),
                     static_cast<void*>(best_), static_cast<void*>(nmax_), static_cast<void*>(nbest_),
-                    static_cast<void*>(pk_)})
+                    static_cast<void*>(pk_), static_cast<void*>(sky_), static_cast<void*>(scr_tab_),
+                    static_cast<void*>(scr_pk_), static_cast<void*>(scr_art_)})
         if (p) (void)hipFree(p);
+    for (void* p : {static_cast<void*>(scr_tab_host_), static_cast<void*>(scr_pk_host_),
+                    static_cast<void*>(scr_art_host_)})
+        if (p) (void)hipHostFree(p);
     for (auto& set : sets_) set.release();
     for (int q = 0; q < 2; ++q)
         for (hipEvent_t e : {ev_built_[q], ev_stacked_[q], ev_counts_[q]})
@@ -385,6 +391,150 @@ PeakList DeviceSearch::peaks(const float* m, const int* b, float floor, int radi
     }
     return order_peaks(c, hx.data(), hy.data(), hs.data(), hv.data(), max_peaks);
 }
+
+// The run's tables of the null screen, in one pinned block and one copy on stream s:
+// velocities by index, sqrt(c) for the S/N normalisation (track_snr, search.cpp:158),
+// the detector test's offsets (search.cpp:217-223: the same accumulation, so the same
+// doubles), the registration offsets and the frame times of every pass.
+void DeviceSearch::screen_tables(const SearchConfig& cfg, const SearchOutput& out,
+                                 const std::vector<std::vector<double>>& times, hipStream_t s) {
+    std::vector<double> steps;
+    const double span = std::max(2.0, 2.0 * cfg.fwhm_px);
+    for (double d = -span; d <= span; d += 0.5) steps.push_back(d);
+    const bool reg = !cfg.reg_dx.empty();
+    const size_t nv = out.grid.size();
+    const size_t need = 2 * nv + (n + 1) + steps.size() + (reg ? 2 * static_cast<size_t>(n) : 0) +
+                        times.size() * static_cast<size_t>(n);
+    if (need > scr_tab_cap_) {
+        if (scr_tab_) scheck(hipFree(scr_tab_), "free screen tables");
+        if (scr_tab_host_) scheck(hipHostFree(scr_tab_host_), "free screen tables");
+        scr_tab_ = scr_tab_host_ = nullptr;
+        scr_tab_cap_ = 0;
+        const size_t cap = need + need / 4;
+        scr_tab_ = salloc<double>(cap, "allocate screen tables");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_tab_host_), sizeof(double) * cap, hipHostMallocDefault),
+               "allocate screen tables");
+        scr_tab_cap_ = cap;
+    }
+    double* q = scr_tab_host_;
+    for (const auto& v : out.grid) {
+        *q++ = v.first;
+        *q++ = v.second;
+    }
+    for (int c = 0; c <= n; ++c) *q++ = std::sqrt(static_cast<double>(c));
+    for (double d : steps) *q++ = d;
+    if (reg) {
+        for (double d : cfg.reg_dx) *q++ = d;
+        for (double d : cfg.reg_dy) *q++ = d;
+    }
+    for (const auto& t : times) q = std::copy(t.begin(), t.end(), q);
+    scheck(hipMemcpyAsync(scr_tab_, scr_tab_host_, sizeof(double) * need, hipMemcpyHostToDevice, s),
+           "upload screen tables");
+    scr_nv_ = static_cast<int>(nv);
+    scr_nsteps_ = static_cast<int>(steps.size());
+    scr_edge_r_ = static_cast<int>(std::ceil(2.0 * cfg.fwhm_px)) + 1;
+    scr_reg_ = reg;
+}
+
+// null_tested_count and the artefact tests of search.cpp:773-778 for several passes
+// in one upload, one launch per pass and one download; returns the flags per pass.
+std::vector<std::vector<unsigned char>> DeviceSearch::screen_passes(const SearchConfig& cfg,
+                                                                    const std::vector<ScreenPass>& sp, hipStream_t s,
+                                                                    SearchOutput& out) {
+    std::vector<std::vector<unsigned char>> flags(sp.size());
+    std::vector<size_t> off(sp.size() + 1, 0);
+    for (size_t k = 0; k < sp.size(); ++k) {
+        const size_t c = cfg.null_tested > 0 ? std::min(sp[k].peaks->x.size(), static_cast<size_t>(cfg.null_tested)) : 0;
+        off[k + 1] = off[k] + c;
+    }
+    const size_t total = off.back();
+    if (total == 0) return flags;
+    if (total > scr_cap_) {
+        for (void* p : {static_cast<void*>(scr_pk_), static_cast<void*>(scr_art_)})
+            if (p) scheck(hipFree(p), "free screen buffers");
+        for (void* p : {static_cast<void*>(scr_pk_host_), static_cast<void*>(scr_art_host_)})
+            if (p) scheck(hipHostFree(p), "free screen buffers");
+        scr_pk_ = scr_pk_host_ = nullptr;
+        scr_art_ = scr_art_host_ = nullptr;
+        scr_cap_ = 0;
+        const size_t cap = std::max<size_t>(total + total / 4, 4096);
+        scr_pk_ = salloc<int>(3 * cap, "allocate screen buffers");
+        scr_art_ = salloc<unsigned char>(cap, "allocate screen buffers");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_pk_host_), sizeof(int) * 3 * cap, hipHostMallocDefault),
+               "allocate screen buffers");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_art_host_), cap, hipHostMallocDefault),
+               "allocate screen buffers");
+        scr_cap_ = cap;
+    }
+    for (size_t k = 0; k < sp.size(); ++k) {
+        const PeakList& pk = *sp[k].peaks;
+        const size_t c = off[k + 1] - off[k];
+        std::copy(pk.x.begin(), pk.x.begin() + c, scr_pk_host_ + off[k]);
+        std::copy(pk.y.begin(), pk.y.begin() + c, scr_pk_host_ + total + off[k]);
+        std::copy(pk.vidx.begin(), pk.vidx.begin() + c, scr_pk_host_ + 2 * total + off[k]);
+    }
+    scheck(hipMemcpyAsync(scr_pk_, scr_pk_host_, sizeof(int) * 3 * total, hipMemcpyHostToDevice, s), "upload peaks");
+    const double* vel = scr_tab_;
+    const double* sqrtc = vel + 2 * static_cast<size_t>(scr_nv_);
+    const double* steps = sqrtc + (n + 1);
+    const double* reg = steps + scr_nsteps_;
+    const double* times = scr_reg_ ? reg + 2 * static_cast<size_t>(n) : reg;
+    for (size_t k = 0; k < sp.size(); ++k) {
+        const size_t c = off[k + 1] - off[k];
+        if (c == 0) continue;
+        for (size_t j = 0; j < c; ++j) {
+            const int v = sp[k].peaks->vidx[j];
+            if (v < 0 || v >= scr_nv_) throw std::runtime_error("null screen: velocity index out of range");
+        }
+        ScreenArgs a;
+        a.frames = sp[k].frames;
+        a.fstride = sp[k].fstride;
+        a.pitch = sp[k].pitch;
+        a.storage = sp[k].storage;
+        a.scale = sp[k].scale;
+        a.sub = sub_;
+        a.sub_stride = static_cast<long long>(w) * h;
+        a.sky = sky_;
+        a.t = times + static_cast<size_t>(sp[k].pass) * n;
+        a.reg_dx = scr_reg_ ? reg : nullptr;
+        a.reg_dy = scr_reg_ ? reg + n : nullptr;
+        a.vel = vel;
+        a.sqrtc = sqrtc;
+        a.steps = steps;
+        a.nsteps = scr_nsteps_;
+        a.px = scr_pk_ + off[k];
+        a.py = scr_pk_ + total + off[k];
+        a.pv = scr_pk_ + 2 * total + off[k];
+        a.count = static_cast<int>(c);
+        a.art = scr_art_ + off[k];
+        a.n = n;
+        a.w = w;
+        a.h = h;
+        a.edge_r = scr_edge_r_;
+        a.max_frame_fraction = cfg.max_frame_fraction;
+        scheck(launch_screen(a, s), "null screen");
+    }
+    scheck(hipMemcpyAsync(scr_art_host_, scr_art_, total, hipMemcpyDeviceToHost, s), "download screen flags");
+    scheck(hipStreamSynchronize(s), "null screen");
+    for (size_t k = 0; k < sp.size(); ++k) {
+        flags[k].assign(scr_art_host_ + off[k], scr_art_host_ + off[k + 1]);
+        out.screened += static_cast<int>(off[k + 1] - off[k]);
+    }
+    return flags;
+}
+
+void pool_null(const PeakList& pk, const std::vector<unsigned char>& art, SearchOutput& out) {
+    for (size_t k = 0; k < pk.snr.size(); ++k) {
+        if (k < art.size() && art[k]) {
+            ++out.artefacts;
+            continue;
+        }
+        out.pooled.push_back(pk.snr[k]);
+    }
+}
+
+bool plan_stored_frames(const ftk_plan* pl, const uint16_t** origin, long long* stride, int* pitch,
+                        const float** scale);
 
 namespace {
 
@@ -441,6 +591,9 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
     if (n < 5) throw std::runtime_error("synthetic tracking needs at least 5 frames");
     if (w < 16 || h < 16) throw std::runtime_error("frames are too small");
     if (cfg.passes_per_launch < 0) throw std::runtime_error("search: passes_per_launch must be >= 0");
+    if ((!cfg.reg_dx.empty() || !cfg.reg_dy.empty()) &&
+        (cfg.reg_dx.size() != static_cast<size_t>(n) || cfg.reg_dy.size() != static_cast<size_t>(n)))
+        throw std::runtime_error("search: registration offsets must be given for every frame (or for none)");
     SearchOutput out;
     // times relative to the middle of the sequence (preprocess.cpp:541-553)
     double tmin = mjd[0], tmax = mjd[0];
@@ -474,7 +627,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     const StageParams prm = stage_params(cfg);
     stage_.background(raw, stride, sub_, static_cast<long long>(plane), nullptr, s);
     stage_.build(sub_, static_cast<long long>(plane), out.t_h.data(), prm, filt_, static_cast<long long>(plane),
-                 nullptr, nullptr, nullptr, nullptr, s);
+                 sky_, nullptr, nullptr, nullptr, s);
     lap("prepare");
 
     const int nv = static_cast<int>(out.grid.size());
@@ -486,7 +639,12 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     out.passes_per_launch = 1;
 
     std::vector<int32_t> dx, dy;
-    auto pass = [&](const std::vector<double>& t, float* m, int* b) {
+    struct PlanDel {
+        void operator()(ftk_plan* p) const { ftk_plan_destroy(p); }
+    };
+    using PlanPtr = std::unique_ptr<ftk_plan, PlanDel>;
+    // one pass with a plan of its own; the plan (and its 16-bit frames) is returned
+    auto pass = [&](const std::vector<double>& t, float* m, int* b) -> PlanPtr {
         shifts_for(out.grid, t, n, dx, dy);
         ftk_options opt;
         opt.device = -1;
@@ -497,6 +655,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         char err[512] = {0};
         if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
             throw std::runtime_error(err);
+        PlanPtr held(plan);
         int st = FTK_OK;
         if (cfg.storage == kStoreF32) {
             st = ftk_plan_run(plan, filt_, plane, m, b, s, err, sizeof err);
@@ -506,8 +665,8 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         }
         // the plan's device tables must outlive the launch
         scheck(hipStreamSynchronize(s), "stack pass");
-        ftk_plan_destroy(plan);
         if (st != FTK_OK) throw std::runtime_error(err);
+        return held;
     };
 
     float* m_real = d_max_snr ? d_max_snr : max_;
@@ -519,6 +678,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     lap("peaks (real)");
 
     const auto orders = null_time_orders(out.t_h, runs, cfg.seed);
+    if (runs > 0 && cfg.null_tested > 0) screen_tables(cfg, out, orders, s);
     float* mn = (m_real == max_) ? nmax_ : max_;
     int* bn = (b_real == best_) ? nbest_ : best_;
     for (int k = 0; k < runs; ++k) {
@@ -526,10 +686,21 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         stage_.build(sub_, static_cast<long long>(plane), t.data(), prm, filt_, static_cast<long long>(plane), nullptr,
                      nullptr, nullptr, nullptr, s);
         lap("null rebuild");
-        pass(t, mn, bn);
+        const PlanPtr held = pass(t, mn, bn);
         lap("stack (null)");
         const PeakList npk = peaks(mn, bn, 3.0f, nms, edge, 20000, s);
-        out.pooled.insert(out.pooled.end(), npk.snr.begin(), npk.snr.end());
+        // the artefact tests read the frames the pass stacked: fp32 as built, or the
+        // plan's 16-bit frames
+        ScreenPass sp{filt_, static_cast<long long>(plane), w, kStoreF32, nullptr, k, &npk};
+        if (cfg.storage != kStoreF32) {
+            const uint16_t* origin = nullptr;
+            if (!plan_stored_frames(held.get(), &origin, &sp.fstride, &sp.pitch, &sp.scale))
+                throw std::runtime_error("null screen: the pass stored no 16-bit frames");
+            sp.frames = origin;
+            sp.storage = cfg.storage;
+        }
+        const auto art = screen_passes(cfg, {sp}, s, out);
+        pool_null(npk, art[0], out);
         lap("peaks (null)");
     }
     finish(cfg, runs, nv, out, s);
@@ -632,13 +803,12 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     for (auto& p : null_time_perms(n, runs, cfg.seed)) perms.push_back(std::move(p));
     std::vector<int> all_perm, all_rank(static_cast<size_t>(passes) * n);
     for (const auto& p : perms) all_perm.insert(all_perm.end(), p.begin(), p.end());
-    {
-        std::vector<double> t(static_cast<size_t>(n));
-        for (int p = 0; p < passes; ++p) {
-            for (int i = 0; i < n; ++i)
-                t[static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
-            time_ranks(t.data(), n, all_rank.data() + static_cast<size_t>(p) * n);
-        }
+    std::vector<std::vector<double>> ptimes(static_cast<size_t>(passes), std::vector<double>(static_cast<size_t>(n)));
+    for (int p = 0; p < passes; ++p) {
+        std::vector<double>& t = ptimes[static_cast<size_t>(p)];
+        for (int i = 0; i < n; ++i)
+            t[static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
+        time_ranks(t.data(), n, all_rank.data() + static_cast<size_t>(p) * n);
     }
 
     // per-run tables in one pinned block and one copy: the shared tables (the ones that
@@ -741,6 +911,9 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     // the copy stream is idle here (the previous run collected everything on it)
     scheck(hipMemcpyAsync(tab_dev_, tab_host_, total, hipMemcpyHostToDevice, copy_stream_), "upload plan tables");
     scheck(hipEventRecord(ev_upload_, copy_stream_), "record event");
+    // the null screen runs on the copy stream, after each group's peak lists
+    const bool screen = runs > 0 && cfg.null_tested > 0;
+    if (screen) screen_tables(cfg, out, ptimes, copy_stream_);
     scheck(hipStreamWaitEvent(s, ev_upload_, 0), "stream wait");
     scheck(hipStreamWaitEvent(build_stream_, ev_upload_, 0), "stream wait");
 
@@ -767,6 +940,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
 
     const float real_floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
     std::vector<PeakList> results(static_cast<size_t>(passes));
+    std::vector<std::vector<unsigned char>> arts(static_cast<size_t>(passes));
 
     auto enqueue = [&](int grp) {
         const int q = grp % nsets;
@@ -788,18 +962,20 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, bs), "permute tables");
             float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
+            // the real pass also leaves the static sky (the same for every frame order)
+            float* sky = p == 0 ? sky_ : nullptr;
             if (st != kStoreF32) {
-                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, nullptr, nullptr, nullptr,
+                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
                 scheck(launch_store(scratch, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
                                     S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n, bs),
                        "store frames");
             } else if (use_pad) {
-                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, nullptr, nullptr, nullptr,
+                stg.build_ranked(sub_, plane, rk, prm, scratch, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
                 scheck(launch_copy_rows(scratch, plane, slot + forigin, fs, pad.pitch, w, h, n, bs), "pad frames");
             } else {
-                stg.build_ranked(sub_, plane, rk, prm, slot, plane, nullptr, nullptr, nullptr,
+                stg.build_ranked(sub_, plane, rk, prm, slot, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
             }
         }
@@ -861,6 +1037,22 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
                 order_peaks(c, v.data(), v.data() + c, reinterpret_cast<const float*>(v.data() + 2 * c), v.data() + 3 * c,
                             p == 0 ? 100000 : 20000);
         }
+        if (!screen) return;
+        // the group's null runs through the artefact tests on their frames in this set,
+        // before the set takes another group
+        std::vector<ScreenPass> sp;
+        for (int g = 0; g < cnt; ++g) {
+            const int p = p0 + g;
+            if (p == 0) continue;
+            float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
+            const void* fr = st == kStoreF32 ? static_cast<const void*>(slot + forigin)
+                                             : static_cast<const void*>(reinterpret_cast<uint16_t*>(slot) + forigin);
+            sp.push_back(ScreenPass{fr, ffloats * 4 / ebytes, use_pad ? pad.pitch : w, st,
+                                    st == kStoreF32 ? nullptr : S.scl + static_cast<long long>(g) * n, p,
+                                    &results[static_cast<size_t>(p)]});
+        }
+        const auto fl = screen_passes(cfg, sp, copy_stream_, out);
+        for (size_t k = 0; k < sp.size(); ++k) arts[static_cast<size_t>(sp[k].pass)] = fl[k];
     };
     for (int grp = 0; grp < groups; ++grp) {
         enqueue(grp);
@@ -874,8 +1066,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     lap("passes (batched)");
 
     out.peaks = std::move(results[0]);
-    for (int p = 1; p < passes; ++p)
-        out.pooled.insert(out.pooled.end(), results[static_cast<size_t>(p)].snr.begin(), results[static_cast<size_t>(p)].snr.end());
+    for (int p = 1; p < passes; ++p) pool_null(results[static_cast<size_t>(p)], arts[static_cast<size_t>(p)], out);
     finish(cfg, runs, nv, out, s);
     lap("threshold fit");
     return out;

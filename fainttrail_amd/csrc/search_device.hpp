@@ -45,6 +45,14 @@ struct SearchConfig {
     // Frame storage of the stacking passes: 0 fp32 (the exact form), 1 fp16, 2 bf16
     // (a power-of-two scale per frame, fp32 accumulation).
     int storage = 0;
+    // Null-peak screen (gauss null model): the strongest null_tested peaks of each null
+    // run go through the artefact tests at their grid position and motion and artefacts
+    // stay out of the pooled peaks (0 = test none); max_frame_fraction is the
+    // single-frame test's limit. reg_dx / reg_dy: the frames' registration offsets
+    // (detector = registered + offset), one per frame, or empty (no detector test).
+    int null_tested = 3000;
+    double max_frame_fraction = 0.25;
+    std::vector<double> reg_dx, reg_dy;
 };
 
 struct PeakList {
@@ -59,7 +67,8 @@ struct SearchOutput {
     double dv = 0.0, t_ref_mjd = 0.0, span_h = 0.0;
     std::vector<double> t_h;
     std::vector<std::pair<double, double>> grid;
-    std::vector<float> pooled;  // null peak S/N values, run after run
+    std::vector<float> pooled;  // null peak S/N values, run after run (artefacts left out)
+    int screened = 0, artefacts = 0;  // null peaks through the artefact tests, and rejected
     PeakList peaks;             // real-pass peaks, sorted, at most 100000
     PeakList selected;          // peaks at or above the threshold, at most max_candidates
     std::vector<std::pair<std::string, double>> timings;
@@ -114,6 +123,23 @@ private:
     SearchOutput run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
                              float* d_max_snr, int* d_best_v, hipStream_t s);
     void finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s);
+    // The null-peak screen of one or more passes: the artefact flags of the first
+    // min(count, null_tested) peaks of each pass, tested on that pass's frames.
+    struct ScreenPass {
+        const void* frames;      // frame i element (x, y) at frames + i fstride + y pitch + x
+        long long fstride;
+        int pitch;
+        int storage;
+        const float* scale;      // 16-bit storage: per-frame scales (device)
+        int pass;                // index of the pass's frame times in the run's table
+        const PeakList* peaks;
+    };
+    std::vector<std::vector<unsigned char>> screen_passes(const SearchConfig& cfg, const std::vector<ScreenPass>& passes,
+                                                          hipStream_t s, SearchOutput& out);
+    // The run's screen tables (velocities, sqrt(c), detector steps, registration
+    // offsets, frame times of every pass), uploaded before the first screen of a run.
+    void screen_tables(const SearchConfig& cfg, const SearchOutput& out, const std::vector<std::vector<double>>& times,
+                       hipStream_t s);
     PeakList peaks(const float* m, const int* b, float floor, int radius, int edge, size_t max_peaks, hipStream_t s);
     DeviceStage stage_;
     float* sub_ = nullptr;
@@ -124,6 +150,18 @@ private:
     int* nbest_ = nullptr;
     int* pk_ = nullptr;  // count + 4 arrays
     size_t pk_cap_ = 0;
+    float* sky_ = nullptr;  // the run's static sky (the screen's blank test)
+    // null screen: run tables (device + pinned staging), peaks and flags
+    double* scr_tab_ = nullptr;
+    double* scr_tab_host_ = nullptr;
+    size_t scr_tab_cap_ = 0;
+    int* scr_pk_ = nullptr;
+    int* scr_pk_host_ = nullptr;
+    unsigned char* scr_art_ = nullptr;
+    unsigned char* scr_art_host_ = nullptr;
+    size_t scr_cap_ = 0;
+    int scr_nv_ = 0, scr_nsteps_ = 0, scr_edge_r_ = 0;
+    bool scr_reg_ = false;
     PassSet sets_[2];
     hipStream_t build_stream_ = nullptr, copy_stream_ = nullptr;
     // more build lanes for small frames (each its own stage scratch, build target and
@@ -160,6 +198,8 @@ std::vector<std::vector<int>> null_time_perms(int n, int runs, uint64_t seed);
 // find_peaks' host order (search.cpp:105-135) of c device peaks: row-major, then a
 // std::sort by descending S/N, then truncation to max_peaks.
 PeakList order_peaks(int c, const int* x, const int* y, const float* snr, const int* vidx, size_t max_peaks);
+// pool_null_peaks (search.cpp:690-693): the run's peaks in order, artefacts left out.
+void pool_null(const PeakList& pk, const std::vector<unsigned char>& art, SearchOutput& out);
 void set_threshold(const SearchConfig& cfg, int width, int height, double fwhm, size_t n_motions, int runs,
                    const std::vector<float>& pooled, SearchOutput& out, const double* device_sums);
 

@@ -19,9 +19,11 @@ double clock_s() {
 double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage) {
     const double P = static_cast<double>(w) * h;
     const double pk_cap = std::max(65536.0, P / 8.0);
-    // DeviceSearch: background-subtracted and filtered frames, four maps, peak buffer;
-    // DeviceStage: residuals, flags, sky / scatter / mask planes, small tables
-    double base = 8.0 * n * P + 16.0 * P + 16.0 * pk_cap + 5.0 * n * P + 12.0 * P + 1048576.0 + 65536.0 * n;
+    // DeviceSearch: background-subtracted and filtered frames, four maps and the static
+    // sky, peak buffer, null-screen tables; DeviceStage: residuals, flags, sky / scatter /
+    // mask planes, small tables
+    double base = 8.0 * n * P + 20.0 * P + 16.0 * pk_cap + 5.0 * n * P + 12.0 * P + 1048576.0 + 65536.0 * n +
+                  16.0 * 3000.0 * passes;
     // small frames get more build lanes: stage scratch and a build target each
     base += (build_lanes(n, static_cast<long long>(w) * h, passes) - 1) * (9.0 * n * P + 12.0 * P + 65536.0 * n);
     // one pass of a group: frames (a padded layout is at most twice a frame for fp32;

@@ -7,7 +7,8 @@ handling (stable sort by MJD, src/c_api.cpp:178-180).
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -23,7 +24,9 @@ class _Params(C.Structure):
                 ("min_coverage", C.c_double), ("threshold_snr", C.c_double), ("min_snr_floor", C.c_double),
                 ("null_runs", C.c_int), ("target_false_per_run", C.c_double), ("max_candidates", C.c_int),
                 ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int),
-                ("passes_per_launch", C.c_int), ("segments", C.c_int), ("storage", C.c_int)]
+                ("passes_per_launch", C.c_int), ("segments", C.c_int), ("storage", C.c_int),
+                ("null_tested", C.c_int), ("max_frame_fraction", C.c_double),
+                ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double))]
 
 
 class _Summary(C.Structure):
@@ -33,7 +36,8 @@ class _Summary(C.Structure):
                 ("n_selected", C.c_int), ("n_pooled", C.c_int), ("t_prepare", C.c_double),
                 ("t_stack_real", C.c_double), ("t_null_rebuild", C.c_double), ("t_stack_null", C.c_double),
                 ("t_peaks", C.c_double), ("t_threshold", C.c_double), ("t_plan", C.c_double),
-                ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int)]
+                ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int),
+                ("n_screened", C.c_int), ("n_artefacts", C.c_int)]
 
 
 METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"]
@@ -88,16 +92,36 @@ class SearchParams:
     # frame storage of the stacking passes: "f32" (exact), "f16" or "bf16" (a
     # power-of-two scale per frame, fp32 accumulation)
     storage: str = "f32"
+    # null-peak screen: the strongest null_tested peaks of each null run go through the
+    # artefact tests (0 = none); max_frame_fraction is the single-frame test's limit
+    null_tested: int = 3000
+    max_frame_fraction: float = 0.25
+    # registration offsets of the frames (detector = registered + offset), one per
+    # frame, or None: with them the detector-fixed test runs
+    reg_dx: Optional[Sequence[float]] = None
+    reg_dy: Optional[Sequence[float]] = None
 
     def c(self) -> _Params:
         from . import STORAGES
         if self.storage not in STORAGES:
             raise FtkError(f"unknown frame storage '{self.storage}' (f32, f16, bf16)")
-        return _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
-                       self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
-                       self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
-                       self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
-                       self.passes_per_launch, self.segments, STORAGES[self.storage])
+        if (self.reg_dx is None) != (self.reg_dy is None):
+            raise FtkError("give both registration offsets (reg_dx, reg_dy) or none")
+        p = _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
+                    self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
+                    self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
+                    self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
+                    self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
+                    self.max_frame_fraction)
+        if self.reg_dx is not None:
+            rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
+            ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)
+            if rx.shape != ry.shape or rx.ndim != 1:
+                raise FtkError("registration offsets: one value per frame in each of reg_dx, reg_dy")
+            p.reg_dx = rx.ctypes.data_as(C.POINTER(C.c_double))
+            p.reg_dy = ry.ctypes.data_as(C.POINTER(C.c_double))
+            p._keep = (rx, ry)  # the arrays live as long as the struct
+        return p
 
 
 @dataclass
@@ -127,6 +151,8 @@ class SearchResult:
     best_v: object = None
     passes_per_launch: int = 1
     segments: int = 0
+    null_screened: int = 0   # null peaks through the artefact tests
+    null_artefacts: int = 0  # of those, left out of null_peaks
 
 
 class Searcher:
@@ -194,7 +220,7 @@ class Searcher:
             {"prepare": summ.t_prepare, "stack (real)": summ.t_stack_real, "null rebuild": summ.t_null_rebuild,
              "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold,
              "plan": summ.t_plan, "passes (batched)": summ.t_passes},
-            mx, bx, summ.passes_per_launch, summ.segments)
+            mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -217,4 +243,8 @@ def search(frames, mjd, params: SearchParams, device: int = 0) -> SearchResult:
     else:
         dev = torch.from_numpy(np.ascontiguousarray(np.asarray(frames, np.float32)[order])).to(f"cuda:{device}")
     n, h, w = dev.shape
+    if params.reg_dx is not None and params.reg_dy is not None:
+        # the offsets belong to the frames: same order
+        params = replace(params, reg_dx=np.asarray(params.reg_dx, np.float64)[order],
+                         reg_dy=np.asarray(params.reg_dy, np.float64)[order])
     return Searcher(n, w, h, device).run(dev, m[order], params)

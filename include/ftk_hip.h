@@ -171,9 +171,9 @@ int ftk_stage_build(ftk_stage* stage, const float* d_sub, size_t sub_stride, con
  * prepare (background + build_stage), the real stacking pass, find_peaks, null_runs
  * time-shuffled null runs (with_times rebuild + pass + peaks, search.cpp:761-782), the
  * null-tail threshold (set_threshold, :573-612) and select_peaks (:614-620), with the
- * frames resident on the device from the first step to the last. Stops before
- * refine/finish_candidates; the null peaks are not screened for artefacts (the
- * reference's SearchParams::null_tested = 0). */
+ * frames resident on the device from the first step to the last, the strongest peaks
+ * of each null run screened for artefacts on the device (is_artefact_peak,
+ * search.cpp:229-243, 773-778). Stops before refine/finish_candidates. */
 typedef struct ftk_search_params {
     double vmin_pxh, vmax_pxh, grid_tol_px;
     double fwhm_px; /* PSF FWHM, must be > 0 (estimated once, outside this path) */
@@ -197,6 +197,18 @@ typedef struct ftk_search_params {
     /* Frame storage of the stacking passes: FTK_STORE_F32 (the exact form), or
      * FTK_STORE_F16 / FTK_STORE_BF16 (see the precision form below). */
     int storage;
+    /* SearchParams::null_tested (default 3000; 0 = test none): the strongest null
+     * peaks of each null run that go through the artefact tests (single frame, field
+     * edge, detector-fixed) at their grid position and motion; artefacts stay out of
+     * the null tail fit. With 16-bit storage the tests read the decoded frames. */
+    int null_tested;
+    double max_frame_fraction; /* the single-frame test's limit (default 0.25) */
+    /* Registration offsets of the frames (n host doubles each, detector = registered +
+     * offset; Frame::reg_dx / reg_dy), or NULL: with them the detector-fixed test
+     * runs (search.cpp:196-225), as in the reference when every frame carries one.
+     * Copied by the call. */
+    const double* reg_dx;
+    const double* reg_dy;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -211,6 +223,7 @@ typedef struct ftk_search_summary {
      * count the run used */
     double t_plan, t_passes;
     int passes_per_launch, segments;
+    int n_screened, n_artefacts; /* null peaks through the artefact tests; of those, left out */
 } ftk_search_summary;
 
 typedef struct ftk_searcher ftk_searcher;

@@ -94,7 +94,7 @@ def ref() -> C.CDLL:
         _r.ref_check_backend_inputs.argtypes = [C.c_int, I32P, F32P, F64P, F64P, C.c_int]
         _r.ref_linear_shifts.argtypes = [C.c_int, F64P, C.c_int, F64P, I32P, I32P]
         _r.ref_search.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, F64P, F32P,
-                                  C.c_int, F32P, C.c_char_p]
+                                  C.c_int, F32P, C.c_char_p, C.c_int, F64P]
     return _r
 
 
@@ -307,9 +307,11 @@ def ref_check_backend_inputs(round_: int):
 
 
 def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, threshold=0.0, floor=5.0, target=0.02,
-               backend="cpu"):
-    """The reference's search() (null_tested = 0) on prepare(align off), with the given
-    stacking backend ("cpu", or "hip" through the linked adapter)."""
+               backend="cpu", null_tested=3000, reg=None):
+    """The reference's search() on prepare(align off), with the given stacking backend
+    ("cpu", or "hip" through the linked adapter), null_tested (the SearchParams
+    default 3000 screens the strongest null peaks; 0 screens none) and optional
+    registration offsets reg = (reg_dx, reg_dy), one per frame each."""
     if backend == "hip":
         import fainttrail_amd  # one HIP runtime: torch's, loaded before the adapter
         fainttrail_amd.lib()
@@ -318,31 +320,36 @@ def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, thresh
     m = f64(mjd)
     prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
     out = np.zeros(6, np.float64)
+    rg = None
+    if reg is not None:
+        rg = f64(np.concatenate([np.asarray(reg[0], np.float64), np.asarray(reg[1], np.float64)]))
+        assert rg.size == 2 * n
+    rgp = _p(rg, F64P) if rg is not None else None
     cnt = ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
-                           _p(out, F64P), None, 0, None, backend.encode())
+                           _p(out, F64P), None, 0, None, backend.encode(), null_tested, rgp)
     pooled = np.empty(max(cnt, 1), np.float32)
     mx = np.empty((h, w), np.float32)
     ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), _p(out, F64P),
-                     _p(pooled, F32P), cnt, _p(mx, F32P), backend.encode())
+                     _p(pooled, F32P), cnt, _p(mx, F32P), backend.encode(), null_tested, rgp)
     return {"threshold": out[0], "null_a": out[1], "null_k": out[2], "null_inv_s2": out[3], "null_b": out[4],
             "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}
 
 
 def ref_search_tiled(frames, mjd, vmin, vmax, tol, fwhm, null_runs, tile, seed=12345, threshold=0.0, floor=5.0,
-                     target=0.02):
-    """The reference's tiled search (search_tiled, null_tested = 0) with core tiles of `tile` px."""
+                     target=0.02, null_tested=3000):
+    """The reference's tiled search (search_tiled) with core tiles of `tile` px."""
     L = ref()
     L.ref_search_tiled.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, C.c_int, F64P,
-                                   F32P, C.c_int, F32P]
+                                   F32P, C.c_int, F32P, C.c_int]
     fr = f32(frames)
     n, h, w = fr.shape
     m = f64(mjd)
     prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
     out = np.zeros(6, np.float64)
     cnt = L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
-                             _p(out, F64P), None, 0, None)
+                             _p(out, F64P), None, 0, None, null_tested)
     pooled = np.empty(max(cnt, 1), np.float32)
     mx = np.empty((h, w), np.float32)
     L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
-                       _p(out, F64P), _p(pooled, F32P), cnt, _p(mx, F32P))
+                       _p(out, F64P), _p(pooled, F32P), cnt, _p(mx, F32P), null_tested)
     return {"threshold": out[0], "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}

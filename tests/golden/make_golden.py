@@ -100,7 +100,7 @@ def main() -> None:
     fr, mjd, _ = synth.make_sequence(12, 96, 64, nstars=12, nmovers=2, mover_snr=(3.0, 3.0), seed=21)
     fr = fr.numpy()
     args = np.array([12.0, 66.0, 0.5, 3.53])
-    r = po.ref_search(fr, mjd, *args, null_runs=4, seed=77)
+    r = po.ref_search(fr, mjd, *args, null_runs=4, seed=77, null_tested=0)
     data["search/frames"], data["search/mjd"], data["search/args"] = fr, mjd, args
     data["search/runs"] = np.int32(4)
     data["search/threshold"] = np.float64(r["threshold"])

@@ -1,7 +1,9 @@
-"""The device-resident search against the reference's own search() (null_tested = 0):
-the real pass's max-S/N map, every pooled null-peak S/N value and the calibrated
-threshold (an FP64 value) must be identical; with the tail-fit sums on the GPU the
-threshold agrees to 1e-12 relative (device log and summation order differ)."""
+"""The device-resident search against the reference's own search() (its defaults: the
+strongest 3000 peaks of each null run screened for artefacts): the real pass's max-S/N
+map, every pooled null-peak S/N value and the calibrated threshold (an FP64 value) must
+be identical; with the tail-fit sums on the GPU the threshold agrees to 1e-12 relative
+(device log and summation order differ). The golden fixture holds a search with no
+screen (null_tested = 0)."""
 import numpy as np
 import pytest
 
@@ -20,7 +22,7 @@ def test_search_matches_reference_golden(hip, golden):
     n, h, w = fr.shape
     res = Searcher(n, w, h).run(torch.from_numpy(fr).cuda(), mjd,
                                 SearchParams(vmin_pxh=vmin, vmax_pxh=vmax, grid_tol_px=tol, fwhm_px=fwhm,
-                                             null_runs=runs, seed=77))
+                                             null_runs=runs, seed=77, null_tested=0))
     torch.cuda.synchronize()
     assert res.n_velocities == int(golden["search/n_velocities"])
     assert same_bits(res.max_snr.cpu().numpy(), golden["search/max_snr"])
@@ -73,6 +75,79 @@ def test_search_matches_reference_tiled_search(hip, tile):
                                                   null_runs=3, seed=9))
     torch.cuda.synchronize()
     assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
+    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
+    assert got.threshold_snr == want["threshold"]
+
+
+def _screen_sequence(n=20, w=160, h=120, seed=5):
+    """A sequence whose null runs hold artefacts of every kind the screen tests for:
+    PSF-wide flashes in single frames (the single-frame share), registration borders
+    (NaN) that grow with the drift (the field edge), and registration offsets that make
+    the detector-fixed test run."""
+    from fainttrail_amd import synth
+    fr, mjd, _ = synth.make_sequence(n, w, h, nstars=40, nmovers=2, mover_snr=(3.0, 3.0), seed=seed)
+    f = fr.numpy().copy()
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    for _ in range(12):
+        i = rng.integers(n)
+        x0, y0 = rng.uniform(10, w - 10), rng.uniform(10, h - 10)
+        f[i] += 400.0 * np.exp(-((xx - x0) ** 2 + (yy - y0) ** 2) / (2 * 1.5 ** 2))
+    for i in range(n):
+        f[i, :, : 2 + i // 3] = np.nan
+        f[i, : 1 + i // 4, :] = np.nan
+    reg = (0.35 * np.arange(n) - 3.0, -0.25 * np.arange(n) + 2.0)
+    return f, mjd, reg
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("with_reg", [False, True])
+@pytest.mark.parametrize("group", [1, 0, 2])
+def test_null_screen_matches_reference(hip, with_reg, group):
+    """is_artefact_peak on the device (src/search.cpp:229-243, 773-778): on a sequence
+    where the screen leaves out about half of the null peaks (426 -> 228 in the
+    reference, 200 with the detector test) and moves the threshold from 15.3 to 5.7,
+    the pooled null peaks and the threshold equal the reference's, one pass at a time
+    and batched, with and without registration offsets."""
+    import torch
+    from fainttrail_amd.search import SearchParams, Searcher
+    f, mjd, reg = _screen_sequence()
+    n, h, w = f.shape
+    rg = reg if with_reg else None
+    want = po.ref_search(f, mjd, 12.0, 66.0, 0.5, 3.53, 4, seed=9, reg=rg)
+    none = po.ref_search(f, mjd, 12.0, 66.0, 0.5, 3.53, 4, seed=9, null_tested=0)
+    print(f"null screen group {group} reg {with_reg}: reference pools {len(want['pooled'])} of "
+          f"{len(none['pooled'])}, threshold {want['threshold']}", flush=True)
+    assert len(want["pooled"]) < 0.7 * len(none["pooled"])
+    prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53, null_runs=4, seed=9,
+                       passes_per_launch=group, reg_dx=rg[0] if rg else None, reg_dy=rg[1] if rg else None)
+    got = Searcher(n, w, h).run(torch.from_numpy(f).cuda(), mjd, prm)
+    torch.cuda.synchronize()
+    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
+    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
+    assert got.threshold_snr == want["threshold"]
+    assert got.null_artefacts == len(none["pooled"]) - len(want["pooled"])
+    assert got.null_screened == len(none["pooled"])  # fewer than 3000 peaks per run: all tested
+    # null_tested = 0 is the unscreened search
+    prm.null_tested = 0
+    got0 = Searcher(n, w, h).run(torch.from_numpy(f).cuda(), mjd, prm)
+    assert np.array_equal(np.sort(got0.null_peaks), np.sort(none["pooled"]))
+    assert got0.threshold_snr == none["threshold"] and got0.null_artefacts == 0
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+def test_null_screen_limit(hip):
+    """null_tested below the run's peak count: only the strongest null_tested peaks of
+    each run (in find_peaks order) are tested (null_tested_count, search.cpp:695-698)."""
+    import torch
+    from fainttrail_amd.search import SearchParams, Searcher
+    f, mjd, _ = _screen_sequence(seed=6)
+    n, h, w = f.shape
+    want = po.ref_search(f, mjd, 12.0, 66.0, 0.5, 3.53, 3, seed=2, null_tested=25)
+    got = Searcher(n, w, h).run(torch.from_numpy(f).cuda(), mjd,
+                                SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53,
+                                             null_runs=3, seed=2, null_tested=25))
+    assert got.null_screened == 3 * 25
     assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
     assert got.threshold_snr == want["threshold"]
 
