@@ -7,20 +7,27 @@
 
 namespace ftk {
 
+// One pass's frames: element (x, y) of frame i at frames + i fstride + y pitch + x
+// (fp32, or 16-bit times scale[i]), and the pass's n frame times (hours).
+struct ScreenFrames {
+    const void* frames;
+    long long fstride;
+    int pitch;
+    int unused;
+    const float* scale;
+    const double* t;
+};
+
+// One launch screens the peaks of several passes: peak k belongs to passes[pslot[k]].
 struct ScreenArgs {
-    // the pass's frames: element (x, y) of frame i at frames + i fstride + y pitch + x
-    // (fp32, or 16-bit times scale[i])
-    const void* frames = nullptr;
-    long long fstride = 0;
-    int pitch = 0;
+    const ScreenFrames* passes = nullptr;  // device table
+    const int* pslot = nullptr;
     int storage = 0;
-    const float* scale = nullptr;
     // background-subtracted frames (w x h each, sub_stride floats apart) and the static
     // sky: where either is non-finite the field is blank
     const float* sub = nullptr;
     long long sub_stride = 0;
     const float* sky = nullptr;
-    const double* t = nullptr;       // n frame times of the pass (hours)
     const double* reg_dx = nullptr;  // n registration offsets each, or null (no detector test)
     const double* reg_dy = nullptr;
     const double* vel = nullptr;     // (vx, vy) per velocity index

@@ -437,7 +437,8 @@ void DeviceSearch::screen_tables(const SearchConfig& cfg, const SearchOutput& ou
 }
 
 // null_tested_count and the artefact tests of search.cpp:773-778 for several passes
-// in one upload, one launch per pass and one download; returns the flags per pass.
+// in one upload (a frames descriptor per pass, then the peaks), one launch and one
+// download; returns the flags per pass.
 std::vector<std::vector<unsigned char>> DeviceSearch::screen_passes(const SearchConfig& cfg,
                                                                     const std::vector<ScreenPass>& sp, hipStream_t s,
                                                                     SearchOutput& out) {
@@ -446,74 +447,76 @@ std::vector<std::vector<unsigned char>> DeviceSearch::screen_passes(const Search
     for (size_t k = 0; k < sp.size(); ++k) {
         const size_t c = cfg.null_tested > 0 ? std::min(sp[k].peaks->x.size(), static_cast<size_t>(cfg.null_tested)) : 0;
         off[k + 1] = off[k] + c;
+        if (sp[k].storage != sp[0].storage) throw std::runtime_error("null screen: passes of one call share a storage");
     }
     const size_t total = off.back();
     if (total == 0) return flags;
-    if (total > scr_cap_) {
+    const size_t desc = sizeof(ScreenFrames) * sp.size();
+    const size_t bytes = desc + sizeof(int) * 4 * total;
+    if (bytes > scr_cap_ || total > scr_art_cap_) {
         for (void* p : {static_cast<void*>(scr_pk_), static_cast<void*>(scr_art_)})
             if (p) scheck(hipFree(p), "free screen buffers");
         for (void* p : {static_cast<void*>(scr_pk_host_), static_cast<void*>(scr_art_host_)})
             if (p) scheck(hipHostFree(p), "free screen buffers");
         scr_pk_ = scr_pk_host_ = nullptr;
         scr_art_ = scr_art_host_ = nullptr;
-        scr_cap_ = 0;
-        const size_t cap = std::max<size_t>(total + total / 4, 4096);
-        scr_pk_ = salloc<int>(3 * cap, "allocate screen buffers");
-        scr_art_ = salloc<unsigned char>(cap, "allocate screen buffers");
-        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_pk_host_), sizeof(int) * 3 * cap, hipHostMallocDefault),
-               "allocate screen buffers");
-        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_art_host_), cap, hipHostMallocDefault),
+        scr_cap_ = scr_art_cap_ = 0;
+        const size_t cap = std::max<size_t>(bytes + bytes / 4, 65536);
+        const size_t acap = std::max<size_t>(total + total / 4, 4096);
+        scr_pk_ = salloc<char>(cap, "allocate screen buffers");
+        scr_art_ = salloc<unsigned char>(acap, "allocate screen buffers");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_pk_host_), cap, hipHostMallocDefault), "allocate screen buffers");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&scr_art_host_), acap, hipHostMallocDefault),
                "allocate screen buffers");
         scr_cap_ = cap;
+        scr_art_cap_ = acap;
     }
-    for (size_t k = 0; k < sp.size(); ++k) {
-        const PeakList& pk = *sp[k].peaks;
-        const size_t c = off[k + 1] - off[k];
-        std::copy(pk.x.begin(), pk.x.begin() + c, scr_pk_host_ + off[k]);
-        std::copy(pk.y.begin(), pk.y.begin() + c, scr_pk_host_ + total + off[k]);
-        std::copy(pk.vidx.begin(), pk.vidx.begin() + c, scr_pk_host_ + 2 * total + off[k]);
-    }
-    scheck(hipMemcpyAsync(scr_pk_, scr_pk_host_, sizeof(int) * 3 * total, hipMemcpyHostToDevice, s), "upload peaks");
     const double* vel = scr_tab_;
     const double* sqrtc = vel + 2 * static_cast<size_t>(scr_nv_);
     const double* steps = sqrtc + (n + 1);
     const double* reg = steps + scr_nsteps_;
     const double* times = scr_reg_ ? reg + 2 * static_cast<size_t>(n) : reg;
+    auto* hdesc = reinterpret_cast<ScreenFrames*>(scr_pk_host_);
+    int* hint = reinterpret_cast<int*>(scr_pk_host_ + desc);
     for (size_t k = 0; k < sp.size(); ++k) {
-        const size_t c = off[k + 1] - off[k];
-        if (c == 0) continue;
-        for (size_t j = 0; j < c; ++j) {
-            const int v = sp[k].peaks->vidx[j];
+        hdesc[k] = ScreenFrames{sp[k].frames, sp[k].fstride, sp[k].pitch, 0, sp[k].scale,
+                                times + static_cast<size_t>(sp[k].pass) * n};
+        const PeakList& pk = *sp[k].peaks;
+        for (size_t j = 0; j < off[k + 1] - off[k]; ++j) {
+            const int v = pk.vidx[j];
             if (v < 0 || v >= scr_nv_) throw std::runtime_error("null screen: velocity index out of range");
+            hint[off[k] + j] = pk.x[j];
+            hint[total + off[k] + j] = pk.y[j];
+            hint[2 * total + off[k] + j] = v;
+            hint[3 * total + off[k] + j] = static_cast<int>(k);
         }
-        ScreenArgs a;
-        a.frames = sp[k].frames;
-        a.fstride = sp[k].fstride;
-        a.pitch = sp[k].pitch;
-        a.storage = sp[k].storage;
-        a.scale = sp[k].scale;
-        a.sub = sub_;
-        a.sub_stride = static_cast<long long>(w) * h;
-        a.sky = sky_;
-        a.t = times + static_cast<size_t>(sp[k].pass) * n;
-        a.reg_dx = scr_reg_ ? reg : nullptr;
-        a.reg_dy = scr_reg_ ? reg + n : nullptr;
-        a.vel = vel;
-        a.sqrtc = sqrtc;
-        a.steps = steps;
-        a.nsteps = scr_nsteps_;
-        a.px = scr_pk_ + off[k];
-        a.py = scr_pk_ + total + off[k];
-        a.pv = scr_pk_ + 2 * total + off[k];
-        a.count = static_cast<int>(c);
-        a.art = scr_art_ + off[k];
-        a.n = n;
-        a.w = w;
-        a.h = h;
-        a.edge_r = scr_edge_r_;
-        a.max_frame_fraction = cfg.max_frame_fraction;
-        scheck(launch_screen(a, s), "null screen");
     }
+    scheck(hipMemcpyAsync(scr_pk_, scr_pk_host_, bytes, hipMemcpyHostToDevice, s), "upload peaks");
+    const int* dint = reinterpret_cast<const int*>(scr_pk_ + desc);
+    ScreenArgs a;
+    a.passes = reinterpret_cast<const ScreenFrames*>(scr_pk_);
+    a.pslot = dint + 3 * total;
+    a.storage = sp[0].storage;
+    a.sub = sub_;
+    a.sub_stride = static_cast<long long>(w) * h;
+    a.sky = sky_;
+    a.reg_dx = scr_reg_ ? reg : nullptr;
+    a.reg_dy = scr_reg_ ? reg + n : nullptr;
+    a.vel = vel;
+    a.sqrtc = sqrtc;
+    a.steps = steps;
+    a.nsteps = scr_nsteps_;
+    a.px = dint;
+    a.py = dint + total;
+    a.pv = dint + 2 * total;
+    a.count = static_cast<int>(total);
+    a.art = scr_art_;
+    a.n = n;
+    a.w = w;
+    a.h = h;
+    a.edge_r = scr_edge_r_;
+    a.max_frame_fraction = cfg.max_frame_fraction;
+    scheck(launch_screen(a, s), "null screen");
     scheck(hipMemcpyAsync(scr_art_host_, scr_art_, total, hipMemcpyDeviceToHost, s), "download screen flags");
     scheck(hipStreamSynchronize(s), "null screen");
     for (size_t k = 0; k < sp.size(); ++k) {

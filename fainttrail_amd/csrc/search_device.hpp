@@ -155,11 +155,11 @@ private:
     double* scr_tab_ = nullptr;
     double* scr_tab_host_ = nullptr;
     size_t scr_tab_cap_ = 0;
-    int* scr_pk_ = nullptr;
-    int* scr_pk_host_ = nullptr;
+    char* scr_pk_ = nullptr;       // frames descriptors, then x, y, velocity index, pass of each peak
+    char* scr_pk_host_ = nullptr;
     unsigned char* scr_art_ = nullptr;
     unsigned char* scr_art_host_ = nullptr;
-    size_t scr_cap_ = 0;
+    size_t scr_cap_ = 0, scr_art_cap_ = 0;
     int scr_nv_ = 0, scr_nsteps_ = 0, scr_edge_r_ = 0;
     bool scr_reg_ = false;
     PassSet sets_[2];
