@@ -96,3 +96,35 @@ def test_product_velocity_grid_matches_reference(golden, key):
     span, vmin, vmax, tol = golden[f"{key}/args"]
     g, dv = ft.velocity_grid(span, vmin, vmax, tol)
     assert np.array_equal(g, golden[f"{key}/vel"]) and dv == golden[f"{key}/dv"]
+
+
+def test_search_params_defaults_are_the_reference_defaults():
+    """ftk_search_params_default and the Python SearchParams carry SearchParams'
+    defaults (include/fainttrail/search.hpp:18-90): among them null_tested = 3000 and
+    max_frame_fraction = 0.25 (:63-65), no registration offsets (fwhm_px = 3: the PSF
+    estimate runs outside this path)."""
+    from fainttrail_amd.search import SearchParams, _Params
+    lib = ctypes.CDLL(str(ROOT / "fainttrail_amd" / "libftk_hip.so"))
+    lib.ftk_search_params_default.argtypes = [ctypes.POINTER(_Params)]
+    c = _Params()
+    lib.ftk_search_params_default(ctypes.byref(c))
+    want = {"vmin_pxh": 4.0, "vmax_pxh": 40.0, "grid_tol_px": 0.5, "clip_sigma": 8.0, "cr_sigma": 5.0,
+            "static_exclude": -1, "mask_scatter": 2.5, "weight_scatter": 1.5, "min_coverage": 0.7,
+            "threshold_snr": 0.0, "min_snr_floor": 5.0, "null_runs": 3, "target_false_per_run": 0.02,
+            "max_candidates": 300, "edge_px": 4, "seed": 12345, "null_tested": 3000, "max_frame_fraction": 0.25}
+    py = SearchParams().c()
+    for k, v in want.items():
+        assert getattr(c, k) == v, k
+        assert getattr(py, k) == v, k
+    assert not c.reg_dx and not c.reg_dy and not py.reg_dx and not py.reg_dy
+
+
+def test_search_params_registration_offsets():
+    from fainttrail_amd.search import SearchParams
+    p = SearchParams(reg_dx=[0.5, 1.5, 2.5], reg_dy=[-1.0, 0.0, 1.0]).c()
+    assert [p.reg_dx[i] for i in range(3)] == [0.5, 1.5, 2.5]
+    assert [p.reg_dy[i] for i in range(3)] == [-1.0, 0.0, 1.0]
+    with pytest.raises(ft.FtkError):
+        SearchParams(reg_dx=[0.0, 1.0]).c()
+    with pytest.raises(ft.FtkError):
+        SearchParams(reg_dx=[0.0, 1.0], reg_dy=[0.0]).c()
