@@ -331,7 +331,10 @@ DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_
         scheck(hipEventCreateWithFlags(&ev_built_[q], hipEventDisableTiming), "create event");
         scheck(hipEventCreateWithFlags(&ev_stacked_[q], hipEventDisableTiming), "create event");
         scheck(hipEventCreateWithFlags(&ev_counts_[q], hipEventDisableTiming), "create event");
+        scheck(hipEventCreateWithFlags(&ev_ring_in_[q], hipEventDisableTiming), "create event");
+        scheck(hipEventCreateWithFlags(&ev_ring_out_[q], hipEventDisableTiming), "create event");
     }
+    scheck(hipStreamCreateWithFlags(&ring_stream_, hipStreamNonBlocking), "create stream");
     scheck(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming), "create event");
     scheck(hipEventCreateWithFlags(&ev_tables_, hipEventDisableTiming), "create event");
     scheck(hipEventCreateWithFlags(&ev_upload_, hipEventDisableTiming), "create event");
@@ -348,7 +351,7 @@ DeviceSearch::~DeviceSearch() {
         if (p) (void)hipHostFree(p);
     for (auto& set : sets_) set.release();
     for (int q = 0; q < 2; ++q)
-        for (hipEvent_t e : {ev_built_[q], ev_stacked_[q], ev_counts_[q]})
+        for (hipEvent_t e : {ev_built_[q], ev_stacked_[q], ev_counts_[q], ev_ring_in_[q], ev_ring_out_[q]})
             if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {ev_start_, ev_tables_, ev_upload_})
         if (e) (void)hipEventDestroy(e);
@@ -363,6 +366,7 @@ DeviceSearch::~DeviceSearch() {
         l.stage.reset();
     }
     if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
+    if (ring_stream_) (void)hipStreamDestroy(ring_stream_);
 }
 
 // find_peaks (src/search.cpp:105-135): device local maxima, then the reference's
@@ -618,6 +622,7 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
         (void)hipStreamSynchronize(s);
         (void)hipStreamSynchronize(build_stream_);
         (void)hipStreamSynchronize(copy_stream_);
+        (void)hipStreamSynchronize(ring_stream_);
         for (BuildLane& l : lanes_) (void)hipStreamSynchronize(l.stream);
         throw;
     }
@@ -994,7 +999,12 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         const StackArgs* dargs = static_cast<const StackArgs*>(dptr(i_args[q]));
         std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
         std::vector<StackArgs> hring(hargs[q].begin() + G, hargs[q].begin() + G + cnt);
-        scheck(launch_stack_multi(hin.data(), hring.data(), dargs, dargs + G, cnt, s), "launch stack kernel");
+        // ring tiles on their own stream, beside the interior launch
+        scheck(hipEventRecord(ev_ring_in_[q], s), "record event");
+        scheck(hipStreamWaitEvent(ring_stream_, ev_ring_in_[q], 0), "stream wait");
+        scheck(launch_stack_multi(hin.data(), hring.data(), dargs, dargs + G, cnt, s, ring_stream_), "launch stack kernel");
+        scheck(hipEventRecord(ev_ring_out_[q], ring_stream_), "record event");
+        scheck(hipStreamWaitEvent(s, ev_ring_out_[q], 0), "stream wait");
         scheck(launch_finalize(S.keys, cnt * plane, S.maxm, S.best, s), "finalize");
         for (int g = 0; g < cnt; ++g) {
             const int p = p0 + g;

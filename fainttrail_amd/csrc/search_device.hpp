@@ -175,6 +175,8 @@ private:
     std::vector<BuildLane> lanes_;  // lanes 1, 2, ... (lane 0: stage_, filt_, build_stream_)
     hipEvent_t ev_built_[2] = {nullptr, nullptr}, ev_stacked_[2] = {nullptr, nullptr};
     hipEvent_t ev_counts_[2] = {nullptr, nullptr};
+    hipStream_t ring_stream_ = nullptr;  // the ring tiles' launches, beside the interior ones
+    hipEvent_t ev_ring_in_[2] = {nullptr, nullptr}, ev_ring_out_[2] = {nullptr, nullptr};
     hipEvent_t ev_start_ = nullptr;
     // per-run tables of the batched form (plan rows, frame orders, time ranks, stacking
     // descriptors): packed in pinned host memory and sent in one copy to a device buffer,

@@ -671,7 +671,7 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s)
 }
 
 hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
-                              const StackArgs* dev_ring, int count, hipStream_t s) {
+                              const StackArgs* dev_ring, int count, hipStream_t s, hipStream_t ring_s) {
     if (count <= 0) return hipSuccess;
     unsigned per_in = 0, per_ring = 0;
     const int lds_rows = host_inner[0].lds_rows;
@@ -686,12 +686,13 @@ hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host
     }
     const size_t lds = lds_bytes_of(lds_rows);
     const dim3 block(kThreads);
+    // the ring tiles first: their workgroups take longer, the interior ones fill in
+    if (per_ring > 0)
+        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage, pre), dim3(per_ring * static_cast<unsigned>(count)),
+                           block, lds, ring_s ? ring_s : s, dev_ring, per_ring);
     if (per_in > 0)
         hipLaunchKernelGGL(multi_kernel(false, false, storage, pre), dim3(per_in * static_cast<unsigned>(count)), block,
                            lds, s, dev_inner, per_in);
-    if (per_ring > 0)
-        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage, pre), dim3(per_ring * static_cast<unsigned>(count)),
-                           block, lds, s, dev_ring, per_ring);
     return hipGetLastError();
 }
 

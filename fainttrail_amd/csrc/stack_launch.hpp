@@ -83,9 +83,11 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s)
 // Several passes in two launches, one over their interior tiles and one over their
 // ring tiles. host_inner / host_ring: count descriptors each, as make_stack_args
 // writes them (for the grid sizes); dev_inner / dev_ring: the same descriptors in
-// device memory. Every descriptor must have the same lds_rows.
+// device memory. Every descriptor must have the same lds_rows. ring_s (optional): the
+// ring tiles' stream, ordered by the caller (both launches merge into the same keys by
+// atomic maximum, so either order gives the same bits).
 hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
-                              const StackArgs* dev_ring, int count, hipStream_t s);
+                              const StackArgs* dev_ring, int count, hipStream_t s, hipStream_t ring_s = nullptr);
 hipError_t launch_fill_keys(unsigned long long* keys, long long count, hipStream_t s);
 hipError_t launch_finalize(const unsigned long long* keys, long long count, float* max_snr, int* best_v,
                            hipStream_t s);
