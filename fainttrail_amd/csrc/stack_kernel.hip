@@ -225,7 +225,8 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                 // uniform row pointers with a per-lane offset (the scalar-base load form);
                 // for the box's columns past 64, lanes beyond the box width re-read its
                 // last column into LDS columns that no motion reads, so every lane stays
-                // inside the box and no exec mask is needed
+                // inside the box and no exec mask is needed (a box exactly 64 wide re-reads
+                // column 63 there: the second load needs no branch)
                 // buffer form of the LDS load: descriptor at the box's first pixel, the
                 // lane's byte offset in a VGPR (fixed for the frame), the row's byte
                 // offset as a scalar, so a row costs no vector instruction
@@ -242,7 +243,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
 #pragma unroll 1
                     for (int rr = wave; rr < R; rr += kWaves) {
                         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, b1, row_b, 0, 0);
-                        if (sx > 0) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d + kTileX, 4, b2, row_b, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d + kTileX, 4, b2, row_b, 0, 0);
                         row_b += kWaves * 4 * pitch;
                         d += kWaves * kPitch;
                     }
@@ -253,8 +254,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                         if (rr >= R) break;
                         const int row_b = rr * 4 * pitch;
                         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
-                        if (sx > 0)
-                            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
                     }
                 }
             }
