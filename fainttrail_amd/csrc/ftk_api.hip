@@ -272,7 +272,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
         for (const Band& b : bands) {
             const int rows = b.r1 - b.r0, out_rows = b.y1 - b.y0;
             const size_t np = static_cast<size_t>(out_rows) * w;
-            auto t0 = clk::now();
+            const double t_up_band0 = t_up;
             plan_tiles(p, out_rows);
             // the band's frame rows inside a zero border that holds every staged box, when
             // that costs at most twice the rows' own size; else plain rows
@@ -284,25 +284,43 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             d_keys.reserve(np, "allocate keys");
             d_max.reserve(np, "allocate output");
             d_best.reserve(np, "allocate output");
-            // frame rows as one contiguous copy each (pitched copies from pageable memory
-            // go row by row); into the zero border through a device-side row copy
-            const size_t frame_floats = static_cast<size_t>(rows) * w;
+            // Frame rows go up in row chunks, each just before the first launch that reads
+            // them, so the stacking of the rows above runs while the rest uploads: every
+            // frame's rows of a chunk as one contiguous copy (pitched copies from pageable
+            // memory go row by row), into the zero border through a device-side row copy.
+            // The copies are synchronous on the null stream; the launches run on
+            // non-blocking streams, so they keep running meanwhile.
+            const size_t chunk_cap = static_cast<size_t>(rows) * w;
             if (use_pad) {
                 check(hipMemset(d_frames.p, 0, sizeof(float) * n * stride), "clear frames");
-                d_stage.reserve(frame_floats, "allocate upload buffer");
+                d_stage.reserve(chunk_cap, "allocate upload buffer");
             }
-            for (int i = 0; i < n; ++i) {
-                const float* src = pass->frames[i] + static_cast<size_t>(b.r0) * w;
-                float* dst = use_pad ? d_stage.p : d_frames.p + i * stride;
-                check(hipMemcpy(dst, src, sizeof(float) * frame_floats, hipMemcpyHostToDevice), "upload frames");
-                if (use_pad)
-                    check(launch_copy_rows(d_stage.p, static_cast<long long>(frame_floats), d_frames.p + i * stride + origin,
-                                           stride, g.pitch, w, rows, 1, nullptr),
-                          "pad frames");
-            }
-            if (use_pad) check(hipDeviceSynchronize(), "pad frames");
+            int up_to = b.r0;  // frame rows [b.r0, up_to) are on the device
+            auto upload_until = [&](int r) {
+                r = std::min(r, b.r1);
+                if (r <= up_to) return;
+                const auto u0 = clk::now();
+                const size_t cnt = static_cast<size_t>(r - up_to) * w;
+                for (int i = 0; i < n; ++i) {
+                    const float* src = pass->frames[i] + static_cast<size_t>(up_to) * w;
+                    if (use_pad) {
+                        // the previous row copy has read the stage buffer (same stream)
+                        check(hipMemcpy(d_stage.p, src, sizeof(float) * cnt, hipMemcpyHostToDevice), "upload frames");
+                        check(launch_copy_rows(d_stage.p, static_cast<long long>(cnt),
+                                               d_frames.p + i * stride + origin + static_cast<long long>(up_to - b.r0) * g.pitch,
+                                               stride, g.pitch, w, r - up_to, 1, nullptr),
+                              "pad frames");
+                    } else {
+                        check(hipMemcpy(d_frames.p + i * stride + static_cast<long long>(up_to - b.r0) * w, src,
+                                        sizeof(float) * cnt, hipMemcpyHostToDevice),
+                              "upload frames");
+                    }
+                }
+                if (use_pad) check(hipStreamSynchronize(nullptr), "pad frames");
+                up_to = r;
+                t_up += std::chrono::duration<double>(clk::now() - u0).count();
+            };
             auto t1 = clk::now();
-            t_up += std::chrono::duration<double>(t1 - t0).count();
 
             // launches of about launch_adds units: whole tile rows of the band with every
             // velocity block (all segments of a tile in one launch share its frame boxes
@@ -316,6 +334,8 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             LaunchQueue lq;
             for (int y = b.y0; y < b.y1; y += rows_per_launch) {
                 const int sub_rows = std::min(rows_per_launch, b.y1 - y);
+                // the launch's tiles read frame rows up to (its last tile row) + max_dy
+                upload_until(y + (sub_rows + kTileY - 1) / kTileY * kTileY + p.max_dy);
                 plan_tiles(p, sub_rows);
                 const double units_per_block = static_cast<double>(sub_rows) * w * n * kVel;
                 const int blocks_per_launch = static_cast<int>(
@@ -360,7 +380,8 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             check(launch_finalize(d_keys.p, static_cast<long long>(np), d_max.p, d_best.p, nullptr), "finalize");
             check(hipDeviceSynchronize(), "finalize");
             auto t2 = clk::now();
-            t_kernel += std::chrono::duration<double>(t2 - t1).count();
+            // stacking time net of the uploads it overlapped
+            t_kernel += std::chrono::duration<double>(t2 - t1).count() - (t_up - t_up_band0);
             check(hipMemcpy(max_snr + static_cast<size_t>(b.y0) * w, d_max.p, np * sizeof(float), hipMemcpyDeviceToHost),
                   "download");
             check(hipMemcpy(best_v + static_cast<size_t>(b.y0) * w, d_best.p, np * sizeof(int), hipMemcpyDeviceToHost),
