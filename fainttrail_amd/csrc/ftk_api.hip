@@ -126,7 +126,7 @@ ftk_options default_options() {
 
 // Plan tables on the device.
 struct DevicePlan {
-    DeviceBuffer<int> shift, box, ubox, lds_off, motion, cov_x, cov_y;
+    DeviceBuffer<int> shift, box, ubox, lds_off, motion, cov_x, cov_y, chain;
     DeviceBuffer<long long> cov_off;
     DeviceBuffer<float> inv_sqrt;
     void upload(const StackPlan& p) {
@@ -137,6 +137,7 @@ struct DevicePlan {
         cov_off.upload(p.cov_off, "upload coverage tables");
         cov_x.upload(p.cov_x, "upload coverage tables");
         cov_y.upload(p.cov_y, "upload coverage tables");
+        chain.upload(p.chain, "upload coverage tables");
         motion.upload(p.motion, "upload motions");
         inv_sqrt.upload(p.inv_sqrt, "upload 1/sqrt table");
     }
@@ -358,6 +359,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                     L.cov_off = dev.cov_off.p;
                     L.cov_x = dev.cov_x.p;
                     L.cov_y = dev.cov_y.p;
+                    L.chain = dev.chain.p;
                     L.motion = dev.motion.p;
                     L.inv_sqrt = dev.inv_sqrt.p;
                     L.y_begin = y;
@@ -454,6 +456,7 @@ StackLaunch plan_launch(ftk_plan* pl) {
     L.cov_off = pl->dev.cov_off.p;
     L.cov_x = pl->dev.cov_x.p;
     L.cov_y = pl->dev.cov_y.p;
+    L.chain = pl->dev.chain.p;
     L.motion = pl->dev.motion.p;
     L.inv_sqrt = pl->dev.inv_sqrt.p;
     L.y_begin = 0;

@@ -842,6 +842,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const size_t i_covoff = part(plan.cov_off.data(), plan.cov_off.size() * 8);
     const size_t i_covx = part(plan.cov_x.data(), plan.cov_x.size() * 4);
     const size_t i_covy = part(plan.cov_y.data(), plan.cov_y.size() * 4);
+    const size_t i_chain = part(plan.chain.data(), plan.chain.size() * 4);
     const size_t i_perm = part(all_perm.data(), all_perm.size() * 4);
     const size_t i_rank = part(all_rank.data(), all_rank.size() * 4);
     std::vector<StackArgs> hargs[2];
@@ -873,6 +874,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const long long* d_covoff = static_cast<const long long*>(dptr(i_covoff));
     const int* d_covx = static_cast<const int*>(dptr(i_covx));
     const int* d_covy = static_cast<const int*>(dptr(i_covy));
+    const int* d_chain = static_cast<const int*>(dptr(i_chain));
     const int* d_perm = static_cast<const int*>(dptr(i_perm));
     const int* d_rank = static_cast<const int*>(dptr(i_rank));
 
@@ -900,6 +902,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             L.cov_off = d_covoff;
             L.cov_x = d_covx;
             L.cov_y = d_covy;
+            L.chain = d_chain;
             L.motion = d_motion;
             L.inv_sqrt = d_inv;
             L.y_begin = 0;

@@ -127,6 +127,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     const long long* __restrict__ cov_off = a.cov_off;
     const int* __restrict__ cov_x = a.cov_x;
     const int* __restrict__ cov_y = a.cov_y;
+    const int* __restrict__ chain = a.chain;
     unsigned long long* __restrict__ keys = a.keys;
     // two frame boxes of lds_rows x kPitch floats (lds_rows = 16 + the plan's largest
     // vertical spread), sized at launch
@@ -158,6 +159,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     cint* const c_motion = as_const(motion);
     cint* const c_shift = as_const(shift);
     cint* const c_covy = as_const(cov_y);
+    cint* const c_chain = as_const(chain);
     const __attribute__((address_space(4))) long long* const c_covoff =
         (const __attribute__((address_space(4))) long long*)(cov_off);
     const int lane_base = wave * RPW * kPitch + lane;
@@ -409,8 +411,10 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
         // pixel's count of frames landing inside [0, w) is F(w-1-x) - F(-1-x). A block that
         // stays inside in y needs only the x table (one count per lane, the same for the
         // lane's rows); one that stays inside in x only the y table (one count per row,
-        // the same for the wave: scalar loads); only corner cases walk the frames.
-        int cover = 0;  // 0: c = n; 1: x table; 2: y table; 3: frame walk
+        // the same for the wave: scalar loads). A block leaving both ranges uses both
+        // tables when each motion's shifts form a chain or anti-chain (straight tracks
+        // do), else walks the frames.
+        int cover = 0;  // 0: c = n; 1: x table; 2: y table; 3: both tables or frame walk
         if (RING && !interior) {
             cint* ub = c_ubox + 4 * blk;
             const bool x_full = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w;
@@ -441,6 +445,25 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                     auto cdf = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= L ? n : G[d]); };
 #pragma unroll
                     for (int r = 0; r < RPW; ++r) cnt[r] = cdf(h - 1 - (yw + r)) - cdf(-1 - (yw + r));
+                } else if (cov_ok && c_chain[V * blk + k] != 0) {
+                    // both ranges cut: for a chain (anti-chain) of (dx, dy) pairs the frames
+                    // in the x range and those in the y range are two intervals of one
+                    // order (by dx, then dy ascending, or descending), so the count is
+                    // their overlap, from the two tables
+                    const int Lx = ub[2] - ub[0] + 1, Ly = ub[3] - ub[1] + 1;
+                    const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * Lx;
+                    cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * Ly;
+                    auto fx = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= Lx ? n : F[d]); };
+                    auto gy = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= Ly ? n : G[d]); };
+                    const int x_lo = fx(-1 - x), x_hi = fx(w - 1 - x);  // positions [x_lo, x_hi)
+                    const bool co = c_chain[V * blk + k] == 1;
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r) {
+                        const int below = gy(-1 - (yw + r)), upto = gy(h - 1 - (yw + r));
+                        // y range positions: [below, upto) ascending, [n - upto, n - below) descending
+                        const int y_lo = co ? below : n - upto, y_hi = co ? upto : n - below;
+                        cnt[r] = max(0, min(x_hi, y_hi) - max(x_lo, y_lo));
+                    }
                 } else {
 #pragma unroll
                     for (int r = 0; r < RPW; ++r) cnt[r] = 0;
@@ -580,6 +603,7 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.cov_off = L.cov_off;
     a.cov_x = L.cov_x;
     a.cov_y = L.cov_y;
+    a.chain = L.chain;
     a.keys = L.keys;
     a.rect = R;
     a.row0 = L.row0;

@@ -35,6 +35,7 @@ struct StackLaunch {
     const long long* cov_off = nullptr;  // per block: offsets into cov_x, cov_y
     const int* cov_x = nullptr;
     const int* cov_y = nullptr;
+    const int* chain = nullptr;          // per block and motion: chain / anti-chain flags
     const int* motion = nullptr;
     const float* inv_sqrt = nullptr;
     int y_begin = 0, y_end = 0, tiles_x = 0, tiles_y = 0;
@@ -64,6 +65,7 @@ struct StackArgs {
     const long long* cov_off;
     const int* cov_x;
     const int* cov_y;
+    const int* chain;
     unsigned long long* keys;
     TileRect rect;
     int row0, row1, n, w, h, pitch, padded, storage, min_count, nb, bps;

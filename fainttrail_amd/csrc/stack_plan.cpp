@@ -204,6 +204,8 @@ void build_coverage(StackPlan& p) {
     p.cov_off.resize(static_cast<size_t>(p.nb) * 2);
     p.cov_x.assign(static_cast<size_t>(tx), 0);
     p.cov_y.assign(static_cast<size_t>(ty), 0);
+    p.chain.assign(static_cast<size_t>(p.nb) * kVel, 0);
+    std::vector<int> lo_y, hi_y;  // per dx value of one motion: least and largest dy
     long long ox = 0, oy = 0;
     for (int b = 0; b < p.nb; ++b) {
         const int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
@@ -223,6 +225,35 @@ void build_coverage(StackPlan& p) {
         for (int k = 0; k < kVel; ++k) {
             for (long long d = 1; d < Lx; ++d) cx[k * Lx + d] += cx[k * Lx + d - 1];
             for (long long d = 1; d < Ly; ++d) cy[k * Ly + d] += cy[k * Ly + d - 1];
+        }
+        // chains: ordered by dx, the dy ranges of successive dx values never step down
+        // (chain) or never step up (anti-chain); a bucket pass over the dx range
+        lo_y.assign(static_cast<size_t>(Lx), 0);
+        hi_y.assign(static_cast<size_t>(Lx), 0);
+        for (int k = 0; k < kVel; ++k) {
+            std::fill(lo_y.begin(), lo_y.end(), std::numeric_limits<int>::max());
+            std::fill(hi_y.begin(), hi_y.end(), std::numeric_limits<int>::min());
+            for (int i = 0; i < n; ++i) {
+                const int32_t* sh = &p.shift[(static_cast<size_t>(b) * n + i) * 2 * kVel];
+                const size_t d = static_cast<size_t>(sh[k] - ux0);
+                lo_y[d] = std::min(lo_y[d], static_cast<int>(sh[kVel + k]));
+                hi_y[d] = std::max(hi_y[d], static_cast<int>(sh[kVel + k]));
+            }
+            bool co = true, anti = true, any = false;
+            int plo = 0, phi = 0;
+            for (long long d = 0; d < Lx; ++d) {
+                if (lo_y[static_cast<size_t>(d)] > hi_y[static_cast<size_t>(d)]) continue;  // no frame at this dx
+                const int clo = lo_y[static_cast<size_t>(d)], chi = hi_y[static_cast<size_t>(d)];
+                if (any) {
+                    co = co && phi <= clo;     // every earlier dy <= every later dy
+                    anti = anti && plo >= chi;  // every earlier dy >= every later dy
+                }
+                // within one dx value the dy order is free in both cases
+                plo = clo;
+                phi = chi;
+                any = true;
+            }
+            p.chain[static_cast<size_t>(b) * kVel + k] = co ? 1 : anti ? 2 : 0;
         }
         ox += kVel * Lx;
         oy += kVel * Ly;

@@ -40,6 +40,11 @@ struct StackPlan {
     // = number of frames with dx <= d (Lx = max dx - min dx + 1); cov_y likewise
     std::vector<long long> cov_off;
     std::vector<int32_t> cov_x, cov_y;
+    // per block and motion (nb x kVel): 1 when the motion's (dx, dy) pairs form a chain
+    // ((dx_i - dx_j)(dy_i - dy_j) >= 0 for all frames i, j), 2 when they form an
+    // anti-chain (<= 0), else 0. Then the frames landing inside both ranges are an
+    // intersection of two intervals of one order, counted from cov_x and cov_y alone.
+    std::vector<int32_t> chain;
     bool cov_ok = true;  // false: the tables would be too large; the kernel walks the frames
     int all_box[4] = {0, 0, 0, 0};   // union of ubox over the blocks: min dx, min dy, max dx, max dy
     std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))
