@@ -167,3 +167,34 @@ def test_missing_device_is_an_error(hip, golden):
     be = hip.make_backend("hip:device=57")
     with pytest.raises(hip.FtkError, match="not present"):
         be.run(hip.StackPass(frames=g["frames"], shifts=hip.ShiftTable.make(g["dx"], g["dy"]), min_count=5))
+
+
+@pytest.mark.parametrize("min_count", [3, 8])
+def test_corner_coverage_chains_anti_chains_and_others(hip, min_count):
+    """Edge tiles that leave the frame in x and y count their covering frames from the
+    two coverage tables when a motion's (dx, dy) pairs form a chain or an anti-chain, and
+    walk the frames otherwise. A 150 x 40 frame (every tile an edge tile, corners
+    included) with shifts up to 22 px: straight tracks in all four quadrants (chains and
+    anti-chains), axis-parallel ones (both), and zig-zag and random tables (neither),
+    against the naive contract of the oracle, bit for bit."""
+    import fainttrail_amd as ft
+    rng = np.random.default_rng(11)
+    n, w, h = 12, 150, 40
+    frames = rng.normal(size=(n, h, w)).astype(np.float32)
+    t = np.arange(n) - 5.5
+    rows_x, rows_y = [], []
+    for a, b in [(3, 2), (-3, 2), (3, -2), (-3, -2), (0, 3.5), (3.5, 0), (1.3, -3.7), (-3.9, 1.1), (2.2, 2.9),
+                 (-1.7, -3.3), (3.8, -0.6), (0.4, 3.9)]:
+        rows_x.append(np.round(a * t).astype(np.int32))  # numpy rounds half to even; any table will do
+        rows_y.append(np.round(b * t).astype(np.int32))
+    for q in range(12):  # zig-zags: dx rises while dy alternates (neither kind)
+        rows_x.append(np.round((q - 6) * 0.6 * t).astype(np.int32))
+        rows_y.append(np.where(np.arange(n) % 2 == 0, 1, -1).astype(np.int32) * (6 + q))
+    for q in range(24):  # random tables
+        rows_x.append(rng.integers(-22, 23, n).astype(np.int32))
+        rows_y.append(rng.integers(-22, 23, n).astype(np.int32))
+    dx, dy = np.stack(rows_x), np.stack(rows_y)
+    want_m, want_b = po.stack_naive(frames, dx, dy, min_count)
+    for options in ("", "pad=0"):
+        m, b = run_hip(ft, frames, dx, dy, min_count, options)
+        assert same_bits(m, want_m) and np.array_equal(b, want_b), options
