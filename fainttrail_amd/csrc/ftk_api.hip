@@ -165,6 +165,15 @@ struct ftk_plan {
     DeviceBuffer<uint16_t> stored;
     DeviceBuffer<float> scales;
     DeviceBuffer<unsigned> maxbits;
+    // the edge tiles' launch runs on a stream of its own beside the interior launch
+    hipStream_t ring_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    ~ftk_plan() {
+        if (ring_stream) (void)hipStreamSynchronize(ring_stream);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (ring_stream) (void)hipStreamDestroy(ring_stream);
+    }
 };
 
 namespace ftk {
@@ -472,8 +481,17 @@ StackLaunch plan_launch(ftk_plan* pl) {
 
 void run_launch(ftk_plan* pl, const StackLaunch& L, float* d_max_snr, int32_t* d_best_v, hipStream_t s) {
     const long long np = static_cast<long long>(pl->plan.w) * pl->plan.h;
+    if (!pl->ring_stream) {
+        check(hipStreamCreateWithFlags(&pl->ring_stream, hipStreamNonBlocking), "create stream");
+        check(hipEventCreateWithFlags(&pl->ev_in, hipEventDisableTiming), "create event");
+        check(hipEventCreateWithFlags(&pl->ev_out, hipEventDisableTiming), "create event");
+    }
     check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
-    check(launch_stack(pl->plan, L, s), "launch stack kernel");
+    check(hipEventRecord(pl->ev_in, s), "record event");
+    check(hipStreamWaitEvent(pl->ring_stream, pl->ev_in, 0), "stream wait");
+    check(launch_stack(pl->plan, L, s, pl->ring_stream), "launch stack kernel");
+    check(hipEventRecord(pl->ev_out, pl->ring_stream), "record event");
+    check(hipStreamWaitEvent(s, pl->ev_out, 0), "stream wait");
     check(launch_finalize(pl->keys.p, np, d_max_snr, d_best_v, s), "finalize");
 }
 }  // namespace

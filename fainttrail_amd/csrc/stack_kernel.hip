@@ -680,11 +680,21 @@ bool storage_ok(const StackArgs& a) {
 }
 }  // namespace
 
-hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s) {
+hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s, hipStream_t ring_s) {
     StackArgs inner, ring;
     make_stack_args(p, L, inner, ring);
     if (!storage_ok(ring)) return hipErrorInvalidValue;
     const size_t lds = lds_bytes_of(inner.lds_rows);
+    if (ring_s) {
+        // beside the interior launch: the edge tiles first, the interior ones fill in
+        if (ring.ntiles > 0)
+            hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage, ring.pre), dim3(pass_groups(ring)),
+                               dim3(kThreads), lds, ring_s, ring);
+        if (inner.ntiles > 0)
+            hipLaunchKernelGGL(single_kernel(false, false, inner.storage, inner.pre), dim3(pass_groups(inner)),
+                               dim3(kThreads), lds, s, inner);
+        return hipGetLastError();
+    }
     if (inner.ntiles > 0)
         hipLaunchKernelGGL(single_kernel(false, false, inner.storage, inner.pre), dim3(pass_groups(inner)), dim3(kThreads),
                            lds, s, inner);

@@ -80,7 +80,9 @@ hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, 
 // Interior and ring descriptors of one launch (ntiles 0 = none of that kind).
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring);
 
-hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s);
+// ring_s (optional): the edge tiles' stream, ordered by the caller (both launches merge into
+// the same keys by atomic maximum, so either order gives the same bits).
+hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s, hipStream_t ring_s = nullptr);
 
 // Several passes in two launches, one over their interior tiles and one over their
 // ring tiles. host_inner / host_ring: count descriptors each, as make_stack_args
