@@ -35,6 +35,10 @@ __device__ inline bool finitef(float v) { return __builtin_isfinite(v); }
 // Global median: the two middle values added in float, halved in double. Leave-out
 // median for frame i: walk the sorted order skipping frames within `exclude` ranks
 // of i (preprocess.cpp:178-199), a and b in double, st = 0.5 (a + b).
+// NMAX > 0 (n <= NMAX): the rank count compares against a copy of the samples held in
+// registers (finite ones; others NaN, which no comparison counts), and only equal values
+// read the time ranks; the order is the same.
+template <int NMAX>
 __global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, long long fstride, int n,
                                                  long long plane, const int* __restrict__ rank, int exclude,
                                                  int min_finite, float* __restrict__ sky, float* __restrict__ resid,
@@ -49,10 +53,22 @@ __global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, 
     const long long p = static_cast<long long>(blockIdx.x) * 64 + t;
     if (p >= plane) return;
     int m = 0;
-    for (int i = 0; i < n; ++i) {
-        const float v = sub[i * fstride + p];
-        vals[i * 64 + t] = v;
-        m += finitef(v) ? 1 : 0;
+    float rv[NMAX > 0 ? NMAX : 1];
+    if constexpr (NMAX > 0) {
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+            const float v = i < n ? sub[i * fstride + p] : 0.0f;
+            if (i < n) vals[i * 64 + t] = v;
+            const bool f = i < n && finitef(v);
+            rv[i] = f ? v : __builtin_nanf("");
+            m += f ? 1 : 0;
+        }
+    } else {
+        for (int i = 0; i < n; ++i) {
+            const float v = sub[i * fstride + p];
+            vals[i * 64 + t] = v;
+            m += finitef(v) ? 1 : 0;
+        }
     }
     if (m < min_finite) {
         sky[p] = __builtin_nanf("");
@@ -64,9 +80,24 @@ __global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, 
         if (!finitef(vi)) continue;
         const int ri = rk[i];
         int pos = 0;
-        for (int j = 0; j < n; ++j) {
-            const float vj = vals[j * 64 + t];
-            pos += (finitef(vj) && (vj < vi || (vj == vi && rk[j] < ri))) ? 1 : 0;
+        if constexpr (NMAX > 0) {
+            int eq = 0;
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j) {
+                pos += rv[j] < vi ? 1 : 0;
+                eq += rv[j] == vi ? 1 : 0;
+            }
+            if (eq > 1) {  // equal values besides its own: the lower time rank first
+                for (int j = 0; j < n; ++j) {
+                    const float vj = vals[j * 64 + t];
+                    pos += (finitef(vj) && vj == vi && rk[j] < ri) ? 1 : 0;
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                const float vj = vals[j * 64 + t];
+                pos += (finitef(vj) && (vj < vi || (vj == vi && rk[j] < ri))) ? 1 : 0;
+            }
         }
         ord[pos * 64 + t] = static_cast<unsigned char>(i);
     }
@@ -627,7 +658,9 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
     // static sky + leave-out residuals
     float* sky = static_sky ? static_sky : sky_;
     const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
-    hipLaunchKernelGGL(sky_kernel, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
+    auto sky_fn = n <= 16 ? sky_kernel<16> : n <= 32 ? sky_kernel<32> : n <= 48 ? sky_kernel<48>
+                : n <= 64 ? sky_kernel<64> : sky_kernel<0>;
+    hipLaunchKernelGGL(sky_fn, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
                        exclude, min_finite, sky, resid_, plane);
     // per-frame residual noise: robust sigma of the grid samples, border 2
     select(resid_, plane, 2, nullptr, false, 1, s);

@@ -65,14 +65,21 @@ def synthetic_sequence(n, h, w, seed):
     return raw
 
 
-@pytest.mark.parametrize("n,h,w,seed", [(20, 96, 128, 1), (33, 64, 200, 2)])
-def test_stage_matches_oracle_random_orders(hip, n, h, w, seed):
+@pytest.mark.parametrize("n,h,w,seed,integer", [(20, 96, 128, 1, False), (33, 64, 200, 2, False),
+                                                (12, 48, 96, 3, True), (40, 48, 96, 4, True), (60, 40, 80, 5, True),
+                                                (70, 32, 72, 6, True)])
+def test_stage_matches_oracle_random_orders(hip, n, h, w, seed, integer):
     """Background and build_stage for the real order and two shuffled orders (the null
     runs of the search) equal the oracle's restatement, which is pinned to the
-    reference by the golden tests."""
+    reference by the golden tests. Frame counts cover every form of the static-sky
+    kernel (samples in registers for n <= 16, 32, 48, 64; in LDS above); integer ADU
+    frames (as FITS data are) make equal samples common, so the time-rank order of
+    equal values is exercised."""
     import torch
     from fainttrail_amd.stage import StageParams
     raw = synthetic_sequence(n, h, w, seed)
+    if integer:
+        raw = np.round(raw).astype(np.float32)
     st = _stage(hip, n, w, h)
     sub_d, bg = st.background(torch.from_numpy(raw).cuda())
     sub_o, bg_o = po.background(raw)
