@@ -99,3 +99,33 @@ def test_tight_budget_serialises_without_changing_results(hip):
         tiny.submit(fr, mjd, prm)
     tiny.close()
     srv.close()
+
+
+def test_job_with_registration_offsets(hip):
+    """A job's registration offsets (the detector-fixed test of the null screen) are
+    copied at submission: the caller's arrays may change or go away while the job
+    queues, and its result is still the one of the same search run alone."""
+    import torch
+    from fainttrail_amd.search import SearchParams, Searcher
+    from fainttrail_amd.serve import Server
+    from fainttrail_amd import synth
+    n, w, h = 14, 160, 120
+    fr, mjd, _ = synth.make_sequence(n, w, h, nstars=30, nmovers=2, mover_snr=(4.0, 8.0), seed=9, device="cuda:0")
+    fr = fr.contiguous()
+    rx = 0.3 * np.arange(n) - 2.0
+    ry = -0.2 * np.arange(n) + 1.0
+    prm = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=3, seed=9,
+                       reg_dx=rx.copy(), reg_dy=ry.copy())
+    alone = Searcher(n, w, h).run(fr, mjd, prm)
+    no_reg = Searcher(n, w, h).run(fr, mjd, SearchParams(**{**prm.__dict__, "reg_dx": None, "reg_dy": None}))
+    torch.cuda.synchronize()
+    srv = Server(slots=2)
+    ids = [srv.submit(fr, mjd, SearchParams(**{**prm.__dict__, "reg_dx": rx, "reg_dy": ry})) for _ in range(2)]
+    rx[:] = 99.0  # the jobs hold their own copies
+    ry[:] = -99.0
+    got = [srv.wait(j) for j in ids]
+    srv.close()
+    for g in got:
+        _same(alone, g)
+    print(f"null screen with offsets: {alone.null_artefacts} artefacts, without: {no_reg.null_artefacts}", flush=True)
+    assert alone.null_artefacts >= no_reg.null_artefacts  # the detector test only adds rejections
