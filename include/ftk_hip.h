@@ -96,7 +96,10 @@ int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, const int32_
                     const ftk_options* opt, ftk_plan** out, char* err, size_t err_len);
 /* d_frames: n frames of w x h fp32 on the device, frame i at d_frames + i * frame_stride.
  * d_max_snr / d_best_v: w x h device outputs. stream: a hipStream_t (NULL = default).
- * Asynchronous with respect to the host. */
+ * Asynchronous with respect to the host. The edge tiles run on a stream the plan owns,
+ * ordered after the work already on `stream` and before what follows on it, so the
+ * call behaves as stream-ordered work; runs of one plan must not overlap (they share
+ * its scratch). */
 int ftk_plan_run(ftk_plan* plan, const float* d_frames, size_t frame_stride, float* d_max_snr, int32_t* d_best_v,
                  void* stream, char* err, size_t err_len);
 /* Units (motions x frames x pixels) one ftk_plan_run stacks, and the number of
