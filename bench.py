@@ -172,7 +172,8 @@ def search_measure(reps: int, with_plain: bool):
             dtp = float("inf")
             for _ in range(2):  # the better of two runs (the first also pays one-time set-up)
                 t0 = time.perf_counter()
-                ref = po.ref_search(host, mjd, 12.0, 66.0, 0.125, prm.fwhm_px, 8, seed=7, backend="hip")
+                ref = po.ref_search(host, mjd, 12.0, 66.0, 0.125, prm.fwhm_px, 8, seed=7, backend="hip",
+                                    extras=False)
                 dtp = min(dtp, time.perf_counter() - t0)
             out["plain_searches_per_s"] = 1.0 / dtp
             out["plain_s_per_search"] = dtp
@@ -386,10 +387,10 @@ def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
         if po.ref_available():
             host = frames.cpu().numpy()
             t0 = time.perf_counter()
-            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, 0, seed=7, backend="hip")
+            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, 0, seed=7, backend="hip", extras=False)
             t_real = time.perf_counter() - t0
             t0 = time.perf_counter()
-            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, plain_nulls, seed=7, backend="hip")
+            po.ref_search(host, mjd, 12.0, 66.0, 0.212, fwhm, plain_nulls, seed=7, backend="hip", extras=False)
             t_k = time.perf_counter() - t0
             per_null = max(0.0, (t_k - t_real) / plain_nulls)
             est = t_real + 256 * per_null

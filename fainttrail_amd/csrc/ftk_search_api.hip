@@ -229,6 +229,69 @@ extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int
     return copy_peaks(s->last, which, cap, x, y, snr, vidx);
 }
 
+// find_peaks (src/search.cpp:105-135) over device maps: the local maxima are found on
+// the device (peaks_kernel), then put in the reference's order on the host. Two peaks
+// never lie within each other's (2 radius + 1)^2 window (the larger one, or the earlier
+// of two equal ones, excludes the other), so there are at most
+// ceil(w / (radius + 1)) * ceil(h / (radius + 1)) of them: the size of the scratch list.
+extern "C" int ftk_find_peaks(const float* d_max_snr, const int32_t* d_best_v, int w, int h, float floor, int radius,
+                              int edge, int max_peaks, int cap, int32_t* x, int32_t* y, float* snr, int32_t* vidx,
+                              int* count, void* stream, char* err, size_t err_len) {
+    int* buf = nullptr;
+    try {
+        if (!d_max_snr || !d_best_v || !count) throw std::runtime_error("ftk_find_peaks: null argument");
+        if (w < 1 || h < 1 || radius < 0 || edge < 0 || max_peaks < 0 || cap < 0)
+            throw std::runtime_error("ftk_find_peaks: bad sizes");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const long long most = static_cast<long long>((w + radius) / (radius + 1)) * ((h + radius) / (radius + 1));
+        if (most > (1ll << 28)) throw std::runtime_error("ftk_find_peaks: map too large");
+        const size_t words = 1 + 4 * static_cast<size_t>(most);
+        if (hipMalloc(reinterpret_cast<void**>(&buf), words * sizeof(int)) != hipSuccess)
+            throw std::runtime_error("ftk_find_peaks: allocate peak list");
+        int* px = buf + 1;
+        int* py = px + most;
+        float* ps = reinterpret_cast<float*>(py + most);
+        int* pv = reinterpret_cast<int*>(ps + most);
+        auto chk = [](hipError_t e, const char* what) {
+            if (e != hipSuccess) throw std::runtime_error(std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+        };
+        chk(hipMemsetAsync(buf, 0, sizeof(int), s), "clear peak count");
+        chk(ftk::launch_peaks(d_max_snr, d_best_v, w, h, floor, radius, edge, static_cast<int>(most), buf, px, py, ps,
+                              pv, s),
+            "peaks kernel");
+        int c = 0;
+        chk(hipMemcpyAsync(&c, buf, sizeof(int), hipMemcpyDeviceToHost, s), "download peak count");
+        chk(hipStreamSynchronize(s), "peaks kernel");
+        if (c > most) throw std::runtime_error("ftk_find_peaks: more local maxima than the bound");
+        std::vector<int> hx(c), hy(c), hv(c);
+        std::vector<float> hs(c);
+        if (c > 0) {
+            chk(hipMemcpyAsync(hx.data(), px, sizeof(int) * c, hipMemcpyDeviceToHost, s), "download peaks");
+            chk(hipMemcpyAsync(hy.data(), py, sizeof(int) * c, hipMemcpyDeviceToHost, s), "download peaks");
+            chk(hipMemcpyAsync(hs.data(), ps, sizeof(float) * c, hipMemcpyDeviceToHost, s), "download peaks");
+            chk(hipMemcpyAsync(hv.data(), pv, sizeof(int) * c, hipMemcpyDeviceToHost, s), "download peaks");
+            chk(hipStreamSynchronize(s), "download peaks");
+        }
+        (void)hipFree(buf);
+        buf = nullptr;
+        const ftk::PeakList pk =
+            ftk::order_peaks(c, hx.data(), hy.data(), hs.data(), hv.data(), static_cast<size_t>(max_peaks));
+        const int total = static_cast<int>(pk.x.size());
+        for (int k = 0; k < total && k < cap; ++k) {
+            if (x) x[k] = pk.x[static_cast<size_t>(k)];
+            if (y) y[k] = pk.y[static_cast<size_t>(k)];
+            if (snr) snr[k] = pk.snr[static_cast<size_t>(k)];
+            if (vidx) vidx[k] = pk.vidx[static_cast<size_t>(k)];
+        }
+        *count = total;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        if (buf) (void)hipFree(buf);
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
 // ------------------------------------------------------------------ server
 
 struct ftk_server {
@@ -292,7 +355,8 @@ extern "C" int ftk_server_wait(ftk_server* s, long long job, double timeout_s, f
             times[0] = j->t_submit;
             times[1] = j->t_admit;
             times[2] = j->t_end;
-            times[3] = j->bytes;
+            times[3] = j->reserved;
+            times[4] = j->held;
         }
         if (j->status != 2) throw std::runtime_error(j->error);
         if (sum) fill_summary(j->out, sum);

@@ -57,6 +57,8 @@ public:
                       int* cr_host, hipStream_t s);
     // Matched-filter taps for prm.fwhm_px (uploaded when they change; synchronises s then).
     void set_taps(const StageParams& prm, hipStream_t s);
+    // Device bytes of the stage's scratch (the sizes asked of hipMalloc).
+    double device_bytes() const;
 
     int n, w, h;
     long long plane;

@@ -563,6 +563,12 @@ DeviceStage::DeviceStage(int n_, int w_, int h_) : n(n_), w(w_), h(h_), plane(st
     taps_ = dalloc<float>(2 * 64 + 2, "allocate taps");
 }
 
+double DeviceStage::device_bytes() const {
+    const double np = static_cast<double>(n) * plane;
+    return np * (4.0 + 1.0) + 3.0 * plane * 4.0 + n * (sizeof(SelectSeg) + 512.0 * 4.0 + 4.0 + 8.0 + 4.0 + 4.0 + 4.0) +
+           (2.0 * 64 + 2) * 4.0;
+}
+
 DeviceStage::~DeviceStage() {
     for (void* p : {static_cast<void*>(resid_), static_cast<void*>(flags_), static_cast<void*>(sky_),
                     static_cast<void*>(scatter_), static_cast<void*>(mask_), static_cast<void*>(seg_),

@@ -314,6 +314,27 @@ void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long
     pk_cap = pk_cap_;
 }
 
+double PassSet::device_bytes() const {
+    if (!filt) return 0.0;
+    const double c = cap;
+    return c * frames * frame_floats * 4.0 + c * plane * (8.0 + 4.0 + 4.0) + c * tab_ints * 4.0 + c * frames * 4.0 +
+           c * 4.0 + c * 4.0 * pk_cap * 4.0 + c * frames * (4.0 + 4.0);
+}
+
+double DeviceSearch::device_bytes() const {
+    const double plane = static_cast<double>(w) * h;
+    double b = 2.0 * plane * n * 4.0 + 5.0 * plane * 4.0 + (1.0 + 4.0 * pk_cap_) * 4.0;
+    b += static_cast<double>(scr_tab_cap_) * 8.0 + static_cast<double>(scr_cap_) + static_cast<double>(scr_art_cap_);
+    b += static_cast<double>(tab_bytes_);
+    b += stage_.device_bytes();
+    for (const PassSet& set : sets_) b += set.device_bytes();
+    for (const BuildLane& l : lanes_) {
+        if (l.stage) b += l.stage->device_bytes();
+        if (l.filt) b += plane * n * 4.0;
+    }
+    return b;
+}
+
 DeviceSearch::DeviceSearch(int n_, int w_, int h_) : n(n_), w(w_), h(h_), stage_(n_, w_, h_) {
     const size_t plane = static_cast<size_t>(w) * h;
     sub_ = salloc<float>(plane * n, "allocate frames");

@@ -94,6 +94,8 @@ struct PassSet {
     void release();
     void reserve(int cap, long long frames, long long frame_floats, long long plane, long long tab_ints,
                  long long pk_cap);
+    // device bytes the set's buffers hold (the sizes asked of hipMalloc)
+    double device_bytes() const;
 };
 
 class DeviceSearch {
@@ -110,6 +112,10 @@ public:
 
     int n, w, h;
     int tuned_passes = 0, tuned_segments = 0;  // from tune(); 0 = none
+
+    // Device bytes the searcher holds now (every buffer it allocated and keeps, as
+    // asked of hipMalloc; buffers only grow between runs).
+    double device_bytes() const;
 
     // Times the batched run over candidate group sizes and segment counts on these
     // frames and keeps the fastest (tuned_passes, tuned_segments). Returns the seconds

@@ -113,12 +113,25 @@ long long SearchServer::submit(const float* raw, long long stride, int n, int w,
     return p->id;
 }
 
-bool SearchServer::head_admissible() {
-    if (queue_.empty() || running_ >= slots_) return false;
+int SearchServer::pick_admissible() {
+    if (queue_.empty() || running_ >= slots_) return -1;
     double idle_total = 0.0;
     for (const Idle& d : idle_) idle_total += d.bytes;
-    // idle searchers can be evicted (or reused) to make room
-    return used_ - idle_total + queue_.front()->bytes <= budget_;
+    for (size_t q = 0; q < queue_.size(); ++q) {
+        const ServerJob* j = queue_[q];
+        // a reused searcher of the job's shape keeps what it holds; idle searchers can be
+        // evicted (or reused) to make room
+        double need = j->bytes;
+        for (const Idle& d : idle_)
+            if (d.n == j->n && d.w == j->w && d.h == j->h) {
+                need = std::max(need, d.bytes);
+                break;
+            }
+        if (used_ - idle_total + need <= budget_) return static_cast<int>(q);
+        // the head has waited through enough overtaking: it goes next
+        if (q == 0 && j->overtaken >= kMaxOvertake) return -1;
+    }
+    return -1;
 }
 
 void SearchServer::worker() {
@@ -127,10 +140,12 @@ void SearchServer::worker() {
     (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
-        cv_.wait(lk, [&] { return stop_ || head_admissible(); });
+        int pos = -1;
+        cv_.wait(lk, [&] { return stop_ || (pos = pick_admissible()) >= 0; });
         if (stop_) break;
-        ServerJob* j = queue_.front();
-        queue_.pop_front();
+        ServerJob* j = queue_[static_cast<size_t>(pos)];
+        for (int q = 0; q < pos; ++q) ++queue_[static_cast<size_t>(q)]->overtaken;
+        queue_.erase(queue_.begin() + pos);
         std::unique_ptr<DeviceSearch> ds;
         double held = 0.0;
         for (auto it = idle_.begin(); it != idle_.end(); ++it)
@@ -140,15 +155,16 @@ void SearchServer::worker() {
                 idle_.erase(it);
                 break;
             }
+        j->reserved = std::max(j->bytes, held);
         std::vector<std::unique_ptr<DeviceSearch>> evict;
         used_ -= held;
-        while (used_ + j->bytes > budget_ && !idle_.empty()) {
+        while (used_ + j->reserved > budget_ && !idle_.empty()) {
             used_ -= idle_.front().bytes;
             evict.push_back(std::move(idle_.front().ds));
             idle_.erase(idle_.begin());
             ++evicted_;
         }
-        used_ += j->bytes;
+        used_ += j->reserved;
         peak_ = std::max(peak_, used_);
         j->status = 1;
         j->t_admit = now();
@@ -159,9 +175,11 @@ void SearchServer::worker() {
         int status = 2;
         std::string error;
         SearchOutput out;
+        double held_now = 0.0;
         try {
             if (!ds) ds = std::make_unique<DeviceSearch>(j->n, j->w, j->h);
             out = ds->run(j->raw, j->stride, j->mjd.data(), j->cfg, nullptr, nullptr, stream);
+            held_now = ds->device_bytes();
         } catch (const std::exception& e) {
             status = 3;
             error = e.what();
@@ -172,12 +190,19 @@ void SearchServer::worker() {
         j->out = std::move(out);
         j->error = error;
         j->t_end = now();
+        j->held = held_now;
         j->status = status;
         --running_;
         if (status == 2) ++completed_;
         else ++failed_;
-        if (ds) idle_.push_back(Idle{j->n, j->w, j->h, std::move(ds), j->bytes});
-        else used_ -= j->bytes;
+        used_ -= j->reserved;
+        if (ds) {
+            // kept for reuse at the bytes it holds, which may exceed this job's estimate
+            // (its buffers only grow): later admissions count them
+            idle_.push_back(Idle{j->n, j->w, j->h, std::move(ds), held_now});
+            used_ += held_now;
+            peak_ = std::max(peak_, used_);
+        }
         done_cv_.notify_all();
         cv_.notify_all();
     }

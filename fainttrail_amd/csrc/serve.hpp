@@ -2,15 +2,19 @@
 // DeviceSearch (search_device.hpp). No reference counterpart (the reference runs one
 // search per process, src/search.cpp:716-809).
 //
-// Jobs (frames already in HBM, times, search parameters) queue in arrival order. A job
-// is admitted when one of `slots` worker slots is free and its HBM footprint fits the
+// Jobs (frames already in HBM, times, search parameters) queue in arrival order. A
+// worker slot that is free admits the first queued job whose HBM reservation fits the
 // memory budget next to the jobs already running and the idle searchers kept for
-// reuse (idle searchers are evicted, oldest first, to make room). Admission is FIFO: a
-// job that does not fit waits, and the jobs behind it wait too (no job overtakes an
-// earlier one, so none starves). Each slot runs its job's batched search on its own
-// streams, so the stage builds, stacking launches and peak transfers of different jobs
-// overlap on the device. A job's result depends on its inputs only: the same alone
-// and in any mix.
+// reuse (idle searchers are evicted, oldest first, to make room). A smaller job may
+// thus overtake a larger one that does not fit yet, but only a bounded number of
+// times: once the head of the queue has been overtaken kMaxOvertake times, nothing
+// else is admitted before it, so no job starves. A job reserves its footprint
+// estimate (search_footprint), or the bytes a reused searcher of its shape already
+// holds if that is more; when the job ends the searcher goes back to the idle list at
+// the bytes it actually holds (DeviceSearch::device_bytes). Each slot runs its job's
+// batched search on its own streams, so the stage builds, stacking launches and peak
+// transfers of different jobs overlap on the device. A job's result depends on its
+// inputs only: the same alone and in any mix.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,7 +46,10 @@ struct ServerJob {
     SearchConfig cfg;
     long long nv = 0;
     int passes = 1;
-    double bytes = 0.0;                  // reserved footprint
+    double bytes = 0.0;                  // footprint estimate (search_footprint)
+    double reserved = 0.0;               // what admission reserved: max(estimate, reused searcher's bytes)
+    double held = 0.0;                   // device bytes the job's searcher held when it ended
+    int overtaken = 0;                   // later jobs admitted while this one waited at the head
     double t_submit = 0, t_admit = 0, t_end = 0;  // seconds on the server's clock
     int status = 0;                      // 0 queued, 1 running, 2 done, 3 failed
     std::string error;
@@ -76,7 +83,9 @@ private:
         double bytes;
     };
     void worker();
-    bool head_admissible();  // with mu_ held
+    // With mu_ held: the queue position of the job to admit next, or -1.
+    int pick_admissible();
+    static constexpr int kMaxOvertake = 16;
 
     int device_;
     double budget_;

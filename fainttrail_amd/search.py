@@ -58,6 +58,9 @@ def _bind():
                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), vp, C.c_char_p, sz]
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    I32 = C.POINTER(C.c_int32)
+    L.ftk_find_peaks.argtypes = [vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, I32, I32,
+                                 C.POINTER(C.c_float), I32, C.POINTER(C.c_int), vp, C.c_char_p, sz]
     _bound = True
     return L
 
@@ -230,6 +233,31 @@ class Searcher:
             except Exception:  # interpreter shutdown: module globals may already be cleared
                 pass
             self._h = None
+
+
+def find_peaks(max_snr, best_v, floor: float, radius: int, edge: int, max_peaks: int, stream=None) -> Peaks:
+    """find_peaks (src/search.cpp:105-135) on device maps: (h, w) float32 max_snr and
+    int32 best_v tensors on the GPU. Returns the peaks in the reference's order."""
+    import torch
+    L = _bind()
+    if (not isinstance(max_snr, torch.Tensor) or not max_snr.is_cuda or max_snr.dtype != torch.float32
+            or not max_snr.is_contiguous() or max_snr.dim() != 2):
+        raise FtkError("find_peaks needs a contiguous (h, w) float32 device tensor")
+    if (not isinstance(best_v, torch.Tensor) or best_v.dtype != torch.int32 or best_v.shape != max_snr.shape
+            or not best_v.is_contiguous() or best_v.device != max_snr.device):
+        raise FtkError("find_peaks needs an int32 best_v tensor of the map's shape on the same device")
+    h, w = max_snr.shape
+    s = stream if stream is not None else torch.cuda.current_stream(max_snr.device)
+    cap = max(1, min(max_peaks, w * h))
+    x, y, v = (np.empty(cap, np.int32) for _ in range(3))
+    q = np.empty(cap, np.float32)
+    cnt, e = C.c_int(), _err()
+    I32 = C.POINTER(C.c_int32)
+    _raise(L.ftk_find_peaks(max_snr.data_ptr(), best_v.data_ptr(), w, h, floor, radius, edge, max_peaks, cap,
+                            x.ctypes.data_as(I32), y.ctypes.data_as(I32), q.ctypes.data_as(C.POINTER(C.c_float)),
+                            v.ctypes.data_as(I32), C.byref(cnt), s.cuda_stream, e, len(e)), e)
+    c = cnt.value
+    return Peaks(x[:c], y[:c], q[:c], v[:c])
 
 
 def search(frames, mjd, params: SearchParams, device: int = 0) -> SearchResult:

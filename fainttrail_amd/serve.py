@@ -1,10 +1,11 @@
 """Device-resident search server (ftk_server_*): continuous batching of search jobs
 of different frame sizes, frame counts and velocity grids on one GPU.
 
-Jobs queue in arrival order and are admitted first come first served when a worker
-slot is free and the job's HBM footprint fits the budget beside the running jobs;
-each slot runs its job's batched search on its own streams, so jobs overlap on the
-device. A job's results depend on its inputs only, the same alone and in any mix.
+Jobs queue in arrival order; a free worker slot admits the first queued job whose HBM
+reservation fits the budget beside the running jobs (a job may overtake the head of
+the queue only while the head has been overtaken fewer than 16 times); each slot runs
+its job's batched search on its own streams, so jobs overlap on the device. A job's
+results depend on its inputs only, the same alone and in any mix.
 """
 from __future__ import annotations
 
@@ -53,6 +54,7 @@ class JobResult:
     t_end: float
     reserved_bytes: float
     passes_per_launch: int
+    held_bytes: float = 0.0  # device bytes the job's searcher held when it ended
     extra: dict = field(default_factory=dict)
 
     @property
@@ -100,18 +102,20 @@ class Server:
 
     def wait(self, job: int, timeout: float = -1.0):
         """The job's result, or None if it has not ended within timeout seconds."""
-        summ, times, e = _Summary(), (C.c_double * 4)(), _err()
+        summ, times, e = _Summary(), (C.c_double * 5)(), _err()
         st = lib().ftk_server_wait(self._h, job, float(timeout), C.byref(summ), times, e, len(e))
         if st == 3:
             return None
         self._frames.pop(job, None)
-        _raise(st, e)
-        res = JobResult(job, summ.threshold_snr, METHODS[summ.method], summ.n_velocities,
-                        self._peaks(job, 0, summ.n_peaks), self._peaks(job, 1, summ.n_selected),
-                        self._peaks(job, 2, summ.n_pooled).snr, times[0], times[1], times[2], times[3],
-                        summ.passes_per_launch)
-        lib().ftk_server_release(self._h, job)
-        return res
+        try:
+            _raise(st, e)
+            return JobResult(job, summ.threshold_snr, METHODS[summ.method], summ.n_velocities,
+                             self._peaks(job, 0, summ.n_peaks), self._peaks(job, 1, summ.n_selected),
+                             self._peaks(job, 2, summ.n_pooled).snr, times[0], times[1], times[2], times[3],
+                             summ.passes_per_launch, times[4])
+        finally:
+            # ended jobs are forgotten by the server, failed ones included
+            lib().ftk_server_release(self._h, job)
 
     def stats(self) -> dict:
         v = (C.c_double * 10)()

@@ -254,15 +254,28 @@ int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride,
  * entries into the non-NULL arrays and returns the total count. */
 int ftk_searcher_peaks(const ftk_searcher* searcher, int which, int cap, int32_t* x, int32_t* y, float* snr,
                        int32_t* vidx);
+/* find_peaks (src/search.cpp:105-135) over device maps (w x h max_snr and best_v):
+ * pixels in [edge, w - edge) x [edge, h - edge) at or above floor that are the maximum
+ * of their (2 radius + 1)^2 window (strict against later pixels, row-major, and
+ * non-strict against earlier ones: one peak per plateau), sorted by descending S/N
+ * after a row-major scan (std::sort, as the reference) and truncated to max_peaks.
+ * Writes up to cap peaks into the non-NULL arrays and the total into *count.
+ * Synchronous on stream. */
+int ftk_find_peaks(const float* d_max_snr, const int32_t* d_best_v, int w, int h, float floor, int radius, int edge,
+                   int max_peaks, int cap, int32_t* x, int32_t* y, float* snr, int32_t* vidx, int* count,
+                   void* stream, char* err, size_t err_len);
 
 /* ------------------------------------------------------------------ serving form
  * No reference counterpart (one search per process there). A device-resident request
- * loop: submitted searches queue in arrival order and are admitted, first come first
- * served, when one of `slots` worker slots is free and the job's HBM footprint (an
- * estimate of its searcher and batched pass buffers) fits the budget beside the running
+ * loop: submitted searches queue in arrival order; a free one of `slots` worker slots
+ * admits the first queued job whose HBM reservation fits the budget beside the running
  * jobs and the idle searchers kept for reuse (evicted, oldest first, when room is
- * needed). Each slot runs its job's batched search on its own streams, so jobs overlap
- * on the device. A job's results depend on its inputs only. */
+ * needed). A job may overtake the head of the queue only while the head has been
+ * overtaken fewer than 16 times. The reservation is the job's footprint estimate (its
+ * searcher and batched pass buffers), or the device bytes a reused searcher of its
+ * shape already holds if that is more; an ended job's searcher is kept at the bytes
+ * it holds. Each slot runs its job's batched search on its own streams, so jobs
+ * overlap on the device. A job's results depend on its inputs only. */
 typedef struct ftk_server ftk_server;
 /* budget_mb <= 0: 85 % of the free device memory. */
 int ftk_server_create(int device, double budget_mb, int slots, ftk_server** out, char* err, size_t err_len);
@@ -274,8 +287,9 @@ void ftk_server_destroy(ftk_server* server);
 int ftk_server_submit(ftk_server* server, const float* d_raw, size_t stride, int n, int w, int h, const double* mjd,
                       const ftk_search_params* params, long long* job, char* err, size_t err_len);
 /* Waits for a job (timeout_s < 0: no limit): FTK_OK with its summary, FTK_ERROR with
- * the job's error, FTK_TIMEOUT. times (may be NULL): submitted, admitted, ended (s on
- * the server's clock), reserved bytes. */
+ * the job's error, FTK_TIMEOUT. times (may be NULL, else 5 doubles): submitted,
+ * admitted, ended (s on the server's clock), reserved bytes, device bytes its searcher
+ * held at the end. */
 int ftk_server_wait(ftk_server* server, long long job, double timeout_s, ftk_search_summary* summary, double* times,
                     char* err, size_t err_len);
 /* Peaks of an ended job, as ftk_searcher_peaks; -1 for an unknown or failed job. */

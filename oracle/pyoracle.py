@@ -93,8 +93,6 @@ def ref() -> C.CDLL:
                                         C.c_double, F32P, I32P]
         _r.ref_check_backend_inputs.argtypes = [C.c_int, I32P, F32P, F64P, F64P, C.c_int]
         _r.ref_linear_shifts.argtypes = [C.c_int, F64P, C.c_int, F64P, I32P, I32P]
-        _r.ref_search.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, F64P, F32P,
-                                  C.c_int, F32P, C.c_char_p, C.c_int, F64P]
     return _r
 
 
@@ -306,12 +304,26 @@ def ref_check_backend_inputs(round_: int):
     return fr, t, v, mc
 
 
-def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, threshold=0.0, floor=5.0, target=0.02,
-               backend="cpu", null_tested=3000, reg=None):
-    """The reference's search() on prepare(align off), with the given stacking backend
-    ("cpu", or "hip" through the linked adapter), null_tested (the SearchParams
-    default 3000 screens the strongest null peaks; 0 screens none) and optional
-    registration offsets reg = (reg_dx, reg_dy), one per frame each."""
+CAND_FIELDS = ["id", "x", "y", "vx_pxh", "vy_pxh", "pos_err_px", "vel_err_pxh", "snr", "snr_coarse", "snr_half1",
+               "snr_half2", "max_frame_fraction", "false_alarms_per_run", "flux", "mag_inst", "ok", "inv_dist_au",
+               "inv_dist_err_au"]
+METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"]
+
+
+def _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, threshold, floor, target, backend,
+                    null_tested, reg, tile, max_candidates, extras=True):
+    L = ref()
+    if not getattr(L, "_search_bound", False):
+        L.ref_search_run.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, C.c_char_p,
+                                     C.c_int, F64P, C.c_int, C.c_int, C.c_int]
+        L.ref_last_error.restype = C.c_char_p
+        L.ref_last_counts.argtypes = [I32P]
+        L.ref_last_summary.argtypes = [F64P]
+        L.ref_last_maps.argtypes = [F32P, I32P]
+        L.ref_last_pooled.argtypes = [F32P]
+        L.ref_last_peaks.argtypes = [C.c_int, I32P, I32P, F32P, I32P]
+        L.ref_last_candidates.argtypes = [F64P, C.c_char_p]
+        L._search_bound = True
     if backend == "hip":
         import fainttrail_amd  # one HIP runtime: torch's, loaded before the adapter
         fainttrail_amd.lib()
@@ -319,37 +331,67 @@ def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, thresh
     n, h, w = fr.shape
     m = f64(mjd)
     prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
-    out = np.zeros(6, np.float64)
     rg = None
     if reg is not None:
         rg = f64(np.concatenate([np.asarray(reg[0], np.float64), np.asarray(reg[1], np.float64)]))
         assert rg.size == 2 * n
-    rgp = _p(rg, F64P) if rg is not None else None
-    cnt = ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
-                           _p(out, F64P), None, 0, None, backend.encode(), null_tested, rgp)
-    pooled = np.empty(max(cnt, 1), np.float32)
+    rc = L.ref_search_run(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
+                          backend.encode(), null_tested, _p(rg, F64P) if rg is not None else None, tile,
+                          max_candidates, int(extras))
+    if rc != 0:
+        raise RuntimeError(f"reference search failed: {L.ref_last_error().decode()}")
+    cnt = np.zeros(5, np.int32)
+    L.ref_last_counts(_p(cnt, I32P))
+    summ = np.zeros(9, np.float64)
+    L.ref_last_summary(_p(summ, F64P))
     mx = np.empty((h, w), np.float32)
-    ref().ref_search(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), _p(out, F64P),
-                     _p(pooled, F32P), cnt, _p(mx, F32P), backend.encode(), null_tested, rgp)
-    return {"threshold": out[0], "null_a": out[1], "null_k": out[2], "null_inv_s2": out[3], "null_b": out[4],
-            "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}
+    bv = np.empty((h, w), np.int32)
+    L.ref_last_maps(_p(mx, F32P), None if tile > 0 else _p(bv, I32P))
+    pooled = np.empty(max(int(cnt[0]), 1), np.float32)
+    L.ref_last_pooled(_p(pooled, F32P))
+
+    def peaks(which, c):
+        x, y, v = (np.empty(max(c, 1), np.int32) for _ in range(3))
+        s = np.empty(max(c, 1), np.float32)
+        L.ref_last_peaks(which, _p(x, I32P), _p(y, I32P), _p(s, F32P), _p(v, I32P))
+        return {"x": x[:c], "y": y[:c], "snr": s[:c], "vidx": v[:c]}
+
+    nc = int(cnt[3])
+    vals = np.zeros((max(nc, 1), len(CAND_FIELDS)), np.float64)
+    flags = C.create_string_buffer(int(cnt[4]))
+    L.ref_last_candidates(_p(vals, F64P), flags)
+    fl = flags.value.decode().split(";")[:nc]
+    cands = [dict(zip(CAND_FIELDS, vals[k]), flags=fl[k]) for k in range(nc)]
+    for c in cands:
+        c["id"], c["ok"] = int(c["id"]), bool(c["ok"])
+    out = {"threshold": summ[0], "null_a": summ[1], "null_k": summ[2], "null_inv_s2": summ[3], "null_b": summ[4],
+           "n_velocities": int(summ[5]), "dv_pxh": summ[6], "t_ref_mjd": summ[7],
+           "method": METHODS[int(summ[8])] if summ[8] >= 0 else "", "pooled": pooled[:int(cnt[0])], "max_snr": mx,
+           "candidates": cands}
+    if tile <= 0 and extras:
+        out.update(best_v=bv, peaks=peaks(0, int(cnt[1])), selected=peaks(1, int(cnt[2])))
+    return out
+
+
+def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, threshold=0.0, floor=5.0, target=0.02,
+               backend="cpu", null_tested=3000, reg=None, max_candidates=0, extras=True):
+    """The reference's search() on prepare(align off), with the given stacking backend
+    ("cpu", or "hip" through the linked adapter), null_tested (the SearchParams
+    default 3000 screens the strongest null peaks; 0 screens none) and optional
+    registration offsets reg = (reg_dx, reg_dy), one per frame each. Returns the
+    threshold and tail fit, the real pass's max_snr and best_v maps, the pooled null
+    peaks in the search's order, the real peak list (find_peaks order), the selected
+    peaks and the refined candidates (dicts of Candidate's fields). extras=False skips
+    the second real pass that exposes best_v and the peak lists (to time search()
+    alone)."""
+    return _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, threshold, floor, target, backend,
+                           null_tested, reg, 0, max_candidates, extras)
 
 
 def ref_search_tiled(frames, mjd, vmin, vmax, tol, fwhm, null_runs, tile, seed=12345, threshold=0.0, floor=5.0,
                      target=0.02, null_tested=3000):
-    """The reference's tiled search (search_tiled) with core tiles of `tile` px."""
-    L = ref()
-    L.ref_search_tiled.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, C.c_int, F64P,
-                                   F32P, C.c_int, F32P, C.c_int]
-    fr = f32(frames)
-    n, h, w = fr.shape
-    m = f64(mjd)
-    prm = f64([vmin, vmax, tol, fwhm, threshold, floor, target])
-    out = np.zeros(6, np.float64)
-    cnt = L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
-                             _p(out, F64P), None, 0, None, null_tested)
-    pooled = np.empty(max(cnt, 1), np.float32)
-    mx = np.empty((h, w), np.float32)
-    L.ref_search_tiled(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed), tile,
-                       _p(out, F64P), _p(pooled, F32P), cnt, _p(mx, F32P), null_tested)
-    return {"threshold": out[0], "n_velocities": int(out[5]), "pooled": pooled[:cnt], "max_snr": mx}
+    """The reference's tiled search (search_tiled) with core tiles of `tile` px: the
+    same outputs as ref_search except best_v and the peak lists, which it keeps
+    internal."""
+    return _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, threshold, floor, target, "cpu",
+                           null_tested, None, tile, 0)

@@ -7,13 +7,16 @@
 // comment and converts between flat arrays and the reference's types.
 #include <algorithm>
 #include <cstdint>
+#include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "backend_cpu.hpp"
 #include "cuda_plan.hpp"
 #include "fainttrail/backend.hpp"
+#include "fainttrail/parallax.hpp"
 #include "fainttrail/search.hpp"
 #include "fainttrail/simulate.hpp"
 #include "fainttrail/tiled.hpp"
@@ -216,17 +219,30 @@ int ref_prepare(int n, int w, int h, const float* frames, const double* mjd, dou
     return 0;
 }
 
-// search() (src/search.cpp:716-809) on prepare(align off) of the given frames, with
-// the given null_tested (artefact screen of the strongest null peaks, 0 = none) and
-// backend ("cpu" when NULL; "hip" is the registered adapter of
-// fainttrail_amd/integration/backend_hip.cpp, linked in). dparams: vmin, vmax, grid_tol,
-// fwhm, threshold_snr, min_snr_floor, target_false_per_run. reg: NULL, or 2n
-// registration offsets (reg_dx of every frame, then reg_dy) given to every frame.
-// out: threshold, null_a, null_k, null_inv_s2, null_b, n_velocities. Returns the pooled
-// null-peak count; pooled (cap values) and max_snr (w x h) are written when non-NULL.
-int ref_search(int n, int w, int h, const float* frames, const double* mjd, const double* dparams, int null_runs,
-               uint64_t seed, double* out, float* pooled, int cap, float* max_snr, const char* backend,
-               int null_tested, const double* reg) {
+// ---------------------------------------------------------------- searches
+// One reference search per call, kept in `g_last` until the next one; the getters
+// below copy its parts out. frames: n planes; mjd: n times; dparams: vmin, vmax,
+// grid_tol, fwhm, threshold_snr, min_snr_floor, target_false_per_run; reg: NULL or 2n
+// registration offsets (reg_dx of every frame, then reg_dy); backend: "cpu" when NULL
+// ("hip" is the registered adapter of fainttrail_amd/integration/backend_hip.cpp,
+// linked in); tile > 0: search_tiled (src/tiled.cpp:272-645) over a memory frame
+// source with that core tile size, else search() (src/search.cpp:716-809) on
+// prepare(align off). For the untiled search the real pass is run once more on the
+// same Prepared through the same backend (unless extras = 0, for timing the search
+// alone) to expose what search() keeps internal:
+// its best_v map, the find_peaks list (search.cpp:105-135, with search()'s floor,
+// radius and edge, :729-748) and select_peaks (:614-620) at the search's threshold.
+struct LastSearch {
+    SearchResult r;
+    bool tiled = false;
+    std::vector<int32_t> best_v;
+    std::vector<detail::Peak> peaks, selected;
+    std::string flags;
+    std::string error;
+};
+static LastSearch g_last;
+
+static std::vector<Frame> to_frames(int n, int w, int h, const float* frames, const double* mjd, const double* reg) {
     std::vector<Frame> fr(static_cast<size_t>(n));
     const size_t plane = static_cast<size_t>(w) * h;
     for (int i = 0; i < n; ++i) {
@@ -240,74 +256,129 @@ int ref_search(int n, int w, int h, const float* frames, const double* mjd, cons
             fr[static_cast<size_t>(i)].reg_dy = reg[n + i];
         }
     }
-    SearchParams s;
-    s.align = "off";
-    s.backend = backend && *backend ? backend : "cpu";
-    s.vmin_pxh = dparams[0];
-    s.vmax_pxh = dparams[1];
-    s.grid_tol_px = dparams[2];
-    s.fwhm_px = dparams[3];
-    s.threshold_snr = dparams[4];
-    s.min_snr_floor = dparams[5];
-    s.target_false_per_run = dparams[6];
-    s.null_runs = null_runs;
-    s.null_tested = null_tested;
-    s.seed = seed;
-    const Prepared p = prepare(fr, s);
-    const SearchResult r = search(p, s);
-    out[0] = r.threshold_snr;
-    out[1] = r.null_a;
-    out[2] = r.null_k;
-    out[3] = r.null_inv_s2;
-    out[4] = r.null_b;
-    out[5] = static_cast<double>(r.n_velocities);
-    for (size_t k = 0; k < r.null_peaks.size() && static_cast<int>(k) < cap; ++k)
-        if (pooled) pooled[k] = static_cast<float>(r.null_peaks[k]);
-    if (max_snr) std::copy(r.max_snr.data.begin(), r.max_snr.data.end(), max_snr);
-    return static_cast<int>(r.null_peaks.size());
+    return fr;
 }
 
-// search_tiled (src/tiled.cpp:272-645) over a memory frame source with the given core
-// tile size; same parameters and outputs as ref_search (no registration offsets). The reference designs the
-// tiled search to equal the untiled one bit for bit (tests/test_main.cpp:1220-1276).
-int ref_search_tiled(int n, int w, int h, const float* frames, const double* mjd, const double* dparams,
-                     int null_runs, uint64_t seed, int tile, double* out, float* pooled, int cap, float* max_snr,
-                     int null_tested) {
-    std::vector<Frame> fr(static_cast<size_t>(n));
-    const size_t plane = static_cast<size_t>(w) * h;
-    for (int i = 0; i < n; ++i) {
-        fr[static_cast<size_t>(i)].image = Image(w, h);
-        std::memcpy(fr[static_cast<size_t>(i)].image.data.data(), frames + i * plane, plane * sizeof(float));
-        fr[static_cast<size_t>(i)].mjd_mid = mjd[i];
-        fr[static_cast<size_t>(i)].exptime_s = 60.0;
+int ref_search_run(int n, int w, int h, const float* frames, const double* mjd, const double* dparams, int null_runs,
+                   uint64_t seed, const char* backend, int null_tested, const double* reg, int tile,
+                   int max_candidates, int extras) {
+    g_last = LastSearch{};
+    try {
+        const auto fr = to_frames(n, w, h, frames, mjd, reg);
+        SearchParams s;
+        s.align = "off";
+        s.backend = backend && *backend ? backend : "cpu";
+        s.vmin_pxh = dparams[0];
+        s.vmax_pxh = dparams[1];
+        s.grid_tol_px = dparams[2];
+        s.fwhm_px = dparams[3];
+        s.threshold_snr = dparams[4];
+        s.min_snr_floor = dparams[5];
+        s.target_false_per_run = dparams[6];
+        s.null_runs = null_runs;
+        s.null_tested = null_tested;
+        s.seed = seed;
+        if (max_candidates > 0) s.max_candidates = max_candidates;
+        if (tile > 0) {
+            const auto src = memory_frame_source(fr);
+            TileOptions opt;
+            opt.tile = tile;
+            g_last.r = search_tiled(*src, s, opt);
+            g_last.tiled = true;
+        } else {
+            const Prepared p = prepare(fr, s);
+            g_last.r = search(p, s);
+            if (!extras) {
+                for (const auto& c : g_last.r.candidates) g_last.flags += c.flags + ";";
+                return 0;
+            }
+            const auto grid = velocity_grid(p, s);
+            double dq = 0.0;
+            const std::vector<double> Q = inverse_distance_grid(p, s, &dq);
+            const std::unique_ptr<SearchBackend> be = make_backend(s.backend);
+            const detail::PassOutput real = detail::run_pass(*be, p, grid, p.t_h, s, {}, 0.0, 1.0, "search", &Q);
+            const bool auto_thr = s.threshold_snr <= 0.0;
+            const float floor = static_cast<float>(auto_thr ? s.min_snr_floor : s.threshold_snr);
+            const int nms = std::max(2, static_cast<int>(std::lround(p.fwhm_px)));
+            const int edge = std::max(s.edge_px, 1);
+            g_last.peaks = detail::find_peaks(real, floor, nms, edge, 100000);
+            g_last.selected = detail::select_peaks(g_last.peaks, g_last.r.threshold_snr, s);
+            g_last.best_v.assign(real.best_v.begin(), real.best_v.end());
+        }
+        for (const auto& c : g_last.r.candidates) g_last.flags += c.flags + ";";
+    } catch (const std::exception& e) {
+        g_last.error = e.what();
+        return -1;
     }
-    SearchParams s;
-    s.align = "off";
-    s.backend = "cpu";
-    s.vmin_pxh = dparams[0];
-    s.vmax_pxh = dparams[1];
-    s.grid_tol_px = dparams[2];
-    s.fwhm_px = dparams[3];
-    s.threshold_snr = dparams[4];
-    s.min_snr_floor = dparams[5];
-    s.target_false_per_run = dparams[6];
-    s.null_runs = null_runs;
-    s.null_tested = null_tested;
-    s.seed = seed;
-    const auto src = memory_frame_source(fr);
-    TileOptions opt;
-    opt.tile = tile;
-    const SearchResult r = search_tiled(*src, s, opt);
+    return 0;
+}
+
+const char* ref_last_error() { return g_last.error.c_str(); }
+
+// counts: pooled null peaks, real peaks, selected peaks, candidates, flags bytes (+1)
+void ref_last_counts(int* counts) {
+    counts[0] = static_cast<int>(g_last.r.null_peaks.size());
+    counts[1] = static_cast<int>(g_last.peaks.size());
+    counts[2] = static_cast<int>(g_last.selected.size());
+    counts[3] = static_cast<int>(g_last.r.candidates.size());
+    counts[4] = static_cast<int>(g_last.flags.size()) + 1;
+}
+
+// threshold, null_a, null_k, null_inv_s2, null_b, n_velocities, dv_pxh, t_ref_mjd,
+// method (0 null-fit gauss, 1 null-fit linear, 2 null-quantile, 3 fixed, 4 gaussian)
+void ref_last_summary(double* out) {
+    const SearchResult& r = g_last.r;
     out[0] = r.threshold_snr;
     out[1] = r.null_a;
     out[2] = r.null_k;
     out[3] = r.null_inv_s2;
     out[4] = r.null_b;
     out[5] = static_cast<double>(r.n_velocities);
-    for (size_t k = 0; k < r.null_peaks.size() && static_cast<int>(k) < cap; ++k)
-        if (pooled) pooled[k] = static_cast<float>(r.null_peaks[k]);
-    if (max_snr) std::copy(r.max_snr.data.begin(), r.max_snr.data.end(), max_snr);
-    return static_cast<int>(r.null_peaks.size());
+    out[6] = r.dv_pxh;
+    out[7] = r.t_ref_mjd;
+    static const char* names[] = {"null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"};
+    out[8] = -1;
+    for (int k = 0; k < 5; ++k)
+        if (r.threshold_method == names[k]) out[8] = k;
+}
+
+// max_snr (w x h) and, untiled only, best_v of the real pass (either may be NULL)
+void ref_last_maps(float* max_snr, int32_t* best_v) {
+    if (max_snr) std::copy(g_last.r.max_snr.data.begin(), g_last.r.max_snr.data.end(), max_snr);
+    if (best_v) std::copy(g_last.best_v.begin(), g_last.best_v.end(), best_v);
+}
+
+// pooled null-peak S/N values in the search's own order (run after run)
+void ref_last_pooled(float* pooled) {
+    for (size_t k = 0; k < g_last.r.null_peaks.size(); ++k) pooled[k] = static_cast<float>(g_last.r.null_peaks[k]);
+}
+
+// which 0: real peaks (find_peaks order), 1: selected peaks
+void ref_last_peaks(int which, int32_t* x, int32_t* y, float* snr, int32_t* vidx) {
+    const auto& pk = which == 0 ? g_last.peaks : g_last.selected;
+    for (size_t k = 0; k < pk.size(); ++k) {
+        x[k] = pk[k].x;
+        y[k] = pk[k].y;
+        snr[k] = pk[k].snr;
+        vidx[k] = pk[k].vidx;
+    }
+}
+
+// Candidates in the result's order, 18 doubles each: id, x, y, vx_pxh, vy_pxh,
+// pos_err_px, vel_err_pxh, snr, snr_coarse, snr_half1, snr_half2,
+// max_frame_fraction, false_alarms_per_run, flux, mag_inst, ok, inv_dist_au,
+// inv_dist_err_au; flags: every candidate's flags followed by ';'.
+void ref_last_candidates(double* vals, char* flags) {
+    size_t k = 0;
+    for (const auto& c : g_last.r.candidates) {
+        const double v[18] = {static_cast<double>(c.id), c.x, c.y, c.vx_pxh, c.vy_pxh, c.pos_err_px, c.vel_err_pxh,
+                              c.snr, c.snr_coarse, c.snr_half1, c.snr_half2, c.max_frame_fraction,
+                              c.false_alarms_per_run, c.flux, c.mag_inst, c.ok ? 1.0 : 0.0, c.inv_dist_au,
+                              c.inv_dist_err_au};
+        std::copy(v, v + 18, vals + 18 * k);
+        ++k;
+    }
+    std::memcpy(flags, g_last.flags.c_str(), g_last.flags.size() + 1);
 }
 
 // The inputs of `fainttrail check-backend` (cli/main.cpp:697-742): simulate() with

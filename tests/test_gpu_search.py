@@ -1,9 +1,10 @@
 """The device-resident search against the reference's own search() (its defaults: the
 strongest 3000 peaks of each null run screened for artefacts): the real pass's max-S/N
-map, every pooled null-peak S/N value and the calibrated threshold (an FP64 value) must
-be identical; with the tail-fit sums on the GPU the threshold agrees to 1e-12 relative
-(device log and summation order differ). The golden fixture holds a search with no
-screen (null_tested = 0)."""
+and best-velocity maps, its peak list (find_peaks order), the selected peaks, every
+pooled null-peak S/N value in the search's own order (run after run) and the
+calibrated threshold (an FP64 value) must be identical; with the tail-fit sums on the
+GPU the threshold agrees to 1e-12 relative (device log and summation order differ).
+The golden fixture holds a search with no screen (null_tested = 0)."""
 import numpy as np
 import pytest
 
@@ -11,6 +12,21 @@ from oracle import pyoracle as po
 from refcases import same_bits
 
 pytestmark = pytest.mark.gpu
+
+
+def _same_as_reference(got, want):
+    """Every output the reference's search exposes, bit for bit and in order."""
+    assert got.n_velocities == want["n_velocities"]
+    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
+    assert same_bits(got.null_peaks, want["pooled"])  # run after run, in the search's order
+    assert got.threshold_snr == want["threshold"]
+    assert got.threshold_method == want["method"]
+    if "best_v" in want:
+        assert np.array_equal(got.best_v.cpu().numpy(), want["best_v"])
+        for mine, theirs in ((got.peaks, want["peaks"]), (got.selected, want["selected"])):
+            for f in ("x", "y", "vidx"):
+                assert np.array_equal(getattr(mine, f), theirs[f]), f
+            assert same_bits(mine.snr, theirs["snr"])
 
 
 def test_search_matches_reference_golden(hip, golden):
@@ -26,7 +42,7 @@ def test_search_matches_reference_golden(hip, golden):
     torch.cuda.synchronize()
     assert res.n_velocities == int(golden["search/n_velocities"])
     assert same_bits(res.max_snr.cpu().numpy(), golden["search/max_snr"])
-    assert np.array_equal(np.sort(res.null_peaks), np.sort(golden["search/pooled"]))
+    assert same_bits(res.null_peaks, golden["search/pooled"])  # in the search's order
     assert res.threshold_snr == float(golden["search/threshold"])
 
 
@@ -44,10 +60,7 @@ def test_search_matches_live_reference(hip, seed, runs):
     s = Searcher(20, 160, 120)
     got = s.run(torch.from_numpy(frames).cuda(), mjd, prm)
     torch.cuda.synchronize()
-    assert got.n_velocities == want["n_velocities"]
-    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
-    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
-    assert got.threshold_snr == want["threshold"]
+    _same_as_reference(got, want)
     # the FP64 tail-fit sums on the GPU: same decision, threshold within 1e-12 relative
     prm.fit_on_device = True
     got2 = s.run(torch.from_numpy(frames).cuda(), mjd, prm)
@@ -74,9 +87,7 @@ def test_search_matches_reference_tiled_search(hip, tile):
                                      SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53,
                                                   null_runs=3, seed=9))
     torch.cuda.synchronize()
-    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
-    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
-    assert got.threshold_snr == want["threshold"]
+    _same_as_reference(got, want)
 
 
 def _screen_sequence(n=20, w=160, h=120, seed=5):
@@ -123,16 +134,14 @@ def test_null_screen_matches_reference(hip, with_reg, group):
                        passes_per_launch=group, reg_dx=rg[0] if rg else None, reg_dy=rg[1] if rg else None)
     got = Searcher(n, w, h).run(torch.from_numpy(f).cuda(), mjd, prm)
     torch.cuda.synchronize()
-    assert same_bits(got.max_snr.cpu().numpy(), want["max_snr"])
-    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
-    assert got.threshold_snr == want["threshold"]
+    _same_as_reference(got, want)
     assert got.null_artefacts == len(none["pooled"]) - len(want["pooled"])
     assert got.null_screened == len(none["pooled"])  # fewer than 3000 peaks per run: all tested
     # null_tested = 0 is the unscreened search
     prm.null_tested = 0
     got0 = Searcher(n, w, h).run(torch.from_numpy(f).cuda(), mjd, prm)
-    assert np.array_equal(np.sort(got0.null_peaks), np.sort(none["pooled"]))
-    assert got0.threshold_snr == none["threshold"] and got0.null_artefacts == 0
+    _same_as_reference(got0, none)
+    assert got0.null_artefacts == 0
 
 
 @pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
@@ -148,8 +157,60 @@ def test_null_screen_limit(hip):
                                 SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53,
                                              null_runs=3, seed=2, null_tested=25))
     assert got.null_screened == 3 * 25
-    assert np.array_equal(np.sort(got.null_peaks), np.sort(want["pooled"]))
-    assert got.threshold_snr == want["threshold"]
+    _same_as_reference(got, want)
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("group", [1, 0])
+def test_search_selects_the_references_peaks(hip, group):
+    """A sequence with 8 movers bright enough to pass the calibrated threshold: the
+    selected list (select_peaks, search.cpp:614-620) is non-trivial (the reference
+    keeps at least 5) and equal to the reference's, one pass at a time and batched."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    fr, mjd, _ = synth.make_sequence(24, 256, 192, nstars=60, nmovers=8, mover_snr=(2.0, 4.0), seed=13)
+    frames = fr.numpy()
+    want = po.ref_search(frames, mjd, 12.0, 66.0, 0.5, 3.53, 5, seed=21)
+    print(f"selected: reference keeps {len(want['selected']['x'])} of {len(want['peaks']['x'])} peaks, "
+          f"threshold {want['threshold']}", flush=True)
+    assert len(want["selected"]["x"]) >= 5
+    got = Searcher(24, 256, 192).run(torch.from_numpy(frames).cuda(), mjd,
+                                     SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.5, fwhm_px=3.53,
+                                                  null_runs=5, seed=21, passes_per_launch=group))
+    torch.cuda.synchronize()
+    _same_as_reference(got, want)
+
+
+def _plateau_map(seed, h, w):
+    """A max-S/N map on a coarse value grid (steps of 0.5): many equal neighbours and
+    flat plateaus, so the one-peak-per-plateau rule decides most peaks."""
+    rng = np.random.default_rng(seed)
+    m = np.round(rng.normal(3.0, 2.0, (h, w)) * 2.0) / 2.0
+    for _ in range(20):  # flat-topped plateaus of a few pixels
+        y, x = rng.integers(0, h - 4), rng.integers(0, w - 4)
+        m[y:y + rng.integers(1, 4), x:x + rng.integers(1, 4)] = 12.0
+    return m.astype(np.float32), rng.integers(-1, 5000, (h, w)).astype(np.int32)
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("radius,edge,floor,max_peaks", [(2, 4, 3.0, 100000), (3, 1, 5.0, 100000),
+                                                         (2, 4, 3.0, 37), (5, 0, -1e30, 20000)])
+def test_find_peaks_matches_reference(hip, radius, edge, floor, max_peaks):
+    """The device peak finder (peaks_kernel + the host's row-major scan order and
+    std::sort, ftk_find_peaks) equals the reference's find_peaks on maps with plateaus
+    and ties: same peaks, same order, same truncation."""
+    import torch
+    from fainttrail_amd.search import find_peaks
+    m, b = _plateau_map(radius * 100 + edge, 157, 203)
+    want = po.ref_find_peaks(m, b, floor, radius, edge, max_peaks)
+    assert len(want[0]) > 20
+    got = find_peaks(torch.from_numpy(m).cuda(), torch.from_numpy(b).cuda(), floor, radius, edge, max_peaks)
+    for a, e in zip((got.x, got.y, got.snr, got.vidx), want):
+        assert np.array_equal(a, e)
+    ora = po.find_peaks(m, b, floor, radius, edge, max_peaks)
+    for a, e in zip(ora, want):
+        assert np.array_equal(a, e)
 
 
 def _same_result(a, b):

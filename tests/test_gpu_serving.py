@@ -1,8 +1,10 @@
 """Serving form (ftk_server_*): jobs of different frame sizes, frame counts and velocity
-grids submitted together run concurrently, and each job's result (peaks, selected
-candidates, pooled null peaks in run order, threshold) is bit-identical to the same
-search run alone; a tight memory budget serialises admission without changing any
-result; a job larger than the budget is refused at submission."""
+grids submitted together or while others run overlap on the device, and each job's
+result (peaks, selected candidates, pooled null peaks in run order, threshold) is
+bit-identical to the same search run alone; a tight memory budget serialises
+admission without changing any result; a small job overtakes a large one that does not
+fit yet; a reused searcher is reserved at the bytes it holds; a job larger than the
+budget is refused at submission."""
 import numpy as np
 import pytest
 
@@ -90,8 +92,8 @@ def test_tight_budget_serialises_without_changing_results(hip):
         _same(a, b)
     assert st["peak_reserved_bytes"] <= st["budget_bytes"]
     assert sum(sizes) > st["budget_bytes"] and st["max_running"] < len(jobs)
-    # admission is first come, first served
-    assert all(got[k].t_admit <= got[k + 1].t_admit for k in range(len(got) - 1))
+    for g in got:  # the reservation covers what the job's searcher holds
+        assert 0 < g.held_bytes <= g.reserved_bytes
     # a job larger than the whole budget is refused when submitted
     tiny = Server(budget_mb=1.0, slots=1)
     fr, mjd, prm = jobs[0]
@@ -129,3 +131,111 @@ def test_job_with_registration_offsets(hip):
         _same(alone, g)
     print(f"null screen with offsets: {alone.null_artefacts} artefacts, without: {no_reg.null_artefacts}", flush=True)
     assert alone.null_artefacts >= no_reg.null_artefacts  # the detector test only adds rejections
+
+
+def test_small_job_overtakes_a_blocked_head(hip):
+    """Under a budget that holds one large job and one small one but not two large
+    ones, a small job queued behind a large job that does not fit yet is admitted
+    first (bounded overtaking); every result equals the same search alone."""
+    import time
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    from fainttrail_amd.serve import Server
+
+    def job(w, h, n, nulls, seed):
+        fr, mjd, _ = synth.make_sequence(n, w, h, nstars=30, nmovers=2, mover_snr=(4.0, 8.0), seed=seed,
+                                         device="cuda:0")
+        return fr.contiguous(), mjd, SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.3, fwhm_px=3.0,
+                                                  null_runs=nulls, seed=seed)
+
+    # the large jobs run for a few tenths of a second: the others arrive while the first runs
+    big_a, big_b, small = job(1024, 1024, 32, 8, 11), job(1024, 1024, 32, 8, 12), job(96, 96, 10, 2, 13)
+    alone = [Searcher(fr.shape[0], fr.shape[2], fr.shape[1]).run(fr, mjd, prm) for fr, mjd, prm in (big_a, big_b, small)]
+    probe = Server(slots=1)
+    sizes = []
+    for fr, mjd, prm in (big_a, small):
+        sizes.append(probe.wait(probe.submit(fr, mjd, prm)).reserved_bytes)
+    probe.close()
+    budget_mb = (sizes[0] + 1.5 * sizes[1]) / 1048576.0
+    assert 2 * sizes[0] > budget_mb * 1048576.0
+    srv = Server(budget_mb=budget_mb, slots=3)
+    ia = srv.submit(*big_a)
+    time.sleep(0.02)  # admitted at once
+    ib = srv.submit(*big_b)
+    ic = srv.submit(*small)
+    ga, gb, gc = srv.wait(ia), srv.wait(ib), srv.wait(ic)
+    st = srv.stats()
+    srv.close()
+    print(f"overtaking: a {ga.t_admit:.3f}-{ga.t_end:.3f}, b {gb.t_admit:.3f}, c {gc.t_admit:.3f}-{gc.t_end:.3f}",
+          flush=True)
+    for a, g in zip(alone, (ga, gb, gc)):
+        _same(a, g)
+    assert gc.t_submit < ga.t_end  # the test's premise: the others arrived while the first ran
+    assert gc.t_admit < gb.t_admit  # the small job went past the blocked one
+    assert gb.t_admit >= ga.t_end - 1e-3  # the second large job waited for the first
+    assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+
+
+def test_reused_searcher_is_reserved_at_what_it_holds(hip):
+    """A large job and then a small one of the same frame shape (n, w, h) under a tight
+    budget with one slot: the small job reuses the large job's searcher, whose buffers
+    only grow, so its reservation is the bytes that searcher holds, not its own smaller
+    estimate; the budget is never exceeded and the result equals the search alone."""
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    from fainttrail_amd.serve import Server
+    n, w, h = 16, 320, 256
+    fr, mjd, _ = synth.make_sequence(n, w, h, nstars=30, nmovers=2, mover_snr=(4.0, 8.0), seed=17, device="cuda:0")
+    fr = fr.contiguous()
+    large = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.2, fwhm_px=3.0, null_runs=8, seed=3)
+    small = SearchParams(vmin_pxh=20.0, vmax_pxh=30.0, grid_tol_px=0.8, fwhm_px=3.0, null_runs=1, seed=3)
+    want = Searcher(n, w, h).run(fr, mjd, small)
+    srv = Server(slots=1)
+    a = srv.wait(srv.submit(fr, mjd, large))
+    b = srv.wait(srv.submit(fr, mjd, small))
+    st = srv.stats()
+    srv.close()
+    print(f"reuse: large reserved {a.reserved_bytes / 1e6:.1f} MB holds {a.held_bytes / 1e6:.1f} MB; "
+          f"small reserved {b.reserved_bytes / 1e6:.1f} MB holds {b.held_bytes / 1e6:.1f} MB", flush=True)
+    assert a.held_bytes <= a.reserved_bytes
+    assert b.reserved_bytes >= a.held_bytes  # reserved at what the reused searcher holds
+    assert b.held_bytes <= b.reserved_bytes
+    assert st["idle_searcher_bytes"] == pytest.approx(b.held_bytes)
+    assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+    _same(want, b)
+
+
+def test_staggered_stream_from_the_serving_distribution(hip):
+    """Jobs drawn from the serving workload's seeded Poisson stream (side 256-4096,
+    10-120 frames, 500-50k motions, 4-64 null runs, 0-15 movers; the first six whose
+    search is at most 3e12 stacked units, to keep the test short) are submitted one by
+    one while earlier ones run; each result equals the same job run alone."""
+    import time
+    from fainttrail_amd.search import Searcher
+    from fainttrail_amd.serve import Server, job_inputs, poisson_jobs
+    specs = [j for j in poisson_jobs(400, 20.0, 5)
+             if j.side * j.side * j.frames * j.velocities * (1 + j.nulls) <= 3e12][:6]
+    assert len(specs) == 6
+    inputs = [job_inputs(sp) for sp in specs]
+    alone = []
+    for fr, mjd, prm in inputs:
+        n, h, w = fr.shape
+        alone.append(Searcher(n, w, h).run(fr, mjd, prm))
+    srv = Server(slots=4)
+    ids = []
+    for k, (fr, mjd, prm) in enumerate(inputs):
+        if k:
+            time.sleep(0.1)  # later jobs arrive while earlier ones run
+        ids.append(srv.submit(fr, mjd, prm))
+    got = [srv.wait(j) for j in ids]
+    st = srv.stats()
+    srv.close()
+    for sp, g in zip(specs, got):
+        print(f"job {sp.side}^2 x {sp.frames} x {sp.velocities} x {sp.nulls} nulls: submitted {g.t_submit:.3f}, "
+              f"admitted {g.t_admit:.3f}, ended {g.t_end:.3f}", flush=True)
+    for a, g in zip(alone, got):
+        _same(a, g)
+        assert g.held_bytes <= g.reserved_bytes
+    # at least one job arrived while an earlier one was on the device
+    assert any(any(e.t_admit <= g.t_submit < e.t_end for e in got[:k]) for k, g in enumerate(got))
+    assert st["max_running"] >= 2 and st["failed"] == 0
