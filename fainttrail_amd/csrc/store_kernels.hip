@@ -13,9 +13,21 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
 
+#include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
 #include "store.hpp"
+
+// Scratch of the allocation-free FP64 evaluation (ftk_stack_fp64_noalloc): the chunks'
+// partial maxima of a strip of rows. Each GPU the library runs on holds its own copy
+// for the life of the process. Host code never uses this name as an address (on the
+// host it is a stand-in, not device memory): it takes the per-device address from
+// hipGetSymbolAddress (scratch_base below).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace ftk {
 
@@ -178,4 +190,96 @@ hipError_t launch_fp64(const float* frames, long long stride, int n, int w, int 
     return e;
 }
 
+// The FP64 evaluation with its partials in `base` (bytes of device scratch): rows go
+// in strips whose chunk partials (8 + 4 bytes per chunk and pixel) fit.
+hipError_t launch_fp64_in(const float* frames, long long stride, int n, int w, int h, int nv, const int* d_dx,
+                          const int* d_dy, int min_count, int y0, int y1, double* out, int* best, void* base,
+                          size_t bytes, hipStream_t s) {
+    if (y1 <= y0) return hipSuccess;
+    const int chunks = (nv + kFp64Chunk - 1) / kFp64Chunk;
+    const size_t row_bytes = static_cast<size_t>(w) * chunks * (sizeof(double) + sizeof(int));
+    if (row_bytes > bytes) return hipErrorInvalidValue;
+    const int rows = static_cast<int>(std::min<size_t>(32768, bytes / row_bytes));
+    for (int r = y0; r < y1; r += rows) {
+        const int r1 = std::min(y1, r + rows);
+        const long long npx = static_cast<long long>(r1 - r) * w;
+        double* cval = static_cast<double*>(base);
+        int* cidx = reinterpret_cast<int*>(cval + static_cast<long long>(chunks) * npx);
+        hipLaunchKernelGGL(fp64_chunk_kernel, dim3((w + 63) / 64, r1 - r, chunks), dim3(64), 0, s, frames, stride, n, w,
+                           h, nv, d_dx, d_dy, min_count, r, r1, r, npx, cval, cidx);
+        hipLaunchKernelGGL(fp64_reduce_kernel, dim3(static_cast<unsigned>((npx + 255) / 256)), dim3(256), 0, s, cval,
+                           cidx, chunks, npx, out + static_cast<long long>(r - y0) * w,
+                           best + static_cast<long long>(r - y0) * w);
+    }
+    return hipGetLastError();
+}
+
+namespace {
+
+// Per-device address of `scratch`, taken once per device with that device current.
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_base[kMaxDevices] = {};
+
+hipError_t scratch_base(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_base[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_base[dev] = p;
+    }
+    *out = g_scratch_base[dev];
+    return hipSuccess;
+}
+
+void put_err3(char* err, size_t len, const std::string& msg) {
+    if (!err || len == 0) return;
+    const size_t k = std::min(len - 1, msg.size());
+    std::memcpy(err, msg.data(), k);
+    err[k] = '\0';
+}
+
+}  // namespace
+
 }  // namespace ftk
+
+extern "C" int ftk_stack_fp64_noalloc(int n, int w, int h, const float* d_frames, size_t frame_stride, int nv,
+                                      const int32_t* d_dx, const int32_t* d_dy, int min_count, int y0, int y1,
+                                      double* d_max_snr, int32_t* d_best_v, void* stream, char* err, size_t err_len) {
+    using namespace ftk;
+    std::string msg;
+    if (!d_frames || !d_dx || !d_dy || !d_max_snr || !d_best_v) msg = "ftk_stack_fp64_noalloc: null argument";
+    else if (n < 1 || w < 1 || h < 1 || nv < 1 || min_count < 1) msg = "ftk_stack_fp64_noalloc: bad sizes";
+    else if (y0 < 0 || y1 > h || y1 < y0) msg = "ftk_stack_fp64_noalloc: row range outside the frame";
+    else if (frame_stride < static_cast<size_t>(w) * h) msg = "ftk_stack_fp64_noalloc: frame stride smaller than a frame";
+    else {
+        const int chunks = (nv + kFp64Chunk - 1) / kFp64Chunk;
+        const size_t row_bytes = static_cast<size_t>(w) * chunks * (sizeof(double) + sizeof(int));
+        if (row_bytes > kNoAllocScratchBytes)
+            msg = "ftk_stack_fp64_noalloc: one row of partial maxima needs " + std::to_string(row_bytes) +
+                  " bytes, more than the " + std::to_string(kNoAllocScratchBytes) + "-byte scratch (w x ceil(nv / " +
+                  std::to_string(kFp64Chunk) + ") x 12 bytes)";
+    }
+    if (!msg.empty()) {
+        put_err3(err, err_len, msg);
+        return FTK_ERROR;
+    }
+    void* base = nullptr;
+    hipError_t e = scratch_base(&base);
+    if (e != hipSuccess) {
+        put_err3(err, err_len, std::string("ftk_stack_fp64_noalloc: scratch address: ") + hipGetErrorString(e));
+        return FTK_ERROR;
+    }
+    e = launch_fp64_in(d_frames, static_cast<long long>(frame_stride), n, w, h, nv, d_dx, d_dy, min_count, y0, y1,
+                       d_max_snr, d_best_v, base, kNoAllocScratchBytes, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        put_err3(err, err_len, std::string("ftk_stack_fp64_noalloc: ") + hipGetErrorString(e));
+        return FTK_ERROR;
+    }
+    return FTK_OK;
+}

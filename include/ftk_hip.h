@@ -133,6 +133,19 @@ int ftk_plan_scales(const ftk_plan* plan, float* scales);
 int ftk_stack_fp64(int n, int w, int h, const float* d_frames, size_t frame_stride, int nv, const int32_t* dx,
                    const int32_t* dy, int min_count, int y0, int y1, double* d_max_snr, int32_t* d_best_v, void* stream,
                    char* err, size_t err_len);
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * ftk_stack_fp64 with the same kernels and results, but it allocates nothing. The shift
+ * tables d_dx / d_dy are device arrays (nv * n, motion-major) and the per-motion-chunk
+ * partial maxima live in a fixed 64 MiB device array of the library (one per GPU, held
+ * until the process ends) instead of memory allocated per call; rows go in strips that
+ * fit it. Fails (FTK_ERROR, nothing launched) when one row of partials,
+ * w * ceil(nv / 128) * 12 bytes, exceeds 64 MiB. Asynchronous on stream. Every call on
+ * one GPU shares that array: calls that overlap on one GPU (different streams or
+ * threads) race; calls in one stream are ordered. */
+int ftk_stack_fp64_noalloc(int n, int w, int h, const float* d_frames, size_t frame_stride, int nv,
+                           const int32_t* d_dx, const int32_t* d_dy, int min_count, int y0, int y1, double* d_max_snr,
+                           int32_t* d_best_v, void* stream, char* err, size_t err_len);
 
 /* ------------------------------------------------------------------ preprocessing stage
  * The time-order-dependent preprocessing of a search, build_stage
