@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
     L.ftk_emulate_pass.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32),
                                    C.c_char_p, C.c_size_t]
+    L.ftk_plan_shape.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int,
+                                 C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
     _lib = L
     return L
 
@@ -285,6 +287,20 @@ def make_backend(name: str = "auto") -> HipBackend:
     if name.startswith("hip:"):
         return HipBackend(name[4:])
     raise FtkError(f"unknown or unavailable search backend '{name}' (available: {', '.join(backend_names() + ['auto'])})")
+
+
+def plan_shape(n: int, w: int, h: int, dx, dy, min_count: int) -> dict:
+    """Host-only planning summary of a pass (ftk_plan_shape): velocity blocks, lattice
+    blocks (stacked by the lattice kernel) and the motions in them, the largest vertical
+    spread of a block, segments per tile, blocks per segment, and the motions not
+    stacked because a lower-index motion has the same shifts in every frame."""
+    dx, dy = _i32(dx), _i32(dy)
+    v = (C.c_int * 7)()
+    e = _err()
+    _raise(lib().ftk_plan_shape(n, w, h, dx.shape[0], _ptr(dx, C.c_int32), _ptr(dy, C.c_int32), int(min_count), v, e,
+                                len(e)), e)
+    return dict(zip(["blocks", "lattice_blocks", "lattice_motions", "max_sy", "segments", "blocks_per_segment",
+                     "shadowed_motions"], list(v)))
 
 
 def emulate_pass(frames: np.ndarray, dx, dy, min_count: int) -> tuple[np.ndarray, np.ndarray]:

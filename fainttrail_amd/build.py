@@ -2,7 +2,8 @@
 
 hipcc compiles for gfx950 only, with -ffp-contract=off (no fused multiply-add, the
 reference's choice for bit-identical results, CMakeLists.txt:56-60 and 86-91),
-denormals kept, and packed-FP32 instruction selection disabled.
+denormals kept, packed-FP32 instruction selection disabled, and wave-uniform branches
+left unstructured.
 """
 from __future__ import annotations
 
@@ -30,6 +31,10 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # branches on wave-uniform values stay plain scalar branches (the lattice stacking
+    # loop's per-row-class switch): structurizing them made the compiler copy the
+    # accumulators at every join
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall",
 ]
 

@@ -126,7 +126,7 @@ ftk_options default_options() {
 
 // Plan tables on the device.
 struct DevicePlan {
-    DeviceBuffer<int> shift, box, ubox, lds_off, motion, cov_x, cov_y, chain;
+    DeviceBuffer<int> shift, box, ubox, lds_off, motion, cov_x, cov_y, chain, lat;
     DeviceBuffer<long long> cov_off;
     DeviceBuffer<float> inv_sqrt;
     void upload(const StackPlan& p) {
@@ -138,6 +138,7 @@ struct DevicePlan {
         cov_x.upload(p.cov_x, "upload coverage tables");
         cov_y.upload(p.cov_y, "upload coverage tables");
         chain.upload(p.chain, "upload coverage tables");
+        lat.upload(p.lat, "upload lattice tables");
         motion.upload(p.motion, "upload motions");
         inv_sqrt.upload(p.inv_sqrt, "upload 1/sqrt table");
     }
@@ -369,6 +370,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                     L.cov_x = dev.cov_x.p;
                     L.cov_y = dev.cov_y.p;
                     L.chain = dev.chain.p;
+                    L.lat = p.nb_lat > 0 ? dev.lat.p : nullptr;
                     L.motion = dev.motion.p;
                     L.inv_sqrt = dev.inv_sqrt.p;
                     L.y_begin = y;
@@ -466,6 +468,7 @@ StackLaunch plan_launch(ftk_plan* pl) {
     L.cov_x = pl->dev.cov_x.p;
     L.cov_y = pl->dev.cov_y.p;
     L.chain = pl->dev.chain.p;
+    L.lat = p.nb_lat > 0 ? pl->dev.lat.p : nullptr;
     L.motion = pl->dev.motion.p;
     L.inv_sqrt = pl->dev.inv_sqrt.p;
     L.y_begin = 0;
@@ -646,6 +649,27 @@ extern "C" int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void
 // The kernel's algorithm on the CPU: same plan, tiles, boxes, -0.0 padding of reads
 // outside the frame, coverage counted from the full-frame bounds only on tiles that
 // are not interior, and the packed-key maximum. Shares no code with the oracle.
+extern "C" int ftk_plan_shape(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count,
+                              int* shape, char* err, size_t err_len) {
+    try {
+        if (!shape) throw std::runtime_error("ftk_plan_shape: null argument");
+        const StackPlan p = make_stack_plan(n, w, h, nv, dx, dy, min_count);
+        int lat_motions = 0;
+        for (size_t q = 0; q < static_cast<size_t>(p.nb_lat) * kVel; ++q) lat_motions += p.motion[q] >= 0;
+        shape[0] = p.nb;
+        shape[1] = p.nb_lat;
+        shape[2] = lat_motions;
+        shape[3] = p.max_sy;
+        shape[4] = p.nseg;
+        shape[5] = p.blocks_per_seg;
+        shape[6] = p.shadowed;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
 extern "C" int ftk_emulate_pass(int n, int w, int h, const float* frames, int nv, const int32_t* dx, const int32_t* dy,
                                 int min_count, float* max_snr, int32_t* best_v, char* err, size_t err_len) {
     try {
@@ -678,6 +702,29 @@ extern "C" int ftk_emulate_pass(int n, int w, int h, const float* frames, int nv
                                         : -0.0f;
                             }
                         const int* sh = &p.shift[(static_cast<size_t>(blk) * n + i) * 2 * kVel];
+                        if (blk < p.nb_lat && interior) {
+                            // the lattice kernel: column a of rows 0 .. 3 + 2 (extra pairs)
+                            // read once per lane row group, rows c_b .. c_b + 3 added to slot
+                            // 4 a + b (an unread row reads as NaN here, so a table that asked
+                            // for one would change the result)
+                            const uint32_t wx = static_cast<uint32_t>(p.lat[(static_cast<size_t>(blk) * n + i) * 2]);
+                            const uint32_t wy = static_cast<uint32_t>(p.lat[(static_cast<size_t>(blk) * n + i) * 2 + 1]);
+                            const int rows = 4 + 2 * static_cast<int>((wy >> 16) & 3u);
+                            for (int a = 0; a < kLatSide; ++a)
+                                for (int b = 0; b < kLatSide; ++b) {
+                                    const int k = kLatSide * a + b;
+                                    const int cb = static_cast<int>((wy >> (4 * b)) & 0xfu);
+                                    const int cx = static_cast<int>((wx >> (8 * a)) & 0xffu);
+                                    for (int ly = 0; ly < kTileY; ++ly)
+                                        for (int lx = 0; lx < kTileX; ++lx) {
+                                            const int q = ly % kRowsPerWave + cb;  // row of the lane's column read
+                                            const float v = q < rows ? box[static_cast<size_t>(ly + cb) * kPitch + lx + cx]
+                                                                     : std::numeric_limits<float>::quiet_NaN();
+                                            acc[(static_cast<size_t>(k) * kTileY + ly) * kTileX + lx] += v;
+                                        }
+                                }
+                            continue;
+                        }
                         for (int k = 0; k < kVel; ++k)
                             for (int ly = 0; ly < kTileY; ++ly)
                                 for (int lx = 0; lx < kTileX; ++lx)

@@ -772,7 +772,8 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const long long n_shift = static_cast<long long>(plan.nb) * n * 2 * kVel;
     const long long n_box = static_cast<long long>(plan.nb) * n * 4;
     const long long n_off = static_cast<long long>(plan.nb) * n * kVel;
-    const long long tab_ints = n_shift + n_box + n_off;
+    const long long n_lat = static_cast<long long>(plan.nb_lat) * n * 2;
+    const long long tab_ints = n_shift + n_box + n_off + n_lat;
 
     // frames of a pass in a zero-bordered buffer that holds every staged box (the ring
     // tiles then stage like interior ones), when that costs at most twice their size;
@@ -864,12 +865,13 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const size_t i_covx = part(plan.cov_x.data(), plan.cov_x.size() * 4);
     const size_t i_covy = part(plan.cov_y.data(), plan.cov_y.size() * 4);
     const size_t i_chain = part(plan.chain.data(), plan.chain.size() * 4);
+    const size_t i_lat = part(plan.lat.data(), plan.lat.size() * 4);
     const size_t i_perm = part(all_perm.data(), all_perm.size() * 4);
     const size_t i_rank = part(all_rank.data(), all_rank.size() * 4);
     std::vector<StackArgs> hargs[2];
     size_t i_args[2] = {0, 0};
     for (int q = 0; q < nsets; ++q) {
-        hargs[q].assign(static_cast<size_t>(2 * G), StackArgs{});
+        hargs[q].assign(static_cast<size_t>(kArgsKinds * G), StackArgs{});
         i_args[q] = part(hargs[q].data(), hargs[q].size() * sizeof(StackArgs));
     }
     // the previous run's copy and kernels are done with both buffers before they change
@@ -896,6 +898,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const int* d_covx = static_cast<const int*>(dptr(i_covx));
     const int* d_covy = static_cast<const int*>(dptr(i_covy));
     const int* d_chain = static_cast<const int*>(dptr(i_chain));
+    const int* d_lat0 = static_cast<const int*>(dptr(i_lat));
     const int* d_perm = static_cast<const int*>(dptr(i_perm));
     const int* d_rank = static_cast<const int*>(dptr(i_rank));
 
@@ -919,6 +922,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             L.shift = tab;
             L.box = tab + n_shift;
             L.lds_off = tab + n_shift + n_box;
+            L.lat = plan.nb_lat > 0 ? tab + n_shift + n_box + n_off : nullptr;
             L.ubox = d_ubox;
             L.cov_off = d_covoff;
             L.cov_x = d_covx;
@@ -934,7 +938,9 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             L.seg0 = 0;
             L.nseg = plan.nseg;
             L.keys = S.keys + static_cast<long long>(g) * plane;
-            make_stack_args(plan, L, hargs[q][static_cast<size_t>(g)], hargs[q][static_cast<size_t>(G + g)]);
+            StackArgs A[kArgsKinds];
+            make_stack_args(plan, L, A);
+            for (int kind = 0; kind < kArgsKinds; ++kind) hargs[q][static_cast<size_t>(kind * G + g)] = A[kind];
         }
     }
     char* const hbase = static_cast<char*>(tab_host_);
@@ -992,6 +998,9 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, bs), "permute tables");
             scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, bs), "permute tables");
             scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, bs), "permute tables");
+            if (n_lat > 0)
+                scheck(launch_permute_rows(d_lat0, tab + n_shift + n_box + n_off, plan.nb_lat, n, 2, perm, bs),
+                       "permute tables");
             float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
             // the real pass also leaves the static sky (the same for every frame order)
@@ -1021,12 +1030,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             scheck(hipStreamWaitEvent(s, L.built[q], 0), "stream wait");
         }
         const StackArgs* dargs = static_cast<const StackArgs*>(dptr(i_args[q]));
-        std::vector<StackArgs> hin(hargs[q].begin(), hargs[q].begin() + cnt);
-        std::vector<StackArgs> hring(hargs[q].begin() + G, hargs[q].begin() + G + cnt);
+        const StackArgs* hk[kArgsKinds];
+        const StackArgs* dk[kArgsKinds];
+        for (int kind = 0; kind < kArgsKinds; ++kind) {
+            hk[kind] = hargs[q].data() + kind * G;
+            dk[kind] = dargs + kind * G;
+        }
         // ring tiles on their own stream, beside the interior launch
         scheck(hipEventRecord(ev_ring_in_[q], s), "record event");
         scheck(hipStreamWaitEvent(ring_stream_, ev_ring_in_[q], 0), "stream wait");
-        scheck(launch_stack_multi(hin.data(), hring.data(), dargs, dargs + G, cnt, s, ring_stream_), "launch stack kernel");
+        scheck(launch_stack_multi(hk, dk, cnt, s, ring_stream_), "launch stack kernel");
         scheck(hipEventRecord(ev_ring_out_[q], ring_stream_), "record event");
         scheck(hipStreamWaitEvent(s, ev_ring_out_[q], 0), "stream wait");
         scheck(launch_finalize(S.keys, cnt * plane, S.maxm, S.best, s), "finalize");

@@ -28,9 +28,10 @@ double search_footprint(int n, int w, int h, long long nv, int G, int passes, in
     base += (build_lanes(n, static_cast<long long>(w) * h, passes) - 1) * (9.0 * n * P + 12.0 * P + 65536.0 * n);
     // one pass of a group: frames (a padded layout is at most twice a frame for fp32;
     // 16-bit frames take half of that), keys and maps, plan rows (shift, box, LDS
-    // offsets: 52 ints per block and frame, blocks of 16 motions, some split), peaks
+    // offsets and lattice offsets: at most 54 ints per block and frame, blocks of 16
+    // motions, some split or partly filled), peaks
     const double frame = storage == 0 ? 8.0 * P : 4.0 * P;
-    const double tab = 52.0 * 4.0 * std::ceil(1.25 * static_cast<double>(nv) / kVel) * n;
+    const double tab = 54.0 * 4.0 * std::ceil(1.25 * static_cast<double>(nv) / kVel) * n;
     const double per_pass = n * frame + 24.0 * P + tab + 16.0 * pk_cap + 12.0 * n + 1024.0;
     // the real pass goes alone first, so any search with null runs uses both sets
     const double sets = (passes > 1 ? 2.0 : 1.0) * std::max(1, G) * per_pass;

@@ -23,6 +23,13 @@
 // block lies in the frame rows present) skip all bounds and coverage work (c = n); the
 // ring of remaining tiles tests each block, and for blocks that reach outside the
 // frame zero-fills the box, loads only in-frame pixels and counts coverage.
+//
+// Lattice blocks (4 x 4 motions with dx = X[a], dy = Y[b] in every frame, slot 4 a + b)
+// have an interior instantiation of their own: per frame a lane reads each of the four
+// shifted columns once, 4 + the frame's vertical spread rows (at most 8), and adds
+// rows c_b .. c_b + 3 of column a to motion (a, b) for the frame's row offset c_b of
+// row class b, selected by a uniform branch per b so the register indices stay fixed.
+// That is at most 32 LDS dwords per lane and frame for 64 adds instead of 64.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -103,10 +110,24 @@ __device__ __forceinline__ float fma_q16(uint32_t w, bool hi, float s, float acc
     }
 }
 
+// Lattice blocks: motion (c, b) of a frame adds rows O .. O + 3 of column c. The adds
+// are opaque VALU instructions: the optimiser would otherwise merge the adds of the
+// switch cases on O into one set fed by register moves.
+template <int O>
+__device__ __forceinline__ void lat_add(float (&acc)[kVel][kRowsPerWave],
+                                        const float (&col)[kLatSide][2 * ((kLatSyMax + 1) / 2) + 4], int b) {
+#pragma unroll
+    for (int c = 0; c < kLatSide; ++c)
+#pragma unroll
+        for (int r = 0; r < kRowsPerWave; ++r)
+            asm("v_add_f32 %0, %1, %0" : "+v"(acc[kLatSide * c + b][r]) : "v"(col[c][O + r]));
+}
+
 // PRE: for 16-bit frames, box rows staged per wavefront (the kernel is instantiated
 // for the few values the planner picks from its largest vertical spread); 0 for fp32.
-template <bool RING, bool PADDED, int ST, int PRE>
+template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
 __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
+    static_assert(!LAT || (!RING && ST == kStoreF32), "lattice blocks: fp32 interior tiles");
     static_assert(ST == kStoreF32 || (ST == kStoreF16 || ST == kStoreBF16), "frame storage");
     static_assert(ST == kStoreF32 || !RING || PADDED, "16-bit frames are always padded");
     using elem_t = typename std::conditional<ST == kStoreF32, float, uint16_t>::type;
@@ -128,6 +149,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     const int* __restrict__ cov_x = a.cov_x;
     const int* __restrict__ cov_y = a.cov_y;
     const int* __restrict__ chain = a.chain;
+    const int* __restrict__ lat = a.lat;
     unsigned long long* __restrict__ keys = a.keys;
     // two frame boxes of lds_rows x kPitch floats (lds_rows = 16 + the plan's largest
     // vertical spread), sized at launch
@@ -160,6 +182,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     cint* const c_shift = as_const(shift);
     cint* const c_covy = as_const(cov_y);
     cint* const c_chain = as_const(chain);
+    cint* const c_lat = as_const(lat);
     const __attribute__((address_space(4))) long long* const c_covoff =
         (const __attribute__((address_space(4))) long long*)(cov_off);
     const int lane_base = wave * RPW * kPitch + lane;
@@ -172,8 +195,8 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
         arg[r] = -1;
     }
 
-    const int blk_begin = seg * bps;
-    const int blk_end = min(nb, blk_begin + bps);
+    const int blk_begin = max(seg * bps, a.blk_lo);
+    const int blk_end = min(min(nb, a.blk_hi), seg * bps + bps);
     for (int blk = blk_begin; blk < blk_end; ++blk) {
         bool interior = true;
         if (RING) {
@@ -250,8 +273,10 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                         d += kWaves * kPitch;
                     }
                 } else {
+                    // a lattice box has at most 16 + kLatSyMax rows: 5 per wavefront
+                    constexpr int kStageRows = LAT ? (kTileY + kLatSyMax + kWaves - 1) / kWaves : kPreRows;
 #pragma unroll
-                    for (int m = 0; m < kPreRows; ++m) {
+                    for (int m = 0; m < kStageRows; ++m) {
                         const int rr = wave + kWaves * m;
                         if (rr >= R) break;
                         const int row_b = rr * 4 * pitch;
@@ -262,12 +287,55 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
             }
         };
         auto consume = [&](const float* L, int i) {
-            cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
+            if constexpr (LAT) {
+                // column a (dx = X[a]) of rows 0 .. 3 + spread, read once; motion (a, b) adds
+                // rows c_b .. c_b + 3 of it
+                cint* lt = c_lat + 2 * (static_cast<long long>(blk) * n + i);
+                const int wx = lt[0], wy = lt[1];
+                constexpr int kRows = 2 * ((kLatSyMax + 1) / 2) + 4;  // row pairs
+                // rows past 3 are read only when the frame's spread needs them; the
+                // unread ones are never added (the plan keeps c_b <= 2 e), so they stay
+                // unset instead of costing a move each
+                float col[kLatSide][kRows];
+                const float* q[kLatSide];
 #pragma unroll
-            for (int k = 0; k < V; ++k) {
-                const float* p = L + lane_base + lo[k];
+                for (int c = 0; c < kLatSide; ++c) {
+                    q[c] = L + lane_base + ((wx >> (8 * c)) & 0xff);
 #pragma unroll
-                for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
+                    for (int r = 0; r < 4; ++r) col[c][r] = q[c][r * kPitch];
+                }
+                const int extra = (wy >> 16) & 3;  // row pairs past the first four rows
+                if (extra >= 1) {
+#pragma unroll
+                    for (int c = 0; c < kLatSide; ++c) {
+                        col[c][4] = q[c][4 * kPitch];
+                        col[c][5] = q[c][5 * kPitch];
+                    }
+                }
+                if (extra >= 2) {
+#pragma unroll
+                    for (int c = 0; c < kLatSide; ++c) {
+                        col[c][6] = q[c][6 * kPitch];
+                        col[c][7] = q[c][7 * kPitch];
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kLatSide; ++b) {
+                    switch ((wy >> (4 * b)) & 0xf) {
+                        case 0: lat_add<0>(acc, col, b); break;
+                        case 1: lat_add<1>(acc, col, b); break;
+                        case 2: lat_add<2>(acc, col, b); break;
+                        default: lat_add<3>(acc, col, b); break;
+                    }
+                }
+            } else {
+                cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const float* p = L + lane_base + lo[k];
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
+                }
             }
         };
 
@@ -502,19 +570,19 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
     }
 }
 
-template <bool RING, bool PADDED, int ST, int PRE>
+template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
 __global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
-    stack_body<RING, PADDED, ST, PRE>(a, blockIdx.x);
+    stack_body<RING, PADDED, ST, PRE, LAT>(a, blockIdx.x);
 }
 
 // Several passes in one launch (the real pass and null runs of a batched search, or
 // regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
 // is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
-template <bool RING, bool PADDED, int ST, int PRE>
+template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
 __global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
     const unsigned p = blockIdx.x / per_pass;
     const StackArgs a = passes[p];
-    stack_body<RING, PADDED, ST, PRE>(a, blockIdx.x - p * per_pass);
+    stack_body<RING, PADDED, ST, PRE, LAT>(a, blockIdx.x - p * per_pass);
 }
 
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
@@ -584,7 +652,7 @@ void interior_rect(const StackPlan& p, const StackLaunch& L, int& tx_lo, int& tx
     if (tx_hi <= tx_lo || ty_hi <= ty_lo) tx_lo = tx_hi = ty_lo = ty_hi = 0;
 }
 
-void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring) {
+void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]) {
     TileRect R{L.tiles_x, L.tiles_y, 0, 0, 0, 0};
     interior_rect(p, L, R.tx_lo, R.tx_hi, R.ty_lo, R.ty_hi);
     const long long all = static_cast<long long>(L.tiles_x) * L.tiles_y;
@@ -604,6 +672,7 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.cov_x = L.cov_x;
     a.cov_y = L.cov_y;
     a.chain = L.chain;
+    a.lat = L.lat;
     a.keys = L.keys;
     a.rect = R;
     a.row0 = L.row0;
@@ -620,6 +689,8 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
     a.y_end = L.y_end;
     a.seg0 = L.seg0;
     a.nseg = L.nseg;
+    a.blk_lo = 0;
+    a.blk_hi = p.nb;
     a.lds_rows = kTileY + p.max_sy;
     if (L.storage != kStoreF32) {
         // word rows: 15 + the largest vertical spread, PRE per wave (5, 6, 8 or 12)
@@ -628,16 +699,36 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner,
         a.lds_rows = std::max(a.lds_rows, kWaves * a.pre);
     }
     a.cov_ok = p.cov_ok ? 1 : 0;
-    inner = a;
-    ring = a;
-    inner.ntiles = static_cast<int>(in);
+    StackArgs& lat = out[kArgsLat];
+    StackArgs& gen = out[kArgsGen];
+    StackArgs& ring = out[kArgsRing];
+    lat = gen = ring = a;
     ring.ntiles = static_cast<int>(all - in);
+    lat.ntiles = gen.ntiles = static_cast<int>(in);
+    const bool split = L.storage == kStoreF32 && L.lat != nullptr && p.nb_lat > 0;
+    const int bps = std::max(1, L.bps), seg_end = L.seg0 + L.nseg;
+    if (split) {
+        // lattice blocks [0, nb_lat) and the others on interior tiles, each launch over
+        // the segments that hold its blocks. A lane of the lattice kernel reads rows
+        // 4 w .. 4 w + 3 + 2 e of a buffer (wavefront w, e = (spread + 1) / 2 row pairs), up
+        // to row 15 + 2 e: the buffers hold 16 + the largest spread rounded up to even.
+        lat.lds_rows = kTileY + 2 * ((p.max_sy_lat + 1) / 2);
+        lat.blk_hi = p.nb_lat;
+        lat.nseg = std::max(0, std::min(seg_end, (p.nb_lat + bps - 1) / bps) - L.seg0);
+        gen.blk_lo = p.nb_lat;
+        gen.seg0 = std::max(L.seg0, p.nb_lat / bps);
+        gen.nseg = std::max(0, seg_end - gen.seg0);
+        if (p.nb_lat >= p.nb) gen.nseg = 0;
+    } else {
+        lat.nseg = 0;
+    }
 }
 
 namespace {
 // workgroups of one pass: tiles padded to whole groups of 8 (one per XCD), times the
 // segments
 unsigned pass_groups(const StackArgs& a) {
+    if (a.ntiles <= 0 || a.nseg <= 0) return 0;
     return static_cast<unsigned>((static_cast<long long>(a.ntiles) + 7) / 8 * 8 * a.nseg);
 }
 // two buffers and the guard word of the 16-bit layout
@@ -645,7 +736,7 @@ size_t lds_bytes_of(int lds_rows) { return sizeof(float) * (2 * lds_rows * kPitc
 }  // namespace
 
 namespace {
-// kernel for (ring tiles, padded frames, storage, 16-bit rows per wave)
+// kernel for (kind, padded frames, storage, 16-bit rows per wave)
 using SingleFn = void (*)(StackArgs);
 using MultiFn = void (*)(const StackArgs*, unsigned);
 template <bool RING, int ST>
@@ -662,15 +753,19 @@ MultiFn multi16(int pre) {
            : pre <= 8 ? stack_multi_kernel<RING, true, ST, 8>
                       : stack_multi_kernel<RING, true, ST, 12>;
 }
-SingleFn single_kernel(bool ring, bool padded, int st, int pre) {
+SingleFn single_kernel(int kind, bool padded, int st, int pre) {
+    const bool ring = kind == kArgsRing;
     if (st == kStoreF16) return ring ? single16<true, kStoreF16>(pre) : single16<false, kStoreF16>(pre);
     if (st == kStoreBF16) return ring ? single16<true, kStoreBF16>(pre) : single16<false, kStoreBF16>(pre);
+    if (kind == kArgsLat) return stack_kernel<false, false, kStoreF32, 0, true>;
     if (!ring) return stack_kernel<false, false, kStoreF32, 0>;
     return padded ? stack_kernel<true, true, kStoreF32, 0> : stack_kernel<true, false, kStoreF32, 0>;
 }
-MultiFn multi_kernel(bool ring, bool padded, int st, int pre) {
+MultiFn multi_kernel(int kind, bool padded, int st, int pre) {
+    const bool ring = kind == kArgsRing;
     if (st == kStoreF16) return ring ? multi16<true, kStoreF16>(pre) : multi16<false, kStoreF16>(pre);
     if (st == kStoreBF16) return ring ? multi16<true, kStoreBF16>(pre) : multi16<false, kStoreBF16>(pre);
+    if (kind == kArgsLat) return stack_multi_kernel<false, false, kStoreF32, 0, true>;
     if (!ring) return stack_multi_kernel<false, false, kStoreF32, 0>;
     return padded ? stack_multi_kernel<true, true, kStoreF32, 0> : stack_multi_kernel<true, false, kStoreF32, 0>;
 }
@@ -681,52 +776,51 @@ bool storage_ok(const StackArgs& a) {
 }  // namespace
 
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s, hipStream_t ring_s) {
-    StackArgs inner, ring;
-    make_stack_args(p, L, inner, ring);
+    StackArgs A[kArgsKinds];
+    make_stack_args(p, L, A);
+    const StackArgs& ring = A[kArgsRing];
     if (!storage_ok(ring)) return hipErrorInvalidValue;
-    const size_t lds = lds_bytes_of(inner.lds_rows);
-    if (ring_s) {
-        // beside the interior launch: the edge tiles first, the interior ones fill in
-        if (ring.ntiles > 0)
-            hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage, ring.pre), dim3(pass_groups(ring)),
-                               dim3(kThreads), lds, ring_s, ring);
-        if (inner.ntiles > 0)
-            hipLaunchKernelGGL(single_kernel(false, false, inner.storage, inner.pre), dim3(pass_groups(inner)),
-                               dim3(kThreads), lds, s, inner);
-        return hipGetLastError();
-    }
-    if (inner.ntiles > 0)
-        hipLaunchKernelGGL(single_kernel(false, false, inner.storage, inner.pre), dim3(pass_groups(inner)), dim3(kThreads),
-                           lds, s, inner);
-    if (ring.ntiles > 0)
-        hipLaunchKernelGGL(single_kernel(true, ring.padded != 0, ring.storage, ring.pre), dim3(pass_groups(ring)),
-                           dim3(kThreads), lds, s, ring);
+    auto go = [&](int kind, hipStream_t st) {
+        const StackArgs& a = A[kind];
+        const unsigned g = pass_groups(a);
+        if (g > 0)
+            hipLaunchKernelGGL(single_kernel(kind, a.padded != 0, a.storage, a.pre), dim3(g), dim3(kThreads),
+                               lds_bytes_of(a.lds_rows), st, a);
+    };
+    // the edge tiles first (on their own stream when given), the interior ones fill in;
+    // the general blocks' interior launch (LDS-bound) goes beside the lattice launch
+    // (VALU-bound) on the edge stream
+    go(kArgsRing, ring_s ? ring_s : s);
+    go(kArgsLat, s);
+    go(kArgsGen, ring_s ? ring_s : s);
     return hipGetLastError();
 }
 
-hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
-                              const StackArgs* dev_ring, int count, hipStream_t s, hipStream_t ring_s) {
+hipError_t launch_stack_multi(const StackArgs* const host[kArgsKinds], const StackArgs* const dev[kArgsKinds], int count,
+                              hipStream_t s, hipStream_t ring_s) {
     if (count <= 0) return hipSuccess;
-    unsigned per_in = 0, per_ring = 0;
-    const int lds_rows = host_inner[0].lds_rows;
-    const int padded = host_ring[0].padded, storage = host_ring[0].storage, pre = host_ring[0].pre;
-    for (int q = 0; q < count; ++q) {
-        const StackArgs &a = host_inner[q], &r = host_ring[q];
-        if (a.lds_rows != lds_rows || r.lds_rows != lds_rows || r.padded != padded || a.storage != storage ||
-            r.storage != storage || a.pre != pre || r.pre != pre || !storage_ok(r))
-            return hipErrorInvalidValue;
-        per_in = std::max(per_in, pass_groups(a));
-        per_ring = std::max(per_ring, pass_groups(r));
-    }
-    const size_t lds = lds_bytes_of(lds_rows);
+    unsigned per[kArgsKinds] = {0, 0, 0};
+    const StackArgs& r0 = host[kArgsRing][0];
+    const int padded = r0.padded, storage = r0.storage, pre = r0.pre;
+    for (int kind = 0; kind < kArgsKinds; ++kind)
+        for (int q = 0; q < count; ++q) {
+            const StackArgs& a = host[kind][q];
+            // one LDS size per launch
+            if (a.lds_rows != host[kind][0].lds_rows || a.storage != storage || a.pre != pre || !storage_ok(a) ||
+                (kind == kArgsRing && a.padded != padded))
+                return hipErrorInvalidValue;
+            per[kind] = std::max(per[kind], pass_groups(a));
+        }
     const dim3 block(kThreads);
     // the ring tiles first: their workgroups take longer, the interior ones fill in
-    if (per_ring > 0)
-        hipLaunchKernelGGL(multi_kernel(true, padded != 0, storage, pre), dim3(per_ring * static_cast<unsigned>(count)),
-                           block, lds, ring_s ? ring_s : s, dev_ring, per_ring);
-    if (per_in > 0)
-        hipLaunchKernelGGL(multi_kernel(false, false, storage, pre), dim3(per_in * static_cast<unsigned>(count)), block,
-                           lds, s, dev_inner, per_in);
+    for (int kind : {static_cast<int>(kArgsRing), static_cast<int>(kArgsLat), static_cast<int>(kArgsGen)}) {
+        if (per[kind] == 0) continue;
+        // the lattice launch on s; the edge tiles and the general blocks' interior launch
+        // beside it on ring_s (when given)
+        hipLaunchKernelGGL(multi_kernel(kind, padded != 0, storage, pre), dim3(per[kind] * static_cast<unsigned>(count)),
+                           block, lds_bytes_of(host[kind][0].lds_rows), kind != kArgsLat && ring_s ? ring_s : s,
+                           dev[kind], per[kind]);
+    }
     return hipGetLastError();
 }
 

@@ -36,6 +36,7 @@ struct StackLaunch {
     const int* cov_x = nullptr;
     const int* cov_y = nullptr;
     const int* chain = nullptr;          // per block and motion: chain / anti-chain flags
+    const int* lat = nullptr;            // lattice blocks' column and row offsets per frame (plan.lat)
     const int* motion = nullptr;
     const float* inv_sqrt = nullptr;
     int y_begin = 0, y_end = 0, tiles_x = 0, tiles_y = 0;
@@ -66,32 +67,40 @@ struct StackArgs {
     const int* cov_x;
     const int* cov_y;
     const int* chain;
+    const int* lat;
     unsigned long long* keys;
     TileRect rect;
     int row0, row1, n, w, h, pitch, padded, storage, min_count, nb, bps;
     int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
     int pre;  // 16-bit frames: box rows staged per wavefront (5, 8 or 12)
-    int pad_;
+    int blk_lo, blk_hi;  // the blocks this launch stacks (of each segment's range)
 };
+
+// The descriptors of one pass: its lattice blocks on interior tiles (the lattice
+// kernel), its other blocks on interior tiles, and all blocks on the ring tiles. A kind
+// with ntiles 0 or nseg 0 has no launch.
+enum { kArgsLat = 0, kArgsGen = 1, kArgsRing = 2, kArgsKinds = 3 };
 
 hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
                             int w, int rows, int n, hipStream_t s);
 
-// Interior and ring descriptors of one launch (ntiles 0 = none of that kind).
-void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs& inner, StackArgs& ring);
+// The three descriptors of one launch (see kArgsLat / kArgsGen / kArgsRing). Lattice
+// blocks get their own launch for fp32 frames with plan.lat present (L.lat set).
+void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]);
 
-// ring_s (optional): the edge tiles' stream, ordered by the caller (both launches merge into
-// the same keys by atomic maximum, so either order gives the same bits).
+// ring_s (optional): the stream of the edge tiles' launch and of the general blocks'
+// interior launch, beside the lattice launch on s; ordered by the caller (every launch
+// merges into the same keys by atomic maximum, so any order gives the same bits).
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s, hipStream_t ring_s = nullptr);
 
-// Several passes in two launches, one over their interior tiles and one over their
-// ring tiles. host_inner / host_ring: count descriptors each, as make_stack_args
-// writes them (for the grid sizes); dev_inner / dev_ring: the same descriptors in
-// device memory. Every descriptor must have the same lds_rows. ring_s (optional): the
-// ring tiles' stream, ordered by the caller (both launches merge into the same keys by
-// atomic maximum, so either order gives the same bits).
-hipError_t launch_stack_multi(const StackArgs* host_inner, const StackArgs* host_ring, const StackArgs* dev_inner,
-                              const StackArgs* dev_ring, int count, hipStream_t s, hipStream_t ring_s = nullptr);
+// Several passes in up to three launches (lattice interior, general interior, ring).
+// host[kind] / dev[kind]: count descriptors of that kind each, as make_stack_args writes
+// them, on the host (for the grid sizes) and in device memory. The descriptors of one
+// kind must have the same lds_rows. ring_s (optional): the ring tiles' stream, ordered by the
+// caller (the launches merge into the same keys by atomic maximum, so any order gives
+// the same bits).
+hipError_t launch_stack_multi(const StackArgs* const host[kArgsKinds], const StackArgs* const dev[kArgsKinds], int count,
+                              hipStream_t s, hipStream_t ring_s = nullptr);
 hipError_t launch_fill_keys(unsigned long long* keys, long long count, hipStream_t s);
 hipError_t launch_finalize(const unsigned long long* keys, long long count, float* max_snr, int* best_v,
                            hipStream_t s);

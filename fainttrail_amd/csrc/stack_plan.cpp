@@ -3,9 +3,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <stdexcept>
+#include <tuple>
+#include <unordered_map>
 
 namespace ftk {
 
@@ -90,9 +94,10 @@ void kd_groups(std::vector<int>& idx, size_t lo, size_t hi, const int32_t* fx, c
     kd_groups(idx, mid, hi, fx, fy, out);
 }
 
-// Emit a group as blocks whose per-frame spread fits the LDS box; groups that do not
-// fit are halved along the worst frame and axis (a single motion always fits).
-void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
+// Emit a group as blocks whose per-frame spread is at most spread_max (<= kSpreadMax, so
+// it fits the LDS box); groups that do not fit are halved along the worst frame and
+// axis (a single motion always fits).
+void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p, int spread_max) {
     const int n = s.n;
     // per-frame extremes over the group (frames in the inner loop: contiguous rows)
     thread_local std::vector<int> ex;
@@ -117,13 +122,13 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
     }
     int worst_frame = -1, worst_axis = 0, worst = 0;
     for (int i = 0; i < n; ++i) {
-        if (x1[i] - x0[i] > kSpreadMax && x1[i] - x0[i] - kSpreadMax > worst) {
-            worst = x1[i] - x0[i] - kSpreadMax;
+        if (x1[i] - x0[i] > spread_max && x1[i] - x0[i] - spread_max > worst) {
+            worst = x1[i] - x0[i] - spread_max;
             worst_frame = i;
             worst_axis = 0;
         }
-        if (y1[i] - y0[i] > kSpreadMax && y1[i] - y0[i] - kSpreadMax > worst) {
-            worst = y1[i] - y0[i] - kSpreadMax;
+        if (y1[i] - y0[i] > spread_max && y1[i] - y0[i] - spread_max > worst) {
+            worst = y1[i] - y0[i] - spread_max;
             worst_frame = i;
             worst_axis = 1;
         }
@@ -133,8 +138,8 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
         auto key = [&](int j) { return tab[static_cast<size_t>(j) * n + worst_frame]; };
         std::sort(g.begin(), g.end(), [&](int a, int b) { return key(a) != key(b) ? key(a) < key(b) : a < b; });
         const size_t half = g.size() / 2;
-        emit_blocks(std::vector<int>(g.begin(), g.begin() + static_cast<ptrdiff_t>(half)), s, p);
-        emit_blocks(std::vector<int>(g.begin() + static_cast<ptrdiff_t>(half), g.end()), s, p);
+        emit_blocks(std::vector<int>(g.begin(), g.begin() + static_cast<ptrdiff_t>(half)), s, p, spread_max);
+        emit_blocks(std::vector<int>(g.begin() + static_cast<ptrdiff_t>(half), g.end()), s, p, spread_max);
         return;
     }
     // ascending motion order inside a block: the kernel's strict '>' then keeps the
@@ -184,6 +189,205 @@ void emit_blocks(std::vector<int> g, const Shifts& s, StackPlan& p) {
     ub[1] = uy0;
     ub[2] = ux1;
     ub[3] = uy1;
+}
+
+// ------------------------------------------------------------------ lattice blocks
+// Motions with the same dx in every frame form an x-class, those with the same dy an
+// y-class (for a straight-line velocity grid: one class per vx, and per vy, or fewer
+// where neighbouring velocities round alike in every frame). Classes are ordered by their
+// shift in the widest frame, then by the whole row. Four consecutive x-classes and four
+// consecutive y-classes make a lattice group; motion (x-class 4 ga + a, y-class 4 gb + b)
+// takes slot 4 a + b of the group (motions of the same classes have the same shifts in
+// every frame: only the lowest index of them is stacked). A layer becomes a lattice block when it holds at least kLatMinFill motions and
+// its four y-classes spread at most kLatSyMax rows in every frame; in a lattice block a
+// lane reads each of the four shifted columns once for all four y-classes.
+
+namespace {
+
+// Class ids of the rows of a motion-major table (n values per motion): equal rows, equal
+// ids. rep[c] is a motion of class c.
+void row_classes(const std::vector<int32_t>& T, int n, int nv, std::vector<int>& cls, std::vector<int>& rep) {
+    std::unordered_map<uint64_t, int> first;  // hash -> class (collisions probe the next hash)
+    first.reserve(static_cast<size_t>(nv) * 2);
+    cls.assign(static_cast<size_t>(nv), 0);
+    rep.clear();
+    for (int j = 0; j < nv; ++j) {
+        const int32_t* r = &T[static_cast<size_t>(j) * n];
+        uint64_t hsh = 1469598103934665603ull;
+        for (int i = 0; i < n; ++i) {
+            hsh ^= static_cast<uint32_t>(r[i]);
+            hsh *= 1099511628211ull;
+        }
+        for (;; ++hsh) {
+            auto it = first.find(hsh);
+            if (it == first.end()) {
+                first.emplace(hsh, static_cast<int>(rep.size()));
+                cls[static_cast<size_t>(j)] = static_cast<int>(rep.size());
+                rep.push_back(j);
+                break;
+            }
+            const int32_t* q = &T[static_cast<size_t>(rep[static_cast<size_t>(it->second)]) * n];
+            if (std::memcmp(q, r, sizeof(int32_t) * static_cast<size_t>(n)) == 0) {
+                cls[static_cast<size_t>(j)] = it->second;
+                break;
+            }
+        }
+    }
+}
+
+// Ranks of the classes: by the shift in frame fi, then by the whole row.
+std::vector<int> class_ranks(const std::vector<int32_t>& T, int n, int fi, const std::vector<int>& rep) {
+    std::vector<int> order(rep.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const int32_t* ra = &T[static_cast<size_t>(rep[static_cast<size_t>(a)]) * n];
+        const int32_t* rb = &T[static_cast<size_t>(rep[static_cast<size_t>(b)]) * n];
+        if (ra[fi] != rb[fi]) return ra[fi] < rb[fi];
+        return std::lexicographical_compare(ra, ra + n, rb, rb + n);
+    });
+    std::vector<int> rank(rep.size());
+    for (size_t r = 0; r < order.size(); ++r) rank[static_cast<size_t>(order[r])] = static_cast<int>(r);
+    return rank;
+}
+
+}  // namespace
+
+// Emits the lattice blocks of the plan (as blocks 0, 1, ...) and returns the motions left
+// for the general grouping, ascending.
+std::vector<int> lattice_blocks(const Shifts& s, int fi, StackPlan& p) {
+    const int n = s.n, nv = s.nv;
+    std::vector<int> cx, cy, repx, repy;
+    row_classes(s.X, n, nv, cx, repx);
+    row_classes(s.Y, n, nv, cy, repy);
+    const std::vector<int> rx = class_ranks(s.X, n, fi, repx), ry = class_ranks(s.Y, n, fi, repy);
+    std::vector<int> by_rank_x(repx.size()), by_rank_y(repy.size());  // rank -> a motion of that class
+    for (size_t c = 0; c < repx.size(); ++c) by_rank_x[static_cast<size_t>(rx[c])] = repx[c];
+    for (size_t c = 0; c < repy.size(); ++c) by_rank_y[static_cast<size_t>(ry[c])] = repy[c];
+    // a motion whose shifts equal a lower-index motion's in every frame (same x- and
+    // y-class) is never stacked: its sums and counts are those of the lower-index motion
+    // bit for bit, so it never beats it, and ties go to the lower index
+    std::map<std::pair<int, int>, int> seen;
+    std::vector<char> stack(static_cast<size_t>(nv), 1);
+    for (int j = 0; j < nv; ++j)
+        if (!seen.emplace(std::make_pair(cx[static_cast<size_t>(j)], cy[static_cast<size_t>(j)]), j).second) {
+            stack[static_cast<size_t>(j)] = 0;
+            ++p.shadowed;
+        }
+    // (ga, gb, layer) -> 16 slots
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    std::map<std::pair<int, int>, int> layers;
+    for (int j = 0; j < nv; ++j) {
+        if (!stack[static_cast<size_t>(j)]) continue;
+        const int ax = rx[static_cast<size_t>(cx[static_cast<size_t>(j)])];
+        const int by = ry[static_cast<size_t>(cy[static_cast<size_t>(j)])];
+        const int ga = ax / kLatSide, gb = by / kLatSide, slot = (ax % kLatSide) * kLatSide + by % kLatSide;
+        int& nl = layers[{ga, gb}];
+        int l = 0;
+        for (; l < nl; ++l) {
+            auto& g = groups[{ga, gb, l}];
+            if (g[static_cast<size_t>(slot)] < 0) {
+                g[static_cast<size_t>(slot)] = j;
+                break;
+            }
+        }
+        if (l == nl) {
+            auto& g = groups[{ga, gb, nl}];
+            g.assign(kVel, -1);
+            g[static_cast<size_t>(slot)] = j;
+            ++nl;
+        }
+    }
+    std::vector<int> left;
+    const int nxr = static_cast<int>(repx.size()), nyr = static_cast<int>(repy.size());
+    for (const auto& kv : groups) {
+        const int ga = std::get<0>(kv.first), gb = std::get<1>(kv.first);
+        const std::vector<int>& g = kv.second;
+        const int fill = static_cast<int>(std::count_if(g.begin(), g.end(), [](int j) { return j >= 0; }));
+        // the classes of the group's columns and rows (past the last class: the first one)
+        const int32_t* X[kLatSide];
+        const int32_t* Y[kLatSide];
+        for (int a = 0; a < kLatSide; ++a) {
+            const int r = kLatSide * ga + a < nxr ? kLatSide * ga + a : kLatSide * ga;
+            X[a] = s.xm(by_rank_x[static_cast<size_t>(r)]);
+        }
+        for (int b = 0; b < kLatSide; ++b) {
+            const int r = kLatSide * gb + b < nyr ? kLatSide * gb + b : kLatSide * gb;
+            Y[b] = s.ym(by_rank_y[static_cast<size_t>(r)]);
+        }
+        bool ok = fill >= kLatMinFill;
+        for (int i = 0; i < n && ok; ++i) {
+            int x0 = X[0][i], x1 = X[0][i], y0 = Y[0][i], y1 = Y[0][i];
+            for (int a = 1; a < kLatSide; ++a) {
+                x0 = std::min(x0, X[a][i]);
+                x1 = std::max(x1, X[a][i]);
+            }
+            for (int b = 1; b < kLatSide; ++b) {
+                y0 = std::min(y0, Y[b][i]);
+                y1 = std::max(y1, Y[b][i]);
+            }
+            ok = x1 - x0 <= kSpreadMax && y1 - y0 <= kLatSyMax;
+        }
+        if (!ok) {
+            for (int j : g)
+                if (j >= 0) left.push_back(j);
+            continue;
+        }
+        const int b = p.nb++;
+        p.motion.resize(static_cast<size_t>(p.nb) * kVel, -1);
+        p.shift.resize(static_cast<size_t>(p.nb) * n * 2 * kVel, 0);
+        p.box.resize(static_cast<size_t>(p.nb) * n * 4, 0);
+        p.ubox.resize(static_cast<size_t>(p.nb) * 4, 0);
+        p.lds_off.resize(static_cast<size_t>(p.nb) * n * kVel, 0);
+        p.lat.resize(static_cast<size_t>(p.nb) * n * 2, 0);
+        for (int k = 0; k < kVel; ++k) p.motion[static_cast<size_t>(b) * kVel + k] = g[static_cast<size_t>(k)];
+        int ux0 = std::numeric_limits<int>::max(), uy0 = ux0, ux1 = std::numeric_limits<int>::min(), uy1 = ux1;
+        for (int i = 0; i < n; ++i) {
+            int x0 = X[0][i], x1 = X[0][i], y0 = Y[0][i], y1 = Y[0][i];
+            for (int a = 1; a < kLatSide; ++a) {
+                x0 = std::min(x0, X[a][i]);
+                x1 = std::max(x1, X[a][i]);
+            }
+            for (int q = 1; q < kLatSide; ++q) {
+                y0 = std::min(y0, Y[q][i]);
+                y1 = std::max(y1, Y[q][i]);
+            }
+            int32_t* sh = &p.shift[(static_cast<size_t>(b) * n + i) * 2 * kVel];
+            int32_t* lo = &p.lds_off[(static_cast<size_t>(b) * n + i) * kVel];
+            uint32_t wx = 0, wy = 0;
+            for (int a = 0; a < kLatSide; ++a) wx |= static_cast<uint32_t>(X[a][i] - x0) << (8 * a);
+            for (int q = 0; q < kLatSide; ++q) wy |= static_cast<uint32_t>(Y[q][i] - y0) << (4 * q);
+            wy |= static_cast<uint32_t>((y1 - y0 + 1) / 2) << 16;
+            for (int a = 0; a < kLatSide; ++a)
+                for (int q = 0; q < kLatSide; ++q) {
+                    // padding slots take their column's and row's shifts: inside the box
+                    const int k = kLatSide * a + q;
+                    sh[k] = X[a][i];
+                    sh[kVel + k] = Y[q][i];
+                    lo[k] = (Y[q][i] - y0) * kPitch + (X[a][i] - x0);
+                }
+            int32_t* bx = &p.box[(static_cast<size_t>(b) * n + i) * 4];
+            bx[0] = x0;
+            bx[1] = y0;
+            bx[2] = x1 - x0;
+            bx[3] = y1 - y0;
+            p.max_sy = std::max(p.max_sy, y1 - y0);
+            p.max_sy_lat = std::max(p.max_sy_lat, y1 - y0);
+            p.lat[(static_cast<size_t>(b) * n + i) * 2] = static_cast<int32_t>(wx);
+            p.lat[(static_cast<size_t>(b) * n + i) * 2 + 1] = static_cast<int32_t>(wy);
+            ux0 = std::min(ux0, x0);
+            ux1 = std::max(ux1, x1);
+            uy0 = std::min(uy0, y0);
+            uy1 = std::max(uy1, y1);
+        }
+        int32_t* ub = &p.ubox[static_cast<size_t>(b) * 4];
+        ub[0] = ux0;
+        ub[1] = uy0;
+        ub[2] = ux1;
+        ub[3] = uy1;
+    }
+    p.nb_lat = p.nb;
+    std::sort(left.begin(), left.end());
+    return left;
 }
 
 // Cumulative shift counts per block and motion over the block's shift range: coverage
@@ -284,18 +488,21 @@ StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const 
         fx[static_cast<size_t>(j)] = s.xm(j)[fi];
         fy[static_cast<size_t>(j)] = s.ym(j)[fi];
     }
-    std::vector<int> idx(static_cast<size_t>(nv));
-    std::iota(idx.begin(), idx.end(), 0);
+    // lattice blocks first, then the remaining motions in compact groups
+    std::vector<int> idx = lattice_blocks(s, fi, p);
     std::vector<std::vector<int>> groups;
     groups.reserve(static_cast<size_t>(nv) / kVel + 1);
-    kd_groups(idx, 0, idx.size(), fx.data(), fy.data(), groups);
-    const size_t est = groups.size() + groups.size() / 4 + 1;  // blocks (some groups split)
+    if (!idx.empty()) kd_groups(idx, 0, idx.size(), fx.data(), fy.data(), groups);
+    const size_t est = static_cast<size_t>(p.nb) + groups.size() + groups.size() / 4 + 1;  // blocks (some groups split)
     p.motion.reserve(est * kVel);
     p.shift.reserve(est * n * 2 * kVel);
     p.box.reserve(est * n * 4);
     p.ubox.reserve(est * 4);
     p.lds_off.reserve(est * n * kVel);
-    for (auto& g : groups) emit_blocks(std::move(g), s, p);
+    // the motions left beside lattice blocks are sparse (the rim of a velocity grid): their
+    // blocks are cut smaller so their boxes stay narrow
+    const int spread = p.nb_lat > 0 ? kLeftSpreadMax : kSpreadMax;
+    for (auto& g : groups) emit_blocks(std::move(g), s, p, spread);
     build_coverage(p);
     p.all_box[0] = p.all_box[1] = std::numeric_limits<int>::max();
     p.all_box[2] = p.all_box[3] = std::numeric_limits<int>::min();

@@ -27,11 +27,18 @@ constexpr int kSpreadMax = 32;                 // largest per-frame shift spread
 constexpr int kPitch = 128;                     // LDS row pitch (floats): rows 0..3 of a lane are 2 x 64 dwords apart (ds_read2st64)
 constexpr int kLdsRows = kTileY + kSpreadMax;  // LDS rows per staged frame
 constexpr int kThreads = kTileX * kWaves;      // 256
+// Lattice blocks: 4 x 4 motions whose shifts factor in every frame as dx = X[a], dy = Y[b]
+// (a, b = 0..3; motion slot k = 4 a + b), with a vertical spread max Y - min Y of at most
+// kLatSyMax in every frame.
+constexpr int kLatSide = 4;
+constexpr int kLatSyMax = 3;
+constexpr int kLatMinFill = 12;                // fewer motions in a lattice group: general blocks
+constexpr int kLeftSpreadMax = 8;              // spread limit of the general blocks beside lattice blocks
 
 struct StackPlan {
     int n = 0, w = 0, h = 0, nv = 0, min_count = 1;
     int nb = 0;                      // velocity blocks
-    std::vector<int32_t> motion;     // nb x kVel motion indices, ascending; -1 = padding
+    std::vector<int32_t> motion;     // nb x kVel motion indices (general blocks: ascending); -1 = padding
     std::vector<int32_t> shift;      // nb x n x (kVel dx, kVel dy), clamped to +-w, +-h
     std::vector<int32_t> box;        // nb x n x 4: min dx, min dy, spread x, spread y
     std::vector<int32_t> ubox;       // nb x 4: min dx, min dy, max dx, max dy over all frames
@@ -45,6 +52,16 @@ struct StackPlan {
     // anti-chain (<= 0), else 0. Then the frames landing inside both ranges are an
     // intersection of two intervals of one order, counted from cov_x and cov_y alone.
     std::vector<int32_t> chain;
+    // Blocks [0, nb_lat) are lattice blocks. lat: nb_lat x n x 2 words per block and frame:
+    // word 0 holds the four column offsets X[a] - min X in bits 8a..8a+7, word 1 the four row
+    // offsets Y[b] - min Y in bits 4b..4b+3 and, in bits 16..17, the row pairs a lane reads
+    // beyond its first two: (spread + 1) / 2 (0, 1 or 2).
+    int nb_lat = 0;
+    std::vector<int32_t> lat;
+    int max_sy_lat = 0;  // largest per-frame vertical spread of a lattice block (<= kLatSyMax)
+    // motions not stacked because a lower-index motion has the same shifts in every frame
+    // (equal sums and counts: the lower index always wins the tie)
+    int shadowed = 0;
     bool cov_ok = true;  // false: the tables would be too large; the kernel walks the frames
     int all_box[4] = {0, 0, 0, 0};   // union of ubox over the blocks: min dx, min dy, max dx, max dy
     std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))

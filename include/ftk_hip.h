@@ -324,6 +324,16 @@ int ftk_copy_f32(const float* d_src, float* d_dst, size_t count, void* stream);
  * kernel's inner loop, which reads one fp32 sample from LDS per stacked unit. */
 int ftk_probe_lds(int width_bytes, double* bytes_per_s);
 
+/* Host-only planning summary of a stacking pass (no device needed): shape[7] receives
+ * the velocity blocks, the lattice blocks among them (4 x 4 motions sharing their
+ * columns' dx and rows' dy in every frame, stacked by the lattice kernel), the motions
+ * in lattice blocks, the largest per-frame vertical spread of a block, the segments
+ * per tile, the blocks per segment and the motions not stacked because a lower-index
+ * motion has the same shifts in every frame (identical sums: never the maximum). No
+ * reference counterpart. */
+int ftk_plan_shape(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count, int* shape,
+                   char* err, size_t err_len);
+
 /* Host emulation of the HIP algorithm (tiles, velocity blocks, -0.0 padding, edge
  * coverage counting, packed-key maximum) on the CPU, for tests without a GPU; the
  * analogue of emulate_cuda_pass (src/cuda_plan.cpp:74-117). frames: n x h x w

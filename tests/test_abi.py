@@ -57,6 +57,53 @@ def test_emulation_matches_reference(golden, name):
     assert same_bits(m, g["max_snr"]) and np.array_equal(b, g["best_v"])
 
 
+LATTICE_CASES = [  # (frames, w, h, vmin, vmax, grid_tol, cadence h, least lattice share of the blocks)
+    (16, 200, 150, 4.0, 60.0, 0.3, 0.05, 0.6),
+    (20, 180, 140, 12.0, 66.0, 0.125, 1 / 60, 0.6),  # many motions with equal shift tables
+    (9, 150, 90, 2.0, 30.0, 0.6, 0.3, 0.2),  # a coarse grid: many groups spread too far, general blocks
+]
+
+
+@pytest.mark.parametrize("case", LATTICE_CASES)
+def test_emulation_with_lattice_blocks_equals_the_oracle(case):
+    """Straight-line grids plan mostly into lattice blocks (4 x 4 motions sharing each
+    column's dx and each row's dy in every frame). The host emulation stacks their
+    interior tiles the lattice kernel's way (four column reads of 4 + spread rows per
+    lane, rows c_b .. c_b + 3 per row class, unread rows as NaN) and motions whose
+    shifts equal a lower-index motion's are not stacked: maps and indices still equal
+    the blocked CPU oracle bit for bit (NaN sample included)."""
+    from oracle import pyoracle as po
+    n, w, h, vmin, vmax, tol, dt, share = case
+    t = (np.arange(n) - (n - 1) / 2.0) * dt
+    frames = po.test_normals(99 + n, (n, h, w))
+    frames[3, 40, 50] = np.nan
+    grid, _ = ft.velocity_grid(float(t[-1] - t[0]), vmin, vmax, tol)
+    tab = ft.linear_shift_table(t, grid)
+    mc = max(3, int(np.ceil(0.7 * n)))
+    shape = ft.plan_shape(n, w, h, tab.dx, tab.dy, mc)
+    assert shape["lattice_blocks"] > share * shape["blocks"]
+    m, b = ft.emulate_pass(frames, tab.dx, tab.dy, mc)
+    wm, wb = po.stack_blocked(frames, tab.dx, tab.dy, mc)
+    assert same_bits(m, wm) and np.array_equal(b, wb)
+
+
+def test_plan_shapes_of_the_benchmark_grids():
+    """Lattice blocks hold nearly all stacked motions of the benchmark grids; equal shift
+    tables (neighbouring velocities that round alike in every frame) are stacked once."""
+    cases = [(60, 6144, 4096, 6.0, 120.0, 0.372, 0), (20, 512, 512, 12.0, 66.0, 0.125, 2528),
+             (40, 2048, 2048, 12.0, 66.0, 0.212, 820)]
+    for n, w, h, vmin, vmax, tol, shadowed in cases:
+        t = (np.arange(n) - (n - 1) / 2.0) / 60.0
+        grid, _ = ft.velocity_grid(float(t[-1] - t[0]), vmin, vmax, tol)
+        tab = ft.linear_shift_table(t, grid)
+        s = ft.plan_shape(n, w, h, tab.dx, tab.dy, max(3, int(np.ceil(0.7 * n))))
+        stacked = len(grid) - s["shadowed_motions"]
+        assert s["shadowed_motions"] == shadowed
+        assert s["lattice_motions"] >= 0.94 * stacked
+        assert s["blocks"] * 16 <= 1.06 * stacked  # little padding
+        assert s["max_sy"] <= 8
+
+
 def test_emulation_check_backend_inputs(golden):
     fr, mc = golden["check0/frames"], int(golden["check0/min_count"])
     for key in ("check0", "check0c"):
