@@ -28,8 +28,8 @@
 // have an interior instantiation of their own: per frame a lane reads each of the four
 // shifted columns once, 4 + the frame's vertical spread rows (at most 8), and adds
 // rows c_b .. c_b + 3 of column a to motion (a, b) for the frame's row offset c_b of
-// row class b, selected by a uniform branch per b so the register indices stay fixed.
-// That is at most 32 LDS dwords per lane and frame for 64 adds instead of 64.
+// row class b, the adds' source registers offset by c_b through GPR indexing (no
+// branch). That is at most 32 LDS dwords per lane and frame for 64 adds instead of 64.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -110,17 +110,35 @@ __device__ __forceinline__ float fma_q16(uint32_t w, bool hi, float s, float acc
     }
 }
 
-// Lattice blocks: motion (c, b) of a frame adds rows O .. O + 3 of column c. The adds
-// are opaque VALU instructions: the optimiser would otherwise merge the adds of the
-// switch cases on O into one set fed by register moves.
-template <int O>
-__device__ __forceinline__ void lat_add(float (&acc)[kVel][kRowsPerWave],
-                                        const float (&col)[kLatSide][2 * ((kLatSyMax + 1) / 2) + 4], int b) {
-#pragma unroll
-    for (int c = 0; c < kLatSide; ++c)
-#pragma unroll
-        for (int r = 0; r < kRowsPerWave; ++r)
-            asm("v_add_f32 %0, %1, %0" : "+v"(acc[kLatSide * c + b][r]) : "v"(col[c][O + r]));
+// Row class B of a lattice frame: motion (a, B) adds rows c .. c + 3 of column a. The
+// column values sit in v40 .. v71 (column a, row q in v(40 + 8 a + q)); s_set_gpr_idx_on
+// adds c to the source register number of the adds that follow, so one instruction
+// sequence serves every row offset without a branch.
+template <int B>
+__device__ __forceinline__ void lat_add_idx(float (&acc)[kVel][kRowsPerWave], const float (&col)[kLatSide][8], int c) {
+    asm volatile(
+        "s_set_gpr_idx_on %16, gpr_idx(SRC0)\n\t"
+        "s_nop 0\n\t"
+        "v_add_f32 %0, v40, %0\n\t"
+        "v_add_f32 %1, v41, %1\n\t"
+        "v_add_f32 %2, v42, %2\n\t"
+        "v_add_f32 %3, v43, %3\n\t"
+        "v_add_f32 %4, v48, %4\n\t"
+        "v_add_f32 %5, v49, %5\n\t"
+        "v_add_f32 %6, v50, %6\n\t"
+        "v_add_f32 %7, v51, %7\n\t"
+        "v_add_f32 %8, v56, %8\n\t"
+        "v_add_f32 %9, v57, %9\n\t"
+        "v_add_f32 %10, v58, %10\n\t"
+        "v_add_f32 %11, v59, %11\n\t"
+        "v_add_f32 %12, v64, %12\n\t"
+        "v_add_f32 %13, v65, %13\n\t"
+        "v_add_f32 %14, v66, %14\n\t"
+        "v_add_f32 %15, v67, %15\n\t"
+        "s_set_gpr_idx_off"
+        : "+v"(acc[0+B][0]), "+v"(acc[0+B][1]), "+v"(acc[0+B][2]), "+v"(acc[0+B][3]), "+v"(acc[4+B][0]), "+v"(acc[4+B][1]), "+v"(acc[4+B][2]), "+v"(acc[4+B][3]), "+v"(acc[8+B][0]), "+v"(acc[8+B][1]), "+v"(acc[8+B][2]), "+v"(acc[8+B][3]), "+v"(acc[12+B][0]), "+v"(acc[12+B][1]), "+v"(acc[12+B][2]), "+v"(acc[12+B][3])
+        : "s"(c), "{v40}"(col[0][0]), "{v41}"(col[0][1]), "{v42}"(col[0][2]), "{v43}"(col[0][3]), "{v44}"(col[0][4]), "{v45}"(col[0][5]), "{v46}"(col[0][6]), "{v47}"(col[0][7]), "{v48}"(col[1][0]), "{v49}"(col[1][1]), "{v50}"(col[1][2]), "{v51}"(col[1][3]), "{v52}"(col[1][4]), "{v53}"(col[1][5]), "{v54}"(col[1][6]), "{v55}"(col[1][7]), "{v56}"(col[2][0]), "{v57}"(col[2][1]), "{v58}"(col[2][2]), "{v59}"(col[2][3]), "{v60}"(col[2][4]), "{v61}"(col[2][5]), "{v62}"(col[2][6]), "{v63}"(col[2][7]), "{v64}"(col[3][0]), "{v65}"(col[3][1]), "{v66}"(col[3][2]), "{v67}"(col[3][3]), "{v68}"(col[3][4]), "{v69}"(col[3][5]), "{v70}"(col[3][6]), "{v71}"(col[3][7])
+        : "m0");
 }
 
 // PRE: for 16-bit frames, box rows staged per wavefront (the kernel is instantiated
@@ -319,15 +337,10 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
                         col[c][7] = q[c][7 * kPitch];
                     }
                 }
-#pragma unroll
-                for (int b = 0; b < kLatSide; ++b) {
-                    switch ((wy >> (4 * b)) & 0xf) {
-                        case 0: lat_add<0>(acc, col, b); break;
-                        case 1: lat_add<1>(acc, col, b); break;
-                        case 2: lat_add<2>(acc, col, b); break;
-                        default: lat_add<3>(acc, col, b); break;
-                    }
-                }
+                lat_add_idx<0>(acc, col, wy & 0xf);
+                lat_add_idx<1>(acc, col, (wy >> 4) & 0xf);
+                lat_add_idx<2>(acc, col, (wy >> 8) & 0xf);
+                lat_add_idx<3>(acc, col, (wy >> 12) & 0xf);
             } else {
                 cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
 #pragma unroll
