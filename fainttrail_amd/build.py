@@ -20,7 +20,7 @@ LIB = PKG / "libftk_hip.so"
 
 HIP_SOURCES = ["stack_kernel.hip", "probe_kernel.hip", "prep_kernels.hip", "search_kernels.hip", "search_device.hip",
                "ftk_api.hip", "ftk_prep_api.hip", "ftk_search_api.hip", "store_kernels.hip", "serve.hip", "screen_kernels.hip",
-               "stack_plan.cpp"]
+               "refine_kernels.hip", "stack_plan.cpp", "candidates.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -52,6 +52,8 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->max_frame_fraction = c.max_frame_fraction;
     p->reg_dx = nullptr;
     p->reg_dy = nullptr;
+    p->candidates = c.candidates;
+    p->half_tolerance = c.half_tolerance;
 }
 
 namespace {
@@ -80,6 +82,8 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.storage = p->storage;
     c.null_tested = p->null_tested;
     c.max_frame_fraction = p->max_frame_fraction;
+    c.candidates = p->candidates;
+    c.half_tolerance = p->half_tolerance;
     return c;
 }
 
@@ -111,6 +115,7 @@ void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
     sum->segments = o.segments;
     sum->n_screened = o.screened;
     sum->n_artefacts = o.artefacts;
+    sum->n_candidates = static_cast<int>(o.candidates.size());
     for (const auto& t : o.timings) {
         if (t.first == "prepare") sum->t_prepare += t.second;
         else if (t.first == "stack (real)") sum->t_stack_real += t.second;
@@ -119,6 +124,7 @@ void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
         else if (t.first.rfind("peaks", 0) == 0) sum->t_peaks += t.second;
         else if (t.first == "plan") sum->t_plan += t.second;
         else if (t.first == "passes (batched)") sum->t_passes += t.second;
+        else if (t.first == "refine") sum->t_refine += t.second;
         else sum->t_threshold += t.second;
     }
 }
@@ -134,6 +140,7 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
     if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
         throw std::runtime_error("ftk_searcher_run: unknown frame storage");
     if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
+    if (p->candidates != 0 && p->candidates != 1) throw std::runtime_error("ftk_searcher_run: candidates must be 0 or 1");
 }
 }  // namespace
 
@@ -227,6 +234,42 @@ extern "C" int ftk_searcher_peaks(const ftk_searcher* s, int which, int cap, int
                                   int32_t* vidx) {
     if (!s) return -1;
     return copy_peaks(s->last, which, cap, x, y, snr, vidx);
+}
+
+namespace {
+int copy_candidates(const ftk::SearchOutput& o, int cap, ftk_candidate* out) {
+    const int c = static_cast<int>(o.candidates.size());
+    for (int k = 0; k < c && k < cap && out; ++k) {
+        const ftk::SearchCandidate& s = o.candidates[static_cast<size_t>(k)];
+        ftk_candidate& r = out[k];
+        std::memset(&r, 0, sizeof r);
+        r.id = s.id;
+        r.ok = s.ok ? 1 : 0;
+        r.x = s.x;
+        r.y = s.y;
+        r.vx_pxh = s.vx_pxh;
+        r.vy_pxh = s.vy_pxh;
+        r.pos_err_px = s.pos_err_px;
+        r.vel_err_pxh = s.vel_err_pxh;
+        r.snr = s.snr;
+        r.snr_coarse = s.snr_coarse;
+        r.snr_half1 = s.snr_half1;
+        r.snr_half2 = s.snr_half2;
+        r.max_frame_fraction = s.max_frame_fraction;
+        r.false_alarms_per_run = s.false_alarms_per_run;
+        r.flux = s.flux;
+        r.mag_inst = s.mag_inst;
+        r.inv_dist_au = s.inv_dist_au;
+        r.inv_dist_err_au = s.inv_dist_err_au;
+        std::strncpy(r.flags, s.flags.c_str(), sizeof(r.flags) - 1);
+    }
+    return c;
+}
+}  // namespace
+
+extern "C" int ftk_searcher_candidates(const ftk_searcher* s, int cap, ftk_candidate* out) {
+    if (!s) return -1;
+    return copy_candidates(s->last, cap, out);
 }
 
 // find_peaks (src/search.cpp:105-135) over device maps: the local maxima are found on
@@ -335,6 +378,7 @@ extern "C" int ftk_server_submit(ftk_server* s, const float* d_raw, size_t strid
             throw std::runtime_error("ftk_server_submit: unknown frame storage");
         if (!(p->fwhm_px > 0.0)) throw std::runtime_error("search: fwhm_px must be given (> 0)");
         if (p->null_tested < 0) throw std::runtime_error("ftk_server_submit: null_tested must be >= 0");
+        if (p->candidates != 0 && p->candidates != 1) throw std::runtime_error("ftk_server_submit: candidates must be 0 or 1");
         ftk::SearchConfig c = config_of(p);
         set_offsets(c, p, n);
         *job = s->srv->submit(d_raw, static_cast<long long>(stride), n, w, h, mjd, std::move(c));
@@ -373,6 +417,13 @@ extern "C" int ftk_server_peaks(ftk_server* s, long long job, int which, int cap
     const ftk::ServerJob* j = s->srv->job(job);
     if (!j || j->status != 2) return -1;
     return copy_peaks(j->out, which, cap, x, y, snr, vidx);
+}
+
+extern "C" int ftk_server_candidates(ftk_server* s, long long job, int cap, ftk_candidate* out) {
+    if (!s) return -1;
+    const ftk::ServerJob* j = s->srv->job(job);
+    if (!j || j->status != 2) return -1;
+    return copy_candidates(j->out, cap, out);
 }
 
 extern "C" void ftk_server_release(ftk_server* s, long long job) {

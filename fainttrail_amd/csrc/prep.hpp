@@ -45,16 +45,17 @@ public:
                     hipStream_t s);
 
     // build_stage for the frame times t_h (only their order matters): writes the
-    // normalised matched-filtered frames; optional outputs may be null.
+    // normalised matched-filtered frames; optional outputs may be null. resid_out
+    // receives the residuals (frame - background - static sky, n planes of w x h).
     void build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm, float* filtered,
                long long filt_stride, float* static_sky, double* sigma_host, float* mask_out, int* cr_host,
-               hipStream_t s);
+               hipStream_t s, float* resid_out = nullptr);
     // The same with the time ranks already on the device (d_rank: n ints, time_ranks of
     // the frame times). Enqueues work only (no host copy) once the filter taps for
     // prm.fwhm_px are in place, so several builds can be queued on several streams.
     void build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
                       float* filtered, long long filt_stride, float* static_sky, double* sigma_host, float* mask_out,
-                      int* cr_host, hipStream_t s);
+                      int* cr_host, hipStream_t s, float* resid_out = nullptr);
     // Matched-filter taps for prm.fwhm_px (uploaded when they change; synchronises s then).
     void set_taps(const StageParams& prm, hipStream_t s);
     // Device bytes of the stage's scratch (the sizes asked of hipMalloc).

@@ -623,11 +623,12 @@ void time_ranks(const double* t_h, int n, int* rank) {
 
 void DeviceStage::build(const float* sub, long long sub_stride, const double* t_h, const StageParams& prm,
                         float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
-                        float* mask_out, int* cr_host, hipStream_t s) {
+                        float* mask_out, int* cr_host, hipStream_t s, float* resid_out) {
     std::vector<int> rank(static_cast<size_t>(n));
     time_ranks(t_h, n, rank.data());
     hcheck(hipMemcpyAsync(rank_, rank.data(), sizeof(int) * n, hipMemcpyHostToDevice, s), "upload ranks");
-    build_ranked(sub, sub_stride, rank_, prm, filtered, filt_stride, static_sky, sigma_host, mask_out, cr_host, s);
+    build_ranked(sub, sub_stride, rank_, prm, filtered, filt_stride, static_sky, sigma_host, mask_out, cr_host, s,
+                 resid_out);
 }
 
 void DeviceStage::set_taps(const StageParams& prm, hipStream_t s) {
@@ -655,7 +656,7 @@ void DeviceStage::set_taps(const StageParams& prm, hipStream_t s) {
 
 void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
                                float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
-                               float* mask_out, int* cr_host, hipStream_t s) {
+                               float* mask_out, int* cr_host, hipStream_t s, float* resid_out) {
     const int exclude = prm.static_exclude >= 0 ? prm.static_exclude : std::max(1, static_cast<int>(std::lround(n / 8.0)));
     const int min_finite = static_cast<int>(std::max<size_t>(5, static_cast<size_t>(0.6 * n)));
     const double psf_sigma = prm.fwhm_px * kFwhmToSigmaD;
@@ -668,6 +669,8 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
                 : n <= 64 ? sky_kernel<64> : sky_kernel<0>;
     hipLaunchKernelGGL(sky_fn, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
                        exclude, min_finite, sky, resid_, plane);
+    if (resid_out)
+        hcheck(hipMemcpyAsync(resid_out, resid_, sizeof(float) * n * plane, hipMemcpyDeviceToDevice, s), "copy residuals");
     // per-frame residual noise: robust sigma of the grid samples, border 2
     select(resid_, plane, 2, nullptr, false, 1, s);
     select(resid_, plane, 2, nullptr, true, 1, s);

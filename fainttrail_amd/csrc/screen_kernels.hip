@@ -11,45 +11,12 @@
 #include <cstdint>
 
 #include "screen.hpp"
-#include "stack_launch.hpp"
+#include "track_device.hpp"
 
 namespace ftk {
 namespace {
 
 constexpr int kWave = 64;
-
-// Frame i's element (x, y) of a pass: fp32, or 16-bit times the frame's power-of-two
-// scale (the value the stacking kernel adds).
-__device__ inline float frame_at(const ScreenFrames& f, int storage, int i, int x, int y) {
-    const long long k = static_cast<long long>(i) * f.fstride + static_cast<long long>(y) * f.pitch + x;
-    if (storage == kStoreF32) return static_cast<const float*>(f.frames)[k];
-    const uint16_t q = static_cast<const uint16_t*>(f.frames)[k];
-    float v;
-    if (storage == kStoreBF16) {
-        v = __uint_as_float(static_cast<unsigned>(q) << 16);
-    } else {
-        _Float16 hv;
-        __builtin_memcpy(&hv, &q, sizeof q);
-        v = static_cast<float>(hv);
-    }
-    return v * f.scale[i];
-}
-
-// bilinear (src/common.hpp:125-138): NaN outside, except exactly on the last row or
-// column; the weights and the sum in double, rounded to float once.
-__device__ inline float bilinear(const ScreenFrames& f, int storage, int w, int h, int i, double x, double y) {
-    const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
-    if (x0 < 0 || y0 < 0 || x0 + 1 >= w || y0 + 1 >= h) {
-        const int xr = static_cast<int>(lround(x)), yr = static_cast<int>(lround(y));
-        if (xr >= 0 && yr >= 0 && xr < w && yr < h && fabs(x - xr) < 1e-9 && fabs(y - yr) < 1e-9)
-            return frame_at(f, storage, i, xr, yr);
-        return __builtin_nanf("");
-    }
-    const double fx = x - x0, fy = y - y0;
-    const float r00 = frame_at(f, storage, i, x0, y0), r01 = frame_at(f, storage, i, x0 + 1, y0);
-    const float r10 = frame_at(f, storage, i, x0, y0 + 1), r11 = frame_at(f, storage, i, x0 + 1, y0 + 1);
-    return static_cast<float>((1 - fy) * ((1 - fx) * r00 + fx * r01) + fy * ((1 - fx) * r10 + fx * r11));
-}
 
 __device__ inline double wave_max(double v) {
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -108,24 +75,7 @@ __global__ void __launch_bounds__(kWave) screen_kernel(ScreenArgs a) {
     // within r of it (the subtracted frame or the static sky undefined); a lane takes
     // one window row of one frame, so its loads are independent of each other
     {
-        const int r = a.edge_r, side = 2 * r + 1;
-        for (int e = lane; e < n * side; e += kWave) {
-            const int i = e / side;
-            const double t = f.t[i];
-            const int xi = static_cast<int>(lround(x + vx * t));
-            const int yy = static_cast<int>(lround(y + vy * t)) + (e - i * side) - r;
-            // a row that leaves the frame holds an outside pixel
-            int any = yy < 0 || yy >= a.h || xi - r < 0 || xi + r >= a.w;
-            if (!any) {
-                const long long o = static_cast<long long>(yy) * a.w + (xi - r);
-                const float* sr = a.sub + static_cast<long long>(i) * a.sub_stride + o;
-                const float* kr = a.sky + o;
-                // no short cut: the loads stay independent
-#pragma unroll 8
-                for (int d = 0; d < side; ++d) any |= static_cast<int>(!isfinite(sr[d])) | static_cast<int>(!isfinite(kr[d]));
-            }
-            if (any) blank[i] = 1;
-        }
+        edge_rows(a.sub, a.sub_stride, a.sky, a.w, a.h, n, f.t, x, y, vx, vy, a.edge_r, lane, kWave, blank);
         __syncthreads();
         int near_blank = 0;
         for (int i = lane; i < n; i += kWave) near_blank += blank[i];
@@ -140,33 +90,10 @@ __global__ void __launch_bounds__(kWave) screen_kernel(ScreenArgs a) {
     // detector_flag: the best S/N of a detector-fixed position on the grid of steps
     // around the track's mean detector position, against 0.8 x the track's S/N
     if (a.reg_dx) {
-        if (lane == 0) {
-            double mx = 0.0, my = 0.0;
-            for (int i = 0; i < n; ++i) {
-                const double t = f.t[i];
-                mx += (x + vx * t) + a.reg_dx[i];
-                my += (y + vy * t) + a.reg_dy[i];
-            }
-            bcast[2] = mx / n;
-            bcast[3] = my / n;
-        }
+        if (lane == 0) detector_mean(n, f.t, a.reg_dx, a.reg_dy, x, y, vx, vy, bcast[2], bcast[3]);
         __syncthreads();
-        const double mx = bcast[2], my = bcast[3];
-        const int ns = a.nsteps, np = ns * ns;
-        double best = 0.0;
-        for (int p = lane; p < np; p += kWave) {
-            const double dxp = mx + a.steps[p % ns], dyp = my + a.steps[p / ns];
-            double sum = 0.0;
-            int count = 0;
-            for (int i = 0; i < n; ++i) {
-                const float v = bilinear(f, a.storage, a.w, a.h, i, dxp - a.reg_dx[i], dyp - a.reg_dy[i]);
-                if (!isfinite(v)) continue;
-                sum += v;
-                ++count;
-            }
-            const double f = count >= 3 ? sum / a.sqrtc[count] : 0.0;
-            best = best < f ? f : best;
-        }
+        double best = detector_best(f, a.storage, a.w, a.h, n, a.reg_dx, a.reg_dy, a.steps, a.nsteps, a.sqrtc,
+                                    bcast[2], bcast[3], lane, kWave);
         best = wave_max(best);
         if (lane == 0) a.art[k] = best >= 0.8 * track_snr_all ? 1 : 0;
         return;

@@ -12,6 +12,7 @@
 #include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
 #include "stack_plan.hpp"
+#include "refine.hpp"
 #include "screen.hpp"
 #include "store.hpp"
 
@@ -157,6 +158,12 @@ struct HostPeak {
 };
 
 }  // namespace
+
+double expected_null_peaks(const SearchOutput& out, double snr) {
+    if (out.null_inv_s2 > 0.0 && snr > 0.0) return std::exp(tail_log_rate(out.null_a, out.null_k, out.null_inv_s2, snr));
+    if (out.null_b > 0.0) return std::exp(out.null_a - out.null_b * snr);
+    return 0.0;
+}
 
 int build_lanes(int n, long long plane, int passes) {
     if (passes < 2) return 1;
@@ -328,6 +335,8 @@ double DeviceSearch::device_bytes() const {
     b += static_cast<double>(tab_bytes_);
     b += stage_.device_bytes();
     for (const PassSet& set : sets_) b += set.device_bytes();
+    if (real_) b += plane * n * 8.0 + plane * 4.0;
+    b += static_cast<double>(ref_cap_);
     for (const BuildLane& l : lanes_) {
         if (l.stage) b += l.stage->device_bytes();
         if (l.filt) b += plane * n * 4.0;
@@ -365,10 +374,11 @@ DeviceSearch::~DeviceSearch() {
     for (void* p : {static_cast<void*>(sub_), static_cast<void*>(filt_), static_cast<void*>(max_),
                     static_cast<void*>(best_), static_cast<void*>(nmax_), static_cast<void*>(nbest_),
                     static_cast<void*>(pk_), static_cast<void*>(sky_), static_cast<void*>(scr_tab_),
-                    static_cast<void*>(scr_pk_), static_cast<void*>(scr_art_)})
+                    static_cast<void*>(scr_pk_), static_cast<void*>(scr_art_), static_cast<void*>(real_),
+                    static_cast<void*>(res_), static_cast<void*>(rmask_), static_cast<void*>(ref_dev_)})
         if (p) (void)hipFree(p);
     for (void* p : {static_cast<void*>(scr_tab_host_), static_cast<void*>(scr_pk_host_),
-                    static_cast<void*>(scr_art_host_)})
+                    static_cast<void*>(scr_art_host_), static_cast<void*>(ref_host_)})
         if (p) (void)hipHostFree(p);
     for (auto& set : sets_) set.release();
     for (int q = 0; q < 2; ++q)
@@ -655,8 +665,11 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     Lap lap(out, s);
     const StageParams prm = stage_params(cfg);
     stage_.background(raw, stride, sub_, static_cast<long long>(plane), nullptr, s);
-    stage_.build(sub_, static_cast<long long>(plane), out.t_h.data(), prm, filt_, static_cast<long long>(plane),
-                 sky_, nullptr, nullptr, nullptr, s);
+    // the candidate stage keeps the real pass's frames, residuals and mask
+    if (cfg.refines()) reserve_real();
+    float* const real_frames = cfg.refines() ? real_ : filt_;
+    stage_.build(sub_, static_cast<long long>(plane), out.t_h.data(), prm, real_frames, static_cast<long long>(plane),
+                 sky_, nullptr, cfg.refines() ? rmask_ : nullptr, nullptr, s, cfg.refines() ? res_ : nullptr);
     lap("prepare");
 
     const int nv = static_cast<int>(out.grid.size());
@@ -673,7 +686,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     };
     using PlanPtr = std::unique_ptr<ftk_plan, PlanDel>;
     // one pass with a plan of its own; the plan (and its 16-bit frames) is returned
-    auto pass = [&](const std::vector<double>& t, float* m, int* b) -> PlanPtr {
+    auto pass = [&](const std::vector<double>& t, const float* frames, float* m, int* b) -> PlanPtr {
         shifts_for(out.grid, t, n, dx, dy);
         ftk_options opt;
         opt.device = -1;
@@ -687,9 +700,9 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         PlanPtr held(plan);
         int st = FTK_OK;
         if (cfg.storage == kStoreF32) {
-            st = ftk_plan_run(plan, filt_, plane, m, b, s, err, sizeof err);
+            st = ftk_plan_run(plan, frames, plane, m, b, s, err, sizeof err);
         } else {
-            st = ftk_plan_store(plan, filt_, plane, cfg.storage, s, err, sizeof err);
+            st = ftk_plan_store(plan, frames, plane, cfg.storage, s, err, sizeof err);
             if (st == FTK_OK) st = ftk_plan_run_stored(plan, m, b, s, err, sizeof err);
         }
         // the plan's device tables must outlive the launch
@@ -700,7 +713,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
 
     float* m_real = d_max_snr ? d_max_snr : max_;
     int* b_real = d_best_v ? d_best_v : best_;
-    pass(out.t_h, m_real, b_real);
+    pass(out.t_h, real_frames, m_real, b_real);
     lap("stack (real)");
     const float floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
     out.peaks = peaks(m_real, b_real, floor, nms, edge, 100000, s);
@@ -715,7 +728,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         stage_.build(sub_, static_cast<long long>(plane), t.data(), prm, filt_, static_cast<long long>(plane), nullptr,
                      nullptr, nullptr, nullptr, s);
         lap("null rebuild");
-        const PlanPtr held = pass(t, mn, bn);
+        const PlanPtr held = pass(t, filt_, mn, bn);
         lap("stack (null)");
         const PeakList npk = peaks(mn, bn, 3.0f, nms, edge, 20000, s);
         // the artefact tests read the frames the pass stacked: fp32 as built, or the
@@ -734,6 +747,10 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     }
     finish(cfg, runs, nv, out, s);
     lap("threshold fit");
+    if (cfg.refines()) {
+        refine(cfg, out, s);
+        lap("refine");
+    }
     return out;
 }
 
@@ -758,6 +775,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const int nms = std::max(2, static_cast<int>(std::lround(cfg.fwhm_px)));
     const int edge = std::max(cfg.edge_px, 1);
     const int passes = 1 + runs;
+    if (cfg.refines()) reserve_real();
 
     // background on the caller's stream (the build lanes wait for it below)
     stage_.background(raw, stride, sub_, plane, nullptr, s);
@@ -1005,7 +1023,21 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
             // the real pass also leaves the static sky (the same for every frame order)
             float* sky = p == 0 ? sky_ : nullptr;
-            if (st != kStoreF32) {
+            if (p == 0 && cfg.refines()) {
+                // the candidate stage keeps the real pass's fp32 frames, residuals and
+                // mask; the pass stacks a copy of the frames in its slot
+                stg.build_ranked(sub_, plane, rk, prm, real_, plane, sky, nullptr, rmask_, nullptr, bs, res_);
+                if (st != kStoreF32)
+                    scheck(launch_store(real_, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs,
+                                        pad.pitch, S.maxb + static_cast<long long>(g) * n,
+                                        S.scl + static_cast<long long>(g) * n, bs),
+                           "store frames");
+                else if (use_pad)
+                    scheck(launch_copy_rows(real_, plane, slot + forigin, fs, pad.pitch, w, h, n, bs), "pad frames");
+                else
+                    scheck(hipMemcpyAsync(slot, real_, sizeof(float) * n * plane, hipMemcpyDeviceToDevice, bs),
+                           "copy frames");
+            } else if (st != kStoreF32) {
                 stg.build_ranked(sub_, plane, rk, prm, scratch, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
                 scheck(launch_store(scratch, plane, w, n, w, h, st, reinterpret_cast<uint16_t*>(slot) + forigin, fs, pad.pitch,
@@ -1119,6 +1151,10 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     for (int p = 1; p < passes; ++p) pool_null(results[static_cast<size_t>(p)], arts[static_cast<size_t>(p)], out);
     finish(cfg, runs, nv, out, s);
     lap("threshold fit");
+    if (cfg.refines()) {
+        refine(cfg, out, s);
+        lap("refine");
+    }
     return out;
 }
 
@@ -1164,9 +1200,110 @@ void DeviceSearch::finish(const SearchConfig& cfg, int runs, int nv, SearchOutpu
     }
 }
 
+void DeviceSearch::reserve_real() {
+    if (real_) return;
+    const size_t plane = static_cast<size_t>(w) * h;
+    real_ = salloc<float>(plane * n, "allocate candidate stage");
+    res_ = salloc<float>(plane * n, "allocate candidate stage");
+    rmask_ = salloc<float>(plane, "allocate candidate stage");
+}
+
+// refine (search.cpp:246-395) of the selected peaks: one pinned block of tables (frame
+// times, sqrt(c), detector offsets, registration offsets, photometry weights, the
+// peaks' grid positions and motions) in one copy, one launch, the figures and per-frame
+// samples back in one copy; finish_candidates on the host.
+void DeviceSearch::refine(const SearchConfig& cfg, SearchOutput& out, hipStream_t s) {
+    const size_t nc = out.selected.x.size();
+    if (nc == 0) return;
+    const long long plane = static_cast<long long>(w) * h;
+    const double s_psf = cfg.fwhm_px * kFwhmToSigmaS;
+    const int R = static_cast<int>(std::ceil(3.0 * s_psf));  // stack_cutout's half size
+    const int size = 2 * R + 1;
+    std::vector<double> steps;
+    const double span = std::max(2.0, 2.0 * cfg.fwhm_px);
+    for (double d = -span; d <= span; d += 0.5) steps.push_back(d);
+    const bool reg = !cfg.reg_dx.empty();
+    const size_t n_tab = static_cast<size_t>(n) + (n + 1) + steps.size() + (reg ? 2 * static_cast<size_t>(n) : 0) +
+                         static_cast<size_t>(size) * size + 4 * nc;
+    const size_t n_fig = nc * kRefOut;
+    const size_t bytes = 8 * (n_tab + n_fig) + 4 * nc * static_cast<size_t>(n);
+    if (bytes > ref_cap_) {
+        if (ref_dev_) scheck(hipFree(ref_dev_), "free candidate buffers");
+        if (ref_host_) scheck(hipHostFree(ref_host_), "free candidate buffers");
+        ref_dev_ = ref_host_ = nullptr;
+        ref_cap_ = 0;
+        const size_t cap = std::max<size_t>(bytes + bytes / 4, 65536);
+        ref_dev_ = salloc<char>(cap, "allocate candidate buffers");
+        scheck(hipHostMalloc(reinterpret_cast<void**>(&ref_host_), cap, hipHostMallocDefault),
+               "allocate candidate buffers");
+        ref_cap_ = cap;
+    }
+    double* q = reinterpret_cast<double*>(ref_host_);
+    for (double t : out.t_h) *q++ = t;
+    for (int c = 0; c <= n; ++c) *q++ = std::sqrt(static_cast<double>(c));
+    for (double d : steps) *q++ = d;
+    if (reg) {
+        for (double d : cfg.reg_dx) *q++ = d;
+        for (double d : cfg.reg_dy) *q++ = d;
+    }
+    // photometry weights exp(-r^2 / 2 s^2) (search.cpp:367-377), with the host's exp
+    for (int y = 0; y < size; ++y)
+        for (int x = 0; x < size; ++x)
+            *q++ = std::exp(-0.5 * ((x - R) * (x - R) + (y - R) * (y - R)) / (s_psf * s_psf));
+    const int nv = static_cast<int>(out.grid.size());
+    for (size_t k = 0; k < nc; ++k) {
+        const int v = out.selected.vidx[k];
+        if (v < 0 || v >= nv) throw std::runtime_error("refine: velocity index out of range");
+        *q++ = out.selected.x[k];
+        *q++ = out.selected.y[k];
+        *q++ = out.grid[static_cast<size_t>(v)].first;
+        *q++ = out.grid[static_cast<size_t>(v)].second;
+    }
+    scheck(hipMemcpyAsync(ref_dev_, ref_host_, 8 * n_tab, hipMemcpyHostToDevice, s), "upload candidate tables");
+    const double* d = reinterpret_cast<const double*>(ref_dev_);
+    RefineArgs a;
+    a.frames = real_;
+    a.fstride = plane;
+    a.resid = res_;
+    a.sub = sub_;
+    a.sky = sky_;
+    a.mask = rmask_;
+    a.t = d;
+    a.sqrtc = d + n;
+    a.steps = a.sqrtc + (n + 1);
+    a.nsteps = static_cast<int>(steps.size());
+    a.reg_dx = reg ? a.steps + steps.size() : nullptr;
+    a.reg_dy = reg ? a.reg_dx + n : nullptr;
+    a.kern = a.steps + steps.size() + (reg ? 2 * static_cast<size_t>(n) : 0);
+    a.R = R;
+    a.cand = a.kern + static_cast<size_t>(size) * size;
+    a.count = static_cast<int>(nc);
+    // the two local-search levels (search.cpp:258-261)
+    const double dv = out.dv;
+    const double levels[2][4] = {{dv / 4, dv, 0.5, 1.0}, {dv / 8, dv / 4, 0.25, 0.5}};
+    for (int l = 0; l < 2; ++l)
+        a.lev[l] = RefineLevel{levels[l][0], levels[l][2], static_cast<int>(std::lround(levels[l][1] / levels[l][0])),
+                               static_cast<int>(std::lround(levels[l][3] / levels[l][2]))};
+    a.dv = dv;
+    a.n = n;
+    a.w = w;
+    a.h = h;
+    a.edge_r = static_cast<int>(std::ceil(2.0 * cfg.fwhm_px)) + 1;
+    a.out = reinterpret_cast<double*>(ref_dev_ + 8 * n_tab);
+    a.contrib = reinterpret_cast<float*>(ref_dev_ + 8 * (n_tab + n_fig));
+    scheck(launch_refine(a, s), "refine");
+    scheck(hipMemcpyAsync(ref_host_ + 8 * n_tab, ref_dev_ + 8 * n_tab, 8 * n_fig + 4 * nc * static_cast<size_t>(n),
+                          hipMemcpyDeviceToHost, s),
+           "download candidates");
+    scheck(hipStreamSynchronize(s), "refine");
+    out.candidates = finish_candidates(cfg, out, reinterpret_cast<const double*>(ref_host_ + 8 * n_tab),
+                                       reinterpret_cast<const float*>(ref_host_ + 8 * (n_tab + n_fig)));
+}
+
 double DeviceSearch::tune(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg_in, int reps,
                           hipStream_t s) {
     SearchConfig cfg = cfg_in;
+    cfg.candidates = 0;  // the choice concerns the passes only
     const int runs = cfg.threshold_snr <= 0.0 ? std::max(0, cfg.null_runs) : 0;
     const int passes = 1 + runs;
     // group sizes: every pass at once, about half (two groups overlap builds with

@@ -53,11 +53,32 @@ struct SearchConfig {
     int null_tested = 3000;
     double max_frame_fraction = 0.25;
     std::vector<double> reg_dx, reg_dy;
+    // Candidate stage: 1 = the selected peaks are refined into candidates (refine and
+    // finish_candidates, src/search.cpp:246-395, 627-675), 0 = the run stops at the
+    // selection; half_tolerance is the halves test's allowed shortfall (sigma).
+    int candidates = 1;
+    double half_tolerance = 3.0;
+    // whether a run has a candidate stage (candidates on and room for at least one)
+    bool refines() const { return candidates != 0 && max_candidates > 0; }
 };
 
 struct PeakList {
     std::vector<int> x, y, vidx;
     std::vector<float> snr;
+};
+
+// A candidate (Candidate, include/fainttrail/search.hpp:131-154): the fields a search
+// without a WCS fills (0-based position at the reference epoch).
+struct SearchCandidate {
+    int id = 0;
+    double x = 0.0, y = 0.0, vx_pxh = 0.0, vy_pxh = 0.0;
+    double pos_err_px = 0.0, vel_err_pxh = 0.0;
+    double snr = 0.0, snr_coarse = 0.0, snr_half1 = 0.0, snr_half2 = 0.0;
+    double max_frame_fraction = 0.0, false_alarms_per_run = 0.0;
+    double flux = 0.0, mag_inst = 0.0;
+    double inv_dist_au = 0.0, inv_dist_err_au = 0.0;
+    bool ok = false;
+    std::string flags;
 };
 
 struct SearchOutput {
@@ -71,6 +92,7 @@ struct SearchOutput {
     int screened = 0, artefacts = 0;  // null peaks through the artefact tests, and rejected
     PeakList peaks;             // real-pass peaks, sorted, at most 100000
     PeakList selected;          // peaks at or above the threshold, at most max_candidates
+    std::vector<SearchCandidate> candidates;  // sorted as the reference sorts them, ids from 1
     std::vector<std::pair<std::string, double>> timings;
     int passes_per_launch = 1, segments = 0;  // what the run used
 };
@@ -129,6 +151,11 @@ private:
     SearchOutput run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
                              float* d_max_snr, int* d_best_v, hipStream_t s);
     void finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s);
+    // The candidate stage: the selected peaks refined on the device against the real
+    // pass's frames, residuals and mask, then finished on the host.
+    void refine(const SearchConfig& cfg, SearchOutput& out, hipStream_t s);
+    // The real pass's buffers of the candidate stage (allocated on first use).
+    void reserve_real();
     // The null-peak screen of one or more passes: the artefact flags of the first
     // min(count, null_tested) peaks of each pass, tested on that pass's frames.
     struct ScreenPass {
@@ -157,6 +184,14 @@ private:
     int* pk_ = nullptr;  // count + 4 arrays
     size_t pk_cap_ = 0;
     float* sky_ = nullptr;  // the run's static sky (the screen's blank test)
+    // candidate stage: the real pass's filtered frames (fp32), residuals and mask; the
+    // refinement's tables and figures (device + pinned staging)
+    float* real_ = nullptr;
+    float* res_ = nullptr;
+    float* rmask_ = nullptr;
+    char* ref_dev_ = nullptr;
+    char* ref_host_ = nullptr;
+    size_t ref_cap_ = 0;
     // null screen: run tables (device + pinned staging), peaks and flags
     double* scr_tab_ = nullptr;
     double* scr_tab_host_ = nullptr;
@@ -210,5 +245,13 @@ PeakList order_peaks(int c, const int* x, const int* y, const float* snr, const 
 void pool_null(const PeakList& pk, const std::vector<unsigned char>& art, SearchOutput& out);
 void set_threshold(const SearchConfig& cfg, int width, int height, double fwhm, size_t n_motions, int runs,
                    const std::vector<float>& pooled, SearchOutput& out, const double* device_sums);
+// expected_null_peaks (search.cpp:696-700): noise peaks per run above snr under the
+// fitted null tail (0 without a fit).
+double expected_null_peaks(const SearchOutput& out, double snr);
+// refine's host tail (search.cpp:319-391) and finish_candidates (:627-675) for the
+// selected peaks of out: fig holds kRefOut device figures per selected peak, contrib n
+// per-frame samples each. Returns the candidates in the reference's order, ids from 1.
+std::vector<SearchCandidate> finish_candidates(const SearchConfig& cfg, const SearchOutput& out, const double* fig,
+                                               const float* contrib);
 
 }  // namespace ftk

@@ -1,5 +1,6 @@
 // Device-resident search server (see serve.hpp).
 #include "serve.hpp"
+#include "refine.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -16,7 +17,7 @@ double clock_s() {
 }
 }  // namespace
 
-double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage) {
+double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage, int max_candidates) {
     const double P = static_cast<double>(w) * h;
     const double pk_cap = std::max(65536.0, P / 8.0);
     // DeviceSearch: background-subtracted and filtered frames, four maps and the static
@@ -38,7 +39,14 @@ double search_footprint(int n, int w, int h, long long nv, int G, int passes, in
     // per-run tables: the real rows, coverage tables (at most 2^26 entries each), orders
     const double run = tab + 2.0 * 4.0 * std::min(67108864.0, 16.0 * std::ceil(1.25 * nv / kVel) * (w + h)) +
                        4.0 * n * passes;
-    return base + sets + run;
+    // candidate stage (max_candidates > 0): the real pass's frames, residuals and mask,
+    // the refinement's tables and figures (a grown buffer: 1.25 x, at least 64 KiB)
+    const double mc = std::max(0, max_candidates);
+    const double cand = mc > 0 ? 8.0 * n * P + 4.0 * P +
+                                     std::max(65536.0, 1.25 * (8.0 * (4.0 * n + 4096.0 + 4.0 * mc + kRefOut * mc) +
+                                                               4.0 * mc * n))
+                               : 0.0;
+    return base + sets + run + cand;
 }
 
 SearchServer::SearchServer(int device, double budget_bytes, int slots)
@@ -97,9 +105,11 @@ long long SearchServer::submit(const float* raw, long long stride, int n, int w,
     // group size: as asked, else up to 8 passes per launch, fewer if the budget needs it
     int G = j->cfg.passes_per_launch > 0 ? j->cfg.passes_per_launch : std::min(8, j->passes);
     G = std::max(1, std::min(G, j->passes));
-    while (G > 1 && search_footprint(n, w, h, j->nv, G, j->passes, j->cfg.storage) > budget_) --G;
+    const int mc = j->cfg.refines() ? j->cfg.max_candidates : 0;
+    auto footprint = [&](int g) { return search_footprint(n, w, h, j->nv, g, j->passes, j->cfg.storage, mc); };
+    while (G > 1 && footprint(G) > budget_) --G;
     j->cfg.passes_per_launch = G;
-    j->bytes = search_footprint(n, w, h, j->nv, G, j->passes, j->cfg.storage);
+    j->bytes = footprint(G);
     if (j->bytes > budget_)
         throw std::runtime_error("server: the job needs " + std::to_string(j->bytes / 1e9) + " GB, more than the " +
                                  std::to_string(budget_ / 1e9) + " GB budget");

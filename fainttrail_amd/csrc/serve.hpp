@@ -34,8 +34,9 @@ namespace ftk {
 
 // Device bytes of a DeviceSearch over n frames of w x h and of its batched run with
 // nv motions in groups of G passes (two sets of pass buffers when the run has more
-// than one group), for the given frame storage: what admission reserves.
-double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage);
+// than one group), for the given frame storage, with a candidate stage for up to
+// max_candidates candidates (0: none): what admission reserves.
+double search_footprint(int n, int w, int h, long long nv, int G, int passes, int storage, int max_candidates);
 
 struct ServerJob {
     long long id = 0;

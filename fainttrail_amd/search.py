@@ -26,7 +26,8 @@ class _Params(C.Structure):
                 ("edge_px", C.c_int), ("seed", C.c_ulonglong), ("fit_on_device", C.c_int),
                 ("passes_per_launch", C.c_int), ("segments", C.c_int), ("storage", C.c_int),
                 ("null_tested", C.c_int), ("max_frame_fraction", C.c_double),
-                ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double))]
+                ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double)),
+                ("candidates", C.c_int), ("half_tolerance", C.c_double)]
 
 
 class _Summary(C.Structure):
@@ -37,7 +38,20 @@ class _Summary(C.Structure):
                 ("t_stack_real", C.c_double), ("t_null_rebuild", C.c_double), ("t_stack_null", C.c_double),
                 ("t_peaks", C.c_double), ("t_threshold", C.c_double), ("t_plan", C.c_double),
                 ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int),
-                ("n_screened", C.c_int), ("n_artefacts", C.c_int)]
+                ("n_screened", C.c_int), ("n_artefacts", C.c_int), ("n_candidates", C.c_int),
+                ("t_refine", C.c_double)]
+
+
+# Candidate's fields (include/fainttrail/search.hpp:131-154) that the record carries, in
+# ftk_candidate's order after id and ok
+CANDIDATE_FIELDS = ["x", "y", "vx_pxh", "vy_pxh", "pos_err_px", "vel_err_pxh", "snr", "snr_coarse", "snr_half1",
+                    "snr_half2", "max_frame_fraction", "false_alarms_per_run", "flux", "mag_inst", "ra_deg",
+                    "dec_deg", "rate_arcsec_min", "pa_deg", "inv_dist_au", "inv_dist_err_au"]
+
+
+class _Candidate(C.Structure):
+    _fields_ = [("id", C.c_int), ("ok", C.c_int)] + [(f, C.c_double) for f in CANDIDATE_FIELDS] + \
+        [("flags", C.c_char * 64)]
 
 
 METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gaussian"]
@@ -58,6 +72,7 @@ def _bind():
                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), vp, C.c_char_p, sz]
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    L.ftk_searcher_candidates.argtypes = [vp, C.c_int, C.POINTER(_Candidate)]
     I32 = C.POINTER(C.c_int32)
     L.ftk_find_peaks.argtypes = [vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, I32, I32,
                                  C.POINTER(C.c_float), I32, C.POINTER(C.c_int), vp, C.c_char_p, sz]
@@ -103,6 +118,10 @@ class SearchParams:
     # frame, or None: with them the detector-fixed test runs
     reg_dx: Optional[Sequence[float]] = None
     reg_dy: Optional[Sequence[float]] = None
+    # candidate stage: refine the selected peaks into candidates (refine and
+    # finish_candidates); half_tolerance is the halves test's allowed shortfall (sigma)
+    candidates: bool = True
+    half_tolerance: float = 3.0
 
     def c(self) -> _Params:
         from . import STORAGES
@@ -115,7 +134,7 @@ class SearchParams:
                     self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
                     self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
-                    self.max_frame_fraction)
+                    self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance)
         if self.reg_dx is not None:
             rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
             ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)
@@ -133,6 +152,48 @@ class Peaks:
     y: np.ndarray
     snr: np.ndarray
     vidx: np.ndarray
+
+
+@dataclass
+class Candidate:
+    """A candidate (Candidate, include/fainttrail/search.hpp:131-154): 0-based position
+    at the reference epoch, motion, S/N figures, photometry and flags."""
+    id: int
+    ok: bool
+    x: float
+    y: float
+    vx_pxh: float
+    vy_pxh: float
+    pos_err_px: float
+    vel_err_pxh: float
+    snr: float
+    snr_coarse: float
+    snr_half1: float
+    snr_half2: float
+    max_frame_fraction: float
+    false_alarms_per_run: float
+    flux: float
+    mag_inst: float
+    ra_deg: float
+    dec_deg: float
+    rate_arcsec_min: float
+    pa_deg: float
+    inv_dist_au: float
+    inv_dist_err_au: float
+    flags: str
+
+
+def read_candidates(fetch, count: int) -> list:
+    """Candidates through fetch(cap, records) -> count (ftk_searcher_candidates or
+    ftk_server_candidates)."""
+    if count <= 0:
+        return []
+    recs = (_Candidate * count)()
+    got = fetch(count, recs)
+    if got != count:
+        raise FtkError(f"candidate count changed ({got} != {count})")
+    return [Candidate(r.id, bool(r.ok), *(getattr(r, f) for f in CANDIDATE_FIELDS), r.flags.decode())
+            for r in recs]
 
 
 @dataclass
@@ -156,6 +217,7 @@ class SearchResult:
     segments: int = 0
     null_screened: int = 0   # null peaks through the artefact tests
     null_artefacts: int = 0  # of those, left out of null_peaks
+    candidates: list = field(default_factory=list)  # Candidate records, the reference's order
 
 
 class Searcher:
@@ -222,8 +284,9 @@ class Searcher:
             self._peaks(1, summ.n_selected), pooled,
             {"prepare": summ.t_prepare, "stack (real)": summ.t_stack_real, "null rebuild": summ.t_null_rebuild,
              "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold,
-             "plan": summ.t_plan, "passes (batched)": summ.t_passes},
-            mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts)
+             "plan": summ.t_plan, "passes (batched)": summ.t_passes, "refine": summ.t_refine},
+            mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts,
+            read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates))
 
     def __del__(self):
         h = getattr(self, "_h", None)

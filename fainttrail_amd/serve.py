@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import FtkError, _err, _raise, lib
-from .search import METHODS, Peaks, SearchParams, _Params, _Summary
+from .search import METHODS, Peaks, SearchParams, _Candidate, _Params, _Summary, read_candidates
 
 _bound = False
 
@@ -34,6 +34,7 @@ def _bind():
     L.ftk_server_wait.argtypes = [vp, ll, C.c_double, C.POINTER(_Summary), C.POINTER(C.c_double), C.c_char_p, sz]
     L.ftk_server_peaks.argtypes = [vp, ll, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+    L.ftk_server_candidates.argtypes = [vp, ll, C.c_int, C.POINTER(_Candidate)]
     L.ftk_server_release.argtypes = [vp, ll]
     L.ftk_server_stats.argtypes = [vp, C.POINTER(C.c_double)]
     _bound = True
@@ -55,6 +56,7 @@ class JobResult:
     reserved_bytes: float
     passes_per_launch: int
     held_bytes: float = 0.0  # device bytes the job's searcher held when it ended
+    candidates: list = field(default_factory=list)  # Candidate records, the reference's order
     extra: dict = field(default_factory=dict)
 
     @property
@@ -112,7 +114,9 @@ class Server:
             return JobResult(job, summ.threshold_snr, METHODS[summ.method], summ.n_velocities,
                              self._peaks(job, 0, summ.n_peaks), self._peaks(job, 1, summ.n_selected),
                              self._peaks(job, 2, summ.n_pooled).snr, times[0], times[1], times[2], times[3],
-                             summ.passes_per_launch, times[4])
+                             summ.passes_per_launch, times[4],
+                             read_candidates(lambda cap, recs: lib().ftk_server_candidates(self._h, job, cap, recs),
+                                             summ.n_candidates))
         finally:
             # ended jobs are forgotten by the server, failed ones included
             lib().ftk_server_release(self._h, job)

@@ -189,7 +189,12 @@ int ftk_stage_build(ftk_stage* stage, const float* d_sub, size_t sub_stride, con
  * null-tail threshold (set_threshold, :573-612) and select_peaks (:614-620), with the
  * frames resident on the device from the first step to the last, the strongest peaks
  * of each null run screened for artefacts on the device (is_artefact_peak,
- * search.cpp:229-243, 773-778). Stops before refine/finish_candidates. */
+ * search.cpp:229-243, 773-778), and the candidate stage: the selected peaks refined on
+ * the device (refine, search.cpp:246-395: local search, sub-step parabolas, halves,
+ * mask, PSF photometry on the stacked residuals, edge and detector tests) and finished
+ * on the host (finish_candidates, :627-675: duplicates, ghosts, flags, false-alarm
+ * rates, order, ids). Straight tracks only (no near-Earth distance grid); no WCS, so
+ * no sky coordinates. */
 typedef struct ftk_search_params {
     double vmin_pxh, vmax_pxh, grid_tol_px;
     double fwhm_px; /* PSF FWHM, must be > 0 (estimated once, outside this path) */
@@ -225,6 +230,11 @@ typedef struct ftk_search_params {
      * Copied by the call. */
     const double* reg_dx;
     const double* reg_dy;
+    /* Candidate stage: 1 (default) = refine the selected peaks into candidates, 0 =
+     * stop at the selection. half_tolerance: SearchParams::half_tolerance (default 3),
+     * the halves test's allowed shortfall in sigma. */
+    int candidates;
+    double half_tolerance;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -240,7 +250,29 @@ typedef struct ftk_search_summary {
     double t_plan, t_passes;
     int passes_per_launch, segments;
     int n_screened, n_artefacts; /* null peaks through the artefact tests; of those, left out */
+    int n_candidates;            /* candidates after the duplicate test */
+    double t_refine;             /* seconds of the candidate stage */
 } ftk_search_summary;
+
+/* A candidate: ft_candidate (include/fainttrail/fainttrail.h:56-69) with the position
+ * 0-based as in Candidate (include/fainttrail/search.hpp:131-154; ft_get_candidate,
+ * src/c_api.cpp:218-246, adds 1 for FITS), and Candidate's inverse distance fields
+ * (0 on straight tracks). ra_deg .. pa_deg stay 0: this path has no WCS. */
+typedef struct ftk_candidate {
+    int id;
+    int ok; /* 1 = passed every consistency test */
+    double x, y;
+    double vx_pxh, vy_pxh;
+    double pos_err_px, vel_err_pxh;
+    double snr, snr_coarse, snr_half1, snr_half2;
+    double max_frame_fraction;
+    double false_alarms_per_run;
+    double flux, mag_inst;
+    double ra_deg, dec_deg;
+    double rate_arcsec_min, pa_deg;
+    double inv_dist_au, inv_dist_err_au;
+    char flags[64]; /* comma separated, as Candidate::flags */
+} ftk_candidate;
 
 typedef struct ftk_searcher ftk_searcher;
 
@@ -267,6 +299,10 @@ int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride,
  * entries into the non-NULL arrays and returns the total count. */
 int ftk_searcher_peaks(const ftk_searcher* searcher, int which, int cap, int32_t* x, int32_t* y, float* snr,
                        int32_t* vidx);
+/* Candidates of the last run, in the reference's order (ok ones first, then by S/N;
+ * ft_candidate_count / ft_get_candidate, src/c_api.cpp:213-246): writes up to cap
+ * records and returns the count, -1 for a null searcher. */
+int ftk_searcher_candidates(const ftk_searcher* searcher, int cap, ftk_candidate* out);
 /* find_peaks (src/search.cpp:105-135) over device maps (w x h max_snr and best_v):
  * pixels in [edge, w - edge) x [edge, h - edge) at or above floor that are the maximum
  * of their (2 radius + 1)^2 window (strict against later pixels, row-major, and
@@ -308,6 +344,8 @@ int ftk_server_wait(ftk_server* server, long long job, double timeout_s, ftk_sea
 /* Peaks of an ended job, as ftk_searcher_peaks; -1 for an unknown or failed job. */
 int ftk_server_peaks(ftk_server* server, long long job, int which, int cap, int32_t* x, int32_t* y, float* snr,
                      int32_t* vidx);
+/* Candidates of an ended job, as ftk_searcher_candidates; -1 for an unknown or failed job. */
+int ftk_server_candidates(ftk_server* server, long long job, int cap, ftk_candidate* out);
 /* Forgets an ended job. */
 void ftk_server_release(ftk_server* server, long long job);
 /* stats[10]: reserved bytes, peak reserved bytes, budget, idle-searcher bytes, running,

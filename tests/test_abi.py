@@ -147,9 +147,9 @@ def test_product_velocity_grid_matches_reference(golden, key):
 
 def test_search_params_defaults_are_the_reference_defaults():
     """ftk_search_params_default and the Python SearchParams carry SearchParams'
-    defaults (include/fainttrail/search.hpp:18-90): among them null_tested = 3000 and
-    max_frame_fraction = 0.25 (:63-65), no registration offsets (fwhm_px = 3: the PSF
-    estimate runs outside this path)."""
+    defaults (include/fainttrail/search.hpp:18-90): among them null_tested = 3000,
+    max_frame_fraction = 0.25 and half_tolerance = 3 (:63-66), no registration offsets
+    (fwhm_px = 3: the PSF estimate runs outside this path), and the candidate stage on."""
     from fainttrail_amd.search import SearchParams, _Params
     lib = ctypes.CDLL(str(ROOT / "fainttrail_amd" / "libftk_hip.so"))
     lib.ftk_search_params_default.argtypes = [ctypes.POINTER(_Params)]
@@ -158,7 +158,8 @@ def test_search_params_defaults_are_the_reference_defaults():
     want = {"vmin_pxh": 4.0, "vmax_pxh": 40.0, "grid_tol_px": 0.5, "clip_sigma": 8.0, "cr_sigma": 5.0,
             "static_exclude": -1, "mask_scatter": 2.5, "weight_scatter": 1.5, "min_coverage": 0.7,
             "threshold_snr": 0.0, "min_snr_floor": 5.0, "null_runs": 3, "target_false_per_run": 0.02,
-            "max_candidates": 300, "edge_px": 4, "seed": 12345, "null_tested": 3000, "max_frame_fraction": 0.25}
+            "max_candidates": 300, "edge_px": 4, "seed": 12345, "null_tested": 3000, "max_frame_fraction": 0.25,
+            "half_tolerance": 3.0, "candidates": 1}
     py = SearchParams().c()
     for k, v in want.items():
         assert getattr(c, k) == v, k
@@ -175,3 +176,30 @@ def test_search_params_registration_offsets():
         SearchParams(reg_dx=[0.0, 1.0]).c()
     with pytest.raises(ft.FtkError):
         SearchParams(reg_dx=[0.0, 1.0], reg_dy=[0.0]).c()
+
+
+def test_ctypes_records_match_the_header_layout(tmp_path):
+    """The Python records of the search C-ABI (ftk_search_params, ftk_search_summary,
+    ftk_candidate) have the header's size and field offsets, as gcc lays them out."""
+    import shutil
+    import subprocess
+    from fainttrail_amd.search import _Candidate, _Params, _Summary
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not present")
+    recs = {"ftk_search_params": _Params, "ftk_search_summary": _Summary, "ftk_candidate": _Candidate}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ftk_hip.h"', "int main(void) {"]
+    for cname, py in recs.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, py in recs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -14,6 +14,22 @@ from refcases import same_bits
 pytestmark = pytest.mark.gpu
 
 
+def _f64_bits(v) -> int:
+    return int(np.array([v], np.float64).view(np.uint64)[0])
+
+
+def _same_candidates(got, want):
+    """The candidate list of refine + finish_candidates (search.cpp:246-395, 627-675):
+    the same candidates in the same order, every field bit for bit, the same flags."""
+    assert len(got) == len(want), (len(got), len(want))
+    for c, r in zip(got, want):
+        assert c.id == r["id"] and c.ok == r["ok"] and c.flags == r["flags"], (c, r)
+        for f in po.CAND_FIELDS:
+            if f in ("id", "ok"):
+                continue
+            assert _f64_bits(getattr(c, f)) == _f64_bits(r[f]), (c.id, f, getattr(c, f), r[f])
+
+
 def _same_as_reference(got, want):
     """Every output the reference's search exposes, bit for bit and in order."""
     assert got.n_velocities == want["n_velocities"]
@@ -27,6 +43,7 @@ def _same_as_reference(got, want):
             for f in ("x", "y", "vidx"):
                 assert np.array_equal(getattr(mine, f), theirs[f]), f
             assert same_bits(mine.snr, theirs["snr"])
+        _same_candidates(got.candidates, want["candidates"])
 
 
 def test_search_matches_reference_golden(hip, golden):
@@ -180,6 +197,9 @@ def test_search_selects_the_references_peaks(hip, group):
                                                   null_runs=5, seed=21, passes_per_launch=group))
     torch.cuda.synchronize()
     _same_as_reference(got, want)
+    print(f"candidates: {len(got.candidates)} ({sum(c.ok for c in got.candidates)} ok), flags "
+          f"{sorted(set(c.flags for c in got.candidates))}", flush=True)
+    assert len(got.candidates) >= 5
 
 
 def _plateau_map(seed, h, w):

@@ -40,6 +40,8 @@ def _same(alone, mixed):
         assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y) and np.array_equal(a.vidx, b.vidx)
         assert same_bits(a.snr, b.snr)
     assert same_bits(alone.null_peaks, mixed.null_peaks)
+    # the candidate stage: the same candidates, every field and flag
+    assert alone.candidates == mixed.candidates
 
 
 def test_jobs_alone_and_in_the_mix(hip):
@@ -61,6 +63,7 @@ def test_jobs_alone_and_in_the_mix(hip):
     for a, b in zip(alone, got):
         _same(a, b)
         assert b.t_submit <= b.t_admit <= b.t_end
+    assert sum(len(a.candidates) for a in alone) >= len(jobs)  # bright movers: the lists are not empty
     assert st["completed"] == len(jobs) and st["failed"] == 0
     assert 1 <= st["max_running"] <= 3
     assert st["peak_reserved_bytes"] <= st["budget_bytes"]
