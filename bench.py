@@ -119,22 +119,24 @@ def host_pass_measure(frames_dev, tab, min_count, dev_max, dev_best):
             "note": "PCIe-inclusive drop-in rate (host frames in, host maps out); never the headline value"}
 
 
-def search_measure(reps: int, with_plain: bool):
+def search_measure(reps: int, with_plain: bool, nmovers: int = 3, mover_snr=(1.5, 1.5), tune: bool = True):
     """Null-calibrated searches/s on the 512^2 configuration: 20 frames of 512 x 512 at
     60 s cadence, sky 1000 ADU + N(0, 32^2), 150 stars (sigma_psf 1.5 px, flux
-    log-uniform 1e3-1e6 ADU), 3 movers of per-frame S/N 1.5 at 0.2-1.1 px/min, grid
-    spaced 0.5 px of total drift (grid_tol 0.125 px -> 5468 motions), 8 null runs,
-    seed 7, the reference's defaults otherwise (the strongest 3000 peaks of each null run
-    screened for artefacts). The device search runs everything on the GPU; the plain
-    form is the reference's own search() with the "hip" stacking backend (host
-    preprocessing, screen and per-pass uploads, as the reference drives its CUDA
-    backend)."""
+    log-uniform 1e3-1e6 ADU), nmovers movers (3 of per-frame S/N 1.5 as stated; the
+    bright form 8 of S/N 2-3, all detected) at 0.2-1.1 px/min, grid spaced 0.5 px of
+    total drift (grid_tol 0.125 px -> 5468 motions), 8 null runs, seed 7, the
+    reference's defaults otherwise (the strongest 3000 peaks of each null run screened
+    for artefacts, the selected peaks refined into candidates). The device search runs
+    everything on the GPU; the plain form is the reference's own search() with the "hip"
+    stacking backend (host preprocessing, screen, refinement and per-pass uploads, as
+    the reference drives its CUDA backend), and its candidates are compared with the
+    device search's field by field."""
     import torch
 
     from fainttrail_amd import synth
     from fainttrail_amd.search import SearchParams, Searcher
     n, w, h = 20, 512, 512
-    frames, mjd, _ = synth.make_sequence(n, w, h, nstars=150, nmovers=3, mover_snr=(1.5, 1.5),
+    frames, mjd, _ = synth.make_sequence(n, w, h, nstars=150, nmovers=nmovers, mover_snr=mover_snr,
                                          rate_px_min=(0.2, 1.1), seed=7, device="cuda:0")
     prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.125, fwhm_px=1.5 * 2.354820045030949,
                        null_runs=8, seed=7)
@@ -150,17 +152,22 @@ def search_measure(reps: int, with_plain: bool):
         return r, (time.perf_counter() - t0) / reps
 
     seq, dt_seq = timed(SearchParams(**{**prm.__dict__, "passes_per_launch": 1}))
-    g, seg, _ = s.tune(frames, mjd, prm, reps=1)
+    g, seg = None, None
+    if tune:
+        g, seg, _ = s.tune(frames, mjd, prm, reps=1)
     res, dt = timed(prm)
     same = bool(seq.threshold_snr == res.threshold_snr and np.array_equal(seq.null_peaks, res.null_peaks)
-                and np.array_equal(seq.peaks.snr, res.peaks.snr)
+                and np.array_equal(seq.peaks.snr, res.peaks.snr) and seq.candidates == res.candidates
                 and torch.equal(seq.max_snr, res.max_snr) and torch.equal(seq.best_v, res.best_v))
     res16, dt16 = timed(SearchParams(**{**prm.__dict__, "storage": "f16"}))
-    out = {"config": "512x512, 20 frames, 5468 motions, 8 null runs, seed 7",
+    out = {"config": f"512x512, 20 frames, 5468 motions, 8 null runs, {nmovers} movers of per-frame S/N "
+                     f"{mover_snr[0]}-{mover_snr[1]}, seed 7",
            "device_searches_per_s": 1.0 / dt, "device_s_per_search": dt, "threshold_snr": res.threshold_snr,
            "method": res.threshold_method, "selected": int(res.selected.x.size),
+           "candidates": len(res.candidates), "candidates_ok": sum(c.ok for c in res.candidates),
            "null_peaks_screened": res.null_screened, "null_artefacts_left_out": res.null_artefacts,
-           "batched": {"passes_per_launch": g, "segments": seg, "equal_to_one_pass_form": same},
+           "batched": {"passes_per_launch": g if tune else res.passes_per_launch,
+                       "segments": seg if tune else res.segments, "equal_to_one_pass_form": same},
            "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
            "one_pass_form_searches_per_s": 1.0 / dt_seq,
            "one_pass_form_stage_seconds": {k: round(v, 4) for k, v in seq.timings.items() if v > 0},
@@ -179,22 +186,56 @@ def search_measure(reps: int, with_plain: bool):
             out["plain_s_per_search"] = dtp
             out["device_over_plain"] = dtp / dt
             out["plain_threshold_equal"] = bool(ref["threshold"] == res.threshold_snr)
+            out["plain_candidates"] = len(ref["candidates"])
+            out["candidates_equal_to_plain"] = same_candidate_lists(res.candidates, ref["candidates"])
             out["f16_over_plain"] = dtp / dt16
     return out
 
 
+def same_candidate_lists(mine, theirs, skip=()) -> bool:
+    """Candidate lists equal in order, ids, flags and every field's bits (fields in
+    skip left out)."""
+    from oracle import pyoracle as po
+    if len(mine) != len(theirs):
+        return False
+    for c, r in zip(mine, theirs):
+        if c.id != r["id"] or c.ok != r["ok"] or c.flags != r["flags"]:
+            return False
+        for f in po.CAND_FIELDS:
+            if f in ("id", "ok") or f in skip:
+                continue
+            if np.float64(getattr(c, f)).tobytes() != np.float64(r[f]).tobytes():
+                return False
+    return True
+
+
 def candidate_agreement(exact, other):
-    """Selected candidates (position, motion) and S/N of a search against the exact one."""
+    """Selected peaks (position, motion) and S/N of a search against the exact one, and
+    its refined candidates: each exact candidate matched to the nearest other one (within
+    1 px at the reference epoch and 1 px/h), the largest S/N and position differences."""
     a = {(int(x), int(y), int(v)): float(q) for x, y, v, q in
          zip(exact.selected.x, exact.selected.y, exact.selected.vidx, exact.selected.snr)}
     b = {(int(x), int(y), int(v)): float(q) for x, y, v, q in
          zip(other.selected.x, other.selected.y, other.selected.vidx, other.selected.snr)}
     err = (other.max_snr.double() - exact.max_snr.double()).abs() / exact.max_snr.double().abs().clamp(min=1.0)
     common = set(a) & set(b)
-    return {"selected": len(b), "selected_exact": len(a), "same_candidates": a.keys() == b.keys(),
+    matched, snr_err, pos_err = 0, 0.0, 0.0
+    for c in exact.candidates:
+        best = min(other.candidates, key=lambda o: (o.x - c.x) ** 2 + (o.y - c.y) ** 2, default=None)
+        if best is None or np.hypot(best.x - c.x, best.y - c.y) > 1.0 or \
+                np.hypot(best.vx_pxh - c.vx_pxh, best.vy_pxh - c.vy_pxh) > 1.0:
+            continue
+        matched += 1
+        snr_err = max(snr_err, abs(best.snr - c.snr) / max(1.0, abs(c.snr)))
+        pos_err = max(pos_err, float(np.hypot(best.x - c.x, best.y - c.y)))
+    return {"selected": len(b), "selected_exact": len(a), "same_selected": a.keys() == b.keys(),
             "threshold": other.threshold_snr, "threshold_exact": exact.threshold_snr,
             "max_rel_snr_error_vs_exact_map": float(err.max()),
-            "max_rel_snr_error_candidates": max((abs(a[k] - b[k]) / max(1.0, abs(a[k])) for k in common), default=0.0)}
+            "max_rel_snr_error_selected": max((abs(a[k] - b[k]) / max(1.0, abs(a[k])) for k in common),
+                                              default=0.0),
+            "candidates": len(other.candidates), "candidates_exact": len(exact.candidates),
+            "candidates_matched": matched, "max_rel_snr_error_candidates": snr_err,
+            "max_position_difference_px": pos_err}
 
 
 def local_maxima(m, floor: float):
@@ -376,12 +417,20 @@ def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     units = float(res.n_velocities) * n * w * h * 257
+    # the same pooled null peaks through the FP64 tail fit with its sums on the GPU
+    t0 = time.perf_counter()
+    thr_gpu, m_gpu = s.refit(prm, on_device=True)
+    t_fit_gpu = time.perf_counter() - t0
     out = {"config": "2048x2048, 40 frames, 7996 motions, 256 null runs + real, seed 7",
            "n_velocities": res.n_velocities, "device_s_per_search": dt, "device_searches_per_s": 1.0 / dt,
            "stacked_units_per_s": units / dt, "threshold_snr": res.threshold_snr, "method": res.threshold_method,
            "selected": int(res.selected.x.size), "tuned_on_16_nulls": {"passes_per_launch": g, "segments": seg},
            "passes_per_launch": res.passes_per_launch, "segments": res.segments,
-           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0}}
+           "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
+           "fit_on_gpu": {"threshold_snr": thr_gpu, "method_equal": m_gpu == res.threshold_method,
+                          "rel_diff_vs_host_fit": abs(thr_gpu - res.threshold_snr) / abs(res.threshold_snr),
+                          "bound_50_m_eps": 50.0 * min(res.null_peaks.size, 3000 * 256) * float(np.finfo(np.float64).eps),
+                          "seconds": t_fit_gpu, "pooled_peaks": int(res.null_peaks.size)}}
     if with_plain:
         from oracle import pyoracle as po
         if po.ref_available():
@@ -398,6 +447,34 @@ def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
                             "seconds_per_null_run": per_null, "extrapolated_s_per_search_256_nulls": est,
                             "note": "extrapolated from the measured runs: real pass + 256 x the per-null cost"}
             out["device_over_plain_extrapolated"] = est / dt
+    # candidates at this scale: the same sky with 10 movers of per-frame S/N 1.5 (all
+    # detectable), searched at the threshold the 256-null calibration set (fixed, so
+    # the plain form needs no null runs), device against plain, field by field
+    fb, mjdb, _ = synth.make_sequence(n, w, h, nstars=300, nmovers=10, mover_snr=(1.5, 1.5),
+                                      rate_px_min=(0.2, 1.1), seed=7, device="cuda:0")
+    pb = SearchParams(**{**prm.__dict__, "threshold_snr": res.threshold_snr, "null_runs": 0})
+    s.run(fb, mjdb, pb)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rb = s.run(fb, mjdb, pb)
+    torch.cuda.synchronize()
+    dtb = time.perf_counter() - t0
+    br = {"config": "the same frames with 10 movers of per-frame S/N 1.5, threshold fixed at the calibrated one",
+          "threshold_snr": res.threshold_snr, "device_s_per_search": dtb, "selected": int(rb.selected.x.size),
+          "candidates": len(rb.candidates), "candidates_ok": sum(c.ok for c in rb.candidates),
+          "stage_seconds": {k: round(v, 4) for k, v in rb.timings.items() if v > 0}}
+    if with_plain:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            host = fb.cpu().numpy()
+            t0 = time.perf_counter()
+            ref = po.ref_search(host, mjdb, 12.0, 66.0, 0.212, fwhm, 0, seed=7, threshold=res.threshold_snr,
+                                backend="hip", extras=False)
+            dtp = time.perf_counter() - t0
+            br.update({"plain_s_per_search": dtp, "device_over_plain": dtp / dtb,
+                       "plain_candidates": len(ref["candidates"]),
+                       "candidates_equal_to_plain": same_candidate_lists(rb.candidates, ref["candidates"])})
+    out["bright_movers_fixed_threshold"] = br
     return out
 
 
@@ -525,6 +602,9 @@ def main() -> int:
     if not args.no_search:
         print("bench: null-calibrated search (512x512 configuration)", file=sys.stderr, flush=True)
         res["search"] = search_measure(args.search_reps, with_plain=not args.no_cpu)
+        print("bench: search (bright movers)", file=sys.stderr, flush=True)
+        res["search"]["bright_movers"] = search_measure(args.search_reps, with_plain=not args.no_cpu, nmovers=8,
+                                                        mover_snr=(2.0, 3.0), tune=False)
         print("bench: search: " + json.dumps(res["search"]), file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_batched:

@@ -44,7 +44,7 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->max_candidates = c.max_candidates;
     p->edge_px = c.edge_px;
     p->seed = c.seed;
-    p->fit_on_device = 0;
+    p->fit_on_device = c.fit_on_device;
     p->passes_per_launch = 0;
     p->segments = 0;
     p->storage = FTK_STORE_F32;
@@ -141,6 +141,7 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
         throw std::runtime_error("ftk_searcher_run: unknown frame storage");
     if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
     if (p->candidates != 0 && p->candidates != 1) throw std::runtime_error("ftk_searcher_run: candidates must be 0 or 1");
+    if (p->fit_on_device < -1 || p->fit_on_device > 1) throw std::runtime_error("ftk_searcher_run: fit_on_device must be -1, 0 or 1");
 }
 }  // namespace
 
@@ -203,6 +204,22 @@ extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t str
         if (passes_per_launch) *passes_per_launch = s->ds->tuned_passes;
         if (segments) *segments = s->ds->tuned_segments;
         if (seconds) *seconds = t;
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        put_err2(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_searcher_refit(ftk_searcher* s, const ftk_search_params* p, int on_device, double* threshold,
+                                  int* method, void* stream, char* err, size_t err_len) {
+    try {
+        if (!s || !p) throw std::runtime_error("ftk_searcher_refit: null argument");
+        hipSetDevice(s->device);
+        int m = 0;
+        const double t = s->ds->refit(config_of(p), s->last, on_device != 0, &m, static_cast<hipStream_t>(stream));
+        if (threshold) *threshold = t;
+        if (method) *method = m;
         return FTK_OK;
     } catch (const std::exception& e) {
         put_err2(err, err_len, e.what());
@@ -379,6 +396,7 @@ extern "C" int ftk_server_submit(ftk_server* s, const float* d_raw, size_t strid
         if (!(p->fwhm_px > 0.0)) throw std::runtime_error("search: fwhm_px must be given (> 0)");
         if (p->null_tested < 0) throw std::runtime_error("ftk_server_submit: null_tested must be >= 0");
         if (p->candidates != 0 && p->candidates != 1) throw std::runtime_error("ftk_server_submit: candidates must be 0 or 1");
+        if (p->fit_on_device < -1 || p->fit_on_device > 1) throw std::runtime_error("ftk_server_submit: fit_on_device must be -1, 0 or 1");
         ftk::SearchConfig c = config_of(p);
         set_offsets(c, p, n);
         *job = s->srv->submit(d_raw, static_cast<long long>(stride), n, w, h, mjd, std::move(c));

@@ -1158,12 +1158,36 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     return out;
 }
 
-// The null-tail threshold (optionally with the FP64 sums on the device) and
-// select_peaks (search.cpp:614-620).
+// The null-tail threshold, with the FP64 tail-fit sums on the GPU for 16-bit storage
+// (or as asked), and select_peaks (search.cpp:614-620).
 void DeviceSearch::finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s) {
+    const bool gpu_fit = cfg.fit_on_device > 0 || (cfg.fit_on_device < 0 && cfg.storage != kStoreF32);
+    fit_threshold(cfg, runs, nv, gpu_fit, out, s);
+    for (size_t q = 0; q < out.peaks.snr.size(); ++q) {
+        if (!(out.peaks.snr[q] >= out.threshold) || out.peaks.vidx[q] < 0) continue;
+        if (static_cast<int>(out.selected.x.size()) >= cfg.max_candidates) break;
+        out.selected.x.push_back(out.peaks.x[q]);
+        out.selected.y.push_back(out.peaks.y[q]);
+        out.selected.snr.push_back(out.peaks.snr[q]);
+        out.selected.vidx.push_back(out.peaks.vidx[q]);
+    }
+}
+
+double DeviceSearch::refit(const SearchConfig& cfg, const SearchOutput& last, bool on_device, int* method,
+                           hipStream_t s) {
+    SearchOutput o;
+    o.pooled = last.pooled;
+    const int runs = cfg.threshold_snr <= 0.0 ? std::max(0, cfg.null_runs) : 0;
+    fit_threshold(cfg, runs, static_cast<int>(last.grid.size()), on_device, o, s);
+    if (method) *method = o.method;
+    return o.threshold;
+}
+
+void DeviceSearch::fit_threshold(const SearchConfig& cfg, int runs, int nv, bool on_device, SearchOutput& out,
+                                 hipStream_t s) {
     double sums[10] = {0};
     const double* dsums = nullptr;
-    if (cfg.fit_on_device && runs > 0) {
+    if (on_device && runs > 0) {
         std::vector<float> desc(out.pooled);
         std::sort(desc.begin(), desc.end(), std::greater<float>());
         const int m = static_cast<int>(std::min(desc.size(), static_cast<size_t>(3000.0 * runs)));
@@ -1190,14 +1214,6 @@ void DeviceSearch::finish(const SearchConfig& cfg, int runs, int nv, SearchOutpu
         }
     }
     set_threshold(cfg, w, h, cfg.fwhm_px, static_cast<size_t>(nv), runs, out.pooled, out, dsums);
-    for (size_t q = 0; q < out.peaks.snr.size(); ++q) {
-        if (!(out.peaks.snr[q] >= out.threshold) || out.peaks.vidx[q] < 0) continue;
-        if (static_cast<int>(out.selected.x.size()) >= cfg.max_candidates) break;
-        out.selected.x.push_back(out.peaks.x[q]);
-        out.selected.y.push_back(out.peaks.y[q]);
-        out.selected.snr.push_back(out.peaks.snr[q]);
-        out.selected.vidx.push_back(out.peaks.vidx[q]);
-    }
 }
 
 void DeviceSearch::reserve_real() {

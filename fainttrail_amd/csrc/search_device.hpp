@@ -34,7 +34,9 @@ struct SearchConfig {
     int max_candidates = 300;
     int edge_px = 4;
     uint64_t seed = 12345;
-    int fit_on_device = 0;  // 1: the FP64 tail-fit sums run on the GPU (tolerance, not bits)
+    // where the FP64 tail-fit sums run: 1 on the GPU (tolerance, not bits), 0 on the
+    // host (the reference's bits), -1 on the GPU for 16-bit storage, else on the host
+    int fit_on_device = -1;
     // Batched form: the real pass and the null runs go through the stacking kernel in
     // groups of passes_per_launch passes per launch (0: the searcher's tuned value, else
     // every pass in one group, within the memory budget; 1: one pass at a time with a
@@ -135,6 +137,10 @@ public:
     int n, w, h;
     int tuned_passes = 0, tuned_segments = 0;  // from tune(); 0 = none
 
+    // The threshold of a run's pooled null peaks fitted again with the FP64 tail-fit
+    // sums on the host (on_device false) or the GPU; method receives the method index.
+    double refit(const SearchConfig& cfg, const SearchOutput& last, bool on_device, int* method, hipStream_t s);
+
     // Device bytes the searcher holds now (every buffer it allocated and keeps, as
     // asked of hipMalloc; buffers only grow between runs).
     double device_bytes() const;
@@ -151,6 +157,7 @@ private:
     SearchOutput run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
                              float* d_max_snr, int* d_best_v, hipStream_t s);
     void finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s);
+    void fit_threshold(const SearchConfig& cfg, int runs, int nv, bool on_device, SearchOutput& out, hipStream_t s);
     // The candidate stage: the selected peaks refined on the device against the real
     // pass's frames, residuals and mask, then finished on the host.
     void refine(const SearchConfig& cfg, SearchOutput& out, hipStream_t s);

@@ -73,6 +73,8 @@ def _bind():
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     L.ftk_searcher_candidates.argtypes = [vp, C.c_int, C.POINTER(_Candidate)]
+    L.ftk_searcher_refit.argtypes = [vp, C.POINTER(_Params), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), vp,
+                                     C.c_char_p, sz]
     I32 = C.POINTER(C.c_int32)
     L.ftk_find_peaks.argtypes = [vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, I32, I32,
                                  C.POINTER(C.c_float), I32, C.POINTER(C.c_int), vp, C.c_char_p, sz]
@@ -101,7 +103,10 @@ class SearchParams:
     max_candidates: int = 300
     edge_px: int = 4
     seed: int = 12345
-    fit_on_device: bool = False
+    # where the FP64 null-tail fit sums run: True on the GPU (threshold within ~m eps
+    # relative of the host fit, same method), False on the host (the reference's bits),
+    # None on the GPU for 16-bit storage, else on the host
+    fit_on_device: Optional[bool] = None
     # batched form (no result depends on these): passes per stacking launch (1 = one
     # pass at a time with a plan per pass; 0 = tuned, else all passes) and
     # velocity-block segments per tile (0 = planner / tuned)
@@ -132,7 +137,8 @@ class SearchParams:
         p = _Params(self.vmin_pxh, self.vmax_pxh, self.grid_tol_px, self.fwhm_px, self.clip_sigma, self.cr_sigma,
                     self.static_exclude, self.mask_scatter, self.weight_scatter, self.min_coverage,
                     self.threshold_snr, self.min_snr_floor, self.null_runs, self.target_false_per_run,
-                    self.max_candidates, self.edge_px, self.seed, int(self.fit_on_device),
+                    self.max_candidates, self.edge_px, self.seed,
+                    -1 if self.fit_on_device is None else int(bool(self.fit_on_device)),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
                     self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance)
         if self.reg_dx is not None:
@@ -287,6 +293,16 @@ class Searcher:
              "plan": summ.t_plan, "passes (batched)": summ.t_passes, "refine": summ.t_refine},
             mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts,
             read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates))
+
+    def refit(self, params: SearchParams, on_device: bool, stream=None):
+        """The last run's null-tail threshold fitted again with the FP64 fit sums on the
+        GPU (on_device) or the host: (threshold, method)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        prm, e, t, m = params.c(), _err(), C.c_double(), C.c_int()
+        _raise(lib().ftk_searcher_refit(self._h, C.byref(prm), int(bool(on_device)), C.byref(t), C.byref(m),
+                                        s.cuda_stream, e, len(e)), e)
+        return t.value, METHODS[m.value]
 
     def __del__(self):
         h = getattr(self, "_h", None)

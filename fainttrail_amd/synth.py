@@ -48,12 +48,13 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
                   star_flux=(1e3, 1e6), psf_sigma: float = 1.5, nmovers: int = 3, mover_snr=(1.5, 1.5),
                   rate_px_min=(0.2, 1.1), cadence_s: float = 60.0, exptime_s: float = 60.0, seed: int = 7,
                   device: str = "cpu", mjd0: float = 61000.0, saturate: float = 0.0, cr_frames: int = 0,
-                  cr_hits: int = 50):
+                  cr_hits: int = 50, mover_box=None):
     """Returns (frames (n, h, w) float32 torch tensor on `device`, mjd (n,), movers).
 
     saturate > 0 clips every pixel at that level (a detector's full well, e.g. 65535
     ADU); cr_frames frames (chosen by the seed) get cr_hits cosmic-ray hits of 1-4
-    pixels and 800-15000 ADU each."""
+    pixels and 800-15000 ADU each. mover_box (x0, x1, y0, y1): where the movers are at
+    the reference epoch (default: 24 px inside the frame)."""
     import torch
     rng = np.random.default_rng(seed)
     static = np.zeros((h, w), np.float32)
@@ -65,12 +66,13 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
     mjd = mjd0 + t_s / 86400.0
     t_ref = 0.5 * (mjd[0] + mjd[-1])
     t_h = (mjd - t_ref) * 24.0
+    bx0, bx1, by0, by1 = mover_box if mover_box is not None else (24, w - 24, 24, h - 24)
     movers = []
     for _ in range(nmovers):
         snr = rng.uniform(*mover_snr)
         rate = rng.uniform(*rate_px_min) * 60.0  # px/h
         pa = rng.uniform(0.0, 2 * math.pi)
-        movers.append(Mover(rng.uniform(24, w - 24), rng.uniform(24, h - 24), rate * math.cos(pa), rate * math.sin(pa),
+        movers.append(Mover(rng.uniform(bx0, bx1), rng.uniform(by0, by1), rate * math.cos(pa), rate * math.sin(pa),
                             snr * noise * math.sqrt(4 * math.pi) * psf_sigma))
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)

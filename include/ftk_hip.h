@@ -207,7 +207,11 @@ typedef struct ftk_search_params {
     double target_false_per_run;
     int max_candidates, edge_px;
     unsigned long long seed;
-    int fit_on_device; /* 1: the FP64 tail-fit sums run on the GPU (agrees to ~1e-12) */
+    /* Where the FP64 null-tail fit sums run: 1 on the GPU (same method; the threshold
+     * agrees with the host fit within the host sums' rounding, ~m eps relative for m
+     * fitted peaks), 0 on the host (the reference's bits), -1 (default) on the GPU for
+     * 16-bit storage, else on the host. */
+    int fit_on_device;
     /* Batched form (no result bit depends on these): stacking passes per launch, the
      * real pass and the null runs in groups of this many (1: one pass at a time, a
      * plan per pass; 0: the searcher's tuned value, else every pass in one group, within
@@ -299,6 +303,11 @@ int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride,
  * entries into the non-NULL arrays and returns the total count. */
 int ftk_searcher_peaks(const ftk_searcher* searcher, int which, int cap, int32_t* x, int32_t* y, float* snr,
                        int32_t* vidx);
+/* The null-tail threshold of the last run's pooled null peaks fitted again (params as
+ * for that run) with the FP64 fit sums on the host (on_device 0) or the GPU (1):
+ * threshold and method index as in ftk_search_summary. */
+int ftk_searcher_refit(ftk_searcher* searcher, const ftk_search_params* params, int on_device, double* threshold,
+                       int* method, void* stream, char* err, size_t err_len);
 /* Candidates of the last run, in the reference's order (ok ones first, then by S/N;
  * ft_candidate_count / ft_get_candidate, src/c_api.cpp:213-246): writes up to cap
  * records and returns the count, -1 for a null searcher. */

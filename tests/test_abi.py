@@ -159,7 +159,7 @@ def test_search_params_defaults_are_the_reference_defaults():
             "static_exclude": -1, "mask_scatter": 2.5, "weight_scatter": 1.5, "min_coverage": 0.7,
             "threshold_snr": 0.0, "min_snr_floor": 5.0, "null_runs": 3, "target_false_per_run": 0.02,
             "max_candidates": 300, "edge_px": 4, "seed": 12345, "null_tested": 3000, "max_frame_fraction": 0.25,
-            "half_tolerance": 3.0, "candidates": 1}
+            "half_tolerance": 3.0, "candidates": 1, "fit_on_device": -1}
     py = SearchParams().c()
     for k, v in want.items():
         assert getattr(c, k) == v, k

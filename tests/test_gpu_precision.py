@@ -187,3 +187,88 @@ def test_storage_error_against_fp64(hip):
     strong = m64 > 6.0
     for k, m in maps.items():
         assert np.array_equal(m > 6.0, strong) or np.sum((m > 6.0) != strong) <= 2, k
+
+
+def test_precision_form_at_the_workload(hip):
+    """The precision workload's sky model at full width: 100 frames of 6144 x 512 (a
+    full-width window of the 6144 x 4096 frame with its star density: 375 stars of
+    1e3-1.6e6 ADU saturating at 65535 ADU), sky 2000 ADU + N(0, 45^2), one frame with
+    50 cosmic-ray hits, prepared on the device, stacked over the 29 972-motion grid
+    (6-120 px/h, grid_tol 0.509 px). 24 movers of per-frame S/N 0.8-1.5 sit inside a
+    64-row band that is also evaluated in FP64.
+
+    * fp16 and bf16 stored passes equal the fp32 pass over the decoded frames, every
+      pixel of the window;
+    * against the FP64 rows: fp32 within 1e-5, fp16 within 1e-3, bf16 within 1e-2 of
+      max(1, |S/N_fp64|) (bf16 keeps 8 mantissa bits: about 2^-9 relative per sample,
+      so 1e-3 is out of its reach);
+    * detections: at least 20 movers are FP64 local maxima (5 x 5) at S/N >= 6, and
+      every one is a maximum at the same pixel in every storage form; the sets of
+      maxima >= 6 in the band agree but for at most two noise maxima at the edge."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.stage import Stage, StageParams
+    n, w, h = 100, 6144, 512
+    y0, y1 = 224, 288
+    fwhm = 1.5 * 2.354820045030949
+    print("precision workload: generating 100 x 6144 x 512", flush=True)
+    raw, mjd, movers = synth.make_sequence(n, w, h, sky=2000.0, noise=45.0, nstars=375, star_flux=(1e3, 1.6e6),
+                                           nmovers=24, mover_snr=(0.8, 1.5), rate_px_min=(0.2, 1.8), seed=41,
+                                           device="cuda:0", saturate=65535.0, cr_frames=1, cr_hits=50,
+                                           mover_box=(150, w - 150, y0 + 6, y1 - 6))
+    t_h = (mjd - 0.5 * (mjd[0] + mjd[-1])) * 24.0
+    stage = Stage(n, w, h)
+    sub, _ = stage.background(raw)
+    del raw
+    filt = stage.build(sub, t_h, StageParams(fwhm_px=fwhm))
+    if isinstance(filt, tuple):
+        filt = filt[0]
+    del sub, stage
+    grid, _ = hip.velocity_grid(float(t_h[-1] - t_h[0]), 6.0, 120.0, 0.509)
+    tab = hip.linear_shift_table(t_h, grid)
+    assert tab.motions == 29972
+    mc = max(3, int(np.ceil(0.7 * n)))
+    print(f"precision workload: {tab.motions} motions, planning", flush=True)
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    maps = {}
+    m32, _ = plan.run(filt)
+    maps["f32"] = m32
+    for st in ("f16", "bf16"):
+        print(f"precision workload: {st} stored pass and the fp32 pass of its decoded frames", flush=True)
+        plan.store(filt, st)
+        ms, bs = plan.run_stored()
+        sc = torch.from_numpy(np.asarray(plan.scales(), np.float32)).cuda().view(n, 1, 1)
+        q = (filt / sc).half() if st == "f16" else (filt / sc).bfloat16()
+        dec = q.float() * sc
+        del q
+        md, bd = plan.run(dec)
+        torch.cuda.synchronize()
+        assert torch.equal(ms.view(torch.int32), md.view(torch.int32)), st
+        assert torch.equal(bs, bd), st
+        del dec, md, bd, bs
+        maps[st] = ms
+    print("precision workload: FP64 rows", flush=True)
+    m64, _ = hip.stack_fp64(filt, tab.dx, tab.dy, mc, y0, y1)
+    torch.cuda.synchronize()
+    errs = {k: float(((m[y0:y1].double() - m64).abs() / m64.abs().clamp(min=1.0)).max()) for k, m in maps.items()}
+    print("max relative S/N error vs FP64:", errs, flush=True)
+    assert errs["f32"] <= 1e-5 and errs["f16"] <= 1e-3 and errs["bf16"] <= 1e-2
+
+    def maxima(m):
+        md = m.double()
+        mx = torch.nn.functional.max_pool2d(md[None, None], 5, stride=1, padding=2)[0, 0]
+        ys, xs = torch.nonzero((md >= 6.0) & (md == mx), as_tuple=True)
+        return {(int(y), int(x)) for y, x in zip(ys.tolist(), xs.tolist()) if 2 <= y < (y1 - y0) - 2}
+
+    ref = maxima(m64)
+    found = set()
+    for mv in movers:
+        near = [p for p in ref if abs(p[0] - (mv.y0 - y0)) <= 2.5 and abs(p[1] - mv.x0) <= 2.5]
+        if near:
+            found.add(min(near, key=lambda p: (p[0] - (mv.y0 - y0)) ** 2 + (p[1] - mv.x0) ** 2))
+    print(f"movers detected in FP64: {len(found)} of {len(movers)}; FP64 maxima >= 6: {len(ref)}", flush=True)
+    assert len(found) >= 20
+    for k, m in maps.items():
+        got = maxima(m[y0:y1])
+        assert found <= got, k
+        assert len(got ^ ref) <= 2, (k, sorted(got ^ ref))

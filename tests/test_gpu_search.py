@@ -3,7 +3,7 @@ strongest 3000 peaks of each null run screened for artefacts): the real pass's m
 and best-velocity maps, its peak list (find_peaks order), the selected peaks, every
 pooled null-peak S/N value in the search's own order (run after run) and the
 calibrated threshold (an FP64 value) must be identical; with the tail-fit sums on the
-GPU the threshold agrees to 1e-12 relative (device log and summation order differ).
+GPU the threshold agrees to 50 m eps relative (m fitted peaks; see fit_tol).
 The golden fixture holds a search with no screen (null_tested = 0)."""
 import numpy as np
 import pytest
@@ -12,6 +12,17 @@ from oracle import pyoracle as po
 from refcases import same_bits
 
 pytestmark = pytest.mark.gpu
+
+# The FP64 null-tail fit with its sums on the GPU against the host fit. The host adds the
+# m fitted terms of each of the nine normal-equation sums one after another (the
+# reference's order), so its sums carry up to ~m eps relative rounding; the GPU adds
+# them in blocks (and takes its own log), which is closer to exact. The 3 x 3 solve and
+# the bisection for the level pass the difference on, amplified by the fit's
+# conditioning: 1.4e-12 at m = 1 877 and 4.2e-10 at m = 768 000 measured. The bound is
+# 50 m eps; the method (the decision) must be the same.
+def fit_tol(pooled, runs) -> float:
+    m = min(len(pooled), 3000 * runs)
+    return 50.0 * m * np.finfo(np.float64).eps
 
 
 def _f64_bits(v) -> int:
@@ -78,11 +89,11 @@ def test_search_matches_live_reference(hip, seed, runs):
     got = s.run(torch.from_numpy(frames).cuda(), mjd, prm)
     torch.cuda.synchronize()
     _same_as_reference(got, want)
-    # the FP64 tail-fit sums on the GPU: same decision, threshold within 1e-12 relative
+    # the FP64 tail-fit sums on the GPU: same decision, threshold within 50 m eps relative
     prm.fit_on_device = True
     got2 = s.run(torch.from_numpy(frames).cuda(), mjd, prm)
     assert got2.threshold_method == got.threshold_method
-    assert abs(got2.threshold_snr - want["threshold"]) <= 1e-12 * abs(want["threshold"])
+    assert abs(got2.threshold_snr - want["threshold"]) <= fit_tol(got.null_peaks, runs) * abs(want["threshold"])
     # selected peaks are the real peaks at or above the threshold, strongest first
     sel = got.selected
     assert np.all(sel.snr >= got.threshold_snr) and np.all(np.diff(sel.snr) <= 0)
@@ -319,8 +330,12 @@ def test_batched_search_fixed_threshold_and_tuner(hip):
 
 def test_sixteen_bit_search_agrees_with_exact_search(hip):
     """Stored-frame searches (fp16, bf16) against the exact fp32 search on a sequence
-    with bright movers: the same selected candidates (position and motion), S/N of each
-    within 1e-3 (fp16) / 1e-2 (bf16) of max(1, S/N), thresholds within 1 %."""
+    with bright movers: the same selected peaks (position and motion), S/N of each
+    within 1e-3 (fp16) / 1e-2 (bf16) of max(1, S/N), thresholds within 1 %. The
+    candidates are refined on the exact fp32 frames, so the refined tracks, S/N and
+    photometry equal the exact search's bit for bit; only the grid S/N differs. The
+    null-tail fit of the 16-bit forms runs on the GPU: within 50 m eps relative of the
+    host fit of the same pooled peaks, same method."""
     import torch
     from fainttrail_amd import synth
     from fainttrail_amd.search import SearchParams, Searcher
@@ -345,3 +360,13 @@ def test_sixteen_bit_search_agrees_with_exact_search(hip):
             assert abs(a[k] - b[k]) <= tol * max(1.0, abs(a[k])), (st, k)
         err = (got.max_snr - ref.max_snr).abs() / ref.max_snr.abs().clamp(min=1.0)
         assert float(err.max()) <= 2 * tol, st
+        assert len(got.candidates) == len(ref.candidates) >= 2, st
+        for c, r in zip(got.candidates, ref.candidates):
+            for f in ("x", "y", "vx_pxh", "vy_pxh", "snr", "snr_half1", "snr_half2", "flux", "mag_inst",
+                      "max_frame_fraction", "pos_err_px", "vel_err_pxh"):
+                assert getattr(c, f) == getattr(r, f), (st, c.id, f)
+            assert abs(c.snr_coarse - r.snr_coarse) <= tol * max(1.0, abs(r.snr_coarse)), (st, c.id)
+        host_t, host_m = s.refit(SearchParams(**{**prm.__dict__, "storage": st}), on_device=False)
+        assert host_m == got.threshold_method, st
+        print(f"{st}: GPU fit vs host fit {abs(host_t - got.threshold_snr) / abs(host_t):.2e} relative", flush=True)
+        assert abs(host_t - got.threshold_snr) <= fit_tol(got.null_peaks, prm.null_runs) * abs(host_t), st
