@@ -67,6 +67,19 @@ def traffic_from_profiles(workload: str):
     return float(m.group(1)) if m else None
 
 
+def lds_bytes_from_profiles(workload: str):
+    """LDS bytes the pass reads (SQ_INSTS_LDS x 64 lanes x 8 B over its kernels, from
+    rocprofv3 --pmc), if a counter summary exists; the headline workload's is in
+    stack_kernel_sq_counters.md."""
+    if workload != "tiled-6144x4096":
+        return None
+    p = ROOT / "profiles" / "stack_kernel_sq_counters.md"
+    if not p.exists():
+        return None
+    m = re.search(r"lds_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(frames_dev, tab, min_count, target_s: float):
     """Time the oracle's blocked CPU pass on output rows in the middle of the frame."""
     from oracle import pyoracle as po
@@ -392,6 +405,90 @@ def serving_measure(count: int, rate_hz: float, seed: int, slots: int, sustain_c
             "at_stated_rate": over, "at_sustainable_rate": sus}
 
 
+def serving_stream_measure(count: int, rate_hz: float, seed: int, slots: int, budget_frac: float = 0.6):
+    """A long stream of the serving distribution for latency percentiles: count jobs
+    (the same per-job distribution as serving_measure) arriving as a seeded Poisson
+    stream at rate_hz, below the server's capacity. Each job's frames are generated on
+    the device by a producer thread at most two jobs ahead of its arrival and handed to
+    the server at the arrival time; a job generated after its arrival time is submitted
+    late (counted, and its latency runs from its actual submission). Ended jobs are
+    collected as the stream runs, so their frames are released. The server's budget is
+    budget_frac of the free HBM (the rest holds the jobs' frames)."""
+    import queue
+    import threading
+
+    import torch
+
+    import fainttrail_amd as ft
+    from fainttrail_amd.serve import Server, job_inputs, poisson_jobs
+    specs = poisson_jobs(count, rate_hz, seed)
+    free_b, _ = torch.cuda.mem_get_info()
+    srv = Server(slots=slots, budget_mb=budget_frac * free_b / 1048576.0)
+    ready = queue.Queue(maxsize=2)
+    failure = []
+
+    def produce():
+        try:
+            for sp in specs:
+                inp = job_inputs(sp)
+                torch.cuda.synchronize()
+                ready.put(inp)
+        except Exception as e:  # surfaced by the consumer
+            failure.append(e)
+            ready.put(None)
+
+    prod = threading.Thread(target=produce, daemon=True)
+    t0 = time.perf_counter()
+    prod.start()
+    pending, results, late, units = {}, {}, [], {}
+
+    def collect(block: bool):
+        for j in list(pending):
+            r = srv.wait(j, -1.0 if block else 0.0)
+            if r is not None:
+                results[pending.pop(j)] = r
+
+    for k, sp in enumerate(specs):
+        inp = ready.get()
+        if inp is None:
+            raise failure[0]
+        fr, mjd, prm = inp
+        span_h = (mjd[-1] - mjd[0]) * 24.0
+        nv = ft.velocity_grid(span_h, prm.vmin_pxh, prm.vmax_pxh, prm.grid_tol_px)[0].shape[0]
+        units[k] = float(nv) * sp.frames * sp.side * sp.side * (1 + sp.nulls)
+        now = time.perf_counter() - t0
+        if now < sp.arrival_s:
+            time.sleep(sp.arrival_s - now)
+        else:
+            late.append(now - sp.arrival_s)
+        pending[srv.submit(fr, mjd, prm)] = k
+        del inp, fr
+        collect(False)
+        if (k + 1) % 20 == 0:
+            print(f"bench: serving stream: {k + 1} of {count} submitted, {len(results)} ended, "
+                  f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    collect(True)
+    prod.join()
+    st = srv.stats()
+    srv.close()
+    res = [results[k] for k in range(count)]
+    lat = np.array([r.latency for r in res])
+    wait = np.array([r.t_admit - r.t_submit for r in res])
+    span = max(r.t_end for r in res) - min(r.t_submit for r in res)
+    busy = sum(r.t_end - r.t_admit for r in res)
+    pct = {f"p{q}_latency_s": float(np.percentile(lat, q)) for q in (50, 90, 95, 99)}
+    return {"config": f"Poisson stream, seed {seed}, {count} jobs at {rate_hz} jobs/s: side 256-4096, 10-120 frames, "
+                      f"500-50000 motions, 4-64 nulls, 0-15 movers; {slots} slots, budget {budget_frac:.0%} of free HBM",
+            "jobs": len(res), **pct, "mean_latency_s": float(lat.mean()), "max_latency_s": float(lat.max()),
+            "p50_queue_wait_s": float(np.percentile(wait, 50)), "p99_queue_wait_s": float(np.percentile(wait, 99)),
+            "jobs_per_s": len(res) / span, "units_per_s": sum(units.values()) / span, "wall_s": span,
+            "mean_job_s_in_slot": busy / len(res), "slot_occupancy": busy / (span * slots),
+            "late_submissions": len(late), "max_late_s": max(late, default=0.0),
+            "candidates": int(sum(len(r.candidates) for r in res)), "max_running": st["max_running"],
+            "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9,
+            "evicted_searchers": st["evicted"], "failed": st["failed"]}
+
+
 def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
     """Batched null calibration on the 2048^2 configuration: 40 frames of 2048 x 2048
     (60 s cadence), 300 stars, 10 movers of per-frame S/N 0.7 at 0.2-1.1 px/min, grid
@@ -498,6 +595,10 @@ def main() -> int:
                     help="skip the host-buffer pass (the drop-in with PCIe upload and download)")
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-seed", type=int, default=3)
+    ap.add_argument("--serving-stream", type=int, default=0, metavar="N",
+                    help="measure only a stream of N serving jobs at --serve-rate (latency percentiles); prints its "
+                         "JSON object")
+    ap.add_argument("--serve-rate", type=float, default=0.3, help="arrival rate of --serving-stream (jobs/s)")
     ap.add_argument("--search-only", action="store_true",
                     help="measure only the null-calibrated search (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -515,6 +616,10 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.serving_stream > 0:
+        print(json.dumps({"serving_stream": serving_stream_measure(args.serving_stream, args.serve_rate,
+                                                                   args.serve_seed, 4)}))
+        return 0
     if args.search_only:
         print(json.dumps({"search": search_measure(args.search_reps, with_plain=not args.no_cpu)}))
         return 0
@@ -560,6 +665,10 @@ def main() -> int:
     # the probe of that form is the kernel's LDS ceiling; 16-byte reads go faster
     lds_peak = ft.probe_lds(8)
     lds_peak16 = ft.probe_lds(16)
+    traffic = traffic_from_profiles(args.workload)
+    lds_bytes = lds_bytes_from_profiles(args.workload) if args.storage == "f32" else None
+    compulsory = float(n) * w * h * (4 if args.storage == "f32" else 2) + 24.0 * w * h  # frames; keys, maps
+    add_peak = 256 * 64 * 2.4e9  # unpacked fp32 adds/s: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz
     res = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -569,15 +678,23 @@ def main() -> int:
                    "grid_tol_px": WORKLOADS[args.workload][5], "dv_pxh": dv, "min_count": min_count,
                    "units_per_step": units},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic_from_profiles(args.workload),
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_over_compulsory": traffic / compulsory if traffic else None,
+                     "traffic_rate_gb_s": traffic / kernel_s / 1e9 if traffic else None,
                      "kernel_ms": kernel_s * 1e3,
-                     "lds_peak_measured": lds_peak / 1e9, "lds_frac": achieved / lds_peak,
+                     "lds_bytes_per_unit": lds_bytes / units if lds_bytes else None,
+                     "lds_peak_measured": lds_peak / 1e9,
+                     "lds_frac": lds_bytes / kernel_s / lds_peak if lds_bytes else None,
                      "lds_peak_b128_measured": lds_peak16 / 1e9,
-                     "note": "algorithmic bytes = 4 B (one fp32 sample) per stacked unit; frac > 1 is reuse "
-                             "from LDS-staged frame boxes; the inner loop reads each unit's sample from LDS (two rows "
-                             "per ds_read2st64_b32), so lds_frac, against the LDS read bandwidth measured with that "
-                             "instruction form, is the binding figure (ds_read_b128 reads reach "
-                             "lds_peak_b128_measured)"},
+                     "lds_frac_best_form": lds_bytes / kernel_s / lds_peak16 if lds_bytes else None,
+                     "add_frac": units / kernel_s / add_peak,
+                     "note": "algorithmic bytes = 4 B (one fp32 sample) per stacked unit, so frac > 1 is the reuse of "
+                             "LDS-staged frame boxes, not an HBM fraction. The binding figures: lds_frac is the LDS "
+                             "bytes the pass reads (rocprofv3 SQ_INSTS_LDS, profiles/) per second against the LDS read "
+                             "rate measured for the loop's ds_read2st64_b32 form, lds_frac_best_form the same against "
+                             "ds_read_b128; add_frac is one fp32 add per unit against 256 CUs x 64 lanes x 2.4 GHz; "
+                             "traffic (FETCH_SIZE + WRITE_SIZE, copy-calibrated) over the compulsory frame and map "
+                             "bytes measures re-reads of staged boxes"},
     }
     if args.copy_probe_mib > 0:
         cnt = args.copy_probe_mib * 262144
