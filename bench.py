@@ -427,11 +427,14 @@ def serving_stream_measure(count: int, rate_hz: float, seed: int, slots: int, bu
     ready = queue.Queue(maxsize=2)
     failure = []
 
+    gen_stream = torch.cuda.Stream()  # the producer's work waits for nothing of the server's
+
     def produce():
         try:
             for sp in specs:
-                inp = job_inputs(sp)
-                torch.cuda.synchronize()
+                with torch.cuda.stream(gen_stream):
+                    inp = job_inputs(sp)
+                gen_stream.synchronize()
                 ready.put(inp)
         except Exception as e:  # surfaced by the consumer
             failure.append(e)
