@@ -39,7 +39,8 @@ class Cancelled(FtkError):
 
 
 class _Options(C.Structure):
-    _fields_ = [("device", C.c_int), ("max_mb", C.c_double), ("launch_adds", C.c_double), ("pad", C.c_int)]
+    _fields_ = [("device", C.c_int), ("max_mb", C.c_double), ("launch_adds", C.c_double), ("pad", C.c_int),
+                ("tile_rows", C.c_int), ("queue", C.c_int)]
 
 
 class _Pass(C.Structure):

@@ -11,15 +11,18 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libftk_hip.so"
+OBJ = PKG / "_build"
 
-HIP_SOURCES = ["stack_kernel.hip", "probe_kernel.hip", "prep_kernels.hip", "search_kernels.hip", "search_device.hip",
-               "ftk_api.hip", "ftk_prep_api.hip", "ftk_search_api.hip", "store_kernels.hip", "serve.hip", "screen_kernels.hip",
+HIP_SOURCES = ["stack_inst_f16.hip", "stack_inst_bf16.hip", "stack_inst_f32.hip", "stack_kernel.hip",
+               "probe_kernel.hip", "prep_kernels.hip", "search_kernels.hip", "search_device.hip", "ftk_api.hip",
+               "ftk_prep_api.hip", "ftk_search_api.hip", "store_kernels.hip", "serve.hip", "screen_kernels.hip",
                "refine_kernels.hip", "stack_plan.cpp", "candidates.cpp"]
 
 HIPCC_FLAGS = [
@@ -27,7 +30,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -53,14 +55,24 @@ def _run(cmd: list[str], cwd: Path) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd[:3])} ... (exit {res.returncode})")
 
 
-def build_backend() -> Path:
-    srcs = []
-    for s in HIP_SOURCES:
+def build_backend(jobs: int | None = None) -> Path:
+    """Compiles every source to an object of its own (in parallel), then links the .so."""
+    OBJ.mkdir(exist_ok=True)
+    inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
+
+    def compile_one(s: str) -> Path:
         p = CSRC / s
+        o = OBJ / (p.name + ".o")
         # .cpp files are host code; hipcc compiles them as HIP too (-x hip) for one .so
-        srcs += (["-x", "hip", str(p)] if p.suffix == ".cpp" else [str(p)])
+        src = ["-x", "hip", str(p)] if p.suffix == ".cpp" else [str(p)]
+        _run([hipcc(), *HIPCC_FLAGS, *inc, "-c", *src, "-o", str(o)], CSRC)
+        return o
+
+    jobs = jobs or max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, HIP_SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-I", str(CSRC), *srcs, "-o", str(tmp)], CSRC)
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)], CSRC)
     os.replace(tmp, LIB)
     return LIB
 

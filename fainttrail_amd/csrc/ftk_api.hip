@@ -121,7 +121,18 @@ ftk_options default_options() {
     // cost ~20 % on the 6144 x 4096 x 60 x 19952 pass when each launch was waited for)
     o.launch_adds = 4e12;
     o.pad = 1;
+    o.tile_rows = kTileY;
+    o.queue = 1;
     return o;
+}
+
+// The plan's tile height from the options (0 = default).
+void set_tile_rows(StackPlan& p, const ftk_options& opt, int out_rows) {
+    const int rows = opt.tile_rows > 0 ? opt.tile_rows : kTileY;
+    if (!tile_rows_ok(rows)) throw std::runtime_error("hip backend option 'tile_rows' must be 8, 16 or 32");
+    if (!opt.queue && rows != kTileY) throw std::runtime_error("hip backend option 'tile_rows' other than 16 needs queue=1");
+    p.tile_rows = rows;
+    plan_tiles(p, out_rows);
 }
 
 // Plan tables on the device.
@@ -166,7 +177,10 @@ struct ftk_plan {
     DeviceBuffer<uint16_t> stored;
     DeviceBuffer<float> scales;
     DeviceBuffer<unsigned> maxbits;
-    // the edge tiles' launch runs on a stream of its own beside the interior launch
+    // persistent launch (queue = 1): its counters and descriptors
+    bool queue = true;
+    DeviceBuffer<int> queue_ws;
+    // otherwise the edge tiles' launch runs on a stream of its own beside the interior launch
     hipStream_t ring_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     ~ftk_plan() {
@@ -205,9 +219,14 @@ extern "C" int ftk_parse_options(const char* spec, ftk_options* out, char* err, 
             else if (key == "launch_adds") o.launch_adds = d;
             else if (key == "pad" && (d == 0.0 || d == 1.0)) o.pad = static_cast<int>(d);
             else if (key == "pad") throw std::runtime_error("hip backend option 'pad' must be 0 or 1");
-            else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds, pad)");
+            else if (key == "tile_rows" && tile_rows_ok(static_cast<int>(d)) && d == static_cast<int>(d)) o.tile_rows = static_cast<int>(d);
+            else if (key == "tile_rows") throw std::runtime_error("hip backend option 'tile_rows' must be 8, 16 or 32");
+            else if (key == "queue" && (d == 0.0 || d == 1.0)) o.queue = static_cast<int>(d);
+            else if (key == "queue") throw std::runtime_error("hip backend option 'queue' must be 0 or 1");
+            else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds, pad, tile_rows, queue)");
         }
         if (o.max_mb < 0 || o.launch_adds <= 0) throw std::runtime_error("hip backend options must be positive");
+        if (!o.queue && o.tile_rows != kTileY) throw std::runtime_error("hip backend option 'tile_rows' other than 16 needs queue=1");
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return FTK_ERROR;
@@ -253,7 +272,8 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             if (!pass->frames[i]) throw std::runtime_error("stack pass: missing frames");
         const ftk_options opt = opt_in ? *opt_in : default_options();
         StackPlan p = make_stack_plan(pass->n, pass->w, pass->h, pass->nv, pass->dx, pass->dy, pass->min_count);
-        const int n = p.n, w = p.w, h = p.h;
+        set_tile_rows(p, opt, p.h);
+        const int n = p.n, w = p.w, h = p.h, ty = p.tile_rows;
         const int device = resolve_device(opt.device);
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
@@ -277,6 +297,13 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
         dev.upload(p);
         DeviceBuffer<float> d_frames, d_max, d_stage;
         DeviceBuffer<int> d_best;
+        // persistent launches: a counter workspace per launch slot; launches cycle through
+        // 8 slots and at most 3 are in flight (LaunchQueue::settle(2) below)
+        constexpr int kSlots = 8;
+        constexpr size_t kSlotInts = kQueueWorkspaceBytes / sizeof(int);
+        DeviceBuffer<int> d_queue;
+        if (opt.queue) d_queue.reserve(kSlots * kSlotInts, "allocate queue counters");
+        int launch_no = 0;
         DeviceBuffer<unsigned long long> d_keys;
         double t_up = 0, t_kernel = 0, t_down = 0;
         double done = 0;  // fraction of the output rows stacked
@@ -339,14 +366,14 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             const double units_per_row = static_cast<double>(w) * n * p.nv;
             int rows_per_launch = static_cast<int>(std::clamp(opt.launch_adds / std::max(1.0, units_per_row), 1.0,
                                                               static_cast<double>(out_rows)));
-            rows_per_launch = std::max(kTileY, rows_per_launch / kTileY * kTileY);
+            rows_per_launch = std::max(ty, rows_per_launch / ty * ty);
             check(launch_fill_keys(d_keys.p, static_cast<long long>(np), nullptr), "clear keys");
             check(hipDeviceSynchronize(), "clear keys");
             LaunchQueue lq;
             for (int y = b.y0; y < b.y1; y += rows_per_launch) {
                 const int sub_rows = std::min(rows_per_launch, b.y1 - y);
                 // the launch's tiles read frame rows up to (its last tile row) + max_dy
-                upload_until(y + (sub_rows + kTileY - 1) / kTileY * kTileY + p.max_dy);
+                upload_until(y + (sub_rows + ty - 1) / ty * ty + p.max_dy);
                 plan_tiles(p, sub_rows);
                 const double units_per_block = static_cast<double>(sub_rows) * w * n * kVel;
                 const int blocks_per_launch = static_cast<int>(
@@ -381,6 +408,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                     L.seg0 = s0;
                     L.nseg = std::min(segs_per_launch, nseg_total - s0);
                     L.keys = d_keys.p + static_cast<size_t>(y - b.y0) * w;
+                    if (opt.queue) L.counters = d_queue.p + (launch_no++ % kSlots) * kSlotInts;
                     hipStream_t st = lq.next();
                     check(launch_stack(p, L, st), "launch stack kernel");
                     const int blocks_done = std::min(p.nb, (s0 + L.nseg) * bps) - std::min(p.nb, s0 * bps);
@@ -428,9 +456,12 @@ extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, c
         auto* pl = new ftk_plan();
         try {
             pl->plan = make_stack_plan(n, w, h, nv, dx, dy, min_count);
+            set_tile_rows(pl->plan, opt, h);
             pl->device = resolve_device(opt.device);
             check(hipSetDevice(pl->device), "hipSetDevice");
             pl->dev.upload(pl->plan);
+            pl->queue = opt.queue != 0;
+            if (pl->queue) pl->queue_ws.reserve(kQueueWorkspaceBytes / sizeof(int), "allocate queue counters");
             pl->keys.reserve(static_cast<size_t>(w) * h, "allocate keys");
             const StackPlan& p = pl->plan;
             pl->pad = pad_geometry(p, 0, h, 0, p.tiles_x, p.tiles_y);
@@ -482,8 +513,17 @@ StackLaunch plan_launch(ftk_plan* pl) {
     return L;
 }
 
-void run_launch(ftk_plan* pl, const StackLaunch& L, float* d_max_snr, int32_t* d_best_v, hipStream_t s) {
+void run_launch(ftk_plan* pl, const StackLaunch& L_in, float* d_max_snr, int32_t* d_best_v, hipStream_t s) {
     const long long np = static_cast<long long>(pl->plan.w) * pl->plan.h;
+    if (pl->queue) {
+        StackLaunch L = L_in;
+        L.counters = pl->queue_ws.p;
+        check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
+        check(launch_stack(pl->plan, L, s), "launch stack kernel");
+        check(launch_finalize(pl->keys.p, np, d_max_snr, d_best_v, s), "finalize");
+        return;
+    }
+    const StackLaunch& L = L_in;
     if (!pl->ring_stream) {
         check(hipStreamCreateWithFlags(&pl->ring_stream, hipStreamNonBlocking), "create stream");
         check(hipEventCreateWithFlags(&pl->ev_in, hipEventDisableTiming), "create event");
@@ -542,6 +582,9 @@ extern "C" int ftk_plan_store(ftk_plan* pl, const float* d_frames, size_t frame_
         if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_store: frame stride smaller than a frame");
         check(hipSetDevice(pl->device), "hipSetDevice");
         hipStream_t s = static_cast<hipStream_t>(stream);
+        if (!tile_rows_fit(p, storage, p.tile_rows))
+            throw std::runtime_error("ftk_plan_store: 16-bit frames cannot be staged in tiles of " +
+                                     std::to_string(p.tile_rows) + " rows for this plan's vertical spread (use tile_rows=16)");
         if (!pl->stored.p) {
             // 16-bit frames are always padded (their fetch reads whole pixel pairs)
             pl->pad16 = pad_geometry16(pad_geometry(p, 0, p.h, 0, p.tiles_x, p.tiles_y));

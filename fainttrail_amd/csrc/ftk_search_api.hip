@@ -7,6 +7,7 @@
 
 #include "../../include/ftk_hip.h"
 #include "search_device.hpp"
+#include "stack_plan.hpp"
 #include "serve.hpp"
 
 struct ftk_searcher {
@@ -54,6 +55,8 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->reg_dy = nullptr;
     p->candidates = c.candidates;
     p->half_tolerance = c.half_tolerance;
+    p->tile_rows = 0;
+    p->queue = c.queue;
 }
 
 namespace {
@@ -84,6 +87,8 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.max_frame_fraction = p->max_frame_fraction;
     c.candidates = p->candidates;
     c.half_tolerance = p->half_tolerance;
+    c.tile_rows = p->tile_rows;
+    c.queue = p->queue;
     return c;
 }
 
@@ -116,6 +121,8 @@ void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
     sum->n_screened = o.screened;
     sum->n_artefacts = o.artefacts;
     sum->n_candidates = static_cast<int>(o.candidates.size());
+    sum->tile_rows = o.tile_rows;
+    sum->queue = o.queue;
     for (const auto& t : o.timings) {
         if (t.first == "prepare") sum->t_prepare += t.second;
         else if (t.first == "stack (real)") sum->t_stack_real += t.second;
@@ -137,6 +144,9 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
         if (mjd[i] < mjd[i - 1]) throw std::runtime_error("ftk_searcher_run: frames must be sorted by time");
     if (p->passes_per_launch < 0 || p->segments < 0)
         throw std::runtime_error("ftk_searcher_run: passes_per_launch and segments must be >= 0");
+    if (p->tile_rows != 0 && !ftk::tile_rows_ok(p->tile_rows))
+        throw std::runtime_error("ftk_searcher_run: tile_rows must be 0, 8, 16 or 32");
+    if (p->queue != 0 && p->queue != 1) throw std::runtime_error("ftk_searcher_run: queue must be 0 or 1");
     if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
         throw std::runtime_error("ftk_searcher_run: unknown frame storage");
     if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
@@ -193,7 +203,7 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
 
 extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
                                  const ftk_search_params* p, int reps, int* passes_per_launch, int* segments,
-                                 double* seconds, void* stream, char* err, size_t err_len) {
+                                 int* tile_rows, double* seconds, void* stream, char* err, size_t err_len) {
     try {
         check_run_args(s, d_raw, stride, mjd, p);
         hipSetDevice(s->device);
@@ -203,6 +213,7 @@ extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t str
                                      static_cast<hipStream_t>(stream));
         if (passes_per_launch) *passes_per_launch = s->ds->tuned_passes;
         if (segments) *segments = s->ds->tuned_segments;
+        if (tile_rows) *tile_rows = s->ds->tuned_tile_rows;
         if (seconds) *seconds = t;
         return FTK_OK;
     } catch (const std::exception& e) {

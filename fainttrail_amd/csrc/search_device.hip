@@ -287,7 +287,7 @@ void PassSet::release() {
     for (void* p : {static_cast<void*>(filt), static_cast<void*>(keys), static_cast<void*>(maxm),
                     static_cast<void*>(best), static_cast<void*>(tabs), static_cast<void*>(perm),
                     static_cast<void*>(counts), static_cast<void*>(pk), static_cast<void*>(scl),
-                    static_cast<void*>(maxb)})
+                    static_cast<void*>(maxb), static_cast<void*>(queue_ws)})
         if (p) (void)hipFree(p);
     if (h_counts) (void)hipHostFree(h_counts);
     *this = PassSet();
@@ -310,6 +310,7 @@ void PassSet::reserve(int cap_, long long frames_, long long frame_floats_, long
     pk = salloc<int>(c * 4 * pk_cap_, "allocate pass peaks");
     scl = salloc<float>(c * frames_, "allocate frame scales");
     maxb = salloc<unsigned>(c * frames_, "allocate frame scratch");
+    queue_ws = salloc<int>(64, "allocate queue counters");
     scheck(hipHostMalloc(reinterpret_cast<void**>(&h_counts), sizeof(int) * c, hipHostMallocDefault),
            "allocate pinned counts");
     cap = cap_;
@@ -325,7 +326,7 @@ double PassSet::device_bytes() const {
     if (!filt) return 0.0;
     const double c = cap;
     return c * frames * frame_floats * 4.0 + c * plane * (8.0 + 4.0 + 4.0) + c * tab_ints * 4.0 + c * frames * 4.0 +
-           c * 4.0 + c * 4.0 * pk_cap * 4.0 + c * frames * (4.0 + 4.0);
+           c * 4.0 + c * 4.0 * pk_cap * 4.0 + c * frames * (4.0 + 4.0) + 64.0 * 4.0;
 }
 
 double DeviceSearch::device_bytes() const {
@@ -679,6 +680,11 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     const int nms = std::max(2, static_cast<int>(std::lround(cfg.fwhm_px)));
     const int edge = std::max(cfg.edge_px, 1);
     out.passes_per_launch = 1;
+    // tile height: asked, tuned, else 16 rows (a height the 16-bit spread does not fit
+    // falls back to 16 in the plan)
+    const int tile_rows = cfg.tile_rows > 0 ? cfg.tile_rows : (tuned_tile_rows > 0 ? tuned_tile_rows : kTileY);
+    out.tile_rows = tile_rows;
+    out.queue = cfg.queue;
 
     std::vector<int32_t> dx, dy;
     struct PlanDel {
@@ -693,6 +699,8 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         opt.max_mb = 0;
         opt.launch_adds = 1e12;
         opt.pad = 1;
+        opt.tile_rows = tile_rows;
+        opt.queue = cfg.queue;
         ftk_plan* plan = nullptr;
         char err[512] = {0};
         if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
@@ -784,6 +792,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     std::vector<int32_t> dx, dy;
     shifts_for(out.grid, out.t_h, n, dx, dy);
     StackPlan plan = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
+    // tile height: asked, tuned, else 16 rows; the one-workgroup-per-item launches take
+    // 16-row tiles only, and 16-bit frames need room for the plan's spread
+    int tile_rows = cfg.tile_rows > 0 ? cfg.tile_rows : (tuned_tile_rows > 0 ? tuned_tile_rows : kTileY);
+    if (!cfg.queue || !tile_rows_fit(plan, cfg.storage, tile_rows)) tile_rows = kTileY;
+    if (tile_rows != plan.tile_rows) {
+        plan.tile_rows = tile_rows;
+        plan_tiles(plan, h);
+    }
+    out.tile_rows = tile_rows;
+    out.queue = cfg.queue;
     const int segments = cfg.segments > 0 ? cfg.segments : tuned_segments;
     set_segments(plan, segments);
     out.segments = plan.nseg;
@@ -1068,12 +1086,18 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             hk[kind] = hargs[q].data() + kind * G;
             dk[kind] = dargs + kind * G;
         }
-        // ring tiles on their own stream, beside the interior launch
-        scheck(hipEventRecord(ev_ring_in_[q], s), "record event");
-        scheck(hipStreamWaitEvent(ring_stream_, ev_ring_in_[q], 0), "stream wait");
-        scheck(launch_stack_multi(hk, dk, cnt, s, ring_stream_), "launch stack kernel");
-        scheck(hipEventRecord(ev_ring_out_[q], ring_stream_), "record event");
-        scheck(hipStreamWaitEvent(s, ev_ring_out_[q], 0), "stream wait");
+        if (cfg.queue) {
+            // the group in one persistent launch: every kind of every pass, work items
+            // from the set's counters
+            scheck(launch_stack_queue(hargs[q].data(), dargs, cnt, G, S.queue_ws, s), "launch stack kernel");
+        } else {
+            // ring tiles on their own stream, beside the interior launch
+            scheck(hipEventRecord(ev_ring_in_[q], s), "record event");
+            scheck(hipStreamWaitEvent(ring_stream_, ev_ring_in_[q], 0), "stream wait");
+            scheck(launch_stack_multi(hk, dk, cnt, s, ring_stream_), "launch stack kernel");
+            scheck(hipEventRecord(ev_ring_out_[q], ring_stream_), "record event");
+            scheck(hipStreamWaitEvent(s, ev_ring_out_[q], 0), "stream wait");
+        }
         scheck(launch_finalize(S.keys, cnt * plane, S.maxm, S.best, s), "finalize");
         for (int g = 0; g < cnt; ++g) {
             const int p = p0 + g;
@@ -1331,8 +1355,10 @@ double DeviceSearch::tune(const float* raw, long long stride, const double* mjd,
     gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
     cfg.passes_per_launch = passes;
     cfg.segments = 0;
+    cfg.tile_rows = kTileY;
     tuned_passes = 0;
     tuned_segments = 0;
+    tuned_tile_rows = 0;
     const SearchOutput probe = run(raw, stride, mjd, cfg, nullptr, nullptr, s);
     const int base = std::max(1, probe.segments);
     std::vector<int> ss{base};
@@ -1359,8 +1385,30 @@ double DeviceSearch::tune(const float* raw, long long stride, const double* mjd,
                 bs = used;
             }
         }
+    // then the tile height at the best group size and segment count (the persistent
+    // launch takes every height; the one-workgroup-per-item launches 16 rows only)
+    int bt = kTileY;
+    if (cfg.queue) {
+        cfg.passes_per_launch = bg;
+        cfg.segments = bs;
+        for (int tr : {8, 32}) {
+            cfg.tile_rows = tr;
+            double t = 1e300;
+            int used = tr;
+            for (int r = 0; r < std::max(1, reps); ++r) {
+                const double t0 = wall();
+                used = run(raw, stride, mjd, cfg, nullptr, nullptr, s).tile_rows;
+                t = std::min(t, wall() - t0);
+            }
+            if (used == tr && t < best) {
+                best = t;
+                bt = tr;
+            }
+        }
+    }
     tuned_passes = bg;
     tuned_segments = bs;
+    tuned_tile_rows = bt;
     return best;
 }
 

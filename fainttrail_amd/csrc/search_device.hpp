@@ -44,6 +44,11 @@ struct SearchConfig {
     // choice). Neither changes any result bit.
     int passes_per_launch = 1;
     int segments = 0;
+    // Stacking work shape: output rows per tile (8, 16 or 32; 0 = tuned, else 16) and
+    // the persistent launch (1: one launch per group whose workgroups take work items
+    // from device counters; 0: one workgroup per work item, 16-row tiles only).
+    int tile_rows = 0;
+    int queue = 1;
     // Frame storage of the stacking passes: 0 fp32 (the exact form), 1 fp16, 2 bf16
     // (a power-of-two scale per frame, fp32 accumulation).
     int storage = 0;
@@ -97,6 +102,7 @@ struct SearchOutput {
     std::vector<SearchCandidate> candidates;  // sorted as the reference sorts them, ids from 1
     std::vector<std::pair<std::string, double>> timings;
     int passes_per_launch = 1, segments = 0;  // what the run used
+    int tile_rows = 16, queue = 0;
 };
 
 // Device buffers of one group of stacking passes (the batched form).
@@ -115,6 +121,7 @@ struct PassSet {
     float* scl = nullptr;          // cap x frames: scales of 16-bit frames
     unsigned* maxb = nullptr;      // cap x frames: scratch of the 16-bit encoder
     int* h_counts = nullptr;       // pinned
+    int* queue_ws = nullptr;       // the persistent launch's 8 counters
     void release();
     void reserve(int cap, long long frames, long long frame_floats, long long plane, long long tab_ints,
                  long long pk_cap);
@@ -135,7 +142,7 @@ public:
                      int* d_best_v, hipStream_t s);
 
     int n, w, h;
-    int tuned_passes = 0, tuned_segments = 0;  // from tune(); 0 = none
+    int tuned_passes = 0, tuned_segments = 0, tuned_tile_rows = 0;  // from tune(); 0 = none
 
     // The threshold of a run's pooled null peaks fitted again with the FP64 tail-fit
     // sums on the host (on_device false) or the GPU; method receives the method index.

@@ -1,40 +1,16 @@
-// gfx950 shift-and-stack kernel with per-pixel max S/N and argmax.
-//
-// Contract (include/fainttrail/backend.hpp:10-40): for output pixel (x, y) and trial
-// motion j, S_j = (sum over covering frames i, ascending, of g_i(x+dx_ji, y+dy_ji))
-// * float(1/sqrt(c)), summed in fp32 from +0.0f; undefined when c < min_count;
-// max over j with ties to the lowest j. Bit-identical to the CPU backend
-// (src/backend_cpu.cpp:77-204) because
-//   * every (pixel, motion) sum adds the covered samples in ascending frame order,
-//     and an uncovered frame adds a zero: a sum that starts at +0.0f is never -0.0f
-//     (IEEE round-to-nearest gives +0 for exact cancellation), and x + (+-0) = x for
-//     every x that is not -0, so the zero is an exact identity;
-//   * the build uses -ffp-contract=off, so no fused multiply-add appears;
-//   * c and 1/sqrt(c) come from the same integer count and the same host table;
-//   * the maximum keeps (value, lowest index), merged across workgroups with a
-//     64-bit atomic max of (ordered value bits, inverted index).
-//
-// One workgroup = one tile (64 x 16 pixels; each wavefront owns 4 rows of 64) and a
-// segment of velocity blocks. Per block and frame, the box the block's shifted tiles
-// can reach is staged in LDS (double buffered, one barrier per frame) and each lane
-// reads its 16 motions x 4 rows from it. No MFMA: this is a gather-sum.
-//
-// Two instantiations: tiles inside the plan's union shift box (every read of every
-// block lies in the frame rows present) skip all bounds and coverage work (c = n); the
-// ring of remaining tiles tests each block, and for blocks that reach outside the
-// frame zero-fills the box, loads only in-frame pixels and counts coverage.
-//
-// Lattice blocks (4 x 4 motions with dx = X[a], dy = Y[b] in every frame, slot 4 a + b)
-// have an interior instantiation of their own: per frame a lane reads each of the four
-// shifted columns once, 4 + the frame's vertical spread rows (at most 8), and adds
-// rows c_b .. c_b + 3 of column a to motion (a, b) for the frame's row offset c_b of
-// row class b, the adds' source registers offset by c_b through GPR indexing (no
-// branch). That is at most 32 LDS dwords per lane and frame for 64 adds instead of 64.
+// Launchers of the gfx950 stacking kernels (stack_body.hpp; instantiated per frame
+// storage in stack_inst_f32.hip, stack_inst_f16.hip and stack_inst_bf16.hip), their
+// descriptors and the small kernels around them (key clearing, finalisation, row
+// copies).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
-#include <type_traits>
+#include <mutex>
+#include <utility>
+#include <vector>
 
+#include "stack_inst.hpp"
 #include "stack_launch.hpp"
 #include "stack_plan.hpp"
 
@@ -42,561 +18,27 @@ namespace ftk {
 
 namespace {
 
-constexpr int V = kVel;
-constexpr int RPW = kRowsPerWave;
-constexpr int kPreRows = kLdsRows / kWaves;  // staged rows per wavefront, at most (16 + 32 rows)
-
-// Plan tables are read at wave-uniform addresses; through the constant address space
-// the compiler issues scalar loads (s_load), which do not wait on the vector memory
-// counter that the in-flight frame prefetch holds.
-using cint = const __attribute__((address_space(4))) int;
-
-// Word 3 of a raw (untyped, stride 0) buffer descriptor on gfx9-family parts: 32-bit
-// data format, as AMD's own gfx90a/gfx94x/gfx950 kernels set it.
-constexpr int kBufferWord3 = 0x00020000;
-__device__ inline cint* as_const(const int* p) { return (cint*)(p); }
-
-__device__ inline uint32_t d_ordered(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+__global__ void zero_counters_kernel(int* __restrict__ c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0;
 }
+
+// The three descriptors of a one-pass persistent launch, from the kernel arguments
+// into device memory (ordered on the launch's stream, no host synchronisation).
+struct ArgsTriple {
+    StackArgs a[kArgsKinds];
+};
+constexpr int kTripleWords = static_cast<int>(sizeof(ArgsTriple) / 4);
+static_assert(sizeof(ArgsTriple) % 4 == 0, "descriptor words");
+__global__ void put_args_kernel(ArgsTriple t, uint32_t* __restrict__ dst) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&t);
+    for (int i = threadIdx.x; i < kTripleWords; i += blockDim.x) dst[i] = src[i];
+}
+
 __device__ inline float d_unordered(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
-__device__ inline void tile_xy(const TileRect& R, bool ring, int t, int& tx, int& ty) {
-    if (!ring) {
-        const int wdt = R.tx_hi - R.tx_lo;
-        tx = R.tx_lo + t % wdt;
-        ty = R.ty_lo + t / wdt;
-        return;
-    }
-    const int top = R.ty_lo * R.tiles_x;
-    const int per = R.tx_lo + (R.tiles_x - R.tx_hi);
-    const int mid = (R.ty_hi - R.ty_lo) * per;
-    if (t < top) {
-        ty = t / R.tiles_x;
-        tx = t % R.tiles_x;
-    } else if (t < top + mid) {
-        const int u = t - top;
-        ty = R.ty_lo + u / per;
-        const int c = u % per;
-        tx = c < R.tx_lo ? c : R.tx_hi + (c - R.tx_lo);
-    } else {
-        const int u = t - top - mid;
-        ty = R.ty_hi + u / R.tiles_x;
-        tx = u % R.tiles_x;
-    }
-}
-
 }  // namespace
-
-// The body of one workgroup of a stacking pass: workgroup `wg` of the pass described
-// by `a` (the kernels below pick the pass and the workgroup index).
-// acc + q s for the 16-bit value q in the low or high half of w and a power-of-two
-// scale s: the product is exact, so the single rounding of the fused operation equals
-// adding the decoded value q s. fp16: one v_fma_mix_f32 (widening inside the FMA);
-// bf16: the value's fp32 bits are its 16 bits shifted up.
-template <int ST>
-__device__ __forceinline__ float fma_q16(uint32_t w, bool hi, float s, float acc) {
-    if constexpr (ST == kStoreF16) {
-        if (hi)
-            asm("v_fma_mix_f32 %0, %1, %2, %0 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(w), "s"(s));
-        else
-            asm("v_fma_mix_f32 %0, %1, %2, %0 op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(w), "s"(s));
-        return acc;
-    } else {
-        return __builtin_fmaf(__uint_as_float(hi ? (w & 0xffff0000u) : (w << 16)), s, acc);
-    }
-}
-
-// Row class B of a lattice frame: motion (a, B) adds rows c .. c + 3 of column a. The
-// column values sit in v40 .. v71 (column a, row q in v(40 + 8 a + q)); s_set_gpr_idx_on
-// adds c to the source register number of the adds that follow, so one instruction
-// sequence serves every row offset without a branch.
-template <int B>
-__device__ __forceinline__ void lat_add_idx(float (&acc)[kVel][kRowsPerWave], const float (&col)[kLatSide][8], int c) {
-    asm volatile(
-        "s_set_gpr_idx_on %16, gpr_idx(SRC0)\n\t"
-        "s_nop 0\n\t"
-        "v_add_f32 %0, v40, %0\n\t"
-        "v_add_f32 %1, v41, %1\n\t"
-        "v_add_f32 %2, v42, %2\n\t"
-        "v_add_f32 %3, v43, %3\n\t"
-        "v_add_f32 %4, v48, %4\n\t"
-        "v_add_f32 %5, v49, %5\n\t"
-        "v_add_f32 %6, v50, %6\n\t"
-        "v_add_f32 %7, v51, %7\n\t"
-        "v_add_f32 %8, v56, %8\n\t"
-        "v_add_f32 %9, v57, %9\n\t"
-        "v_add_f32 %10, v58, %10\n\t"
-        "v_add_f32 %11, v59, %11\n\t"
-        "v_add_f32 %12, v64, %12\n\t"
-        "v_add_f32 %13, v65, %13\n\t"
-        "v_add_f32 %14, v66, %14\n\t"
-        "v_add_f32 %15, v67, %15\n\t"
-        "s_set_gpr_idx_off"
-        : "+v"(acc[0+B][0]), "+v"(acc[0+B][1]), "+v"(acc[0+B][2]), "+v"(acc[0+B][3]), "+v"(acc[4+B][0]), "+v"(acc[4+B][1]), "+v"(acc[4+B][2]), "+v"(acc[4+B][3]), "+v"(acc[8+B][0]), "+v"(acc[8+B][1]), "+v"(acc[8+B][2]), "+v"(acc[8+B][3]), "+v"(acc[12+B][0]), "+v"(acc[12+B][1]), "+v"(acc[12+B][2]), "+v"(acc[12+B][3])
-        : "s"(c), "{v40}"(col[0][0]), "{v41}"(col[0][1]), "{v42}"(col[0][2]), "{v43}"(col[0][3]), "{v44}"(col[0][4]), "{v45}"(col[0][5]), "{v46}"(col[0][6]), "{v47}"(col[0][7]), "{v48}"(col[1][0]), "{v49}"(col[1][1]), "{v50}"(col[1][2]), "{v51}"(col[1][3]), "{v52}"(col[1][4]), "{v53}"(col[1][5]), "{v54}"(col[1][6]), "{v55}"(col[1][7]), "{v56}"(col[2][0]), "{v57}"(col[2][1]), "{v58}"(col[2][2]), "{v59}"(col[2][3]), "{v60}"(col[2][4]), "{v61}"(col[2][5]), "{v62}"(col[2][6]), "{v63}"(col[2][7]), "{v64}"(col[3][0]), "{v65}"(col[3][1]), "{v66}"(col[3][2]), "{v67}"(col[3][3]), "{v68}"(col[3][4]), "{v69}"(col[3][5]), "{v70}"(col[3][6]), "{v71}"(col[3][7])
-        : "m0");
-}
-
-// PRE: for 16-bit frames, box rows staged per wavefront (the kernel is instantiated
-// for the few values the planner picks from its largest vertical spread); 0 for fp32.
-template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
-__device__ __forceinline__ void stack_body(const StackArgs& a, unsigned wg) {
-    static_assert(!LAT || (!RING && ST == kStoreF32), "lattice blocks: fp32 interior tiles");
-    static_assert(ST == kStoreF32 || (ST == kStoreF16 || ST == kStoreBF16), "frame storage");
-    static_assert(ST == kStoreF32 || !RING || PADDED, "16-bit frames are always padded");
-    using elem_t = typename std::conditional<ST == kStoreF32, float, uint16_t>::type;
-    const elem_t* __restrict__ frames = static_cast<const elem_t*>(a.frames);
-    const float* __restrict__ scale = a.scale;
-    const long long frame_stride = a.frame_stride;
-    const int row0 = a.row0, row1 = a.row1, n = a.n, w = a.w, h = a.h, pitch = a.pitch;
-    const int* __restrict__ shift = a.shift;
-    const int* __restrict__ box = a.box;
-    const int* __restrict__ ubox = a.ubox;
-    const int* __restrict__ lds_off = a.lds_off;
-    const int* __restrict__ motion = a.motion;
-    const float* __restrict__ inv_sqrt = a.inv_sqrt;
-    const int min_count = a.min_count, nb = a.nb, bps = a.bps;
-    const TileRect rect = a.rect;
-    const int y_begin = a.y_begin, y_end = a.y_end, seg0 = a.seg0, nseg = a.nseg, ntiles = a.ntiles;
-    const int lds_rows = a.lds_rows, cov_ok = a.cov_ok;
-    const long long* __restrict__ cov_off = a.cov_off;
-    const int* __restrict__ cov_x = a.cov_x;
-    const int* __restrict__ cov_y = a.cov_y;
-    const int* __restrict__ chain = a.chain;
-    const int* __restrict__ lat = a.lat;
-    unsigned long long* __restrict__ keys = a.keys;
-    // two frame boxes of lds_rows x kPitch floats (lds_rows = 16 + the plan's largest
-    // vertical spread), sized at launch
-    extern __shared__ float lds_dyn[];
-    float* const lds0 = lds_dyn;
-    float* const lds1 = lds_dyn + lds_rows * kPitch;
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    // Workgroups go to the 8 XCDs round-robin (workgroup b to XCD b mod 8). The k-th
-    // workgroup an XCD receives works on tile 8 (k / nseg) + XCD, segment k mod nseg:
-    // the segments of a tile run together on one XCD (sharing its L2 copy of the
-    // tile's frame boxes) and the eight XCDs work on neighbouring tiles (sharing frame
-    // rows in the MALL). The grid is padded to whole groups of 8 tiles.
-    const unsigned b = wg, xcd = b & 7u, kq = b >> 3;
-    const int tile = static_cast<int>(8u * (kq / static_cast<unsigned>(nseg)) + xcd);
-    const int seg = seg0 + static_cast<int>(kq % static_cast<unsigned>(nseg));
-    if (tile >= ntiles) return;
-    int tx, ty;
-    tile_xy(rect, RING, tile, tx, ty);
-    const int x0 = tx * kTileX;
-    const int y0 = y_begin + ty * kTileY;
-    const int x = x0 + lane;
-    const int yw = y0 + wave * RPW;
-    const float inv_n = inv_sqrt[n];
-    cint* const c_box = as_const(box);
-    cint* const c_ubox = as_const(ubox);
-    cint* const c_off = as_const(lds_off);
-    cint* const c_motion = as_const(motion);
-    cint* const c_shift = as_const(shift);
-    cint* const c_covy = as_const(cov_y);
-    cint* const c_chain = as_const(chain);
-    cint* const c_lat = as_const(lat);
-    const __attribute__((address_space(4))) long long* const c_covoff =
-        (const __attribute__((address_space(4))) long long*)(cov_off);
-    const int lane_base = wave * RPW * kPitch + lane;
-
-    float best[RPW];
-    int arg[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        best[r] = -__builtin_inff();
-        arg[r] = -1;
-    }
-
-    const int blk_begin = max(seg * bps, a.blk_lo);
-    const int blk_end = min(min(nb, a.blk_hi), seg * bps + bps);
-    for (int blk = blk_begin; blk < blk_end; ++blk) {
-        bool interior = true;
-        if (RING) {
-            cint* ub = c_ubox + 4 * blk;
-            interior = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w && y0 + ub[1] >= row0 &&
-                       y0 + kTileY - 1 + ub[3] < row1;
-        }
-
-        float acc[V][RPW];
-#pragma unroll
-        for (int k = 0; k < V; ++k)
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) acc[k][r] = 0.0f;
-
-        // ---- staging of frame i's box into LDS buffer `dst` with gfx950's direct
-        // global -> LDS loads, one 64-float row chunk per wavefront instruction (lane l
-        // writes the chunk's l-th float). In a padded frame buffer (a zero border wide
-        // enough for every block's box) every box is loaded whole. Otherwise a block that
-        // is not interior to this tile zeroes its box first and then loads only the lanes
-        // whose pixel lies in the frame rows present and in [0, w). Either way the zeros
-        // stand for frames that do not cover.
-        auto stage = [&](int i, float* dst) {
-            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
-            const int sx = bx[2], R = kTileY + bx[3];
-            const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
-            if (RING && !PADDED && !interior) {
-#pragma unroll
-                for (int m = 0; m < kPreRows; ++m) {
-                    const int rr = wave + kWaves * m;
-                    if (rr >= R) break;
-                    dst[rr * kPitch + lane] = 0.0f;
-                    if (lane < kPitch - kTileX) dst[rr * kPitch + kTileX + lane] = 0.0f;
-                }
-                __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
-            }
-            const float* f = static_cast<const float*>(a.frames) + static_cast<long long>(i) * frame_stride;
-            if (RING && !PADDED && !interior) {
-#pragma unroll
-                for (int m = 0; m < kPreRows; ++m) {
-                    const int rr = wave + kWaves * m;
-                    if (rr >= R) break;
-                    const int gy = by0 + rr;
-                    const float* src = f + static_cast<long long>(gy - row0) * pitch + bx0;
-                    if (gy < row0 || gy >= row1) continue;
-                    const int gx = bx0 + lane;
-                    if (gx >= 0 && gx < w) __builtin_amdgcn_global_load_lds(src + lane, dst + rr * kPitch, 4, 0, 0);
-                    if (lane < sx && gx + kTileX >= 0 && gx + kTileX < w)
-                        __builtin_amdgcn_global_load_lds(src + kTileX + lane, dst + rr * kPitch + kTileX, 4, 0, 0);
-                }
-            } else {
-                // uniform row pointers with a per-lane offset (the scalar-base load form);
-                // for the box's columns past 64, lanes beyond the box width re-read its
-                // last column into LDS columns that no motion reads, so every lane stays
-                // inside the box and no exec mask is needed (a box exactly 64 wide re-reads
-                // column 63 there: the second load needs no branch)
-                // buffer form of the LDS load: descriptor at the box's first pixel, the
-                // lane's byte offset in a VGPR (fixed for the frame), the row's byte
-                // offset as a scalar, so a row costs no vector instruction
-                const int b1 = 4 * lane;
-                const int b2 = 4 * (kTileX + min(lane, sx - 1));
-                const float* boxp = f + static_cast<long long>(by0 - row0) * pitch + bx0;
-                const __amdgpu_buffer_rsrc_t rs =
-                    __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(boxp), 0, 0x7fffffff, kBufferWord3);
-                if constexpr (RING) {
-                    // a rolled loop with running offsets: the edge kernel has no scalar
-                    // registers to spare for the unrolled loop's per-row offsets
-                    int row_b = wave * 4 * pitch;
-                    float* d = dst + wave * kPitch;
-#pragma unroll 1
-                    for (int rr = wave; rr < R; rr += kWaves) {
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, b1, row_b, 0, 0);
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d + kTileX, 4, b2, row_b, 0, 0);
-                        row_b += kWaves * 4 * pitch;
-                        d += kWaves * kPitch;
-                    }
-                } else {
-                    // a lattice box has at most 16 + kLatSyMax rows: 5 per wavefront
-                    constexpr int kStageRows = LAT ? (kTileY + kLatSyMax + kWaves - 1) / kWaves : kPreRows;
-#pragma unroll
-                    for (int m = 0; m < kStageRows; ++m) {
-                        const int rr = wave + kWaves * m;
-                        if (rr >= R) break;
-                        const int row_b = rr * 4 * pitch;
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
-                    }
-                }
-            }
-        };
-        auto consume = [&](const float* L, int i) {
-            if constexpr (LAT) {
-                // column a (dx = X[a]) of rows 0 .. 3 + spread, read once; motion (a, b) adds
-                // rows c_b .. c_b + 3 of it
-                cint* lt = c_lat + 2 * (static_cast<long long>(blk) * n + i);
-                const int wx = lt[0], wy = lt[1];
-                constexpr int kRows = 2 * ((kLatSyMax + 1) / 2) + 4;  // row pairs
-                // rows past 3 are read only when the frame's spread needs them; the
-                // unread ones are never added (the plan keeps c_b <= 2 e), so they stay
-                // unset instead of costing a move each
-                float col[kLatSide][kRows];
-                const float* q[kLatSide];
-#pragma unroll
-                for (int c = 0; c < kLatSide; ++c) {
-                    q[c] = L + lane_base + ((wx >> (8 * c)) & 0xff);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) col[c][r] = q[c][r * kPitch];
-                }
-                const int extra = (wy >> 16) & 3;  // row pairs past the first four rows
-                if (extra >= 1) {
-#pragma unroll
-                    for (int c = 0; c < kLatSide; ++c) {
-                        col[c][4] = q[c][4 * kPitch];
-                        col[c][5] = q[c][5 * kPitch];
-                    }
-                }
-                if (extra >= 2) {
-#pragma unroll
-                    for (int c = 0; c < kLatSide; ++c) {
-                        col[c][6] = q[c][6 * kPitch];
-                        col[c][7] = q[c][7 * kPitch];
-                    }
-                }
-                lat_add_idx<0>(acc, col, wy & 0xf);
-                lat_add_idx<1>(acc, col, (wy >> 4) & 0xf);
-                lat_add_idx<2>(acc, col, (wy >> 8) & 0xf);
-                lat_add_idx<3>(acc, col, (wy >> 12) & 0xf);
-            } else {
-                cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
-#pragma unroll
-                for (int k = 0; k < V; ++k) {
-                    const float* p = L + lane_base + lo[k];
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) acc[k][r] += p[r * kPitch];
-                }
-            }
-        };
-
-        if constexpr (ST == kStoreF32) {
-            stage(0, lds0);
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            for (int i = 0; i < n; i += 2) {
-                if (i + 1 < n) stage(i + 1, lds1);
-                consume(lds0, i);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-                if (i + 1 >= n) break;
-                if (i + 2 < n) stage(i + 2, lds0);
-                consume(lds1, i + 1);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-            }
-        } else {
-            // 16-bit frames, kept 16-bit in LDS as row pairs: LDS word (r, c) holds the
-            // values of box rows r (low half) and r + 1 (high half) at box column c. Each
-            // buffer starts one word into the LDS area, so a pair fetched from one column
-            // left of the box (odd start) writes its first word to the word before the
-            // buffer (the area's first word, or the previous buffer's last word, which no
-            // motion reads). A lane's four rows of a motion are then two words, read
-            // with one ds_read2st64, half the LDS traffic of fp32 rows; each value is
-            // widened and multiplied by the frame's power-of-two scale inside a fused
-            // multiply-add whose product is exact (q times a power of two), so every sum
-            // is the fp32 sum of the decoded frames (q * s), added in frame order.
-            //
-            // Staging: wave w fetches box rows w PRE .. w PRE + PRE (clamped to the box)
-            // as dwords of two pixels from the even column at or left of the box start,
-            // while the current frame is consumed, then packs the pairs of rows into the
-            // word rows w PRE .. w PRE + PRE - 1 of the other buffer (word rows past the
-            // box are written but never read; the buffer has at least 4 PRE rows).
-            uint32_t* const lw0 = reinterpret_cast<uint32_t*>(lds0) + 1;
-            uint32_t* const lw1 = reinterpret_cast<uint32_t*>(lds1) + 1;
-            uint32_t pre[PRE + 1];
-            int pre_col = 0;
-            float sc_next = 1.0f;
-            auto fetch = [&](int i) {
-                cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
-                const int sx = bx[2], R = kTileY + bx[3];
-                const int bxs = x0 + bx[0];
-                const int off = bxs & 1;  // 1: the pair starts one column left of the box
-                const int lmax = (kTileX - 1 + sx + off) >> 1;
-                const int l = min(lane, lmax);  // lanes past the box repeat its last pair
-                const elem_t* rowp =
-                    frames + static_cast<long long>(i) * frame_stride + static_cast<long long>(y0 + bx[1] - row0) * pitch + (bxs - off);
-                const __amdgpu_buffer_rsrc_t rs =
-                    __builtin_amdgcn_make_buffer_rsrc(const_cast<elem_t*>(rowp), 0, 0x7fffffff, kBufferWord3);
-                pre_col = 2 * l - off;  // word column of the pair's first pixel (-1 for the guard word)
-                sc_next = reinterpret_cast<const __attribute__((address_space(4))) float*>(
-                    (const __attribute__((address_space(4))) char*)(scale))[i];
-#pragma unroll
-                for (int m = 0; m <= PRE; ++m) {
-                    const int rr = min(wave * PRE + m, R - 1);
-                    pre[m] = __builtin_amdgcn_raw_buffer_load_b32(rs, 4 * l, rr * 2 * pitch, 0);
-                }
-            };
-            auto commit = [&](uint32_t* dst) {
-                uint32_t* d = dst + wave * PRE * kPitch + pre_col;
-#pragma unroll
-                for (int m = 0; m < PRE; ++m) {
-                    // v_perm_b32 picks the bytes: low halves of rows m, m + 1, then high halves
-                    d[m * kPitch] = __builtin_amdgcn_perm(pre[m + 1], pre[m], 0x05040100u);
-                    d[m * kPitch + 1] = __builtin_amdgcn_perm(pre[m + 1], pre[m], 0x07060302u);
-                }
-            };
-            // per-lane read bases of the two buffers, formed once (the empty asm keeps the
-            // compiler from splitting the guard-word offset back out of them, which would
-            // cost one more add per motion)
-            int lb = lane_base + 1;
-            asm volatile("" : "+v"(lb));
-            const uint32_t* const rb0 = reinterpret_cast<const uint32_t*>(lds0) + lb;
-            const uint32_t* const rb1 = reinterpret_cast<const uint32_t*>(lds1) + lb;
-            auto consume16 = [&](const uint32_t* L, int i, float sc) {
-                cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
-                // the words of motions k + 2, k + 3 are read while motions k, k + 1 are
-                // added, so a wavefront keeps four LDS reads in flight
-                uint32_t cur[2][2], nxt[2][2];
-#pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const uint32_t* p = L + lo[t];
-                    cur[t][0] = p[0];
-                    cur[t][1] = p[2 * kPitch];
-                }
-#pragma unroll
-                for (int k = 0; k < V; k += 2) {
-                    if (k + 2 < V) {
-#pragma unroll
-                        for (int t = 0; t < 2; ++t) {
-                            const uint32_t* p = L + lo[k + 2 + t];
-                            nxt[t][0] = p[0];
-                            nxt[t][1] = p[2 * kPitch];
-                        }
-                    }
-#pragma unroll
-                    for (int t = 0; t < 2; ++t) {
-                        acc[k + t][0] = fma_q16<ST>(cur[t][0], false, sc, acc[k + t][0]);
-                        acc[k + t][1] = fma_q16<ST>(cur[t][0], true, sc, acc[k + t][1]);
-                        acc[k + t][2] = fma_q16<ST>(cur[t][1], false, sc, acc[k + t][2]);
-                        acc[k + t][3] = fma_q16<ST>(cur[t][1], true, sc, acc[k + t][3]);
-                    }
-                    if (k + 2 < V) {
-#pragma unroll
-                        for (int t = 0; t < 2; ++t) {
-                            cur[t][0] = nxt[t][0];
-                            cur[t][1] = nxt[t][1];
-                        }
-                    }
-                }
-            };
-            fetch(0);
-            commit(lw0);
-            float sc = sc_next;
-            __syncthreads();
-            // (straight-line code: past the last frame the last one is staged again into
-            // the buffer that is not read any more)
-            for (int i = 0; i < n; i += 2) {
-                fetch(min(i + 1, n - 1));
-                consume16(rb0, i, sc);
-                commit(lw1);
-                sc = sc_next;
-                __syncthreads();
-                if (i + 1 >= n) break;
-                fetch(min(i + 2, n - 1));
-                consume16(rb1, i + 1, sc);
-                commit(lw0);
-                sc = sc_next;
-                __syncthreads();
-            }
-        }
-
-        // S/N of the block's motions (ascending index) and the running maximum
-        cint* mo = c_motion + V * blk;
-        const __attribute__((address_space(4))) float* const c_inv =
-            (const __attribute__((address_space(4))) float*)(inv_sqrt);
-        // coverage from the full-frame bounds. cov_x / cov_y hold, per block and motion,
-        // the number of frames whose shift is <= d for every d of the block's range, so a
-        // pixel's count of frames landing inside [0, w) is F(w-1-x) - F(-1-x). A block that
-        // stays inside in y needs only the x table (one count per lane, the same for the
-        // lane's rows); one that stays inside in x only the y table (one count per row,
-        // the same for the wave: scalar loads). A block leaving both ranges uses both
-        // tables when each motion's shifts form a chain or anti-chain (straight tracks
-        // do), else walks the frames.
-        int cover = 0;  // 0: c = n; 1: x table; 2: y table; 3: both tables or frame walk
-        if (RING && !interior) {
-            cint* ub = c_ubox + 4 * blk;
-            const bool x_full = x0 + ub[0] >= 0 && x0 + kTileX - 1 + ub[2] < w;
-            const bool y_full = y0 + ub[1] >= 0 && y0 + kTileY - 1 + ub[3] < h;
-            cover = (x_full && y_full) ? 0 : (cov_ok && y_full) ? 1 : (cov_ok && x_full) ? 2 : 3;
-        }
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            const int j = mo[k];
-            if (j < 0) continue;
-            int cnt[RPW];
-            float ix = inv_n;  // the factor of every row when the count is the same for them
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) cnt[r] = n;
-            if (RING && cover != 0) {
-                cint* ub = c_ubox + 4 * blk;
-                if (cover == 1) {
-                    const int L = ub[2] - ub[0] + 1;
-                    const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * L;
-                    auto cdf = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= L ? n : F[d]); };
-                    const int cx = cdf(w - 1 - x) - cdf(-1 - x);
-                    ix = inv_sqrt[cx];
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) cnt[r] = cx;
-                } else if (cover == 2) {
-                    const int L = ub[3] - ub[1] + 1;
-                    cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * L;
-                    auto cdf = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= L ? n : G[d]); };
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) cnt[r] = cdf(h - 1 - (yw + r)) - cdf(-1 - (yw + r));
-                } else if (cov_ok && c_chain[V * blk + k] != 0) {
-                    // both ranges cut: for a chain (anti-chain) of (dx, dy) pairs the frames
-                    // in the x range and those in the y range are two intervals of one
-                    // order (by dx, then dy ascending, or descending), so the count is
-                    // their overlap, from the two tables
-                    const int Lx = ub[2] - ub[0] + 1, Ly = ub[3] - ub[1] + 1;
-                    const int* F = cov_x + cov_off[2 * blk] + static_cast<long long>(k) * Lx;
-                    cint* G = c_covy + c_covoff[2 * blk + 1] + static_cast<long long>(k) * Ly;
-                    auto fx = [&](int q) { const int d = q - ub[0]; return d < 0 ? 0 : (d >= Lx ? n : F[d]); };
-                    auto gy = [&](int q) { const int d = q - ub[1]; return d < 0 ? 0 : (d >= Ly ? n : G[d]); };
-                    const int x_lo = fx(-1 - x), x_hi = fx(w - 1 - x);  // positions [x_lo, x_hi)
-                    const bool co = c_chain[V * blk + k] == 1;
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) {
-                        const int below = gy(-1 - (yw + r)), upto = gy(h - 1 - (yw + r));
-                        // y range positions: [below, upto) ascending, [n - upto, n - below) descending
-                        const int y_lo = co ? below : n - upto, y_hi = co ? upto : n - below;
-                        cnt[r] = max(0, min(x_hi, y_hi) - max(x_lo, y_lo));
-                    }
-                } else {
-#pragma unroll
-                    for (int r = 0; r < RPW; ++r) cnt[r] = 0;
-                    for (int i = 0; i < n; ++i) {
-                        cint* sh = c_shift + (static_cast<long long>(blk) * n + i) * (2 * V);
-                        const int sdx = sh[k], sdy = sh[V + k];
-                        const int xin = static_cast<unsigned>(x + sdx) < static_cast<unsigned>(w) ? 1 : 0;
-#pragma unroll
-                        for (int r = 0; r < RPW; ++r)
-                            cnt[r] += static_cast<unsigned>(yw + r + sdy) < static_cast<unsigned>(h) ? xin : 0;
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                if (cnt[r] < min_count) continue;
-                // x table or c = n: one factor; y table: the count is the wave's (scalar
-                // load); frame walk: a per-lane count
-                const float f = (!RING || cover <= 1) ? ix : cover == 2 ? c_inv[cnt[r]] : inv_sqrt[cnt[r]];
-                const float snr = acc[k][r] * f;
-                if (snr > best[r] || (snr == best[r] && j < arg[r])) {
-                    best[r] = snr;
-                    arg[r] = j;
-                }
-            }
-        }
-    }
-
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int y = yw + r;
-        if (arg[r] < 0 || x >= w || y >= y_end) continue;
-        const unsigned long long key =
-            (static_cast<unsigned long long>(d_ordered(best[r])) << 32) | static_cast<uint32_t>(~static_cast<uint32_t>(arg[r]));
-        atomicMax(&keys[static_cast<long long>(y - y_begin) * w + x], key);
-    }
-}
-
-template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
-__global__ __launch_bounds__(kThreads) void stack_kernel(StackArgs a) {
-    stack_body<RING, PADDED, ST, PRE, LAT>(a, blockIdx.x);
-}
-
-// Several passes in one launch (the real pass and null runs of a batched search, or
-// regions of a frame): workgroup b works on pass b / per_pass. The descriptor address
-// is uniform and the kernel never writes it, so the compiler reads it with scalar loads.
-template <bool RING, bool PADDED, int ST, int PRE, bool LAT = false>
-__global__ __launch_bounds__(kThreads) void stack_multi_kernel(const StackArgs* __restrict__ passes, unsigned per_pass) {
-    const unsigned p = blockIdx.x / per_pass;
-    const StackArgs a = passes[p];
-    stack_body<RING, PADDED, ST, PRE, LAT>(a, blockIdx.x - p * per_pass);
-}
 
 __global__ void fill_keys_kernel(unsigned long long* __restrict__ keys, long long count) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -655,14 +97,27 @@ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 }  // namespace
 
 // Interior tiles: x0 + min dx >= 0, x0 + 63 + max dx < w, y0 + min dy >= row0 and
-// y0 + 15 + max dy < row1 for the union box of every block.
+// y0 + tile_rows - 1 + max dy < row1 for the union box of every block.
 void interior_rect(const StackPlan& p, const StackLaunch& L, int& tx_lo, int& tx_hi, int& ty_lo, int& ty_hi) {
     const int* u = p.all_box;
+    const int ty = p.tile_rows;
     tx_lo = std::max(0, ceil_div_pos(-u[0], kTileX));
     tx_hi = std::min(L.tiles_x, floor_div(p.w - kTileX - u[2], kTileX) + 1);
-    ty_lo = std::max(0, ceil_div_pos(L.row0 - u[1] - L.y_begin, kTileY));
-    ty_hi = std::min(L.tiles_y, floor_div(L.row1 - kTileY - u[3] - L.y_begin, kTileY) + 1);
+    ty_lo = std::max(0, ceil_div_pos(L.row0 - u[1] - L.y_begin, ty));
+    ty_hi = std::min(L.tiles_y, floor_div(L.row1 - ty - u[3] - L.y_begin, ty) + 1);
     if (tx_hi <= tx_lo || ty_hi <= ty_lo) tx_lo = tx_hi = ty_lo = ty_hi = 0;
+}
+
+int rows16_per_wave(const StackPlan& p, int tile_rows) {
+    const int nw = tile_rows / kRowsPerWave;
+    // word rows: tile_rows - 1 + the largest vertical spread, pre per wavefront
+    const int need = (tile_rows - 1 + p.max_sy + nw - 1) / nw;
+    return need <= 5 ? 5 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 12 ? 12 : 0;
+}
+
+bool tile_rows_fit(const StackPlan& p, int storage, int tile_rows) {
+    if (!tile_rows_ok(tile_rows)) return false;
+    return storage == kStoreF32 || rows16_per_wave(p, tile_rows) > 0;
 }
 
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]) {
@@ -670,6 +125,7 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kAr
     interior_rect(p, L, R.tx_lo, R.tx_hi, R.ty_lo, R.ty_hi);
     const long long all = static_cast<long long>(L.tiles_x) * L.tiles_y;
     const long long in = static_cast<long long>(R.tx_hi - R.tx_lo) * (R.ty_hi - R.ty_lo);
+    const int ty = p.tile_rows, nw = ty / kRowsPerWave;
     StackArgs a{};
     a.frames = L.frames;
     a.scale = L.scale;
@@ -704,12 +160,12 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kAr
     a.nseg = L.nseg;
     a.blk_lo = 0;
     a.blk_hi = p.nb;
-    a.lds_rows = kTileY + p.max_sy;
+    a.tile_rows = ty;
+    a.lds_rows = ty + p.max_sy;
     if (L.storage != kStoreF32) {
-        // word rows: 15 + the largest vertical spread, PRE per wave (5, 6, 8 or 12)
-        const int need = (kTileY - 1 + p.max_sy + kWaves - 1) / kWaves;
-        a.pre = need <= 5 ? 5 : need <= 6 ? 6 : need <= 8 ? 8 : 12;
-        a.lds_rows = std::max(a.lds_rows, kWaves * a.pre);
+        // 0 when this tile height cannot hold the spread (the launchers refuse it)
+        a.pre = rows16_per_wave(p, ty);
+        a.lds_rows = std::max(a.lds_rows, nw * a.pre);
     }
     a.cov_ok = p.cov_ok ? 1 : 0;
     StackArgs& lat = out[kArgsLat];
@@ -724,8 +180,9 @@ void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kAr
         // lattice blocks [0, nb_lat) and the others on interior tiles, each launch over
         // the segments that hold its blocks. A lane of the lattice kernel reads rows
         // 4 w .. 4 w + 3 + 2 e of a buffer (wavefront w, e = (spread + 1) / 2 row pairs), up
-        // to row 15 + 2 e: the buffers hold 16 + the largest spread rounded up to even.
-        lat.lds_rows = kTileY + 2 * ((p.max_sy_lat + 1) / 2);
+        // to row tile_rows - 1 + 2 e: the buffers hold tile_rows + the largest spread
+        // rounded up to even.
+        lat.lds_rows = ty + 2 * ((p.max_sy_lat + 1) / 2);
         lat.blk_hi = p.nb_lat;
         lat.nseg = std::max(0, std::min(seg_end, (p.nb_lat + bps - 1) / bps) - L.seg0);
         gen.blk_lo = p.nb_lat;
@@ -746,45 +203,43 @@ unsigned pass_groups(const StackArgs& a) {
 }
 // two buffers and the guard word of the 16-bit layout
 size_t lds_bytes_of(int lds_rows) { return sizeof(float) * (2 * lds_rows * kPitch + 1); }
-}  // namespace
 
-namespace {
-// kernel for (kind, padded frames, storage, 16-bit rows per wave)
-using SingleFn = void (*)(StackArgs);
-using MultiFn = void (*)(const StackArgs*, unsigned);
-template <bool RING, int ST>
-SingleFn single16(int pre) {
-    return pre <= 5   ? stack_kernel<RING, true, ST, 5>
-           : pre <= 6 ? stack_kernel<RING, true, ST, 6>
-           : pre <= 8 ? stack_kernel<RING, true, ST, 8>
-                      : stack_kernel<RING, true, ST, 12>;
-}
-template <bool RING, int ST>
-MultiFn multi16(int pre) {
-    return pre <= 5   ? stack_multi_kernel<RING, true, ST, 5>
-           : pre <= 6 ? stack_multi_kernel<RING, true, ST, 6>
-           : pre <= 8 ? stack_multi_kernel<RING, true, ST, 8>
-                      : stack_multi_kernel<RING, true, ST, 12>;
-}
-SingleFn single_kernel(int kind, bool padded, int st, int pre) {
-    const bool ring = kind == kArgsRing;
-    if (st == kStoreF16) return ring ? single16<true, kStoreF16>(pre) : single16<false, kStoreF16>(pre);
-    if (st == kStoreBF16) return ring ? single16<true, kStoreBF16>(pre) : single16<false, kStoreBF16>(pre);
-    if (kind == kArgsLat) return stack_kernel<false, false, kStoreF32, 0, true>;
-    if (!ring) return stack_kernel<false, false, kStoreF32, 0>;
-    return padded ? stack_kernel<true, true, kStoreF32, 0> : stack_kernel<true, false, kStoreF32, 0>;
-}
-MultiFn multi_kernel(int kind, bool padded, int st, int pre) {
-    const bool ring = kind == kArgsRing;
-    if (st == kStoreF16) return ring ? multi16<true, kStoreF16>(pre) : multi16<false, kStoreF16>(pre);
-    if (st == kStoreBF16) return ring ? multi16<true, kStoreBF16>(pre) : multi16<false, kStoreBF16>(pre);
-    if (kind == kArgsLat) return stack_multi_kernel<false, false, kStoreF32, 0, true>;
-    if (!ring) return stack_multi_kernel<false, false, kStoreF32, 0>;
-    return padded ? stack_multi_kernel<true, true, kStoreF32, 0> : stack_multi_kernel<true, false, kStoreF32, 0>;
-}
 bool storage_ok(const StackArgs& a) {
     if (a.storage == kStoreF32) return true;
-    return (a.storage == kStoreF16 || a.storage == kStoreBF16) && a.padded && a.scale != nullptr;
+    return (a.storage == kStoreF16 || a.storage == kStoreBF16) && a.padded && a.scale != nullptr && a.pre > 0;
+}
+SingleFn single_kernel(int kind, const StackArgs& a) {
+    return a.storage == kStoreF32 ? single_kernel_f32(kind, a.padded != 0) : single_kernel_16(a.storage, kind == kArgsRing, a.pre);
+}
+MultiFn multi_kernel(int kind, const StackArgs& a) {
+    return a.storage == kStoreF32 ? multi_kernel_f32(kind, a.padded != 0) : multi_kernel_16(a.storage, kind == kArgsRing, a.pre);
+}
+QueueFn queue_kernel(const StackArgs& a) {
+    const int nw = a.tile_rows / kRowsPerWave;
+    return a.storage == kStoreF32 ? queue_kernel_f32(nw, a.padded != 0) : queue_kernel_16(a.storage, nw, a.pre);
+}
+
+// Workgroups of one persistent launch: as many as the GPU holds at once (occupancy per
+// CU for the kernel and its LDS, times the CUs), a multiple of 8, at most what the
+// items need. Occupancy is asked once per kernel and LDS size.
+int resident_groups(QueueFn fn, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    const void* key = reinterpret_cast<const void*>(fn);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (const auto& c : cache)
+            if (c.first.first == key && c.first.second == (lds << 8 | static_cast<size_t>(dev))) return c.second;
+    }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, key, threads, lds) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const int groups = std::max(1, per_cu) * std::max(1, cus);
+    std::lock_guard<std::mutex> g(mu);
+    cache.push_back({{key, lds << 8 | static_cast<size_t>(dev)}, groups});
+    return groups;
 }
 }  // namespace
 
@@ -792,13 +247,22 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s,
     StackArgs A[kArgsKinds];
     make_stack_args(p, L, A);
     const StackArgs& ring = A[kArgsRing];
-    if (!storage_ok(ring)) return hipErrorInvalidValue;
+    if (!storage_ok(ring) || !tile_rows_ok(ring.tile_rows)) return hipErrorInvalidValue;
+    if (L.counters) {
+        // descriptors after the counters (256 bytes in) in the launch's workspace
+        static_assert(256 + sizeof(ArgsTriple) <= kQueueWorkspaceBytes, "workspace size");
+        ArgsTriple t;
+        for (int k = 0; k < kArgsKinds; ++k) t.a[k] = A[k];
+        uint32_t* dst = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(L.counters) + 256);
+        hipLaunchKernelGGL(put_args_kernel, dim3(1), dim3(256), 0, s, t, dst);
+        return launch_stack_queue(A, reinterpret_cast<const StackArgs*>(dst), 1, 1, L.counters, s);
+    }
+    if (ring.tile_rows != kTileY) return hipErrorInvalidValue;  // one-workgroup-per-item kernels: 16-row tiles
     auto go = [&](int kind, hipStream_t st) {
         const StackArgs& a = A[kind];
         const unsigned g = pass_groups(a);
         if (g > 0)
-            hipLaunchKernelGGL(single_kernel(kind, a.padded != 0, a.storage, a.pre), dim3(g), dim3(kThreads),
-                               lds_bytes_of(a.lds_rows), st, a);
+            hipLaunchKernelGGL(single_kernel(kind, a), dim3(g), dim3(kThreads), lds_bytes_of(a.lds_rows), st, a);
     };
     // the edge tiles first (on their own stream when given), the interior ones fill in;
     // the general blocks' interior launch (LDS-bound) goes beside the lattice launch
@@ -815,12 +279,13 @@ hipError_t launch_stack_multi(const StackArgs* const host[kArgsKinds], const Sta
     unsigned per[kArgsKinds] = {0, 0, 0};
     const StackArgs& r0 = host[kArgsRing][0];
     const int padded = r0.padded, storage = r0.storage, pre = r0.pre;
+    if (r0.tile_rows != kTileY) return hipErrorInvalidValue;
     for (int kind = 0; kind < kArgsKinds; ++kind)
         for (int q = 0; q < count; ++q) {
             const StackArgs& a = host[kind][q];
             // one LDS size per launch
             if (a.lds_rows != host[kind][0].lds_rows || a.storage != storage || a.pre != pre || !storage_ok(a) ||
-                (kind == kArgsRing && a.padded != padded))
+                a.tile_rows != kTileY || (kind == kArgsRing && a.padded != padded))
                 return hipErrorInvalidValue;
             per[kind] = std::max(per[kind], pass_groups(a));
         }
@@ -830,10 +295,51 @@ hipError_t launch_stack_multi(const StackArgs* const host[kArgsKinds], const Sta
         if (per[kind] == 0) continue;
         // the lattice launch on s; the edge tiles and the general blocks' interior launch
         // beside it on ring_s (when given)
-        hipLaunchKernelGGL(multi_kernel(kind, padded != 0, storage, pre), dim3(per[kind] * static_cast<unsigned>(count)),
-                           block, lds_bytes_of(host[kind][0].lds_rows), kind != kArgsLat && ring_s ? ring_s : s,
-                           dev[kind], per[kind]);
+        hipLaunchKernelGGL(multi_kernel(kind, host[kind][0]), dim3(per[kind] * static_cast<unsigned>(count)), block,
+                           lds_bytes_of(host[kind][0].lds_rows), kind != kArgsLat && ring_s ? ring_s : s, dev[kind],
+                           per[kind]);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stack_queue(const StackArgs* host, const StackArgs* dev, int count, int stride, int* counters,
+                              hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (!counters || !dev || stride < count) return hipErrorInvalidValue;
+    QueueDesc d{};
+    d.args = dev;
+    d.counters = counters;
+    d.count = count;
+    d.stride = stride;
+    const StackArgs& r0 = host[kArgsRing * stride];
+    int lds_rows = 0;
+    long long items = 0;
+    for (int kind = 0; kind < kArgsKinds; ++kind) {
+        const StackArgs& k0 = host[kind * stride];
+        d.ntiles[kind] = k0.nseg > 0 ? k0.ntiles : 0;
+        d.nseg[kind] = d.ntiles[kind] > 0 ? k0.nseg : 0;
+        for (int q = 0; q < count; ++q) {
+            const StackArgs& a = host[kind * stride + q];
+            // one kernel and one work shape for every pass of the launch
+            if (a.storage != r0.storage || a.pre != r0.pre || a.padded != r0.padded || a.tile_rows != r0.tile_rows ||
+                !storage_ok(a) || (d.ntiles[kind] > 0 && (a.ntiles != k0.ntiles || a.nseg != k0.nseg)))
+                return hipErrorInvalidValue;
+            if (d.ntiles[kind] > 0) lds_rows = std::max(lds_rows, a.lds_rows);
+        }
+        items += static_cast<long long>(d.ntiles[kind]) * d.nseg[kind] * count;
+    }
+    if (items == 0) return hipSuccess;
+    if (items > 0x7fffffffLL - 64 * 1024 * 1024) return hipErrorInvalidValue;  // int counters
+    const QueueFn fn = queue_kernel(r0);
+    if (!fn) return hipErrorInvalidValue;
+    const int threads = 64 * (r0.tile_rows / kRowsPerWave);
+    const size_t lds = lds_bytes_of(lds_rows);
+    int groups = resident_groups(fn, threads, lds);
+    if (groups <= 0) return hipErrorInvalidValue;
+    groups = static_cast<int>(std::min<long long>(groups, (items + 7) / 8 * 8)) / 8 * 8;
+    groups = std::max(groups, 8);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(groups)), dim3(static_cast<unsigned>(threads)), lds, s, d);
     return hipGetLastError();
 }
 

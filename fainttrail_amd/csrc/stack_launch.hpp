@@ -42,6 +42,10 @@ struct StackLaunch {
     int y_begin = 0, y_end = 0, tiles_x = 0, tiles_y = 0;
     int bps = 1, seg0 = 0, nseg = 1;
     unsigned long long* keys = nullptr;  // (y_end - y_begin) x w packed maxima
+    // persistent form (launch_stack with counters set): every kind of the pass in one
+    // launch taking work items from 8 device counters (cleared by the launch); counters
+    // points at kQueueWorkspaceBytes of device memory (counters, then descriptors)
+    int* counters = nullptr;
 };
 
 // Tile coordinates of a launch: interior tiles (every block's reads inside the frame
@@ -72,8 +76,9 @@ struct StackArgs {
     TileRect rect;
     int row0, row1, n, w, h, pitch, padded, storage, min_count, nb, bps;
     int y_begin, y_end, seg0, nseg, ntiles, lds_rows, cov_ok;
-    int pre;  // 16-bit frames: box rows staged per wavefront (5, 8 or 12)
+    int pre;  // 16-bit frames: box rows staged per wavefront (5, 6, 8 or 12)
     int blk_lo, blk_hi;  // the blocks this launch stacks (of each segment's range)
+    int tile_rows;       // 8, 16 or 32: 4 rows per wavefront of the kernel
 };
 
 // The descriptors of one pass: its lattice blocks on interior tiles (the lattice
@@ -81,16 +86,39 @@ struct StackArgs {
 // with ntiles 0 or nseg 0 has no launch.
 enum { kArgsLat = 0, kArgsGen = 1, kArgsRing = 2, kArgsKinds = 3 };
 
+// The kernel argument of a persistent launch (stack_queue_kernel): per kind, the tiles
+// and segments of each of `count` passes (the passes of one launch share their plan);
+// the descriptor of pass p and kind k is args[k * stride + p] in device memory;
+// counters: the 8 queue counters.
+struct QueueDesc {
+    const StackArgs* args;
+    int* counters;
+    int count, stride;
+    int ntiles[kArgsKinds], nseg[kArgsKinds];
+};
+
+// Device workspace of a persistent one-pass launch (StackLaunch::counters): the 8 queue
+// counters, then the pass's three descriptors. 256-byte aligned.
+constexpr size_t kQueueWorkspaceBytes = 4096;
+
 hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
                             int w, int rows, int n, hipStream_t s);
 
 // The three descriptors of one launch (see kArgsLat / kArgsGen / kArgsRing). Lattice
-// blocks get their own launch for fp32 frames with plan.lat present (L.lat set).
+// blocks get their own launch for fp32 frames with plan.lat present (L.lat set). The
+// tiles are the plan's (p.tile_rows rows).
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]);
+// 16-bit frames: box rows staged per wavefront for tiles of tile_rows rows (5, 6, 8 or
+// 12), 0 when the plan's vertical spread needs more.
+int rows16_per_wave(const StackPlan& p, int tile_rows);
+// Whether the kernels can stack the plan with tiles of tile_rows rows in this storage.
+bool tile_rows_fit(const StackPlan& p, int storage, int tile_rows);
 
-// ring_s (optional): the stream of the edge tiles' launch and of the general blocks'
-// interior launch, beside the lattice launch on s; ordered by the caller (every launch
-// merges into the same keys by atomic maximum, so any order gives the same bits).
+// One pass. With L.counters set: one persistent launch on s (any tile height; see
+// launch_stack_queue). Otherwise up to three launches (16-row tiles); ring_s
+// (optional): the stream of the edge tiles' launch and of the general blocks' interior
+// launch, beside the lattice launch on s; ordered by the caller (every launch merges
+// into the same keys by atomic maximum, so any order gives the same bits).
 hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s, hipStream_t ring_s = nullptr);
 
 // Several passes in up to three launches (lattice interior, general interior, ring).
@@ -101,6 +129,13 @@ hipError_t launch_stack(const StackPlan& p, const StackLaunch& L, hipStream_t s,
 // the same bits).
 hipError_t launch_stack_multi(const StackArgs* const host[kArgsKinds], const StackArgs* const dev[kArgsKinds], int count,
                               hipStream_t s, hipStream_t ring_s = nullptr);
+// Several passes in ONE persistent launch of about as many workgroups as the GPU holds
+// at once, which take (pass, kind, tile, segment) work items from the 8 device counters
+// (XCD-aware queues, see stack_queue_kernel); the counters are cleared on s first.
+// host / dev: descriptors kind-major, stride apart (host for the shapes, dev for the
+// kernel); every pass of the launch must have the same tiles and segments per kind.
+hipError_t launch_stack_queue(const StackArgs* host, const StackArgs* dev, int count, int stride, int* counters,
+                              hipStream_t s);
 hipError_t launch_fill_keys(unsigned long long* keys, long long count, hipStream_t s);
 hipError_t launch_finalize(const unsigned long long* keys, long long count, float* max_snr, int* best_v,
                            hipStream_t s);

@@ -522,13 +522,14 @@ StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const 
 
 void plan_tiles(StackPlan& p, int out_rows) {
     p.tiles_x = (p.w + kTileX - 1) / kTileX;
-    p.tiles_y = (out_rows + kTileY - 1) / kTileY;
+    p.tiles_y = (out_rows + p.tile_rows - 1) / p.tile_rows;
     const long tiles = static_cast<long>(p.tiles_x) * p.tiles_y;
-    // Enough workgroups to fill 256 CUs many times over, and at least 16 segments per
+    // Enough work items to fill 256 CUs many times over, and at least 16 segments per
     // tile: the segments of one tile run at the same time on one XCD (the kernel's
     // workgroup order) and read the same frame neighbourhood, so the tile's frames are
     // fetched from HBM about once per 16 workgroups instead of once per workgroup.
-    const long target = 8192;
+    // (8192 items of 16-row tiles; the same work per item for other tile heights.)
+    const long target = 8192L * kTileY / p.tile_rows;
     long nseg = std::max<long>({1, (target + tiles - 1) / tiles, 16});
     nseg = std::min<long>(nseg, p.nb);
     p.blocks_per_seg = static_cast<int>((p.nb + nseg - 1) / nseg);
@@ -557,11 +558,12 @@ PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, in
     const int* u = p.all_box;
     PadGeometry g;
     // columns read: [min dx, 64 tiles_x - 1 + max dx] (a tile's staged box ends 63 + its
-    // largest dx past the tile start); rows: [y_begin + min dy, y_begin + 16 tiles_y - 1 + max dy]
+    // largest dx past the tile start); rows: [y_begin + min dy, y_begin + rows tiles_y - 1 + max dy]
+    // for tiles of `rows` = tile_rows rows
     g.left = std::max(0, -u[0]);
     g.pitch = g.left + std::max(p.w, kTileX * tiles_x + std::max(0, u[2]));
     const int first = std::min(row0, y_begin + std::min(0, u[1]));
-    const int last = std::max(row1, y_begin + kTileY * tiles_y + std::max(0, u[3]));
+    const int last = std::max(row1, y_begin + p.tile_rows * tiles_y + std::max(0, u[3]));
     g.top = row0 - first;
     g.rows = last - first;
     return g;

@@ -19,9 +19,9 @@
 namespace ftk {
 
 constexpr int kTileX = 64;                     // pixels per tile row = wavefront width
-constexpr int kWaves = 4;                      // wavefronts per workgroup
+constexpr int kWaves = 4;                      // wavefronts per workgroup of the default tile
 constexpr int kRowsPerWave = 4;                // tile rows per wavefront
-constexpr int kTileY = kWaves * kRowsPerWave;  // 16
+constexpr int kTileY = kWaves * kRowsPerWave;  // 16: the default tile height (StackPlan::tile_rows)
 constexpr int kVel = 16;                       // motions per block
 constexpr int kSpreadMax = 32;                 // largest per-frame shift spread of a block, each axis
 constexpr int kPitch = 128;                     // LDS row pitch (floats): rows 0..3 of a lane are 2 x 64 dwords apart (ds_read2st64)
@@ -67,6 +67,9 @@ struct StackPlan {
     std::vector<float> inv_sqrt;     // c = 0..n: float(1 / sqrt(double c))
     int min_dy = 0, max_dy = 0;      // extremes of the clamped vertical shifts
     int max_sy = 0;                  // largest per-frame vertical spread of a block (<= kSpreadMax)
+    // tile height: 4 rows per wavefront, 2, 4 or 8 wavefronts per workgroup (8, 16 or
+    // 32 rows); set before plan_tiles
+    int tile_rows = kTileY;
     int tiles_x = 0, tiles_y = 0;
     int blocks_per_seg = 1, nseg = 1;
     double units = 0.0;              // nv * n * w * h
@@ -76,8 +79,11 @@ struct StackPlan {
 // Throws std::runtime_error with the reference's messages.
 StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const int32_t* dy, int min_count);
 
-// Tile rows for output rows [y0, y1) (row bands) and the segment size for a grid.
+// Tile rows for output rows [y0, y1) (row bands) and the segment size for a grid, at
+// the plan's tile height.
 void plan_tiles(StackPlan& p, int out_rows);
+// Whether a tile height is one the kernels are built for (8, 16 or 32 rows).
+inline bool tile_rows_ok(int rows) { return rows == 8 || rows == 16 || rows == 32; }
 
 // Packed (value, lowest index) key: larger key = larger S/N, then lower index.
 // Never called with NaN; 0 means "no defined motion".

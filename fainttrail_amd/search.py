@@ -27,7 +27,8 @@ class _Params(C.Structure):
                 ("passes_per_launch", C.c_int), ("segments", C.c_int), ("storage", C.c_int),
                 ("null_tested", C.c_int), ("max_frame_fraction", C.c_double),
                 ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double)),
-                ("candidates", C.c_int), ("half_tolerance", C.c_double)]
+                ("candidates", C.c_int), ("half_tolerance", C.c_double), ("tile_rows", C.c_int),
+                ("queue", C.c_int)]
 
 
 class _Summary(C.Structure):
@@ -39,7 +40,7 @@ class _Summary(C.Structure):
                 ("t_peaks", C.c_double), ("t_threshold", C.c_double), ("t_plan", C.c_double),
                 ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int),
                 ("n_screened", C.c_int), ("n_artefacts", C.c_int), ("n_candidates", C.c_int),
-                ("t_refine", C.c_double)]
+                ("t_refine", C.c_double), ("tile_rows", C.c_int), ("queue", C.c_int)]
 
 
 # Candidate's fields (include/fainttrail/search.hpp:131-154) that the record carries, in
@@ -69,7 +70,8 @@ def _bind():
     L.ftk_searcher_run.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.POINTER(_Summary), vp,
                                    vp, vp, C.c_char_p, sz]
     L.ftk_searcher_tune.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.c_int,
-                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), vp, C.c_char_p, sz]
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    vp, C.c_char_p, sz]
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     L.ftk_searcher_candidates.argtypes = [vp, C.c_int, C.POINTER(_Candidate)]
@@ -127,6 +129,11 @@ class SearchParams:
     # finish_candidates); half_tolerance is the halves test's allowed shortfall (sigma)
     candidates: bool = True
     half_tolerance: float = 3.0
+    # stacking work shape (no result depends on these): output rows per tile (8, 16
+    # or 32; 0 = tuned, else 16) and the persistent launch (True: one launch per group
+    # of passes whose workgroups take work items from device counters)
+    tile_rows: int = 0
+    queue: bool = True
 
     def c(self) -> _Params:
         from . import STORAGES
@@ -140,7 +147,8 @@ class SearchParams:
                     self.max_candidates, self.edge_px, self.seed,
                     -1 if self.fit_on_device is None else int(bool(self.fit_on_device)),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
-                    self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance)
+                    self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance,
+                    self.tile_rows, int(bool(self.queue)))
         if self.reg_dx is not None:
             rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
             ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)
@@ -224,6 +232,8 @@ class SearchResult:
     null_screened: int = 0   # null peaks through the artefact tests
     null_artefacts: int = 0  # of those, left out of null_peaks
     candidates: list = field(default_factory=list)  # Candidate records, the reference's order
+    tile_rows: int = 16      # the stacking work shape the run used
+    queue: bool = False
 
 
 class Searcher:
@@ -256,18 +266,19 @@ class Searcher:
         return m
 
     def tune(self, frames, mjd, params: SearchParams, reps: int = 2, stream=None):
-        """Times the batched form over group sizes and segment counts on these frames and
-        keeps the fastest for runs with passes_per_launch = 0 / segments = 0. Returns
-        (passes_per_launch, segments, seconds per search)."""
+        """Times the batched form over group sizes and segment counts, then tile heights,
+        on these frames and keeps the fastest for runs with passes_per_launch = 0 /
+        segments = 0 / tile_rows = 0. Returns (passes_per_launch, segments, tile_rows,
+        seconds per search)."""
         import torch
         m = self._check(frames, mjd)
         s = stream if stream is not None else torch.cuda.current_stream(frames.device)
         prm, e = params.c(), _err()
-        g, sg, t = C.c_int(), C.c_int(), C.c_double()
+        g, sg, tr, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _raise(lib().ftk_searcher_tune(self._h, frames.data_ptr(), frames.stride(0),
                                        m.ctypes.data_as(C.POINTER(C.c_double)), C.byref(prm), reps, C.byref(g),
-                                       C.byref(sg), C.byref(t), s.cuda_stream, e, len(e)), e)
-        return g.value, sg.value, t.value
+                                       C.byref(sg), C.byref(tr), C.byref(t), s.cuda_stream, e, len(e)), e)
+        return g.value, sg.value, tr.value, t.value
 
     def run(self, frames, mjd, params: SearchParams, keep_maps: bool = True, stream=None) -> SearchResult:
         """frames: (n, h, w) float32 device tensor sorted by time; mjd: n mid-exposure MJDs."""
@@ -292,7 +303,8 @@ class Searcher:
              "stack (null)": summ.t_stack_null, "peaks": summ.t_peaks, "threshold fit": summ.t_threshold,
              "plan": summ.t_plan, "passes (batched)": summ.t_passes, "refine": summ.t_refine},
             mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts,
-            read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates))
+            read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates),
+            summ.tile_rows, bool(summ.queue))
 
     def refit(self, params: SearchParams, on_device: bool, stream=None):
         """The last run's null-tail threshold fitted again with the FP64 fit sums on the

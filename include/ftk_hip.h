@@ -34,19 +34,26 @@ extern "C" {
  * Same shape as ft_progress_fn (fainttrail.h:71) and ProgressFn (types.hpp:76). */
 typedef int (*ftk_progress_fn)(double fraction, const char* stage, void* user);
 
-/* Backend options, "hip:device=N,max_mb=M,launch_adds=A,pad=P".
+/* Backend options, "hip:device=N,max_mb=M,launch_adds=A,pad=P,tile_rows=R,queue=Q".
  * Replaces CudaOptions / parse_cuda_options (src/cuda_plan.hpp:27-32,
  * src/cuda_plan.cpp:14-33): device -1 = FAINTTRAIL_HIP_DEVICE or 0; max_mb caps the
  * device memory for frame rows (0 = 80 % of free memory; small values force row
  * bands); launch_adds = stacked units per kernel launch (progress and cancel
- * granularity; default 4e12, about 0.25 s on MI355X); pad = 1 (default) keeps the frames in HBM inside a zero border that holds
- * every shifted read (when that costs at most twice their size), 0 = plain rows. The
- * results do not depend on pad. */
+ * granularity; default 4e12, about 0.25 s on MI355X); pad = 1 (default) keeps the
+ * frames in HBM inside a zero border that holds every shifted read (when that costs
+ * at most twice their size), 0 = plain rows. tile_rows and queue have no reference
+ * counterpart: tile_rows = output rows per stacking tile (8, 16 or 32; default 16;
+ * 64 columns wide); queue = 1 (default) stacks each launch as one persistent kernel
+ * whose workgroups take (kind, tile, velocity segment) work items from device
+ * counters, 0 = one workgroup per work item in up to three launches (16-row tiles
+ * only). The results do not depend on pad, tile_rows or queue. */
 typedef struct ftk_options {
     int device;
     double max_mb;
     double launch_adds;
     int pad;
+    int tile_rows;
+    int queue;
 } ftk_options;
 
 /* Parses the part after "hip:" (NULL or "" = defaults). Unknown keys or
@@ -239,6 +246,12 @@ typedef struct ftk_search_params {
      * the halves test's allowed shortfall in sigma. */
     int candidates;
     double half_tolerance;
+    /* Stacking work shape (no result bit depends on these): output rows per tile (8,
+     * 16 or 32; 0 = the searcher's tuned value, else 16) and the persistent launch
+     * (1, default: one launch per group of passes whose workgroups take work items
+     * from device counters; 0: one workgroup per work item, 16-row tiles only). */
+    int tile_rows;
+    int queue;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -256,6 +269,7 @@ typedef struct ftk_search_summary {
     int n_screened, n_artefacts; /* null peaks through the artefact tests; of those, left out */
     int n_candidates;            /* candidates after the duplicate test */
     double t_refine;             /* seconds of the candidate stage */
+    int tile_rows, queue;        /* the stacking work shape the run used */
 } ftk_search_summary;
 
 /* A candidate: ft_candidate (include/fainttrail/fainttrail.h:56-69) with the position
@@ -291,13 +305,14 @@ int ftk_searcher_run(ftk_searcher* searcher, const float* d_raw, size_t stride, 
                      const ftk_search_params* params, ftk_search_summary* summary, float* d_max_snr,
                      int32_t* d_best_v, void* stream, char* err, size_t err_len);
 /* Autotuner of the batched form: runs the search on these frames (params as for
- * ftk_searcher_run) over candidate group sizes and segment counts, reps times each,
- * and keeps the fastest for later runs with passes_per_launch = 0 / segments = 0.
+ * ftk_searcher_run) over candidate group sizes and velocity segments per tile, then
+ * tile heights (8, 16, 32 rows) at the best of those, reps times each, and keeps the
+ * fastest for later runs with passes_per_launch = 0 / segments = 0 / tile_rows = 0.
  * Writes the choice and its seconds per search (any output may be NULL). No extension
  * of the reference: its searches run one pass at a time. */
 int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride, const double* mjd,
                       const ftk_search_params* params, int reps, int* passes_per_launch, int* segments,
-                      double* seconds, void* stream, char* err, size_t err_len);
+                      int* tile_rows, double* seconds, void* stream, char* err, size_t err_len);
 /* Peaks of the last run: which 0 = real-pass peaks (sorted as find_peaks sorts them),
  * 1 = selected peaks, 2 = pooled null peak S/N values (snr only). Copies up to cap
  * entries into the non-NULL arrays and returns the total count. */

@@ -26,13 +26,16 @@ def run_hip(ft, frames, dx, dy, min_count, options=""):
 
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("options", ["", "pad=0", "max_mb=0.01,launch_adds=1", "max_mb=0.003,launch_adds=3000",
-                                     "max_mb=0.01,launch_adds=1,pad=0"])
+                                     "max_mb=0.01,launch_adds=1,pad=0", "queue=0", "queue=0,pad=0",
+                                     "max_mb=0.01,launch_adds=1,queue=0", "tile_rows=8", "tile_rows=32,pad=0",
+                                     "tile_rows=8,max_mb=0.01,launch_adds=1,pad=0", "tile_rows=32,max_mb=0.003"])
 def test_reference_test_inputs_bit_identical(hip, golden, name, options):
     """test_backend_contract / test_shift_tables / test_cuda_algorithm inputs (NaN,
     shifts beyond the frame, 171 motions, curved tables), unbanded and forced into
     row bands with tiny launches (test_main.cpp:602-609 runs "cuda:max_mb=0.01,
     launch_adds=1"); pad=0 keeps plain frame rows (masked edge staging) instead of
-    the zero-bordered layout."""
+    the zero-bordered layout; queue=0 launches one workgroup per work item instead of
+    the persistent kernel; tile_rows 8 and 32 change the tile height."""
     g = case(golden, name)
     print(f"case {name} options '{options}'", flush=True)
     m, b = run_hip(hip, g["frames"], g["dx"], g["dy"], g["min_count"], options)
@@ -80,7 +83,7 @@ def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
     print("cfg1 shape: oracle", flush=True)
     om, ob = po.stack_blocked(frames, tab.dx, tab.dy, mc)
     d = torch.from_numpy(frames).cuda()
-    for opts in ("", "pad=0"):
+    for opts in ("", "pad=0", "queue=0", "tile_rows=8", "tile_rows=32", "tile_rows=32,pad=0"):
         print(f"cfg1 shape: device plan '{opts}'", flush=True)
         plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc, opts)
         m, b = plan.run(d)
