@@ -165,9 +165,9 @@ def search_measure(reps: int, with_plain: bool, nmovers: int = 3, mover_snr=(1.5
         return r, (time.perf_counter() - t0) / reps
 
     seq, dt_seq = timed(SearchParams(**{**prm.__dict__, "passes_per_launch": 1}))
-    g, seg, tr = None, None, None
+    g, seg, tr, q = None, None, None, None
     if tune:
-        g, seg, tr, _ = s.tune(frames, mjd, prm, reps=1)
+        g, seg, tr, q, _ = s.tune(frames, mjd, prm, reps=1)
     res, dt = timed(prm)
     same = bool(seq.threshold_snr == res.threshold_snr and np.array_equal(seq.null_peaks, res.null_peaks)
                 and np.array_equal(seq.peaks.snr, res.peaks.snr) and seq.candidates == res.candidates
@@ -180,8 +180,8 @@ def search_measure(reps: int, with_plain: bool, nmovers: int = 3, mover_snr=(1.5
            "candidates": len(res.candidates), "candidates_ok": sum(c.ok for c in res.candidates),
            "null_peaks_screened": res.null_screened, "null_artefacts_left_out": res.null_artefacts,
            "batched": {"passes_per_launch": g if tune else res.passes_per_launch,
-                       "segments": seg if tune else res.segments, "tile_rows": tr if tune else res.tile_rows,
-                       "persistent_launch": res.queue, "equal_to_one_pass_form": same},
+                       "segments": seg if tune else res.segments, "tile_rows": res.tile_rows,
+                       "persistent_launch": bool(res.queue), "equal_to_one_pass_form": same},
            "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
            "one_pass_form_searches_per_s": 1.0 / dt_seq,
            "one_pass_form_stage_seconds": {k: round(v, 4) for k, v in seq.timings.items() if v > 0},
@@ -511,9 +511,9 @@ def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
     fwhm = 1.5 * 2.354820045030949
     prm = SearchParams(vmin_pxh=12.0, vmax_pxh=66.0, grid_tol_px=0.212, fwhm_px=fwhm, null_runs=256, seed=7)
     s = Searcher(n, w, h)
-    g, seg, tr, t_tune = s.tune(frames, mjd, SearchParams(**{**prm.__dict__, "null_runs": 16}), reps=1)
-    print(f"bench: batched nulls: tuned {g} passes per launch, {seg} segments, {tr}-row tiles", file=sys.stderr,
-          flush=True)
+    g, seg, tr, q, t_tune = s.tune(frames, mjd, SearchParams(**{**prm.__dict__, "null_runs": 16}), reps=1)
+    print(f"bench: batched nulls: tuned {g} passes per launch, {seg} segments, tile rows {tr}, launch form {q}",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     res = s.run(frames, mjd, prm)
     torch.cuda.synchronize()
@@ -527,9 +527,9 @@ def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
            "n_velocities": res.n_velocities, "device_s_per_search": dt, "device_searches_per_s": 1.0 / dt,
            "stacked_units_per_s": units / dt, "threshold_snr": res.threshold_snr, "method": res.threshold_method,
            "selected": int(res.selected.x.size),
-           "tuned_on_16_nulls": {"passes_per_launch": g, "segments": seg, "tile_rows": tr},
+           "tuned_on_16_nulls": {"passes_per_launch": g, "segments": seg, "tile_rows": tr, "queue": q},
            "passes_per_launch": res.passes_per_launch, "segments": res.segments, "tile_rows": res.tile_rows,
-           "persistent_launch": res.queue,
+           "persistent_launch": bool(res.queue),
            "stage_seconds": {k: round(v, 4) for k, v in res.timings.items() if v > 0},
            "fit_on_gpu": {"threshold_snr": thr_gpu, "method_equal": m_gpu == res.threshold_method,
                           "rel_diff_vs_host_fit": abs(thr_gpu - res.threshold_snr) / abs(res.threshold_snr),
@@ -593,6 +593,8 @@ def main() -> int:
     ap.add_argument("--no-search", action="store_true", help="skip the null-calibrated search measurement")
     ap.add_argument("--search-reps", type=int, default=3)
     ap.add_argument("--options", default="", help="backend options of the timed pass (hip:...), e.g. tile_rows=32")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="time the plan at its options' work shape instead of the shape ftk_plan_tune picks")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
                     help="frame storage of the timed stacking passes (16-bit: a power-of-two scale per frame)")
     ap.add_argument("--no-precision", action="store_true", help="skip the fp16/bf16/FP64 precision measurement")
@@ -643,14 +645,23 @@ def main() -> int:
     stream = torch.cuda.current_stream(dev)
     print(f"bench: {args.workload}: {n} x {w}x{h}, {tab.motions} motions, {plan.blocks} blocks, "
           f"{plan.units:.3e} units/pass", file=sys.stderr, flush=True)
+    shape = None
     if args.storage == "f32":
+        if not args.no_tune:
+            shape = plan.tune(frames, reps=1, stream=stream)
+
         def one_pass():
             plan.run(frames, out_m, out_b, stream)
     else:
         plan.store(frames, args.storage, stream)
+        if not args.no_tune:
+            shape = plan.tune(None, reps=1, stream=stream)
 
         def one_pass():
             plan.run_stored(out_m, out_b, stream)
+    if shape is not None:
+        print(f"bench: tuned work shape: {shape[0]}-row tiles, launch form {shape[1]} ({shape[2] * 1e3:.1f} ms)",
+              file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         one_pass()
     torch.cuda.synchronize()
@@ -683,6 +694,8 @@ def main() -> int:
         "dtype": "f32" if args.storage == "f32" else f"{args.storage} storage, f32 accumulation",
         "data": "synthetic (unit-variance fp32 frames, seed 7)",
         "config": {"workload": args.workload, "options": args.options or "defaults",
+                   "work_shape": {"tile_rows": shape[0], "persistent_launch": bool(shape[1]), "tuned": True}
+                   if shape else "options",
                    "frames": n, "width": w, "height": h, "motions": int(tab.motions),
                    "grid_tol_px": WORKLOADS[args.workload][5], "dv_pxh": dv, "min_count": min_count,
                    "units_per_step": units},

@@ -88,6 +88,8 @@ def lib() -> C.CDLL:
     L.ftk_plan_destroy.argtypes = [C.c_void_p]
     L.ftk_plan_store.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_char_p, C.c_size_t]
     L.ftk_plan_run_stored.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+    L.ftk_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_char_p, C.c_size_t]
     L.ftk_plan_scales.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
     L.ftk_stack_fp64.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -389,6 +391,28 @@ class Plan:
         _raise(lib().ftk_plan_run_stored(self._h, C.c_void_p(max_snr.data_ptr()), C.c_void_p(best_v.data_ptr()),
                                          C.c_void_p(s.cuda_stream), e, len(e)), e)
         return max_snr, best_v
+
+    def tune(self, frames=None, reps: int = 2, stream=None):
+        """Times the pass at each launch form and tile height (ftk_plan_tune) on these
+        fp32 frames, or on the stored 16-bit frames when frames is None, and keeps the
+        fastest for that storage's later runs. Returns (tile_rows, queue, seconds)."""
+        import torch
+        if frames is not None:
+            self._check_frames(frames)
+            dev = frames.device
+        else:
+            dev = getattr(self, "_stored_device", None)
+            if dev is None:
+                raise FtkError("Plan.tune: give fp32 frames or store 16-bit frames first")
+        m = torch.empty((self.h, self.w), dtype=torch.float32, device=dev)
+        b = torch.empty((self.h, self.w), dtype=torch.int32, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        tr, q, t, e = C.c_int(), C.c_int(), C.c_double(), _err()
+        _raise(lib().ftk_plan_tune(self._h, C.c_void_p(frames.data_ptr() if frames is not None else None),
+                                   frames.stride(0) if frames is not None else 0, C.c_void_p(m.data_ptr()),
+                                   C.c_void_p(b.data_ptr()), reps, C.c_void_p(s.cuda_stream), C.byref(tr), C.byref(q),
+                                   C.byref(t), e, len(e)), e)
+        return tr.value, q.value, t.value
 
     def scales(self) -> np.ndarray:
         """The per-frame scales of the stored frames."""

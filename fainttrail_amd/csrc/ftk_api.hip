@@ -121,18 +121,17 @@ ftk_options default_options() {
     // cost ~20 % on the 6144 x 4096 x 60 x 19952 pass when each launch was waited for)
     o.launch_adds = 4e12;
     o.pad = 1;
-    o.tile_rows = kTileY;
-    o.queue = 1;
+    o.tile_rows = 0;
+    o.queue = -1;
     return o;
 }
 
-// The plan's tile height from the options (0 = default).
-void set_tile_rows(StackPlan& p, const ftk_options& opt, int out_rows) {
-    const int rows = opt.tile_rows > 0 ? opt.tile_rows : kTileY;
-    if (!tile_rows_ok(rows)) throw std::runtime_error("hip backend option 'tile_rows' must be 8, 16 or 32");
-    if (!opt.queue && rows != kTileY) throw std::runtime_error("hip backend option 'tile_rows' other than 16 needs queue=1");
-    p.tile_rows = rows;
-    plan_tiles(p, out_rows);
+// The plan's tiles at a work shape.
+void use_shape(StackPlan& p, const StackShape& sh, int out_rows) {
+    if (p.tile_rows != sh.tile_rows || p.tiles_y == 0) {
+        p.tile_rows = sh.tile_rows;
+        plan_tiles(p, out_rows);
+    }
 }
 
 // Plan tables on the device.
@@ -177,8 +176,11 @@ struct ftk_plan {
     DeviceBuffer<uint16_t> stored;
     DeviceBuffer<float> scales;
     DeviceBuffer<unsigned> maxbits;
-    // persistent launch (queue = 1): its counters and descriptors
-    bool queue = true;
+    // work shapes of fp32 runs and of 16-bit runs (from the options, or ftk_plan_tune)
+    // and the asked values (tile_rows 0, queue -1: automatic)
+    StackShape shape32, shape16;
+    int ask_tile_rows = 0, ask_queue = -1;
+    // the persistent launch's counters and descriptors
     DeviceBuffer<int> queue_ws;
     // otherwise the edge tiles' launch runs on a stream of its own beside the interior launch
     hipStream_t ring_stream = nullptr;
@@ -219,14 +221,16 @@ extern "C" int ftk_parse_options(const char* spec, ftk_options* out, char* err, 
             else if (key == "launch_adds") o.launch_adds = d;
             else if (key == "pad" && (d == 0.0 || d == 1.0)) o.pad = static_cast<int>(d);
             else if (key == "pad") throw std::runtime_error("hip backend option 'pad' must be 0 or 1");
-            else if (key == "tile_rows" && tile_rows_ok(static_cast<int>(d)) && d == static_cast<int>(d)) o.tile_rows = static_cast<int>(d);
-            else if (key == "tile_rows") throw std::runtime_error("hip backend option 'tile_rows' must be 8, 16 or 32");
-            else if (key == "queue" && (d == 0.0 || d == 1.0)) o.queue = static_cast<int>(d);
-            else if (key == "queue") throw std::runtime_error("hip backend option 'queue' must be 0 or 1");
+            else if (key == "tile_rows" && (d == 0.0 || (tile_rows_ok(static_cast<int>(d)) && d == static_cast<int>(d))))
+                o.tile_rows = static_cast<int>(d);
+            else if (key == "tile_rows") throw std::runtime_error("hip backend option 'tile_rows' must be 0, 8, 16 or 32");
+            else if (key == "queue" && (d == -1.0 || d == 0.0 || d == 1.0)) o.queue = static_cast<int>(d);
+            else if (key == "queue") throw std::runtime_error("hip backend option 'queue' must be -1, 0 or 1");
             else throw std::runtime_error("unknown hip backend option '" + key + "' (device, max_mb, launch_adds, pad, tile_rows, queue)");
         }
         if (o.max_mb < 0 || o.launch_adds <= 0) throw std::runtime_error("hip backend options must be positive");
-        if (!o.queue && o.tile_rows != kTileY) throw std::runtime_error("hip backend option 'tile_rows' other than 16 needs queue=1");
+        if (o.queue == 0 && o.tile_rows != 0 && o.tile_rows != kTileY)
+            throw std::runtime_error("hip backend option 'tile_rows' other than 16 needs queue=1");
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());
         return FTK_ERROR;
@@ -272,7 +276,8 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             if (!pass->frames[i]) throw std::runtime_error("stack pass: missing frames");
         const ftk_options opt = opt_in ? *opt_in : default_options();
         StackPlan p = make_stack_plan(pass->n, pass->w, pass->h, pass->nv, pass->dx, pass->dy, pass->min_count);
-        set_tile_rows(p, opt, p.h);
+        const StackShape shape = resolve_shape(p, kStoreF32, opt.tile_rows, opt.queue, p.h);
+        use_shape(p, shape, p.h);
         const int n = p.n, w = p.w, h = p.h, ty = p.tile_rows;
         const int device = resolve_device(opt.device);
         int count = 0;
@@ -302,7 +307,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
         constexpr int kSlots = 8;
         constexpr size_t kSlotInts = kQueueWorkspaceBytes / sizeof(int);
         DeviceBuffer<int> d_queue;
-        if (opt.queue) d_queue.reserve(kSlots * kSlotInts, "allocate queue counters");
+        if (shape.queue) d_queue.reserve(kSlots * kSlotInts, "allocate queue counters");
         int launch_no = 0;
         DeviceBuffer<unsigned long long> d_keys;
         double t_up = 0, t_kernel = 0, t_down = 0;
@@ -408,7 +413,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                     L.seg0 = s0;
                     L.nseg = std::min(segs_per_launch, nseg_total - s0);
                     L.keys = d_keys.p + static_cast<size_t>(y - b.y0) * w;
-                    if (opt.queue) L.counters = d_queue.p + (launch_no++ % kSlots) * kSlotInts;
+                    if (shape.queue) L.counters = d_queue.p + (launch_no++ % kSlots) * kSlotInts;
                     hipStream_t st = lq.next();
                     check(launch_stack(p, L, st), "launch stack kernel");
                     const int blocks_done = std::min(p.nb, (s0 + L.nseg) * bps) - std::min(p.nb, s0 * bps);
@@ -456,12 +461,14 @@ extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, c
         auto* pl = new ftk_plan();
         try {
             pl->plan = make_stack_plan(n, w, h, nv, dx, dy, min_count);
-            set_tile_rows(pl->plan, opt, h);
+            pl->ask_tile_rows = opt.tile_rows;
+            pl->ask_queue = opt.queue;
+            pl->shape32 = resolve_shape(pl->plan, kStoreF32, opt.tile_rows, opt.queue, h);
+            use_shape(pl->plan, pl->shape32, h);
             pl->device = resolve_device(opt.device);
             check(hipSetDevice(pl->device), "hipSetDevice");
             pl->dev.upload(pl->plan);
-            pl->queue = opt.queue != 0;
-            if (pl->queue) pl->queue_ws.reserve(kQueueWorkspaceBytes / sizeof(int), "allocate queue counters");
+            pl->queue_ws.reserve(kQueueWorkspaceBytes / sizeof(int), "allocate queue counters");
             pl->keys.reserve(static_cast<size_t>(w) * h, "allocate keys");
             const StackPlan& p = pl->plan;
             pl->pad = pad_geometry(p, 0, h, 0, p.tiles_x, p.tiles_y);
@@ -513,9 +520,12 @@ StackLaunch plan_launch(ftk_plan* pl) {
     return L;
 }
 
+// One pass of the plan at its shape for the launch's storage (the caller built L with
+// plan_launch after use_shape for that shape).
 void run_launch(ftk_plan* pl, const StackLaunch& L_in, float* d_max_snr, int32_t* d_best_v, hipStream_t s) {
     const long long np = static_cast<long long>(pl->plan.w) * pl->plan.h;
-    if (pl->queue) {
+    const StackShape& sh = L_in.storage == kStoreF32 ? pl->shape32 : pl->shape16;
+    if (sh.queue) {
         StackLaunch L = L_in;
         L.counters = pl->queue_ws.p;
         check(launch_fill_keys(pl->keys.p, np, s), "clear keys");
@@ -548,6 +558,7 @@ extern "C" int ftk_plan_run(ftk_plan* pl, const float* d_frames, size_t frame_st
         if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_run: frame stride smaller than a frame");
         check(hipSetDevice(pl->device), "hipSetDevice");
         hipStream_t s = static_cast<hipStream_t>(stream);
+        use_shape(pl->plan, pl->shape32, p.h);
         StackLaunch L = plan_launch(pl);
         L.frames = d_frames;
         L.frame_stride = static_cast<long long>(frame_stride);
@@ -582,9 +593,10 @@ extern "C" int ftk_plan_store(ftk_plan* pl, const float* d_frames, size_t frame_
         if (frame_stride < static_cast<size_t>(p.w) * p.h) throw std::runtime_error("ftk_plan_store: frame stride smaller than a frame");
         check(hipSetDevice(pl->device), "hipSetDevice");
         hipStream_t s = static_cast<hipStream_t>(stream);
-        if (!tile_rows_fit(p, storage, p.tile_rows))
-            throw std::runtime_error("ftk_plan_store: 16-bit frames cannot be staged in tiles of " +
-                                     std::to_string(p.tile_rows) + " rows for this plan's vertical spread (use tile_rows=16)");
+        // the 16-bit runs' shape (a tile height the 16-bit staging cannot hold for this
+        // plan becomes 16 rows; the padded layout is the same for every tile height)
+        if (!pl->storage || pl->storage != storage)
+            pl->shape16 = resolve_shape(p, storage, pl->ask_tile_rows, pl->ask_queue, p.h);
         if (!pl->stored.p) {
             // 16-bit frames are always padded (their fetch reads whole pixel pairs)
             pl->pad16 = pad_geometry16(pad_geometry(p, 0, p.h, 0, p.tiles_x, p.tiles_y));
@@ -615,6 +627,7 @@ extern "C" int ftk_plan_run_stored(ftk_plan* pl, float* d_max_snr, int32_t* d_be
         if (!pl->storage) throw std::runtime_error("ftk_plan_run_stored: no frames stored (ftk_plan_store)");
         check(hipSetDevice(pl->device), "hipSetDevice");
         const PadGeometry& g = pl->pad16;
+        use_shape(pl->plan, pl->shape16, pl->plan.h);
         StackLaunch L = plan_launch(pl);
         L.frames = pl->stored.p + static_cast<long long>(g.top) * g.pitch + g.left;
         L.frame_stride = static_cast<long long>(g.rows) * g.pitch;
@@ -623,6 +636,67 @@ extern "C" int ftk_plan_run_stored(ftk_plan* pl, float* d_max_snr, int32_t* d_be
         L.storage = pl->storage;
         L.scale = pl->scales.p;
         run_launch(pl, L, d_max_snr, d_best_v, static_cast<hipStream_t>(stream));
+        return FTK_OK;
+    } catch (const std::exception& e) {
+        set_err(err, err_len, e.what());
+        return FTK_ERROR;
+    }
+}
+
+extern "C" int ftk_plan_tune(ftk_plan* pl, const float* d_frames, size_t frame_stride, float* d_max_snr,
+                             int32_t* d_best_v, int reps, void* stream, int* tile_rows, int* queue, double* seconds,
+                             char* err, size_t err_len) {
+    try {
+        if (!pl || !d_max_snr || !d_best_v) throw std::runtime_error("ftk_plan_tune: null argument");
+        if (!d_frames && !pl->storage) throw std::runtime_error("ftk_plan_tune: no frames (give fp32 frames or store 16-bit ones)");
+        check(hipSetDevice(pl->device), "hipSetDevice");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int storage = d_frames ? kStoreF32 : pl->storage;
+        StackShape& target = storage == kStoreF32 ? pl->shape32 : pl->shape16;
+        const StackShape keep = target;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        check(hipEventCreate(&e0), "create event");
+        check(hipEventCreate(&e1), "create event");
+        struct Events {
+            hipEvent_t a, b;
+            ~Events() {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } held{e0, e1};
+        StackShape best = keep;
+        double best_t = 1e300;
+        char inner[512] = {0};
+        // the launch forms and tile heights the kernels take (a height the 16-bit staging
+        // cannot hold for this plan is skipped)
+        const StackShape cands[] = {{16, 0}, {8, 1}, {16, 1}, {32, 1}};
+        for (const StackShape& c : cands) {
+            if (!tile_rows_fit(pl->plan, storage, c.tile_rows)) continue;
+            target = c;
+            double t = 1e300;
+            for (int r = 0; r <= std::max(1, reps); ++r) {  // the first run warms up
+                check(hipEventRecord(e0, s), "record event");
+                const int st = d_frames ? ftk_plan_run(pl, d_frames, frame_stride, d_max_snr, d_best_v, stream, inner, sizeof inner)
+                                        : ftk_plan_run_stored(pl, d_max_snr, d_best_v, stream, inner, sizeof inner);
+                if (st != FTK_OK) {
+                    target = keep;
+                    throw std::runtime_error(inner);
+                }
+                check(hipEventRecord(e1, s), "record event");
+                check(hipEventSynchronize(e1), "stack pass");
+                float ms = 0.0f;
+                check(hipEventElapsedTime(&ms, e0, e1), "event time");
+                if (r > 0) t = std::min(t, 1e-3 * ms);
+            }
+            if (t < best_t) {
+                best_t = t;
+                best = c;
+            }
+        }
+        target = best;
+        if (tile_rows) *tile_rows = best.tile_rows;
+        if (queue) *queue = best.queue;
+        if (seconds) *seconds = best_t;
         return FTK_OK;
     } catch (const std::exception& e) {
         set_err(err, err_len, e.what());

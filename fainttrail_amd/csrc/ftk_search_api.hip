@@ -146,7 +146,9 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
         throw std::runtime_error("ftk_searcher_run: passes_per_launch and segments must be >= 0");
     if (p->tile_rows != 0 && !ftk::tile_rows_ok(p->tile_rows))
         throw std::runtime_error("ftk_searcher_run: tile_rows must be 0, 8, 16 or 32");
-    if (p->queue != 0 && p->queue != 1) throw std::runtime_error("ftk_searcher_run: queue must be 0 or 1");
+    if (p->queue < -1 || p->queue > 1) throw std::runtime_error("ftk_searcher_run: queue must be -1, 0 or 1");
+    if (p->queue == 0 && p->tile_rows != 0 && p->tile_rows != 16)
+        throw std::runtime_error("ftk_searcher_run: tile_rows other than 16 need queue 1 (or -1)");
     if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
         throw std::runtime_error("ftk_searcher_run: unknown frame storage");
     if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
@@ -203,7 +205,8 @@ extern "C" int ftk_searcher_run(ftk_searcher* s, const float* d_raw, size_t stri
 
 extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t stride, const double* mjd,
                                  const ftk_search_params* p, int reps, int* passes_per_launch, int* segments,
-                                 int* tile_rows, double* seconds, void* stream, char* err, size_t err_len) {
+                                 int* tile_rows, int* queue, double* seconds, void* stream, char* err,
+                                 size_t err_len) {
     try {
         check_run_args(s, d_raw, stride, mjd, p);
         hipSetDevice(s->device);
@@ -214,6 +217,7 @@ extern "C" int ftk_searcher_tune(ftk_searcher* s, const float* d_raw, size_t str
         if (passes_per_launch) *passes_per_launch = s->ds->tuned_passes;
         if (segments) *segments = s->ds->tuned_segments;
         if (tile_rows) *tile_rows = s->ds->tuned_tile_rows;
+        if (queue) *queue = s->ds->tuned_queue;
         if (seconds) *seconds = t;
         return FTK_OK;
     } catch (const std::exception& e) {

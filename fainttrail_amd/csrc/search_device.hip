@@ -680,11 +680,9 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     const int nms = std::max(2, static_cast<int>(std::lround(cfg.fwhm_px)));
     const int edge = std::max(cfg.edge_px, 1);
     out.passes_per_launch = 1;
-    // tile height: asked, tuned, else 16 rows (a height the 16-bit spread does not fit
-    // falls back to 16 in the plan)
-    const int tile_rows = cfg.tile_rows > 0 ? cfg.tile_rows : (tuned_tile_rows > 0 ? tuned_tile_rows : kTileY);
-    out.tile_rows = tile_rows;
-    out.queue = cfg.queue;
+    // work shape: asked, else tuned, else automatic (each plan resolves it)
+    const int ask_rows = cfg.tile_rows > 0 ? cfg.tile_rows : tuned_tile_rows;
+    const int ask_queue = cfg.queue >= 0 ? cfg.queue : tuned_queue;
 
     std::vector<int32_t> dx, dy;
     struct PlanDel {
@@ -699,8 +697,8 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         opt.max_mb = 0;
         opt.launch_adds = 1e12;
         opt.pad = 1;
-        opt.tile_rows = tile_rows;
-        opt.queue = cfg.queue;
+        opt.tile_rows = ask_rows;
+        opt.queue = ask_queue;
         ftk_plan* plan = nullptr;
         char err[512] = {0};
         if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
@@ -721,6 +719,14 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
 
     float* m_real = d_max_snr ? d_max_snr : max_;
     int* b_real = d_best_v ? d_best_v : best_;
+    {
+        // the shape the plans resolve (the same rule for every pass of the run)
+        shifts_for(out.grid, out.t_h, n, dx, dy);
+        const StackPlan probe = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
+        const StackShape sh = resolve_shape(probe, cfg.storage, ask_rows, ask_queue, h);
+        out.tile_rows = sh.tile_rows;
+        out.queue = sh.queue;
+    }
     pass(out.t_h, real_frames, m_real, b_real);
     lap("stack (real)");
     const float floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
@@ -792,16 +798,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     std::vector<int32_t> dx, dy;
     shifts_for(out.grid, out.t_h, n, dx, dy);
     StackPlan plan = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
-    // tile height: asked, tuned, else 16 rows; the one-workgroup-per-item launches take
-    // 16-row tiles only, and 16-bit frames need room for the plan's spread
-    int tile_rows = cfg.tile_rows > 0 ? cfg.tile_rows : (tuned_tile_rows > 0 ? tuned_tile_rows : kTileY);
-    if (!cfg.queue || !tile_rows_fit(plan, cfg.storage, tile_rows)) tile_rows = kTileY;
-    if (tile_rows != plan.tile_rows) {
-        plan.tile_rows = tile_rows;
+    // work shape: asked, else tuned, else automatic; the one-workgroup-per-item
+    // launches take 16-row tiles only, and 16-bit frames need room for the plan's spread
+    const StackShape shape = resolve_shape(plan, cfg.storage, cfg.tile_rows > 0 ? cfg.tile_rows : tuned_tile_rows,
+                                           cfg.queue >= 0 ? cfg.queue : tuned_queue, h);
+    if (shape.tile_rows != plan.tile_rows) {
+        plan.tile_rows = shape.tile_rows;
         plan_tiles(plan, h);
     }
-    out.tile_rows = tile_rows;
-    out.queue = cfg.queue;
+    out.tile_rows = shape.tile_rows;
+    out.queue = shape.queue;
     const int segments = cfg.segments > 0 ? cfg.segments : tuned_segments;
     set_segments(plan, segments);
     out.segments = plan.nseg;
@@ -1086,7 +1092,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             hk[kind] = hargs[q].data() + kind * G;
             dk[kind] = dargs + kind * G;
         }
-        if (cfg.queue) {
+        if (shape.queue) {
             // the group in one persistent launch: every kind of every pass, work items
             // from the set's counters
             scheck(launch_stack_queue(hargs[q].data(), dargs, cnt, G, S.queue_ws, s), "launch stack kernel");
@@ -1355,10 +1361,11 @@ double DeviceSearch::tune(const float* raw, long long stride, const double* mjd,
     gs.erase(std::unique(gs.begin(), gs.end()), gs.end());
     cfg.passes_per_launch = passes;
     cfg.segments = 0;
-    cfg.tile_rows = kTileY;
+    const int ask_rows = cfg.tile_rows, ask_queue = cfg.queue;  // a fixed shape is not tuned
     tuned_passes = 0;
     tuned_segments = 0;
     tuned_tile_rows = 0;
+    tuned_queue = -1;
     const SearchOutput probe = run(raw, stride, mjd, cfg, nullptr, nullptr, s);
     const int base = std::max(1, probe.segments);
     std::vector<int> ss{base};
@@ -1385,30 +1392,34 @@ double DeviceSearch::tune(const float* raw, long long stride, const double* mjd,
                 bs = used;
             }
         }
-    // then the tile height at the best group size and segment count (the persistent
-    // launch takes every height; the one-workgroup-per-item launches 16 rows only)
-    int bt = kTileY;
-    if (cfg.queue) {
-        cfg.passes_per_launch = bg;
-        cfg.segments = bs;
-        for (int tr : {8, 32}) {
-            cfg.tile_rows = tr;
-            double t = 1e300;
-            int used = tr;
-            for (int r = 0; r < std::max(1, reps); ++r) {
-                const double t0 = wall();
-                used = run(raw, stride, mjd, cfg, nullptr, nullptr, s).tile_rows;
-                t = std::min(t, wall() - t0);
-            }
-            if (used == tr && t < best) {
-                best = t;
-                bt = tr;
-            }
+    // then the launch forms and tile heights at the best group size and segment count
+    // (a form or height the caller fixed stays fixed)
+    int bt = 0, bq = -1;
+    cfg.passes_per_launch = bg;
+    cfg.segments = bs;
+    const StackShape cands[] = {{16, 0}, {8, 1}, {16, 1}, {32, 1}};
+    for (const StackShape& c : cands) {
+        if ((ask_rows > 0 && c.tile_rows != ask_rows) || (ask_queue >= 0 && c.queue != ask_queue)) continue;
+        cfg.tile_rows = c.tile_rows;
+        cfg.queue = c.queue;
+        double t = 1e300;
+        SearchOutput o;
+        for (int r = 0; r < std::max(1, reps); ++r) {
+            const double t0 = wall();
+            o = run(raw, stride, mjd, cfg, nullptr, nullptr, s);
+            t = std::min(t, wall() - t0);
+        }
+        // a height the 16-bit staging cannot hold ran at another: not this candidate
+        if (o.tile_rows == c.tile_rows && o.queue == c.queue && t < best) {
+            best = t;
+            bt = c.tile_rows;
+            bq = c.queue;
         }
     }
     tuned_passes = bg;
     tuned_segments = bs;
     tuned_tile_rows = bt;
+    tuned_queue = bq;
     return best;
 }
 

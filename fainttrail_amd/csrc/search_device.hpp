@@ -44,11 +44,12 @@ struct SearchConfig {
     // choice). Neither changes any result bit.
     int passes_per_launch = 1;
     int segments = 0;
-    // Stacking work shape: output rows per tile (8, 16 or 32; 0 = tuned, else 16) and
-    // the persistent launch (1: one launch per group whose workgroups take work items
-    // from device counters; 0: one workgroup per work item, 16-row tiles only).
+    // Stacking work shape: output rows per tile (8, 16 or 32; 0 = tuned, else
+    // automatic) and the launch form (1: one persistent launch per group whose
+    // workgroups take work items from device counters; 0: one workgroup per work item,
+    // 16-row tiles only; -1 = tuned, else automatic, see resolve_shape).
     int tile_rows = 0;
-    int queue = 1;
+    int queue = -1;
     // Frame storage of the stacking passes: 0 fp32 (the exact form), 1 fp16, 2 bf16
     // (a power-of-two scale per frame, fp32 accumulation).
     int storage = 0;
@@ -143,6 +144,7 @@ public:
 
     int n, w, h;
     int tuned_passes = 0, tuned_segments = 0, tuned_tile_rows = 0;  // from tune(); 0 = none
+    int tuned_queue = -1;                                           // from tune(); -1 = none
 
     // The threshold of a run's pooled null peaks fitted again with the FP64 tail-fit
     // sums on the host (on_device false) or the GPU; method receives the method index.
@@ -152,8 +154,9 @@ public:
     // asked of hipMalloc; buffers only grow between runs).
     double device_bytes() const;
 
-    // Times the batched run over candidate group sizes and segment counts on these
-    // frames and keeps the fastest (tuned_passes, tuned_segments). Returns the seconds
+    // Times the batched run over candidate group sizes and segment counts, then over
+    // the launch forms and tile heights, on these frames and keeps the fastest
+    // (tuned_passes, tuned_segments, tuned_tile_rows, tuned_queue). Returns the seconds
     // per search of the choice.
     double tune(const float* raw, long long stride, const double* mjd, const SearchConfig& cfg, int reps,
                 hipStream_t s);

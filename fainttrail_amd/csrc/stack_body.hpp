@@ -624,7 +624,7 @@ __global__ __launch_bounds__(64 * NW) void stack_multi_kernel(const StackArgs* _
 // workgroup ends after at most (its items + 8) counter reads, whatever the others do.
 // The counters are zero before the launch.
 template <int NW, bool PADDED, int ST, int PRE>
-__global__ __launch_bounds__(64 * NW) void stack_queue_kernel(QueueDesc d) {
+__global__ __launch_bounds__(64 * NW, ST == kStoreF32 ? 1 : 5) void stack_queue_kernel(QueueDesc d) {
     __shared__ int s_item;
     const int home = static_cast<int>(blockIdx.x & 7u);
     for (int q = 0; q < 8; ++q) {

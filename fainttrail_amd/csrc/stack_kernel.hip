@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <mutex>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -118,6 +119,23 @@ int rows16_per_wave(const StackPlan& p, int tile_rows) {
 bool tile_rows_fit(const StackPlan& p, int storage, int tile_rows) {
     if (!tile_rows_ok(tile_rows)) return false;
     return storage == kStoreF32 || rows16_per_wave(p, tile_rows) > 0;
+}
+
+StackShape resolve_shape(const StackPlan& p, int storage, int tile_rows, int queue, int out_rows) {
+    if (tile_rows != 0 && !tile_rows_ok(tile_rows)) throw std::runtime_error("tile height must be 8, 16 or 32 rows");
+    if (queue < -1 || queue > 1) throw std::runtime_error("launch form (queue) must be -1, 0 or 1");
+    if (queue == 0 && tile_rows != 0 && tile_rows != kTileY)
+        throw std::runtime_error("tile heights other than 16 rows need the persistent launch (queue 1)");
+    StackShape sh;
+    if (queue < 0) {
+        const long long tiles = static_cast<long long>((p.w + kTileX - 1) / kTileX) * ((out_rows + kTileY - 1) / kTileY);
+        queue = storage != kStoreF32 || tiles >= 1024 ? 1 : 0;
+    }
+    sh.queue = queue;
+    sh.tile_rows = tile_rows > 0 ? tile_rows : kTileY;
+    if (!tile_rows_fit(p, storage, sh.tile_rows)) sh.tile_rows = kTileY;
+    if (!sh.queue) sh.tile_rows = kTileY;
+    return sh;
 }
 
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]) {

@@ -108,6 +108,21 @@ hipError_t launch_copy_rows(const float* src, long long src_stride, float* dst, 
 // blocks get their own launch for fp32 frames with plan.lat present (L.lat set). The
 // tiles are the plan's (p.tile_rows rows).
 void make_stack_args(const StackPlan& p, const StackLaunch& L, StackArgs out[kArgsKinds]);
+// The work shape of a stacking pass: tile height and launch form (queue 1: persistent
+// launches; 0: one workgroup per work item, 16-row tiles only). No result bit depends
+// on it.
+struct StackShape {
+    int tile_rows = 16;
+    int queue = 1;
+};
+// The shape for asked values (tile_rows 0 = automatic, queue -1 = automatic) of a pass
+// of out_rows output rows in this storage. Automatic: 16-row tiles; the persistent
+// launch for 16-bit frames, and for fp32 frames of at least 1024 tiles (measured on
+// MI355X: faster there, a few % slower on smaller frames, whose edge tiles run at one
+// wavefront per SIMD more in their own launch). A tile height that the 16-bit staging
+// cannot hold for this plan becomes 16. Throws on a tile height other than 16 with
+// queue 0.
+StackShape resolve_shape(const StackPlan& p, int storage, int tile_rows, int queue, int out_rows);
 // 16-bit frames: box rows staged per wavefront for tiles of tile_rows rows (5, 6, 8 or
 // 12), 0 when the plan's vertical spread needs more.
 int rows16_per_wave(const StackPlan& p, int tile_rows);

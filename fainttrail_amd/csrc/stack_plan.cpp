@@ -562,8 +562,11 @@ PadGeometry pad_geometry(const StackPlan& p, int row0, int row1, int y_begin, in
     // for tiles of `rows` = tile_rows rows
     g.left = std::max(0, -u[0]);
     g.pitch = g.left + std::max(p.w, kTileX * tiles_x + std::max(0, u[2]));
+    // the tiles' rows rounded up to whole tiles of the tallest height (32 rows), so the
+    // same buffer holds the reads of every tile height
+    const int tiled = (p.tile_rows * tiles_y + kTileRowsMax - 1) / kTileRowsMax * kTileRowsMax;
     const int first = std::min(row0, y_begin + std::min(0, u[1]));
-    const int last = std::max(row1, y_begin + p.tile_rows * tiles_y + std::max(0, u[3]));
+    const int last = std::max(row1, y_begin + tiled + std::max(0, u[3]));
     g.top = row0 - first;
     g.rows = last - first;
     return g;

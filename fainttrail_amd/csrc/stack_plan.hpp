@@ -84,6 +84,7 @@ StackPlan make_stack_plan(int n, int w, int h, int nv, const int32_t* dx, const 
 void plan_tiles(StackPlan& p, int out_rows);
 // Whether a tile height is one the kernels are built for (8, 16 or 32 rows).
 inline bool tile_rows_ok(int rows) { return rows == 8 || rows == 16 || rows == 32; }
+constexpr int kTileRowsMax = 32;
 
 // Packed (value, lowest index) key: larger key = larger S/N, then lower index.
 // Never called with NaN; 0 means "no defined motion".
@@ -114,7 +115,8 @@ std::vector<Band> plan_row_bands(int w, int h, int n, int min_dy, int max_dy, do
 // Zero border of a padded frame buffer for frame rows [row0, row1) and the tiles of
 // output rows from y_begin: `left` zero columns, rows of `pitch` floats, `top` zero
 // rows above row0, `rows` rows in all; frames points at (x 0, row0), i.e. top * pitch +
-// left floats into a frame's buffer.
+// left floats into a frame's buffer. The rows hold the tiles of any tile height over
+// the same output rows (the layout does not change with the tile height).
 struct PadGeometry {
     int left = 0, pitch = 0, top = 0, rows = 0;
 };

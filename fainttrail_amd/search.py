@@ -70,8 +70,8 @@ def _bind():
     L.ftk_searcher_run.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.POINTER(_Summary), vp,
                                    vp, vp, C.c_char_p, sz]
     L.ftk_searcher_tune.argtypes = [vp, vp, sz, C.POINTER(C.c_double), C.POINTER(_Params), C.c_int,
-                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
-                                    vp, C.c_char_p, sz]
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double), vp, C.c_char_p, sz]
     L.ftk_searcher_peaks.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_float), C.POINTER(C.c_int32)]
     L.ftk_searcher_candidates.argtypes = [vp, C.c_int, C.POINTER(_Candidate)]
@@ -130,10 +130,11 @@ class SearchParams:
     candidates: bool = True
     half_tolerance: float = 3.0
     # stacking work shape (no result depends on these): output rows per tile (8, 16
-    # or 32; 0 = tuned, else 16) and the persistent launch (True: one launch per group
-    # of passes whose workgroups take work items from device counters)
+    # or 32; 0 = tuned, else automatic) and the launch form (1: one persistent launch
+    # per group of passes whose workgroups take work items from device counters; 0:
+    # one workgroup per work item, 16-row tiles; -1 = tuned, else automatic)
     tile_rows: int = 0
-    queue: bool = True
+    queue: int = -1
 
     def c(self) -> _Params:
         from . import STORAGES
@@ -148,7 +149,7 @@ class SearchParams:
                     -1 if self.fit_on_device is None else int(bool(self.fit_on_device)),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
                     self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance,
-                    self.tile_rows, int(bool(self.queue)))
+                    self.tile_rows, int(self.queue))
         if self.reg_dx is not None:
             rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
             ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)
@@ -233,7 +234,7 @@ class SearchResult:
     null_artefacts: int = 0  # of those, left out of null_peaks
     candidates: list = field(default_factory=list)  # Candidate records, the reference's order
     tile_rows: int = 16      # the stacking work shape the run used
-    queue: bool = False
+    queue: int = 0
 
 
 class Searcher:
@@ -266,19 +267,20 @@ class Searcher:
         return m
 
     def tune(self, frames, mjd, params: SearchParams, reps: int = 2, stream=None):
-        """Times the batched form over group sizes and segment counts, then tile heights,
-        on these frames and keeps the fastest for runs with passes_per_launch = 0 /
-        segments = 0 / tile_rows = 0. Returns (passes_per_launch, segments, tile_rows,
-        seconds per search)."""
+        """Times the batched form over group sizes and segment counts, then launch forms
+        and tile heights, on these frames and keeps the fastest for runs with
+        passes_per_launch = 0 / segments = 0 / tile_rows = 0 / queue = -1. Returns
+        (passes_per_launch, segments, tile_rows, queue, seconds per search); tile_rows 0
+        / queue -1 when the automatic shape was fastest."""
         import torch
         m = self._check(frames, mjd)
         s = stream if stream is not None else torch.cuda.current_stream(frames.device)
         prm, e = params.c(), _err()
-        g, sg, tr, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        g, sg, tr, q, t = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _raise(lib().ftk_searcher_tune(self._h, frames.data_ptr(), frames.stride(0),
                                        m.ctypes.data_as(C.POINTER(C.c_double)), C.byref(prm), reps, C.byref(g),
-                                       C.byref(sg), C.byref(tr), C.byref(t), s.cuda_stream, e, len(e)), e)
-        return g.value, sg.value, tr.value, t.value
+                                       C.byref(sg), C.byref(tr), C.byref(q), C.byref(t), s.cuda_stream, e, len(e)), e)
+        return g.value, sg.value, tr.value, q.value, t.value
 
     def run(self, frames, mjd, params: SearchParams, keep_maps: bool = True, stream=None) -> SearchResult:
         """frames: (n, h, w) float32 device tensor sorted by time; mjd: n mid-exposure MJDs."""
@@ -304,7 +306,7 @@ class Searcher:
              "plan": summ.t_plan, "passes (batched)": summ.t_passes, "refine": summ.t_refine},
             mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts,
             read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates),
-            summ.tile_rows, bool(summ.queue))
+            summ.tile_rows, summ.queue)
 
     def refit(self, params: SearchParams, on_device: bool, stream=None):
         """The last run's null-tail threshold fitted again with the FP64 fit sums on the

@@ -42,11 +42,13 @@ typedef int (*ftk_progress_fn)(double fraction, const char* stage, void* user);
  * granularity; default 4e12, about 0.25 s on MI355X); pad = 1 (default) keeps the
  * frames in HBM inside a zero border that holds every shifted read (when that costs
  * at most twice their size), 0 = plain rows. tile_rows and queue have no reference
- * counterpart: tile_rows = output rows per stacking tile (8, 16 or 32; default 16;
- * 64 columns wide); queue = 1 (default) stacks each launch as one persistent kernel
- * whose workgroups take (kind, tile, velocity segment) work items from device
- * counters, 0 = one workgroup per work item in up to three launches (16-row tiles
- * only). The results do not depend on pad, tile_rows or queue. */
+ * counterpart: tile_rows = output rows per stacking tile (8, 16 or 32; 0 = automatic,
+ * 16; 64 columns wide); queue = 1 stacks each launch as one persistent kernel whose
+ * workgroups take (kind, tile, velocity segment) work items from device counters, 0 =
+ * one workgroup per work item in up to three launches (16-row tiles only), -1
+ * (default) = automatic: persistent for 16-bit frames and for fp32 frames of at least
+ * 1024 tiles of 64 x 16. ftk_plan_tune picks both by timing. The results do not
+ * depend on pad, tile_rows or queue. */
 typedef struct ftk_options {
     int device;
     double max_mb;
@@ -132,6 +134,16 @@ int ftk_plan_store(ftk_plan* plan, const float* d_frames, size_t frame_stride, i
 int ftk_plan_run_stored(ftk_plan* plan, float* d_max_snr, int32_t* d_best_v, void* stream, char* err, size_t err_len);
 /* The n per-frame scales of the stored frames (host copy; synchronises the device). */
 int ftk_plan_scales(const ftk_plan* plan, float* scales);
+
+/* Autotuner of a plan's work shape, no reference counterpart: times the plan's pass
+ * (reps runs after one warm-up) on these fp32 frames (as ftk_plan_run), or on the
+ * stored 16-bit frames when d_frames is NULL (as ftk_plan_run_stored), for each launch
+ * form and tile height the kernels take (one workgroup per work item at 16 rows; the
+ * persistent launch at 8, 16 and 32 rows), and keeps the fastest for that storage's
+ * later runs. Writes the choice and its seconds per pass (outputs may be NULL); the
+ * maps receive the pass's result (the same for every shape). Synchronous. */
+int ftk_plan_tune(ftk_plan* plan, const float* d_frames, size_t frame_stride, float* d_max_snr, int32_t* d_best_v,
+                  int reps, void* stream, int* tile_rows, int* queue, double* seconds, char* err, size_t err_len);
 /* The stacking contract evaluated in FP64 on the GPU, the precision reference for
  * stored and fp32 passes: output rows [y0, y1) of n fp32 frames (plain rows of w),
  * sums of the covering frames in double, times 1/sqrt(c) in double, strict maximum
@@ -247,9 +259,10 @@ typedef struct ftk_search_params {
     int candidates;
     double half_tolerance;
     /* Stacking work shape (no result bit depends on these): output rows per tile (8,
-     * 16 or 32; 0 = the searcher's tuned value, else 16) and the persistent launch
-     * (1, default: one launch per group of passes whose workgroups take work items
-     * from device counters; 0: one workgroup per work item, 16-row tiles only). */
+     * 16 or 32; 0 = the searcher's tuned value, else automatic) and the launch form
+     * (1: one persistent launch per group of passes whose workgroups take work items
+     * from device counters; 0: one workgroup per work item, 16-row tiles only; -1,
+     * default: tuned, else automatic as the plan options' queue -1). */
     int tile_rows;
     int queue;
 } ftk_search_params;
@@ -306,13 +319,15 @@ int ftk_searcher_run(ftk_searcher* searcher, const float* d_raw, size_t stride, 
                      int32_t* d_best_v, void* stream, char* err, size_t err_len);
 /* Autotuner of the batched form: runs the search on these frames (params as for
  * ftk_searcher_run) over candidate group sizes and velocity segments per tile, then
- * tile heights (8, 16, 32 rows) at the best of those, reps times each, and keeps the
- * fastest for later runs with passes_per_launch = 0 / segments = 0 / tile_rows = 0.
- * Writes the choice and its seconds per search (any output may be NULL). No extension
- * of the reference: its searches run one pass at a time. */
+ * over the launch forms and tile heights (one workgroup per work item at 16 rows;
+ * persistent at 8, 16, 32 rows) at the best of those, reps times each, and keeps the
+ * fastest for later runs with passes_per_launch = 0 / segments = 0 / tile_rows = 0 /
+ * queue = -1 (a tile height or launch form the params fix is kept). Writes the choice
+ * and its seconds per search (any output may be NULL). No extension of the
+ * reference: its searches run one pass at a time. */
 int ftk_searcher_tune(ftk_searcher* searcher, const float* d_raw, size_t stride, const double* mjd,
                       const ftk_search_params* params, int reps, int* passes_per_launch, int* segments,
-                      int* tile_rows, double* seconds, void* stream, char* err, size_t err_len);
+                      int* tile_rows, int* queue, double* seconds, void* stream, char* err, size_t err_len);
 /* Peaks of the last run: which 0 = real-pass peaks (sorted as find_peaks sorts them),
  * 1 = selected peaks, 2 = pooled null peak S/N values (snr only). Copies up to cap
  * entries into the non-NULL arrays and returns the total count. */

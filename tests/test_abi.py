@@ -39,11 +39,11 @@ def test_options_parse_like_the_reference():
     assert o.device == 1 and o.max_mb == 64
     o = ft.parse_options("")
     assert o.device == -1 and o.max_mb == 0 and o.launch_adds > 0 and o.pad == 1
-    assert o.tile_rows == 16 and o.queue == 1
+    assert o.tile_rows == 0 and o.queue == -1  # automatic work shape
     assert ft.parse_options("pad=0").pad == 0
     o = ft.parse_options("tile_rows=32,queue=1")
     assert o.tile_rows == 32 and o.queue == 1
-    assert ft.parse_options("queue=0").queue == 0
+    assert ft.parse_options("queue=0").queue == 0 and ft.parse_options("queue=0,tile_rows=16").tile_rows == 16
     for bad in ("speed=11", "device=", "max_mb=abc", "max_mb=-1", "launch_adds=0", "pad=2", "tile_rows=12",
                 "tile_rows=8.5", "queue=2", "tile_rows=8,queue=0"):
         with pytest.raises(ft.FtkError):

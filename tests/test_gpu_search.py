@@ -257,14 +257,15 @@ def _same_result(a, b):
 
 
 @pytest.mark.parametrize("group,segments,storage,tile_rows,queue",
-                         [(0, 0, "f32", 0, True), (2, 0, "f32", 0, True), (3, 5, "f32", 0, False),
-                          (4, 1, "f32", 8, True), (6, 64, "f32", 32, True), (0, 0, "f32", 16, False),
-                          (0, 0, "f16", 0, True), (4, 0, "bf16", 0, False), (3, 0, "f16", 8, True),
-                          (0, 0, "bf16", 32, True)])
+                         [(0, 0, "f32", 0, 1), (2, 0, "f32", 0, -1), (3, 5, "f32", 0, 0),
+                          (4, 1, "f32", 8, 1), (6, 64, "f32", 32, 1), (0, 0, "f32", 16, 0),
+                          (0, 0, "f16", 0, -1), (4, 0, "bf16", 0, 0), (3, 0, "f16", 8, 1),
+                          (0, 0, "bf16", 32, 1), (2, 0, "bf16", 16, 1)])
 def test_batched_search_equals_one_pass_at_a_time(hip, group, segments, storage, tile_rows, queue):
     """The batched form (one plan whose per-frame rows are permuted for each null run,
-    passes fused into one persistent launch per group, or multi-pass launches of one
-    workgroup per work item, stage builds on a second stream) gives the one-pass-at-a-
+    passes fused into one persistent launch per group (queue 1), or multi-pass launches
+    of one workgroup per work item (queue 0), or the automatic choice (-1), stage
+    builds on a second stream) gives the one-pass-at-a-
     time form's results bit for bit, for any group size, segment count and tile height:
     maps, peak lists, the pooled null peaks in run order and the threshold."""
     import torch
@@ -277,13 +278,13 @@ def test_batched_search_equals_one_pass_at_a_time(hip, group, segments, storage,
     s = Searcher(18, 200, 136)
     base = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=5, seed=4,
                         storage=storage)
-    one = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 1, "queue": False}))
+    one = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": 1, "queue": 0}))
     many = s.run(frames, mjd, SearchParams(**{**base.__dict__, "passes_per_launch": group, "segments": segments,
                                               "tile_rows": tile_rows, "queue": queue}))
     torch.cuda.synchronize()
-    assert one.passes_per_launch == 1 and not one.queue
+    assert one.passes_per_launch == 1 and one.queue == 0
     assert many.passes_per_launch == (group if group else 6)
-    assert many.tile_rows == (tile_rows or 16) and many.queue == queue
+    assert many.tile_rows == (tile_rows or 16) and (queue < 0 or many.queue == queue)
     assert one.null_peaks.size > 0 and one.peaks.x.size > 0
     _same_result(one, many)
 
@@ -326,12 +327,14 @@ def test_batched_search_fixed_threshold_and_tuner(hip):
     _same_result(a, b)
     nulls = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=4, seed=2)
     ref = s.run(frames, mjd, SearchParams(**{**nulls.__dict__, "passes_per_launch": 1}))
-    g, seg, tr, secs = s.tune(frames, mjd, nulls, reps=1)
-    print(f"tuned: {g} passes per launch, {seg} segments, {tr}-row tiles, {secs * 1e3:.2f} ms per search", flush=True)
-    assert 1 <= g <= 5 and seg >= 1 and tr in (8, 16, 32) and secs > 0
+    g, seg, tr, q, secs = s.tune(frames, mjd, nulls, reps=1)
+    print(f"tuned: {g} passes per launch, {seg} segments, tile rows {tr}, launch form {q}, "
+          f"{secs * 1e3:.2f} ms per search", flush=True)
+    assert 1 <= g <= 5 and seg >= 1 and tr in (0, 8, 16, 32) and q in (-1, 0, 1) and secs > 0
     tuned = s.run(frames, mjd, nulls)
     torch.cuda.synchronize()
-    assert tuned.passes_per_launch == g and tuned.segments == seg and tuned.tile_rows == tr and tuned.queue
+    assert tuned.passes_per_launch == g and tuned.segments == seg
+    assert (tr == 0 or tuned.tile_rows == tr) and (q < 0 or tuned.queue == q)
     _same_result(ref, tuned)
 
 

@@ -27,15 +27,17 @@ def run_hip(ft, frames, dx, dy, min_count, options=""):
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("options", ["", "pad=0", "max_mb=0.01,launch_adds=1", "max_mb=0.003,launch_adds=3000",
                                      "max_mb=0.01,launch_adds=1,pad=0", "queue=0", "queue=0,pad=0",
-                                     "max_mb=0.01,launch_adds=1,queue=0", "tile_rows=8", "tile_rows=32,pad=0",
+                                     "max_mb=0.01,launch_adds=1,queue=0", "queue=1", "queue=1,pad=0",
+                                     "max_mb=0.01,launch_adds=1,queue=1", "tile_rows=8", "tile_rows=32,pad=0",
                                      "tile_rows=8,max_mb=0.01,launch_adds=1,pad=0", "tile_rows=32,max_mb=0.003"])
 def test_reference_test_inputs_bit_identical(hip, golden, name, options):
     """test_backend_contract / test_shift_tables / test_cuda_algorithm inputs (NaN,
     shifts beyond the frame, 171 motions, curved tables), unbanded and forced into
     row bands with tiny launches (test_main.cpp:602-609 runs "cuda:max_mb=0.01,
     launch_adds=1"); pad=0 keeps plain frame rows (masked edge staging) instead of
-    the zero-bordered layout; queue=0 launches one workgroup per work item instead of
-    the persistent kernel; tile_rows 8 and 32 change the tile height."""
+    the zero-bordered layout; queue=0 launches one workgroup per work item, queue=1 the
+    persistent kernel (the default picks by frame size); tile_rows 8 and 32 change the
+    tile height."""
     g = case(golden, name)
     print(f"case {name} options '{options}'", flush=True)
     m, b = run_hip(hip, g["frames"], g["dx"], g["dy"], g["min_count"], options)
@@ -67,6 +69,36 @@ def test_check_backend_round1_simple_and_bands(hip):
         assert chk["ok"] and chk["max_abs_diff"] == 0.0
 
 
+def test_plan_tune_keeps_every_bit(hip):
+    """ftk_plan_tune times the launch forms and tile heights and keeps the fastest; the
+    tuned plan's fp32 and fp16 passes equal the untuned ones bit for bit (and the fp32
+    pass equals the oracle's blocked CPU pass)."""
+    import torch
+    n, h, w = 24, 200, 328
+    rng = np.random.default_rng(5)
+    frames = rng.standard_normal((n, h, w), dtype=np.float32)
+    frames[3, 10:14, 20:30] = np.nan
+    t = (np.arange(n) - (n - 1) / 2.0) / 60.0
+    grid, _ = hip.velocity_grid(t[-1] - t[0], 10.0, 60.0, 0.3)
+    tab = hip.linear_shift_table(t, grid)
+    mc = max(3, int(np.ceil(0.7 * n)))
+    om, ob = po.stack_blocked(frames, tab.dx, tab.dy, mc)
+    d = torch.from_numpy(frames).cuda()
+    plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc)
+    plan.store(d, "f16")
+    m16, b16 = (x.cpu().numpy() for x in plan.run_stored())
+    tr, q, secs = plan.tune(d, reps=1)
+    print(f"plan tune fp32: tile rows {tr}, launch form {q}, {secs * 1e3:.3f} ms", flush=True)
+    assert tr in (8, 16, 32) and q in (0, 1) and secs > 0 and (q == 1 or tr == 16)
+    m, b = plan.run(d)
+    assert same_bits(m.cpu().numpy(), om) and np.array_equal(b.cpu().numpy(), ob)
+    tr16, q16, secs16 = plan.tune(None, reps=1)
+    print(f"plan tune fp16: tile rows {tr16}, launch form {q16}, {secs16 * 1e3:.3f} ms", flush=True)
+    m2, b2 = plan.run_stored()
+    torch.cuda.synchronize()
+    assert same_bits(m2.cpu().numpy(), m16) and np.array_equal(b2.cpu().numpy(), b16)
+
+
 def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
     """512 x 512, 20 frames, the 5468-motion grid of the 512^2 configuration
     (dv * span = 0.5 px), unit-variance frames: device-resident plan and the host
@@ -83,7 +115,7 @@ def test_cfg1_shape_bit_identical_to_blocked_oracle(hip):
     print("cfg1 shape: oracle", flush=True)
     om, ob = po.stack_blocked(frames, tab.dx, tab.dy, mc)
     d = torch.from_numpy(frames).cuda()
-    for opts in ("", "pad=0", "queue=0", "tile_rows=8", "tile_rows=32", "tile_rows=32,pad=0"):
+    for opts in ("", "pad=0", "queue=0", "queue=1", "tile_rows=8", "tile_rows=32", "tile_rows=32,pad=0"):
         print(f"cfg1 shape: device plan '{opts}'", flush=True)
         plan = hip.Plan(n, w, h, tab.dx, tab.dy, mc, opts)
         m, b = plan.run(d)
