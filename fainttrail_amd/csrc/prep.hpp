@@ -31,6 +31,10 @@ struct StageParams {
     double weight_scatter = 1.5;
 };
 
+// Frames of one device stage: the static-sky kernel's per-pixel LDS columns (6 n bytes
+// per lane beyond 64 frames) fit a workgroup's 160 KB of LDS up to this count.
+constexpr int kMaxStageFrames = 420;
+
 // Scratch and launch sequence for one frame stack of n x h x w.
 class DeviceStage {
 public:

@@ -28,47 +28,165 @@ __device__ inline bool finitef(float v) { return __builtin_isfinite(v); }
 }  // namespace
 
 // ------------------------------------------------------------------ static sky
-// One pixel per thread, 64 threads per workgroup. The pixel's n samples sit in an LDS
-// column (vals[i][lane]); each finite sample's position in the (value, time rank)
-// order is counted (ranks are distinct, so the order is total, as std::sort of the
-// pairs makes it, preprocess.cpp:171), giving the sorted index -> frame map `ord`.
-// Global median: the two middle values added in float, halved in double. Leave-out
-// median for frame i: walk the sorted order skipping frames within `exclude` ranks
-// of i (preprocess.cpp:178-199), a and b in double, st = 0.5 (a + b).
-// NMAX > 0 (n <= NMAX): the rank count compares against a copy of the samples held in
-// registers (finite ones; others NaN, which no comparison counts), and only equal values
-// read the time ranks; the order is the same.
+// Per pixel: the finite samples in (value, time rank) order (ranks are distinct, so the
+// order is total, as std::sort of the pairs makes it, preprocess.cpp:171). Global
+// median: the two middle values added in float, halved in double. Leave-out median for
+// frame i: the sorted samples without the frames within `exclude` time ranks of i
+// (preprocess.cpp:178-199), a and b in double, st = 0.5 (a + b); fewer than 5 left:
+// the global median. Residual: float(double(v) - st).
+//
+// sky_rank_kernel (n <= NMAX <= 64): one pixel per lane, its samples in registers in
+// TIME-RANK order (rv[r] = frame fr[r]'s sample), so the (value, rank) order needs no
+// rank lookups: sample r's sorted position is the count of samples s < r with
+// rv[s] <= rv[r] plus those s > r with rv[s] < rv[r] (non-finite samples are NaN and
+// never counted). The sorted samples go to an LDS column (sv), each rank's sorted
+// position to a byte column (pb). Frames are then visited in time-rank order with a
+// 64-bit mask W of the sorted positions of the ranks within `exclude` of the current
+// one, updated by one bit in and one out per step: the kept samples are the set bits of
+// valid & ~W, their count a popcount, and the lo-th / hi-th kept sample the position
+// of the lo-th set bit (a select by halving popcounts) and of the next set bit. Two LDS
+// reads per frame instead of a walk over the sorted order.
+namespace {
+// f(integral_constant<int, I>) for I = B .. E - 1: the indices are constants before any
+// loop pass runs, so register arrays indexed with them are split into registers at once.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+__device__ inline int select_bit64(uint64_t k_mask, int k) {
+    // position of the k-th (from 0) set bit; the caller guarantees k < popcount
+    uint32_t x = static_cast<uint32_t>(k_mask);
+    int c = __builtin_popcount(x), pos = 0;
+    const bool up = k >= c;
+    k -= up ? c : 0;
+    pos = up ? 32 : 0;
+    x = up ? static_cast<uint32_t>(k_mask >> 32) : x;
+#pragma unroll
+    for (int half = 16; half >= 1; half >>= 1) {
+        const uint32_t lowmask = (1u << half) - 1u;
+        c = __builtin_popcount(x & lowmask);
+        const bool u = k >= c;
+        k -= u ? c : 0;
+        pos += u ? half : 0;
+        x = u ? (x >> half) : x;
+    }
+    return pos;
+}
+}  // namespace
+
 template <int NMAX>
+__global__ __launch_bounds__(64) void sky_rank_kernel(const float* __restrict__ sub, long long fstride, int n,
+                                                      long long plane, const int* __restrict__ rank, int exclude,
+                                                      int min_finite, float* __restrict__ sky, float* __restrict__ resid,
+                                                      long long rstride) {
+    static_assert(NMAX >= 1 && NMAX <= 64, "one 64-bit mask of sorted positions");
+    extern __shared__ unsigned char smem[];
+    int* fr = reinterpret_cast<int*>(smem);                                   // [n]: frame of time rank r
+    float* sv = reinterpret_cast<float*>(smem + 4 * n);                       // [n][64]: sorted samples
+    unsigned char* pb = smem + 4 * n + 4 * 64 * n;                            // [n][64]: sorted position of rank r
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += 64) fr[rank[i]] = i;
+    __syncthreads();  // the only barrier: lanes past the plane may leave below
+    const long long p = static_cast<long long>(blockIdx.x) * 64 + t;
+    if (p >= plane) return;
+    float rv[NMAX];
+    int m = 0;
+    static_for<0, NMAX>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        float v = __builtin_nanf("");
+        if (r < n) v = sub[fr[r] * fstride + p];
+        const bool f = r < n && finitef(v);
+        rv[r] = f ? v : __builtin_nanf("");
+        m += f ? 1 : 0;
+    });
+    if (m < min_finite) {
+        sky[p] = __builtin_nanf("");
+        for (int i = 0; i < n; ++i) resid[i * rstride + p] = __builtin_nanf("");
+        return;
+    }
+    static_for<0, NMAX>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const float vr = rv[r];
+        int pos = 0;
+        static_for<0, NMAX>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q < r) pos += rv[q] <= vr ? 1 : 0;
+            else if constexpr (q > r) pos += rv[q] < vr ? 1 : 0;
+        });
+        const bool f = vr == vr;
+        if (r < n) pb[r * 64 + t] = f ? static_cast<unsigned char>(pos) : 255u;
+        if (f) sv[pos * 64 + t] = vr;
+    });
+    const float ga = sv[((m - 1) / 2) * 64 + t];
+    const float gb = sv[(m / 2) * 64 + t];
+    const float gsum = ga + gb;
+    const double global = 0.5 * static_cast<double>(gsum);
+    sky[p] = static_cast<float>(global);
+    const uint64_t valid = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
+    auto bit_of = [&](int r) -> uint64_t {
+        const unsigned q = pb[r * 64 + t];
+        return q < 64u ? (1ull << q) : 0ull;
+    };
+    // W: sorted positions of the ranks in [r - exclude, r + exclude] (frame r included)
+    uint64_t W = 0ull;
+    if (exclude > 0)
+        for (int q = 0; q <= exclude && q < n; ++q) W |= bit_of(q);
+    static_for<0, NMAX>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if (r >= n) return;
+        if (r > 0 && exclude > 0) {
+            if (r + exclude < n) W |= bit_of(r + exclude);
+            if (r - exclude - 1 >= 0) W &= ~bit_of(r - exclude - 1);
+        }
+        const float v = rv[r];
+        float out = __builtin_nanf("");
+        if (v == v) {
+            double st = global;
+            if (exclude > 0) {
+                const uint64_t K = valid & ~W;
+                const int kept = __builtin_popcountll(K);
+                if (kept >= 5) {
+                    const int lo = (kept - 1) / 2, hi = kept / 2;
+                    const int qlo = select_bit64(K, lo);
+                    // hi is lo or lo + 1: the next set bit after qlo (qlo < 63 then)
+                    const int qhi = hi == lo ? qlo : __builtin_ctzll(K & ~((2ull << qlo) - 1ull));
+                    const double a = sv[qlo * 64 + t];
+                    const double b = sv[qhi * 64 + t];
+                    st = 0.5 * (a + b);
+                }
+            }
+            out = static_cast<float>(static_cast<double>(v) - st);
+        }
+        resid[fr[r] * rstride + p] = out;
+    });
+}
+
+// sky_kernel (any n): the pixel's n samples in an LDS column (vals[i][lane]); each
+// finite sample's position in the (value, time rank) order is counted, giving the
+// sorted index -> frame map `ord`; the leave-out median walks that order skipping the
+// frames within `exclude` ranks of i.
 __global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, long long fstride, int n,
                                                  long long plane, const int* __restrict__ rank, int exclude,
                                                  int min_finite, float* __restrict__ sky, float* __restrict__ resid,
                                                  long long rstride) {
     extern __shared__ unsigned char smem[];
-    float* vals = reinterpret_cast<float*>(smem);             // [n][64]
-    int* rk = reinterpret_cast<int*>(smem + 4 * 64 * n);      // [n]
-    unsigned char* ord = smem + 4 * 64 * n + 4 * n;           // [n][64]
+    float* vals = reinterpret_cast<float*>(smem);                        // [n][64]
+    int* rk = reinterpret_cast<int*>(smem + 4 * 64 * n);                 // [n]
+    uint16_t* ord = reinterpret_cast<uint16_t*>(smem + 4 * 64 * n + 4 * n);  // [n][64]
     const int t = threadIdx.x;
     for (int i = t; i < n; i += 64) rk[i] = rank[i];
     __syncthreads();
     const long long p = static_cast<long long>(blockIdx.x) * 64 + t;
     if (p >= plane) return;
     int m = 0;
-    float rv[NMAX > 0 ? NMAX : 1];
-    if constexpr (NMAX > 0) {
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-            const float v = i < n ? sub[i * fstride + p] : 0.0f;
-            if (i < n) vals[i * 64 + t] = v;
-            const bool f = i < n && finitef(v);
-            rv[i] = f ? v : __builtin_nanf("");
-            m += f ? 1 : 0;
-        }
-    } else {
-        for (int i = 0; i < n; ++i) {
-            const float v = sub[i * fstride + p];
-            vals[i * 64 + t] = v;
-            m += finitef(v) ? 1 : 0;
-        }
+    for (int i = 0; i < n; ++i) {
+        const float v = sub[i * fstride + p];
+        vals[i * 64 + t] = v;
+        m += finitef(v) ? 1 : 0;
     }
     if (m < min_finite) {
         sky[p] = __builtin_nanf("");
@@ -80,26 +198,11 @@ __global__ __launch_bounds__(64) void sky_kernel(const float* __restrict__ sub, 
         if (!finitef(vi)) continue;
         const int ri = rk[i];
         int pos = 0;
-        if constexpr (NMAX > 0) {
-            int eq = 0;
-#pragma unroll
-            for (int j = 0; j < NMAX; ++j) {
-                pos += rv[j] < vi ? 1 : 0;
-                eq += rv[j] == vi ? 1 : 0;
-            }
-            if (eq > 1) {  // equal values besides its own: the lower time rank first
-                for (int j = 0; j < n; ++j) {
-                    const float vj = vals[j * 64 + t];
-                    pos += (finitef(vj) && vj == vi && rk[j] < ri) ? 1 : 0;
-                }
-            }
-        } else {
-            for (int j = 0; j < n; ++j) {
-                const float vj = vals[j * 64 + t];
-                pos += (finitef(vj) && (vj < vi || (vj == vi && rk[j] < ri))) ? 1 : 0;
-            }
+        for (int j = 0; j < n; ++j) {
+            const float vj = vals[j * 64 + t];
+            pos += (finitef(vj) && (vj < vi || (vj == vi && rk[j] < ri))) ? 1 : 0;
         }
-        ord[pos * 64 + t] = static_cast<unsigned char>(i);
+        ord[pos * 64 + t] = static_cast<uint16_t>(i);
     }
     const float ga = vals[ord[((m - 1) / 2) * 64 + t] * 64 + t];
     const float gb = vals[ord[(m / 2) * 64 + t] * 64 + t];
@@ -507,6 +610,7 @@ __global__ void inv_sigma_kernel(const SelectSeg* __restrict__ seg, int nseg, fl
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace ftk {
 
@@ -545,7 +649,10 @@ unsigned blocks_for(long long count, int threads) { return static_cast<unsigned>
 
 DeviceStage::DeviceStage(int n_, int w_, int h_) : n(n_), w(w_), h(h_), plane(static_cast<long long>(w_) * h_) {
     if (n < 1 || w < 3 || h < 3) throw std::runtime_error("build_stage: bad frame stack size");
-    if (n > 255) throw std::runtime_error("build_stage: at most 255 frames on the device path");
+    // the static-sky kernel keeps each pixel's samples and order in LDS columns (6 n bytes
+    // per lane for n > 64) and the scatter map one column (4 n): both within 160 KB
+    if (n > kMaxStageFrames)
+        throw std::runtime_error("build_stage: at most " + std::to_string(kMaxStageFrames) + " frames on the device path");
     const size_t np = static_cast<size_t>(n) * plane;
     resid_ = dalloc<float>(np, "allocate residuals");
     flags_ = dalloc<unsigned char>(np, "allocate flags");
@@ -664,11 +771,18 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
 
     // static sky + leave-out residuals
     float* sky = static_sky ? static_sky : sky_;
-    const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
-    auto sky_fn = n <= 16 ? sky_kernel<16> : n <= 32 ? sky_kernel<32> : n <= 48 ? sky_kernel<48>
-                : n <= 64 ? sky_kernel<64> : sky_kernel<0>;
-    hipLaunchKernelGGL(sky_fn, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
-                       exclude, min_finite, sky, resid_, plane);
+    if (n <= 64) {
+        // samples in registers, sorted samples and positions in LDS (5 n bytes per lane)
+        const size_t lds = static_cast<size_t>(n) * 64 * 5 + 4 * static_cast<size_t>(n);
+        auto sky_fn = n <= 16 ? sky_rank_kernel<16> : n <= 32 ? sky_rank_kernel<32> : n <= 48 ? sky_rank_kernel<48>
+                                                                                               : sky_rank_kernel<64>;
+        hipLaunchKernelGGL(sky_fn, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
+                           exclude, min_finite, sky, resid_, plane);
+    } else {
+        const size_t lds = static_cast<size_t>(n) * 64 * 6 + 4 * static_cast<size_t>(n);
+        hipLaunchKernelGGL(sky_kernel, dim3(blocks_for(plane, 64)), dim3(64), lds, s, sub, sub_stride, n, plane, d_rank,
+                           exclude, min_finite, sky, resid_, plane);
+    }
     if (resid_out)
         hcheck(hipMemcpyAsync(resid_out, resid_, sizeof(float) * n * plane, hipMemcpyDeviceToDevice, s), "copy residuals");
     // per-frame residual noise: robust sigma of the grid samples, border 2
