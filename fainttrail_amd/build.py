@@ -23,7 +23,7 @@ OBJ = PKG / "_build"
 HIP_SOURCES = ["stack_inst_f16.hip", "stack_inst_bf16.hip", "stack_inst_f32.hip", "stack_kernel.hip",
                "probe_kernel.hip", "prep_kernels.hip", "search_kernels.hip", "search_device.hip", "ftk_api.hip",
                "ftk_prep_api.hip", "ftk_search_api.hip", "store_kernels.hip", "serve.hip", "screen_kernels.hip",
-               "refine_kernels.hip", "stack_plan.cpp", "candidates.cpp"]
+               "refine_kernels.hip", "fuse.hip", "stack_plan.cpp", "candidates.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -57,6 +57,7 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->half_tolerance = c.half_tolerance;
     p->tile_rows = 0;
     p->queue = c.queue;
+    p->build_lanes = 0;
 }
 
 namespace {
@@ -89,6 +90,7 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.half_tolerance = p->half_tolerance;
     c.tile_rows = p->tile_rows;
     c.queue = p->queue;
+    c.build_lanes = p->build_lanes;
     return c;
 }
 
@@ -147,6 +149,7 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
     if (p->tile_rows != 0 && !ftk::tile_rows_ok(p->tile_rows))
         throw std::runtime_error("ftk_searcher_run: tile_rows must be 0, 8, 16 or 32");
     if (p->queue < -1 || p->queue > 1) throw std::runtime_error("ftk_searcher_run: queue must be -1, 0 or 1");
+    if (p->build_lanes < 0 || p->build_lanes > 4) throw std::runtime_error("ftk_searcher_run: build_lanes must be 0-4");
     if (p->queue == 0 && p->tile_rows != 0 && p->tile_rows != 16)
         throw std::runtime_error("ftk_searcher_run: tile_rows other than 16 need queue 1 (or -1)");
     if (p->storage != FTK_STORE_F32 && p->storage != FTK_STORE_F16 && p->storage != FTK_STORE_BF16)
@@ -466,10 +469,17 @@ extern "C" void ftk_server_release(ftk_server* s, long long job) {
 extern "C" int ftk_server_stats(ftk_server* s, double* stats) {
     if (!s || !stats) return FTK_ERROR;
     const auto st = s->srv->stats();
-    const double v[10] = {st.used, st.peak, st.budget, st.idle_bytes, static_cast<double>(st.running),
+    const double v[13] = {st.used, st.peak, st.budget, st.idle_bytes, static_cast<double>(st.running),
                           static_cast<double>(st.queued), static_cast<double>(st.completed),
                           static_cast<double>(st.failed), static_cast<double>(st.evicted),
-                          static_cast<double>(st.max_running)};
-    for (int k = 0; k < 10; ++k) stats[k] = v[k];
+                          static_cast<double>(st.max_running), static_cast<double>(st.stack_launches),
+                          static_cast<double>(st.fused_launches), static_cast<double>(st.fused_passes)};
+    for (int k = 0; k < 13; ++k) stats[k] = v[k];
+    return FTK_OK;
+}
+
+extern "C" int ftk_server_fusion(ftk_server* s, int on, double window_ms) {
+    if (!s || window_ms < 0) return FTK_ERROR;
+    s->srv->set_fusion(on != 0, 1e-3 * window_ms);
     return FTK_OK;
 }

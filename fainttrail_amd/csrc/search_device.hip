@@ -7,7 +7,9 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <exception>
 #include <stdexcept>
+#include <thread>
 
 #include "../../include/ftk_hip.h"
 #include "stack_launch.hpp"
@@ -15,6 +17,7 @@
 #include "refine.hpp"
 #include "screen.hpp"
 #include "store.hpp"
+#include "fuse.hpp"
 
 namespace ftk {
 
@@ -165,8 +168,9 @@ double expected_null_peaks(const SearchOutput& out, double snr) {
     return 0.0;
 }
 
-int build_lanes(int n, long long plane, int passes) {
+int build_lanes(int n, long long plane, int passes, int asked) {
     if (passes < 2) return 1;
+    if (asked > 0) return std::min({asked, 4, passes});
     const double floats = static_cast<double>(n) * static_cast<double>(plane);
     return floats <= 16777216.0 ? 4 : floats <= 67108864.0 ? 2 : 1;
 }
@@ -379,7 +383,7 @@ DeviceSearch::~DeviceSearch() {
                     static_cast<void*>(res_), static_cast<void*>(rmask_), static_cast<void*>(ref_dev_)})
         if (p) (void)hipFree(p);
     for (void* p : {static_cast<void*>(scr_tab_host_), static_cast<void*>(scr_pk_host_),
-                    static_cast<void*>(scr_art_host_), static_cast<void*>(ref_host_)})
+                    static_cast<void*>(scr_art_host_), static_cast<void*>(ref_host_), static_cast<void*>(pk_host_)})
         if (p) (void)hipHostFree(p);
     for (auto& set : sets_) set.release();
     for (int q = 0; q < 2; ++q)
@@ -797,7 +801,18 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     // the plan of the real pass, shared by all passes
     std::vector<int32_t> dx, dy;
     shifts_for(out.grid, out.t_h, n, dx, dy);
-    StackPlan plan = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
+    // the host plan: the previous run's when its shift tables and coverage count are the
+    // same (the plan depends on nothing else; its tiles are set below), else a new one
+    if (!plan_cache_ || plan_key_mc_ != min_count || plan_key_dx_ != dx || plan_key_dy_ != dy) {
+        plan_cache_.reset();
+        plan_cache_ = std::make_unique<StackPlan>(make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count));
+        plan_key_dx_ = dx;
+        plan_key_dy_ = dy;
+        plan_key_mc_ = min_count;
+    }
+    StackPlan& plan = *plan_cache_;
+    plan.tile_rows = kTileY;
+    plan_tiles(plan, h);
     // work shape: asked, else tuned, else automatic; the one-workgroup-per-item
     // launches take 16-row tiles only, and 16-bit frames need room for the plan's spread
     const StackShape shape = resolve_shape(plan, cfg.storage, cfg.tile_rows > 0 ? cfg.tile_rows : tuned_tile_rows,
@@ -848,9 +863,11 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     }
     out.passes_per_launch = G;
     // group k holds passes [gstart[k], gstart[k + 1]): the real pass alone first (the
-    // only stage build the GPU waits for before it stacks), then groups of G
+    // only stage build the GPU waits for before it stacks; the next group's builds run
+    // beside its launches), then groups of G. A persistent launch holds every CU until
+    // it ends, so nothing runs beside it: then the real pass opens the first group.
     std::vector<int> gstart{0};
-    if (passes > 1) gstart.push_back(1);
+    if (passes > 1 && !shape.queue) gstart.push_back(1);
     while (gstart.back() < passes) gstart.push_back(std::min(passes, gstart.back() + G));
     const int groups = static_cast<int>(gstart.size()) - 1;
     const int nsets = groups > 1 ? 2 : 1;
@@ -999,7 +1016,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
 
     // build lanes: more for small frames, where a stage build's many small kernels
     // leave most of the GPU idle
-    const int lanes = build_lanes(n, plane, passes);
+    const int lanes = build_lanes(n, plane, passes, cfg.build_lanes);
     while (static_cast<int>(lanes_.size()) < lanes - 1) {
         BuildLane l;
         l.stage = std::make_unique<DeviceStage>(n, w, h);
@@ -1026,7 +1043,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         const int q = grp % nsets;
         PassSet& S = sets_[q];
         const int p0 = gstart[static_cast<size_t>(grp)], cnt = gstart[static_cast<size_t>(grp) + 1] - p0;
-        for (int g = 0; g < cnt; ++g) {
+        auto build_pass = [&](int g) {
             const int p = p0 + g;
             // pass g of the group on build lane g mod lanes
             const int lane = g % lanes;
@@ -1037,12 +1054,11 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             int* tab = S.tabs + static_cast<long long>(g) * S.tab_ints;
             const int* perm = d_perm + static_cast<long long>(p) * n;
             const int* rk = d_rank + static_cast<long long>(p) * n;
-            scheck(launch_permute_rows(d_shift0, tab, plan.nb, n, 2 * kVel, perm, bs), "permute tables");
-            scheck(launch_permute_rows(d_box0, tab + n_shift, plan.nb, n, 4, perm, bs), "permute tables");
-            scheck(launch_permute_rows(d_off0, tab + n_shift + n_box, plan.nb, n, kVel, perm, bs), "permute tables");
-            if (n_lat > 0)
-                scheck(launch_permute_rows(d_lat0, tab + n_shift + n_box + n_off, plan.nb_lat, n, 2, perm, bs),
-                       "permute tables");
+            const PermuteTable pt[4] = {{d_shift0, tab, plan.nb, 2 * kVel},
+                                        {d_box0, tab + n_shift, plan.nb, 4},
+                                        {d_off0, tab + n_shift + n_box, plan.nb, kVel},
+                                        {d_lat0, tab + n_shift + n_box + n_off, plan.nb_lat, 2}};
+            scheck(launch_permute_rows(pt, n_lat > 0 ? 4 : 3, n, perm, bs), "permute tables");
             float* slot = S.filt + static_cast<long long>(g) * n * ffloats;
             const long long fs = ffloats * 4 / ebytes;  // frame stride in elements
             // the real pass also leaves the static sky (the same for every frame order)
@@ -1075,6 +1091,36 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
                 stg.build_ranked(sub_, plane, rk, prm, slot, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
             }
+        };
+        // Each build lane's passes are enqueued by a host thread of its own: a stage build
+        // is some fifty small launches, and one thread enqueueing every lane's builds in
+        // turn kept the later lanes waiting for the host. The lanes use disjoint stage
+        // scratch, streams and pass slots; the device is the caller's.
+        const int used = std::min(lanes, cnt);
+        if (used > 1) {
+            int device = 0;
+            scheck(hipGetDevice(&device), "current device");
+            std::vector<std::exception_ptr> errs(static_cast<size_t>(used));
+            std::vector<std::thread> pool;
+            for (int l = 1; l < used; ++l)
+                pool.emplace_back([&, l] {
+                    try {
+                        scheck(hipSetDevice(device), "set device");
+                        for (int g = l; g < cnt; g += lanes) build_pass(g);
+                    } catch (...) {
+                        errs[static_cast<size_t>(l)] = std::current_exception();
+                    }
+                });
+            try {
+                for (int g = 0; g < cnt; g += lanes) build_pass(g);
+            } catch (...) {
+                errs[0] = std::current_exception();
+            }
+            for (std::thread& th : pool) th.join();
+            for (const std::exception_ptr& e : errs)
+                if (e) std::rethrow_exception(e);
+        } else {
+            for (int g = 0; g < cnt; ++g) build_pass(g);
         }
         scheck(launch_fill_keys(S.keys, cnt * plane, build_stream_), "clear keys");
         scheck(hipMemsetAsync(S.counts, 0, sizeof(int) * cnt, build_stream_), "clear peak counts");
@@ -1094,8 +1140,11 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         }
         if (shape.queue) {
             // the group in one persistent launch: every kind of every pass, work items
-            // from the set's counters
-            scheck(launch_stack_queue(hargs[q].data(), dargs, cnt, G, S.queue_ws, s), "launch stack kernel");
+            // from the set's counters (or, under a server, perhaps with other jobs' passes)
+            if (fuser_)
+                fuser_->stack(hargs[q].data(), dargs, cnt, G, S.queue_ws, s, w, h);
+            else
+                scheck(launch_stack_queue(hargs[q].data(), dargs, cnt, G, S.queue_ws, s), "launch stack kernel");
         } else {
             // ring tiles on their own stream, beside the interior launch
             scheck(hipEventRecord(ev_ring_in_[q], s), "record event");
@@ -1128,26 +1177,38 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         PassSet& S = sets_[q];
         const int p0 = gstart[static_cast<size_t>(grp)], cnt = gstart[static_cast<size_t>(grp) + 1] - p0;
         scheck(hipEventSynchronize(ev_counts_[q]), "peak counts");
-        std::vector<std::vector<int>> hv(static_cast<size_t>(cnt));
+        // the group's peak lists into pinned host memory (DMA copies, no host staging)
+        std::vector<size_t> at(static_cast<size_t>(cnt) + 1, 0);
         for (int g = 0; g < cnt; ++g) {
             const int c = S.h_counts[g];
             if (c < 0 || c > S.pk_cap) throw std::runtime_error("too many local maxima for the peak buffer");
-            auto& v = hv[static_cast<size_t>(g)];
-            v.resize(static_cast<size_t>(4) * c);
+            at[static_cast<size_t>(g) + 1] = at[static_cast<size_t>(g)] + static_cast<size_t>(4) * c;
+        }
+        if (at.back() > pk_host_cap_) {
+            if (pk_host_) scheck(hipHostFree(pk_host_), "free pinned peaks");
+            pk_host_ = nullptr;
+            pk_host_cap_ = 0;
+            const size_t cap = at.back() + at.back() / 2;
+            scheck(hipHostMalloc(reinterpret_cast<void**>(&pk_host_), sizeof(int) * cap, hipHostMallocDefault),
+                   "allocate pinned peaks");
+            pk_host_cap_ = cap;
+        }
+        for (int g = 0; g < cnt; ++g) {
+            const int c = S.h_counts[g];
+            int* v = pk_host_ + at[static_cast<size_t>(g)];
             const int* px = S.pk + static_cast<long long>(g) * 4 * S.pk_cap;
             if (c > 0)
                 for (int a = 0; a < 4; ++a)
-                    scheck(hipMemcpyAsync(v.data() + static_cast<size_t>(a) * c, px + a * S.pk_cap, sizeof(int) * c,
+                    scheck(hipMemcpyAsync(v + static_cast<size_t>(a) * c, px + a * S.pk_cap, sizeof(int) * c,
                                           hipMemcpyDeviceToHost, copy_stream_),
                            "download peaks");
         }
         scheck(hipStreamSynchronize(copy_stream_), "download peaks");
         for (int g = 0; g < cnt; ++g) {
             const int p = p0 + g, c = S.h_counts[g];
-            const auto& v = hv[static_cast<size_t>(g)];
+            const int* v = pk_host_ + at[static_cast<size_t>(g)];
             results[static_cast<size_t>(p)] =
-                order_peaks(c, v.data(), v.data() + c, reinterpret_cast<const float*>(v.data() + 2 * c), v.data() + 3 * c,
-                            p == 0 ? 100000 : 20000);
+                order_peaks(c, v, v + c, reinterpret_cast<const float*>(v + 2 * c), v + 3 * c, p == 0 ? 100000 : 20000);
         }
         if (!screen) return;
         // the group's null runs through the artefact tests on their frames in this set,

@@ -13,13 +13,21 @@
 #include <vector>
 
 #include "prep.hpp"
+#include "stack_plan.hpp"
 
 namespace ftk {
 
 hipError_t launch_peaks(const float* m, const int* best_v, int w, int h, float floor, int radius, int edge, int cap,
                         int* count, int* px, int* py, float* ps, int* pv, hipStream_t s);
 hipError_t launch_tail_sums(const float* pooled_desc, int m, double runs, double* partial, int* bad, hipStream_t s);
-hipError_t launch_permute_rows(const int* src, int* dst, int nb, int n, int row, const int* perm, hipStream_t s);
+// A per-frame plan table to permute: nb blocks of n frame rows of `row` ints.
+struct PermuteTable {
+    const int* src;
+    int* dst;
+    int nb, row;
+};
+// Row i of every block of each table's dst = row perm[i] of its src (up to 4 tables, one launch).
+hipError_t launch_permute_rows(const PermuteTable* tables, int count, int n, const int* perm, hipStream_t s);
 
 // SearchParams fields on the path (include/fainttrail/search.hpp:18-90).
 struct SearchConfig {
@@ -50,6 +58,9 @@ struct SearchConfig {
     // 16-row tiles only; -1 = tuned, else automatic, see resolve_shape).
     int tile_rows = 0;
     int queue = -1;
+    // Stage builds of a group in parallel on this many streams (1-4; 0 = by frame size:
+    // 4 up to 16 M samples per pass, 2 up to 64 M, else 1).
+    int build_lanes = 0;
     // Frame storage of the stacking passes: 0 fp32 (the exact form), 1 fp16, 2 bf16
     // (a power-of-two scale per frame, fp32 accumulation).
     int storage = 0;
@@ -154,6 +165,11 @@ public:
     // asked of hipMalloc; buffers only grow between runs).
     double device_bytes() const;
 
+    // Persistent stacking launches of the batched form go through this fuser (the
+    // search server's), which may stack them together with other jobs' (null: launched
+    // directly).
+    void set_fuser(class StackFuser* f) { fuser_ = f; }
+
     // Times the batched run over candidate group sizes and segment counts, then over
     // the launch forms and tile heights, on these frames and keeps the fastest
     // (tuned_passes, tuned_segments, tuned_tile_rows, tuned_queue). Returns the seconds
@@ -244,10 +260,21 @@ private:
     size_t tab_bytes_ = 0;
     hipEvent_t ev_tables_ = nullptr, ev_upload_ = nullptr;
     bool tables_used_ = false;
+    // the batched form's host plan of the last run, reused while its inputs (frame times,
+    // velocity grid, coverage count) are the same: planning is the run's largest host cost
+    // at small frames
+    std::unique_ptr<StackPlan> plan_cache_;
+    std::vector<int32_t> plan_key_dx_, plan_key_dy_;
+    int plan_key_mc_ = -1;
+    // pinned host staging of a group's peak lists (x, y, snr, vidx of each pass)
+    int* pk_host_ = nullptr;
+    size_t pk_host_cap_ = 0;
+    class StackFuser* fuser_ = nullptr;
 };
 
-// Build lanes of the batched form for n frames of `plane` pixels and `passes` passes.
-int build_lanes(int n, long long plane, int passes);
+// Build lanes of the batched form for n frames of `plane` pixels and `passes` passes
+// (asked > 0: that many, at most 4 and at most the passes).
+int build_lanes(int n, long long plane, int passes, int asked = 0);
 
 // Host pieces of the loop, restated from the reference.
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv);

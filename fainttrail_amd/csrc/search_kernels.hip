@@ -75,23 +75,50 @@ __global__ void tail_sums_kernel(const float* __restrict__ pooled_desc, int m, d
 // frame i taking the shifts of frame perm[i], so its plan is the real plan with these
 // rows permuted (the blocks, coverage tables and bounds depend only on the set of
 // shifts per motion, which the permutation does not change).
-__global__ void permute_rows_kernel(const int* __restrict__ src, int* __restrict__ dst, long long total, int n,
-                                    int row, const int* __restrict__ perm) {
-    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
-    for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < total; q += stride) {
-        const long long r = q / row;
-        const int e = static_cast<int>(q - r * row);
-        const long long b = r / n;
-        const int i = static_cast<int>(r - b * n);
-        dst[q] = src[(b * n + perm[i]) * row + e];
+// Up to four per-frame tables permuted in one launch: table t has nb[t] blocks of n
+// frame rows of row[t] ints; row i of block b of dst is row perm[i] of block b of src.
+// 32-bit indices (every plan table here is far below 2^31 ints).
+struct PermuteTables {
+    const int* src[4];
+    int* dst[4];
+    int nb[4], row[4];
+    int total[4];  // nb * n * row
+    int count;
+};
+
+__global__ void permute_rows_kernel(PermuteTables t, int n, const int* __restrict__ perm) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = 0; k < t.count; ++k) {
+        const int row = t.row[k];
+        const int* __restrict__ src = t.src[k];
+        int* __restrict__ dst = t.dst[k];
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < t.total[k]; q += stride) {
+            const int r = q / row, e = q - r * row;
+            const int b = r / n, i = r - b * n;
+            dst[q] = src[(b * n + perm[i]) * row + e];
+        }
     }
 }
 
-hipError_t launch_permute_rows(const int* src, int* dst, int nb, int n, int row, const int* perm, hipStream_t s) {
-    const long long total = static_cast<long long>(nb) * n * row;
-    if (total <= 0) return hipSuccess;
-    const unsigned blocks = static_cast<unsigned>(std::min<long long>((total + 255) / 256, 4096));
-    hipLaunchKernelGGL(permute_rows_kernel, dim3(blocks), dim3(256), 0, s, src, dst, total, n, row, perm);
+hipError_t launch_permute_rows(const PermuteTable* tables, int count, int n, const int* perm, hipStream_t s) {
+    PermuteTables t{};
+    long long most = 0;
+    t.count = 0;
+    for (int k = 0; k < count && k < 4; ++k) {
+        const long long total = static_cast<long long>(tables[k].nb) * n * tables[k].row;
+        if (total <= 0) continue;
+        if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+        const int j = t.count++;
+        t.src[j] = tables[k].src;
+        t.dst[j] = tables[k].dst;
+        t.nb[j] = tables[k].nb;
+        t.row[j] = tables[k].row;
+        t.total[j] = static_cast<int>(total);
+        most = std::max(most, total);
+    }
+    if (t.count == 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>(std::min<long long>((most + 255) / 256, 4096));
+    hipLaunchKernelGGL(permute_rows_kernel, dim3(blocks), dim3(256), 0, s, t, n, perm);
     return hipGetLastError();
 }
 

@@ -98,6 +98,8 @@ long long SearchServer::submit(const float* raw, long long stride, int n, int w,
         if (j->mjd[static_cast<size_t>(i)] < j->mjd[static_cast<size_t>(i) - 1])
             throw std::runtime_error("server: frames must be sorted by time");
     j->cfg = cfg_in;
+    // the persistent stacking launch (fusable with other jobs') unless the job asks
+    if (j->cfg.queue < 0) j->cfg.queue = 1;
     const double span = (j->mjd.back() - j->mjd.front()) * 24.0;
     double dv = 0.0;
     j->nv = static_cast<long long>(velocity_grid(span, j->cfg.vmin_pxh, j->cfg.vmax_pxh, j->cfg.grid_tol_px, &dv).size());
@@ -187,8 +189,15 @@ void SearchServer::worker() {
         std::string error;
         SearchOutput out;
         double held_now = 0.0;
+        bool fuse = false;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fuse = fuse_;
+        }
+        if (fuse) fuser_.enter(j->w, j->h);
         try {
             if (!ds) ds = std::make_unique<DeviceSearch>(j->n, j->w, j->h);
+            ds->set_fuser(fuse ? &fuser_ : nullptr);
             out = ds->run(j->raw, j->stride, j->mjd.data(), j->cfg, nullptr, nullptr, stream);
             held_now = ds->device_bytes();
         } catch (const std::exception& e) {
@@ -197,6 +206,7 @@ void SearchServer::worker() {
             ds.reset();  // a searcher that failed is not reused
             (void)hipStreamSynchronize(stream);
         }
+        if (fuse) fuser_.leave(j->w, j->h);
         lk.lock();
         j->out = std::move(out);
         j->error = error;
@@ -259,7 +269,17 @@ SearchServer::Stats SearchServer::stats() {
     s.failed = failed_;
     s.evicted = evicted_;
     s.max_running = max_running_;
+    const StackFuser::Stats f = fuser_.stats();
+    s.fused_launches = f.fused_launches;
+    s.fused_passes = f.fused_passes;
+    s.stack_launches = f.launches;
     return s;
+}
+
+void SearchServer::set_fusion(bool on, double window_s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    fuse_ = on;
+    fuser_.set_window(window_s);
 }
 
 }  // namespace ftk

@@ -13,8 +13,9 @@
 // holds if that is more; when the job ends the searcher goes back to the idle list at
 // the bytes it actually holds (DeviceSearch::device_bytes). Each slot runs its job's
 // batched search on its own streams, so the stage builds, stacking launches and peak
-// transfers of different jobs overlap on the device. A job's result depends on its
-// inputs only: the same alone and in any mix.
+// transfers of different jobs overlap on the device, and jobs of the same frame size
+// whose stacking launches meet are stacked in one persistent launch (fuse.hpp). A job's
+// result depends on its inputs only: the same alone and in any mix.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +29,7 @@
 #include <thread>
 #include <vector>
 
+#include "fuse.hpp"
 #include "search_device.hpp"
 
 namespace ftk {
@@ -73,7 +75,10 @@ public:
     struct Stats {
         double used = 0, peak = 0, budget = 0, idle_bytes = 0;
         int running = 0, queued = 0, completed = 0, failed = 0, evicted = 0, max_running = 0;
+        long long fused_launches = 0, fused_passes = 0, stack_launches = 0;
     };
+    // Cross-job fusion of persistent stacking launches (on by default, 3 ms window).
+    void set_fusion(bool on, double window_s);
     Stats stats();
     double now() const;
 
@@ -101,6 +106,8 @@ private:
     long long next_id_ = 1;
     bool stop_ = false;
     double t0_ = 0.0;
+    StackFuser fuser_{0.003};
+    bool fuse_ = true;
     std::vector<std::thread> workers_;
 };
 

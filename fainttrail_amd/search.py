@@ -28,7 +28,7 @@ class _Params(C.Structure):
                 ("null_tested", C.c_int), ("max_frame_fraction", C.c_double),
                 ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double)),
                 ("candidates", C.c_int), ("half_tolerance", C.c_double), ("tile_rows", C.c_int),
-                ("queue", C.c_int)]
+                ("queue", C.c_int), ("build_lanes", C.c_int)]
 
 
 class _Summary(C.Structure):
@@ -135,6 +135,8 @@ class SearchParams:
     # one workgroup per work item, 16-row tiles; -1 = tuned, else automatic)
     tile_rows: int = 0
     queue: int = -1
+    # stage builds of a group in parallel on this many streams (1-4; 0 = by frame size)
+    build_lanes: int = 0
 
     def c(self) -> _Params:
         from . import STORAGES
@@ -149,7 +151,7 @@ class SearchParams:
                     -1 if self.fit_on_device is None else int(bool(self.fit_on_device)),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
                     self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance,
-                    self.tile_rows, int(self.queue))
+                    self.tile_rows, int(self.queue), int(self.build_lanes))
         if self.reg_dx is not None:
             rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
             ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)

@@ -37,6 +37,7 @@ def _bind():
     L.ftk_server_candidates.argtypes = [vp, ll, C.c_int, C.POINTER(_Candidate)]
     L.ftk_server_release.argtypes = [vp, ll]
     L.ftk_server_stats.argtypes = [vp, C.POINTER(C.c_double)]
+    L.ftk_server_fusion.argtypes = [vp, C.c_int, C.c_double]
     _bound = True
     return L
 
@@ -122,11 +123,16 @@ class Server:
             lib().ftk_server_release(self._h, job)
 
     def stats(self) -> dict:
-        v = (C.c_double * 10)()
+        v = (C.c_double * 13)()
         lib().ftk_server_stats(self._h, v)
         keys = ["reserved_bytes", "peak_reserved_bytes", "budget_bytes", "idle_searcher_bytes", "running", "queued",
-                "completed", "failed", "evicted", "max_running"]
+                "completed", "failed", "evicted", "max_running", "stack_launches", "fused_launches", "fused_passes"]
         return {k: (int(x) if i >= 4 else float(x)) for i, (k, x) in enumerate(zip(keys, v))}
+
+    def fusion(self, on: bool = True, window_ms: float = 3.0) -> None:
+        """Cross-job fusion of persistent stacking launches (ftk_server_fusion)."""
+        if lib().ftk_server_fusion(self._h, int(bool(on)), float(window_ms)) != 0:
+            raise FtkError("ftk_server_fusion: window_ms must be >= 0")
 
     def close(self):
         h = getattr(self, "_h", None)

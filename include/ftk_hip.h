@@ -265,6 +265,9 @@ typedef struct ftk_search_params {
      * default: tuned, else automatic as the plan options' queue -1). */
     int tile_rows;
     int queue;
+    /* Stage builds of a group of passes run in parallel on this many streams, each fed
+     * by a host thread of its own (1-4; 0 = by frame size). No result bit depends on it. */
+    int build_lanes;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -387,9 +390,15 @@ int ftk_server_peaks(ftk_server* server, long long job, int which, int cap, int3
 int ftk_server_candidates(ftk_server* server, long long job, int cap, ftk_candidate* out);
 /* Forgets an ended job. */
 void ftk_server_release(ftk_server* server, long long job);
-/* stats[10]: reserved bytes, peak reserved bytes, budget, idle-searcher bytes, running,
- * queued, completed, failed, evicted searchers, most jobs running at once. */
+/* stats[13]: reserved bytes, peak reserved bytes, budget, idle-searcher bytes, running,
+ * queued, completed, failed, evicted searchers, most jobs running at once, persistent
+ * stacking launches, of those fused across jobs, passes stacked in fused launches. */
 int ftk_server_stats(ftk_server* server, double* stats);
+/* Cross-job fusion (default on, 3 ms): jobs of the same frame size whose persistent
+ * stacking launches have the same work shape and meet within window_ms are stacked in
+ * one launch (each job's result is unchanged). Jobs use the persistent launch unless
+ * their params set queue = 0. */
+int ftk_server_fusion(ftk_server* server, int on, double window_ms);
 
 /* Plain device copy (n floats) on `stream`: the known-size kernel that calibrates
  * HBM traffic counters. */

@@ -2,8 +2,8 @@
 grids submitted together or while others run overlap on the device, and each job's
 result (peaks, selected candidates, pooled null peaks in run order, threshold) is
 bit-identical to the same search run alone; a tight memory budget serialises
-admission without changing any result; a small job overtakes a large one that does not
-fit yet; a reused searcher is reserved at the bytes it holds; a job larger than the
+admission without changing any result; jobs of the same frame size are stacked in
+shared launches; a small job overtakes a large one that does not fit yet; a reused searcher is reserved at the bytes it holds; a job larger than the
 budget is refused at submission."""
 import numpy as np
 import pytest
@@ -67,6 +67,39 @@ def test_jobs_alone_and_in_the_mix(hip):
     assert st["completed"] == len(jobs) and st["failed"] == 0
     assert 1 <= st["max_running"] <= 3
     assert st["peak_reserved_bytes"] <= st["budget_bytes"]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_same_geometry_jobs_fused_in_one_stacking_launch(hip, storage):
+    """Jobs of one frame size and velocity grid (different frames, seeds and null-run
+    counts) submitted together: their persistent stacking launches meet and are stacked
+    in shared launches (the server's fused-launch count), and every job's result is
+    still the same as run alone."""
+    import torch
+    from fainttrail_amd import synth
+    from fainttrail_amd.search import SearchParams, Searcher
+    from fainttrail_amd.serve import Server
+    jobs = []
+    for seed, nulls in ((11, 6), (12, 5), (13, 7), (14, 6)):
+        fr, mjd, _ = synth.make_sequence(16, 256, 192, nstars=30, nmovers=2, mover_snr=(4.0, 8.0), seed=seed,
+                                         device="cuda:0")
+        prm = SearchParams(vmin_pxh=10.0, vmax_pxh=60.0, grid_tol_px=0.5, fwhm_px=3.0, null_runs=nulls, seed=seed,
+                           passes_per_launch=3, queue=1, storage=storage)
+        jobs.append((fr.contiguous(), mjd, prm))
+    alone = [Searcher(16, 256, 192).run(fr, mjd, prm) for fr, mjd, prm in jobs]
+    torch.cuda.synchronize()
+    srv = Server(slots=4)
+    srv.fusion(True, window_ms=20.0)
+    ids = [srv.submit(fr, mjd, prm) for fr, mjd, prm in jobs]
+    got = [srv.wait(j) for j in ids]
+    st = srv.stats()
+    print(f"fusion {storage}: server stats {st}", flush=True)
+    srv.close()
+    for a, b in zip(alone, got):
+        _same(a, b)
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    assert st["fused_launches"] >= 1 and st["fused_passes"] >= 2
+    assert st["stack_launches"] >= st["fused_launches"]
 
 
 def test_tight_budget_serialises_without_changing_results(hip):
