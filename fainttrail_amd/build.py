@@ -80,7 +80,7 @@ def build_backend(jobs: int | None = None) -> Path:
 def build_oracle(with_reference: bool = True) -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8"], ROOT)
     if with_reference and Path("/root/reference/src").is_dir():
-        _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8", "ref", "cli"], ROOT)
+        _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j8", "ref", "cli", "cli-hip"], ROOT)
 
 
 def build_all() -> None:

@@ -12,8 +12,12 @@
 //   ft_search() with ft_params.backend = "hip"
 // run unchanged. Options come from FAINTTRAIL_HIP_OPTIONS ("device=0,max_mb=2,...",
 // the same keys as "cuda:...", src/cuda_plan.cpp:14-33), since registered names are
-// matched whole. Making "auto" prefer it is a two-line change to make_backend and
-// backend_names (src/backend.cpp:46-87).
+// matched whole.
+//
+// Built with FAINTTRAIL_HAVE_HIP (the reference's backend.cpp then knows the backend as
+// it knows CUDA: "hip" and "hip:key=value,..." names, "auto" preferring it, listed
+// first by backend_names; see oracle/hip_integration.sh for the edits), it does not
+// register itself and exports hip_backend_available() and make_hip_backend() instead.
 #include <cstdlib>
 #include <memory>
 #include <stdexcept>
@@ -92,6 +96,7 @@ private:
     std::string desc_;
 };
 
+#ifndef FAINTTRAIL_HAVE_HIP
 struct Registration {
     Registration() {
         register_backend("hip", [] {
@@ -101,7 +106,23 @@ struct Registration {
     }
 };
 const Registration registration;
+#endif
 
 }  // namespace
+
+// The built-in form's entry points (the counterparts of cuda_device_count and
+// make_cuda_backend, src/backend_cuda.cu:198-209, 80-97).
+bool hip_backend_available(std::string* why) {
+    char buf[512] = {0};
+    const bool ok = ftk_device_count(buf, sizeof buf) > 0;
+    if (!ok && why) *why = buf;
+    return ok;
+}
+
+std::unique_ptr<SearchBackend> make_hip_backend(const std::string& options) {
+    // the options after "hip:", else FAINTTRAIL_HIP_OPTIONS
+    const char* env = std::getenv("FAINTTRAIL_HIP_OPTIONS");
+    return std::unique_ptr<SearchBackend>(new HipBackend(!options.empty() ? options : (env ? env : "")));
+}
 
 }  // namespace fainttrail::detail

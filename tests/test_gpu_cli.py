@@ -73,6 +73,41 @@ def test_search_csv_and_map_identical(hip, sequence, name, extra):
     assert gpu[1] == cpu[1], f"{name}: max-S/N map differs between --backend hip and --backend cpu"
 
 
+CLI_HIP = ROOT / "oracle" / "_ref" / "fainttrail-hip"
+
+
+def _run_hip_cli(args, tmp, timeout=300):
+    env = dict(os.environ)
+    env.pop("FAINTTRAIL_HIP_OPTIONS", None)
+    r = subprocess.run([str(CLI_HIP)] + [str(a) for a in args], cwd=tmp, env=env, capture_output=True, text=True,
+                       timeout=timeout)
+    assert r.returncode == 0, f"fainttrail-hip {' '.join(map(str, args))} failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+    return r.stdout
+
+
+@pytest.mark.skipif(not CLI_HIP.exists(), reason="reference CLI with the built-in hip backend not built (make -C oracle cli-hip)")
+def test_builtin_hip_backend_auto_options_and_four_checks(hip, sequence):
+    """The reference built with the "hip" backend in, as it builds CUDA in
+    (oracle/hip_integration.sh: FAINTTRAIL_HAVE_HIP): `backends` lists hip first, "auto"
+    picks it, "hip:key=value" options reach it, and check-backend runs all four checks,
+    the row-band one included, bit-identical; a search with --backend auto writes the
+    CPU backend's candidates CSV and map."""
+    out = _run_hip_cli(["backends"], sequence)
+    print(out, flush=True)
+    assert out.split()[0] == "hip"
+    for name in ("auto", "hip", "hip:max_mb=64,launch_adds=1e9"):
+        out = _run_hip_cli(["check-backend", "--backend", name], sequence)
+        print(out, flush=True)
+        assert "checking backend 'hip'" in out
+        assert "bit-identical to the reference in 4 of 4 checks" in out
+    csv_cpu, csv_auto = sequence / "auto_cpu.csv", sequence / "auto_auto.csv"
+    for backend, csv in (("cpu", csv_cpu), ("auto", csv_auto)):
+        _run_hip_cli(["search", sequence / "seq", "--backend", backend, "-o", csv, "--snr-map",
+                      sequence / f"auto_{backend}_snr.fits", "--quiet", "--photometry", "off"], sequence)
+    assert csv_auto.read_bytes() == csv_cpu.read_bytes()
+    assert (sequence / "auto_auto_snr.fits").read_bytes() == (sequence / "auto_cpu_snr.fits").read_bytes()
+
+
 def test_reference_regression_cases_strict(hip, tmp_path):
     """The reference's real-data regression suite (three registered sequences: Ceres,
     the Leo triplet with (21996) 1999 XP31, (12631) Mariekebaan; fixtures copied by
