@@ -205,6 +205,8 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
     const __attribute__((address_space(4))) long long* const c_covoff =
         (const __attribute__((address_space(4))) long long*)(cov_off);
     const int lane_base = wave * RPW * kPitch + lane;
+    // the tile origin in a frame (rows from row0, the first row present)
+    const long long tile_org = static_cast<long long>(y0 - row0) * pitch + x0;
 
     float best[RPW];
     int arg[RPW];
@@ -237,8 +239,9 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
         // is not interior to this tile zeroes its box first and then loads only the lanes
         // whose pixel lies in the frame rows present and in [0, w). Either way the zeros
         // stand for frames that do not cover.
-        auto stage = [&](int i, float* dst) {
-            cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
+        // bx: the frame's box (x and y offset from the tile, extra columns, extra rows),
+        // ft: the frame at the tile origin (frame i + (y0 - row0) pitch + x0)
+        auto stage = [&](int i, const int (&bx)[4], const float* ft, float* dst) {
             const int sx = bx[2], R = TY + bx[3];
             const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
             if (RING && !PADDED && !interior) {
@@ -251,8 +254,8 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
                 }
                 __builtin_amdgcn_s_waitcnt(0);  // the zeros land before the loads
             }
-            const float* f = static_cast<const float*>(a.frames) + static_cast<long long>(i) * frame_stride;
             if (RING && !PADDED && !interior) {
+                const float* f = static_cast<const float*>(a.frames) + static_cast<long long>(i) * frame_stride;
 #pragma unroll
                 for (int m = 0; m < kPreRows; ++m) {
                     const int rr = wave + NW * m;
@@ -275,8 +278,10 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
                 // lane's byte offset in a VGPR (fixed for the frame), the row's byte
                 // offset as a scalar, so a row costs no vector instruction
                 const int b1 = 4 * lane;
-                const int b2 = 4 * (kTileX + min(lane, sx - 1));
-                const float* boxp = f + static_cast<long long>(by0 - row0) * pitch + bx0;
+                // second chunk: the instruction offset (4 kTileX bytes) applies to both the
+                // frame and the LDS address, so both loads of a row share one M0 value
+                const int b2 = 4 * min(lane, sx - 1);
+                const float* boxp = ft + (bx[1] * pitch + bx[0]);
                 const __amdgpu_buffer_rsrc_t rs =
                     __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(boxp), 0, 0x7fffffff, kBufferWord3);
                 if constexpr (RING) {
@@ -287,7 +292,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
 #pragma unroll 1
                     for (int rr = wave; rr < R; rr += NW) {
                         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, b1, row_b, 0, 0);
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d + kTileX, 4, b2, row_b, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, d, 4, b2, row_b, 4 * kTileX, 0);
                         row_b += NW * 4 * pitch;
                         d += NW * kPitch;
                     }
@@ -300,17 +305,17 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
                         if (rr >= R) break;
                         const int row_b = rr * 4 * pitch;
                         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b1, row_b, 0, 0);
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch + kTileX, 4, b2, row_b, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, dst + rr * kPitch, 4, b2, row_b, 4 * kTileX, 0);
                     }
                 }
             }
         };
-        auto consume = [&](const float* L, int i) {
+        // lw: a lattice frame's words (column offsets, row classes; unused otherwise)
+        auto consume = [&](const float* L, int i, const int (&lw)[2]) {
             if constexpr (LAT) {
                 // column a (dx = X[a]) of rows 0 .. 3 + spread, read once; motion (a, b) adds
                 // rows c_b .. c_b + 3 of it
-                cint* lt = c_lat + 2 * (static_cast<long long>(blk) * n + i);
-                const int wx = lt[0], wy = lt[1];
+                const int wx = lw[0], wy = lw[1];
                 constexpr int kRows = 2 * ((kLatSyMax + 1) / 2) + 4;  // row pairs
                 // rows past 3 are read only when the frame's spread needs them; the
                 // unread ones are never added (the plan keeps c_b <= 2 e), so they stay
@@ -354,17 +359,54 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
         };
 
         if constexpr (ST == kStoreF32) {
-            stage(0, lds0);
+            // The scalar loads of a frame's box and lattice words are issued one frame
+            // ahead (while the frame before is staged and consumed), so no frame waits
+            // on its own descriptor; the frame pointer at the tile origin advances by the
+            // frame stride.
+            auto load_box = [&](int i, int (&d)[4]) {
+                cint* bx = c_box + 4 * (static_cast<long long>(blk) * n + i);
+                d[0] = bx[0];
+                d[1] = bx[1];
+                d[2] = bx[2];
+                d[3] = bx[3];
+            };
+            auto load_lat = [&](int i, int (&d)[2]) {
+                if constexpr (LAT) {
+                    cint* lt = c_lat + 2 * (static_cast<long long>(blk) * n + i);
+                    d[0] = lt[0];
+                    d[1] = lt[1];
+                }
+            };
+            const float* ft = frames + tile_org;
+            int bn[4], ln[2] = {0, 0}, lc[2];
+            load_box(0, bn);
+            stage(0, bn, ft, lds0);
+            if (n > 1) load_box(1, bn);
+            load_lat(0, ln);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             for (int i = 0; i < n; i += 2) {
-                if (i + 1 < n) stage(i + 1, lds1);
-                consume(lds0, i);
+                lc[0] = ln[0];
+                lc[1] = ln[1];
+                if (i + 1 < n) {
+                    ft += frame_stride;
+                    stage(i + 1, bn, ft, lds1);
+                    if (i + 2 < n) load_box(i + 2, bn);
+                    load_lat(i + 1, ln);
+                }
+                consume(lds0, i, lc);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
                 if (i + 1 >= n) break;
-                if (i + 2 < n) stage(i + 2, lds0);
-                consume(lds1, i + 1);
+                lc[0] = ln[0];
+                lc[1] = ln[1];
+                if (i + 2 < n) {
+                    ft += frame_stride;
+                    stage(i + 2, bn, ft, lds0);
+                    if (i + 3 < n) load_box(i + 3, bn);
+                    load_lat(i + 2, ln);
+                }
+                consume(lds1, i + 1, lc);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
             }
