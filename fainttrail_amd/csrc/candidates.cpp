@@ -64,7 +64,7 @@ std::vector<SearchCandidate> finish_candidates(const SearchConfig& cfg, const Se
         c.vx_pxh = o[kRefVx];
         c.vy_pxh = o[kRefVy];
         c.snr = o[kRefSnr];
-        c.inv_dist_au = 0.0;  // straight tracks
+        c.inv_dist_au = o[kRefQ];  // 0 on straight tracks
         {
             // centroid precision of a matched-filter fit, with the empirical factor 1.25
             constexpr double kEmpirical = 1.25;
@@ -73,6 +73,15 @@ std::vector<SearchCandidate> finish_candidates(const SearchConfig& cfg, const Se
             for (double t : out.t_h) t2 += t * t;
             t2 /= std::max(1, n);
             c.vel_err_pxh = t2 > 0.0 ? c.pos_err_px / std::sqrt(t2) : 0.0;
+            if (out.dq > 0.0) {
+                // likewise for the inverse distance: the parallax curve takes the place of t
+                double p2 = 0.0;
+                for (int i = 0; i < n; ++i)
+                    p2 += 0.5 * (cfg.par_x[static_cast<size_t>(i)] * cfg.par_x[static_cast<size_t>(i)] +
+                                 cfg.par_y[static_cast<size_t>(i)] * cfg.par_y[static_cast<size_t>(i)]);
+                p2 /= std::max(1, n);
+                c.inv_dist_err_au = p2 > 0.0 ? c.pos_err_px / std::sqrt(p2) : 0.0;
+            }
         }
         c.snr_half1 = o[kRefHalf1];
         c.snr_half2 = o[kRefHalf2];
@@ -105,6 +114,16 @@ std::vector<SearchCandidate> finish_candidates(const SearchConfig& cfg, const Se
         for (size_t i = 0; i < flags.size(); ++i) c.flags += (i ? "," : "") + flags[i];
     }
 
+    // track_pos (src/internal.hpp:107-116) of a candidate's track in frame f
+    auto track_pos = [&](int f, const SearchCandidate& c, double& ox, double& oy) {
+        const double t = out.t_h[static_cast<size_t>(f)];
+        ox = c.x + c.vx_pxh * t;
+        oy = c.y + c.vy_pxh * t;
+        if (c.inv_dist_au != 0.0) {
+            ox += c.inv_dist_au * cfg.par_x[static_cast<size_t>(f)];
+            oy += c.inv_dist_au * cfg.par_y[static_cast<size_t>(f)];
+        }
+    };
     // duplicates and ghosts: against every stronger candidate kept so far
     std::sort(rc.begin(), rc.end(), [](const Refined& a, const Refined& b) { return a.c.snr > b.c.snr; });
     const double r_overlap = 1.5 * cfg.fwhm_px;
@@ -119,9 +138,9 @@ std::vector<SearchCandidate> finish_candidates(const SearchConfig& cfg, const Se
             if (drop[i]) continue;
             int overlap = 0;
             for (int f = 0; f < n; ++f) {
-                const double t = out.t_h[static_cast<size_t>(f)];
-                const double xi = rc[i].c.x + rc[i].c.vx_pxh * t, yi = rc[i].c.y + rc[i].c.vy_pxh * t;
-                const double xj = rc[j].c.x + rc[j].c.vx_pxh * t, yj = rc[j].c.y + rc[j].c.vy_pxh * t;
+                double xi, yi, xj, yj;
+                track_pos(f, rc[i].c, xi, yi);
+                track_pos(f, rc[j].c, xj, yj);
                 const double dx = xi - xj, dy = yi - yj;
                 if (dx * dx + dy * dy < r_overlap * r_overlap) {
                     ++overlap;

@@ -58,6 +58,10 @@ extern "C" void ftk_search_params_default(ftk_search_params* p) {
     p->tile_rows = 0;
     p->queue = c.queue;
     p->build_lanes = 0;
+    p->neo_min_dist_au = c.neo_min_dist_au;
+    p->neo_max_trials = c.neo_max_trials;
+    p->par_x = nullptr;
+    p->par_y = nullptr;
 }
 
 namespace {
@@ -91,12 +95,19 @@ ftk::SearchConfig config_of(const ftk_search_params* p) {
     c.tile_rows = p->tile_rows;
     c.queue = p->queue;
     c.build_lanes = p->build_lanes;
+    c.neo_min_dist_au = p->neo_min_dist_au;
+    c.neo_max_trials = p->neo_max_trials;
     return c;
 }
 
-// The registration offsets belong to the frames: n of each, copied (a queued job may
-// outlive the caller's arrays).
+// The registration offsets and parallax displacements belong to the frames: n of each,
+// copied (a queued job may outlive the caller's arrays).
 void set_offsets(ftk::SearchConfig& c, const ftk_search_params* p, int n) {
+    if (!p->par_x != !p->par_y) throw std::runtime_error("search: give both parallax displacements (par_x, par_y) or none");
+    if (p->par_x) {
+        c.par_x.assign(p->par_x, p->par_x + n);
+        c.par_y.assign(p->par_y, p->par_y + n);
+    }
     if (!p->reg_dx && !p->reg_dy) return;
     if (!p->reg_dx || !p->reg_dy) throw std::runtime_error("search: give both registration offsets (reg_dx, reg_dy) or none");
     c.reg_dx.assign(p->reg_dx, p->reg_dx + n);
@@ -125,6 +136,8 @@ void fill_summary(const ftk::SearchOutput& o, ftk_search_summary* sum) {
     sum->n_candidates = static_cast<int>(o.candidates.size());
     sum->tile_rows = o.tile_rows;
     sum->queue = o.queue;
+    sum->n_distances = static_cast<int>(o.inv_dist.size());
+    sum->dq_inv_au = o.dq;
     for (const auto& t : o.timings) {
         if (t.first == "prepare") sum->t_prepare += t.second;
         else if (t.first == "stack (real)") sum->t_stack_real += t.second;
@@ -157,6 +170,9 @@ void check_run_args(const ftk_searcher* s, const float* d_raw, size_t stride, co
     if (p->null_tested < 0) throw std::runtime_error("ftk_searcher_run: null_tested must be >= 0");
     if (p->candidates != 0 && p->candidates != 1) throw std::runtime_error("ftk_searcher_run: candidates must be 0 or 1");
     if (p->fit_on_device < -1 || p->fit_on_device > 1) throw std::runtime_error("ftk_searcher_run: fit_on_device must be -1, 0 or 1");
+    if (!(p->neo_min_dist_au >= 0.0)) throw std::runtime_error("ftk_searcher_run: neo_min_dist_au must be >= 0");
+    if (p->neo_min_dist_au > 0.0 && (!p->par_x || !p->par_y))
+        throw std::runtime_error("ftk_searcher_run: the near-Earth search needs par_x and par_y for every frame");
 }
 }  // namespace
 

@@ -22,6 +22,7 @@ enum RefineOut {
     kRefMasked,   // of those, frames whose rounded track position is masked
     kRefEdge,     // 1: the field-edge test fires
     kRefDetector, // 1: the detector-fixed test fires
+    kRefQ,        // refined inverse distance (1/AU; 0 on straight tracks)
     kRefOut
 };
 
@@ -49,10 +50,15 @@ struct RefineArgs {
     const double* reg_dy = nullptr;
     const double* kern = nullptr;   // photometry weights, (2 R + 1)^2, row-major
     int R = 0;                      // cutout half size
-    const double* cand = nullptr;   // per candidate: x, y (grid position), vx, vy (grid motion)
+    const double* cand = nullptr;   // per candidate: x, y (grid position), vx, vy, q (grid motion)
     int count = 0;
     RefineLevel lev[2] = {};
     double dv = 0.0;                // velocity grid step (px/h)
+    // near-Earth search: the frames' parallax displacements at 1 AU (n each) and the
+    // inverse-distance step dq (> 0); null and 0 for straight tracks
+    const double* par_x = nullptr;
+    const double* par_y = nullptr;
+    double dq = 0.0;
     int n = 0, w = 0, h = 0;
     int edge_r = 0;                 // ceil(2 fwhm) + 1
     double* out = nullptr;          // count x kRefOut

@@ -76,17 +76,18 @@ __global__ void __launch_bounds__(kRefineThreads) refine_kernel(RefineArgs a) {
     float* const cut = lds + 2 * n;
     __shared__ double rv[kRefineThreads];
     __shared__ int ri[kRefineThreads];
-    __shared__ double sb[20];
+    __shared__ double sb[24];
     __shared__ int counts[2];
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
-    const ScreenFrames F{a.frames, a.fstride, a.w, 0, nullptr, a.t};
-    const ScreenFrames Rs{a.resid, a.fstride, a.w, 0, nullptr, a.t};
+    const ScreenFrames F{a.frames, a.fstride, a.w, 0, nullptr, a.t, a.par_x, a.par_y};
+    const ScreenFrames Rs{a.resid, a.fstride, a.w, 0, nullptr, a.t, a.par_x, a.par_y};
+    const double* c0 = a.cand + 5 * static_cast<long long>(k);
+    double bx = c0[0], by = c0[1], bvx = c0[2], bvy = c0[3], bq = c0[4];
+    // S/N of the track at the current inverse distance bq
     auto S = [&](double x, double y, double vx, double vy, int which) {
-        return track_snr(F, kStoreF32, a.w, a.h, n, a.sqrtc, x, y, vx, vy, which);
+        return track_snr(F, kStoreF32, a.w, a.h, n, a.sqrtc, x, y, vx, vy, which, bq);
     };
-
-    double bx = a.cand[4 * k], by = a.cand[4 * k + 1], bvx = a.cand[4 * k + 2], bvy = a.cand[4 * k + 3];
     if (tid == 0) {
         sb[0] = S(bx, by, bvx, bvy, 0);
         counts[0] = counts[1] = 0;
@@ -128,6 +129,29 @@ __global__ void __launch_bounds__(kRefineThreads) refine_kernel(RefineArgs a) {
             bx = cx + i * L.pstep;
             by = cy + j * L.pstep;
         }
+        if (a.dq > 0.0) {
+            // near-Earth: the inverse distance in steps of dq L.vstep / dv (dq/4, then
+            // dq/8), never negative, in the reference's order (:274-285)
+            if (tid == 0) {
+                const double step = a.dq * L.vstep / a.dv, cq = bq;
+                double bb = best, qb = bq;
+                for (int kk = -2; kk <= 2; ++kk) {
+                    const double qq = cq + kk * step;
+                    if (kk == 0 || qq < 0.0) continue;
+                    const double sv2 = track_snr(F, kStoreF32, a.w, a.h, n, a.sqrtc, bx, by, bvx, bvy, 0, qq);
+                    if (sv2 > bb) {
+                        bb = sv2;
+                        qb = qq;
+                    }
+                }
+                sb[15] = bb;
+                sb[16] = qb;
+            }
+            __syncthreads();
+            best = sb[15];
+            bq = sb[16];
+            __syncthreads();
+        }
     }
 
     // sub-pixel position from a parabola through three samples per axis (:287-296)
@@ -157,12 +181,30 @@ __global__ void __launch_bounds__(kRefineThreads) refine_kernel(RefineArgs a) {
         if (denx < 0) bvx += clampd(0.5 * hh * (s0 - s2) / denx, -hh, hh);
         if (deny < 0) bvy += clampd(0.5 * hh * (t0 - t2) / deny, -hh, hh);
     }
+    if (a.dq > 0.0) {
+        // and the inverse distance (:308-316); std::max(0.0, v) as (0 < v ? v : 0)
+        const double hh = a.dq / 8;
+        __syncthreads();  // every thread has read sb[5..9]
+        if (tid == 0) sb[17] = S(bx, by, bvx, bvy, 0);
+        if (tid == 1) {
+            const double lo = bq - hh;
+            sb[18] = track_snr(F, kStoreF32, a.w, a.h, n, a.sqrtc, bx, by, bvx, bvy, 0, 0.0 < lo ? lo : 0.0);
+        }
+        if (tid == 2) sb[19] = track_snr(F, kStoreF32, a.w, a.h, n, a.sqrtc, bx, by, bvx, bvy, 0, bq + hh);
+        __syncthreads();
+        const double sc = sb[17], s0 = sb[18], s2 = sb[19];
+        const double den = s0 - 2 * sc + s2;
+        if (den < 0 && bq - hh >= 0.0) {
+            const double nq = bq + clampd(0.5 * hh * (s0 - s2) / den, -hh, hh);
+            bq = 0.0 < nq ? nq : 0.0;
+        }
+    }
 
     // the refined track's samples, and the frames where it is inside and masked (:318-368)
     int inside = 0, masked = 0;
     for (int i = tid; i < n; i += kRefineThreads) {
-        const double t = a.t[i];
-        const double px = bx + bvx * t, py = by + bvy * t;
+        double px, py;
+        track_at(a.t, a.par_x, a.par_y, bq, i, bx, by, bvx, bvy, px, py);
         const float v = bilinear(F, kStoreF32, a.w, a.h, i, px, py);
         per_frame[i] = v;
         blank[i] = 0;
@@ -192,16 +234,18 @@ __global__ void __launch_bounds__(kRefineThreads) refine_kernel(RefineArgs a) {
     }
     if (tid == 1) sb[11] = S(bx, by, bvx, bvy, 1);
     if (tid == 2) sb[12] = S(bx, by, bvx, bvy, 2);
-    if (tid == 3 && a.reg_dx) detector_mean(n, a.t, a.reg_dx, a.reg_dy, bx, by, bvx, bvy, sb[13], sb[14]);
-    edge_rows(a.sub, a.fstride, a.sky, a.w, a.h, n, a.t, bx, by, bvx, bvy, a.edge_r, tid, kRefineThreads, blank);
+    if (tid == 3 && a.reg_dx)
+        detector_mean(n, a.t, a.par_x, a.par_y, bq, a.reg_dx, a.reg_dy, bx, by, bvx, bvy, sb[13], sb[14]);
+    edge_rows(a.sub, a.fstride, a.sky, a.w, a.h, n, a.t, a.par_x, a.par_y, bq, bx, by, bvx, bvy, a.edge_r, tid,
+              kRefineThreads, blank);
     const int size = 2 * a.R + 1;
     for (int p = tid; p < size * size; p += kRefineThreads) {
         const int u = p % size, v = p / size;
         float acc = 0.0f;
         int c = 0;
         for (int i = 0; i < n; ++i) {
-            const double t = a.t[i];
-            const double cx = bx + bvx * t, cy = by + bvy * t;
+            double cx, cy;
+            track_at(a.t, a.par_x, a.par_y, bq, i, bx, by, bvx, bvy, cx, cy);
             const float s = bilinear(Rs, kStoreF32, a.w, a.h, i, cx + u - a.R, cy + v - a.R);
             if (!isfinite(s)) continue;
             acc += s;
@@ -245,6 +289,7 @@ __global__ void __launch_bounds__(kRefineThreads) refine_kernel(RefineArgs a) {
         o[kRefMasked] = counts[1];
         o[kRefEdge] = n > 0 && near_blank > lim ? 1.0 : 0.0;
         o[kRefDetector] = det;
+        o[kRefQ] = bq;
     }
     for (int i = tid; i < n; i += kRefineThreads) {
         const float v = per_frame[i];

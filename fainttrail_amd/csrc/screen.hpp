@@ -8,7 +8,8 @@
 namespace ftk {
 
 // One pass's frames: element (x, y) of frame i at frames + i fstride + y pitch + x
-// (fp32, or 16-bit times scale[i]), and the pass's n frame times (hours).
+// (fp32, or 16-bit times scale[i]), the pass's n frame times (hours) and, for a
+// near-Earth search, its n parallax displacements at 1 AU (pixels; else null).
 struct ScreenFrames {
     const void* frames;
     long long fstride;
@@ -16,6 +17,8 @@ struct ScreenFrames {
     int unused;
     const float* scale;
     const double* t;
+    const double* par_x = nullptr;
+    const double* par_y = nullptr;
 };
 
 // One launch screens the peaks of several passes: peak k belongs to passes[pslot[k]].
@@ -31,6 +34,8 @@ struct ScreenArgs {
     const double* reg_dx = nullptr;  // n registration offsets each, or null (no detector test)
     const double* reg_dy = nullptr;
     const double* vel = nullptr;     // (vx, vy) per velocity index
+    int nv = 0;                      // velocities; motion v is velocity v mod nv at inverse distance
+    const double* qs = nullptr;      // qs[v / nv] (null: straight tracks, motion = velocity)
     const double* sqrtc = nullptr;   // sqrt(c) for c = 0 .. n
     const double* steps = nullptr;   // detector-test offsets: -span, -span + 0.5, ... <= span
     int nsteps = 0;

@@ -37,13 +37,17 @@ __global__ void __launch_bounds__(kWave) screen_kernel(ScreenArgs a) {
     const int n = a.n;
     const ScreenFrames f = a.passes[a.pslot[k]];
     const double x = a.px[k], y = a.py[k];
-    const int vi = a.pv[k];
+    // motion j = k nv + iv: velocity iv at inverse distance qs[k] (search.cpp:732-733)
+    const int mj = a.pv[k];
+    const int vi = a.qs ? mj % a.nv : mj;
+    const double q = a.qs ? a.qs[mj / a.nv] : 0.0;
     const double vx = a.vel[2 * vi], vy = a.vel[2 * vi + 1];
 
     // track_snr(which = 0) with the per-frame values (search.cpp:139-159)
     for (int i = lane; i < n; i += kWave) {
-        const double t = f.t[i];
-        per_frame[i] = bilinear(f, a.storage, a.w, a.h, i, x + vx * t, y + vy * t);
+        double tx, ty;
+        track_at(f.t, f.par_x, f.par_y, q, i, x, y, vx, vy, tx, ty);
+        per_frame[i] = bilinear(f, a.storage, a.w, a.h, i, tx, ty);
         blank[i] = 0;
     }
     __syncthreads();
@@ -75,7 +79,8 @@ __global__ void __launch_bounds__(kWave) screen_kernel(ScreenArgs a) {
     // within r of it (the subtracted frame or the static sky undefined); a lane takes
     // one window row of one frame, so its loads are independent of each other
     {
-        edge_rows(a.sub, a.sub_stride, a.sky, a.w, a.h, n, f.t, x, y, vx, vy, a.edge_r, lane, kWave, blank);
+        edge_rows(a.sub, a.sub_stride, a.sky, a.w, a.h, n, f.t, f.par_x, f.par_y, q, x, y, vx, vy, a.edge_r, lane,
+                  kWave, blank);
         __syncthreads();
         int near_blank = 0;
         for (int i = lane; i < n; i += kWave) near_blank += blank[i];
@@ -90,7 +95,7 @@ __global__ void __launch_bounds__(kWave) screen_kernel(ScreenArgs a) {
     // detector_flag: the best S/N of a detector-fixed position on the grid of steps
     // around the track's mean detector position, against 0.8 x the track's S/N
     if (a.reg_dx) {
-        if (lane == 0) detector_mean(n, f.t, a.reg_dx, a.reg_dy, x, y, vx, vy, bcast[2], bcast[3]);
+        if (lane == 0) detector_mean(n, f.t, f.par_x, f.par_y, q, a.reg_dx, a.reg_dy, x, y, vx, vy, bcast[2], bcast[3]);
         __syncthreads();
         double best = detector_best(f, a.storage, a.w, a.h, n, a.reg_dx, a.reg_dy, a.steps, a.nsteps, a.sqrtc,
                                     bcast[2], bcast[3], lane, kWave);

@@ -175,6 +175,55 @@ int build_lanes(int n, long long plane, int passes, int asked) {
     return floats <= 16777216.0 ? 4 : floats <= 67108864.0 ? 2 : 1;
 }
 
+// inverse_distance_grid (src/parallax.cpp:167-185): {0} when the near-Earth search is
+// off or the largest parallax displacement at the nearest distance stays within the
+// grid tolerance (straight tracks suffice); else steps + 1 equal steps from 0 to
+// 1 / neo_min_dist_au, neighbours 2 tol apart at the frame of largest displacement, at
+// most neo_max_trials of them.
+std::vector<double> inverse_distance_grid(const SearchConfig& cfg, int n, double* dq_out) {
+    if (dq_out) *dq_out = 0.0;
+    if (cfg.neo_min_dist_au <= 0.0) return {0.0};
+    if (static_cast<int>(cfg.par_x.size()) != n || static_cast<int>(cfg.par_y.size()) != n)
+        throw std::runtime_error("the near-Earth search needs the parallax model: par_x and par_y for every frame");
+    double amp = 0.0;  // parallax_amplitude_px (src/parallax.cpp:159-163)
+    for (int i = 0; i < n; ++i)
+        amp = std::max(amp, std::hypot(cfg.par_x[static_cast<size_t>(i)], cfg.par_y[static_cast<size_t>(i)]));
+    const double qmax = 1.0 / cfg.neo_min_dist_au;
+    const double tol = std::max(0.05, cfg.grid_tol_px);
+    if (amp * qmax <= tol) return {0.0};
+    int steps = static_cast<int>(std::ceil(amp * qmax / (2.0 * tol)));
+    steps = std::min(steps, std::max(1, cfg.neo_max_trials - 1));
+    std::vector<double> q(static_cast<size_t>(steps) + 1);
+    for (int k = 0; k <= steps; ++k) q[static_cast<size_t>(k)] = qmax * k / steps;
+    if (dq_out) *dq_out = qmax / steps;
+    return q;
+}
+
+// The frame whose own time each of a pass's frame times is: the first frame with that
+// time (with_times, src/preprocess.cpp:503-515, gives a null run's frame i the parallax
+// of that frame). For distinct times this is the pass's frame order.
+static std::vector<int> time_sources(const std::vector<double>& t_h, const std::vector<double>& times) {
+    std::vector<int> src(times.size());
+    for (size_t i = 0; i < times.size(); ++i) {
+        const auto it = std::find(t_h.begin(), t_h.end(), times[i]);
+        if (it == t_h.end()) throw std::runtime_error("null run times are not a permutation of the frame times");
+        src[i] = static_cast<int>(it - t_h.begin());
+    }
+    return src;
+}
+
+// A pass's parallax displacements: those of the frames whose times it takes.
+static void pass_parallax(const SearchConfig& cfg, const std::vector<int>& src, std::vector<double>& px,
+                   std::vector<double>& py) {
+    px.assign(src.size(), 0.0);
+    py.assign(src.size(), 0.0);
+    if (cfg.par_x.empty()) return;
+    for (size_t i = 0; i < src.size(); ++i) {
+        px[i] = cfg.par_x[static_cast<size_t>(src[i])];
+        py[i] = cfg.par_y[static_cast<size_t>(src[i])];
+    }
+}
+
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv_out) {
     if (!(span_h > 0.0)) throw std::runtime_error("invalid time span");
     if (vmax <= 0.0 || vmax < vmin) throw std::runtime_error("invalid velocity range");
@@ -435,16 +484,19 @@ PeakList DeviceSearch::peaks(const float* m, const int* b, float floor, int radi
 // The run's tables of the null screen, in one pinned block and one copy on stream s:
 // velocities by index, sqrt(c) for the S/N normalisation (track_snr, search.cpp:158),
 // the detector test's offsets (search.cpp:217-223: the same accumulation, so the same
-// doubles), the registration offsets and the frame times of every pass.
+// doubles), the registration offsets and the frame times of every pass; for a
+// near-Earth search also every pass's parallax displacements (those of the frames
+// whose times it takes) and the inverse distances.
 void DeviceSearch::screen_tables(const SearchConfig& cfg, const SearchOutput& out,
                                  const std::vector<std::vector<double>>& times, hipStream_t s) {
     std::vector<double> steps;
     const double span = std::max(2.0, 2.0 * cfg.fwhm_px);
     for (double d = -span; d <= span; d += 0.5) steps.push_back(d);
     const bool reg = !cfg.reg_dx.empty();
-    const size_t nv = out.grid.size();
+    const bool curved = out.dq > 0.0;
+    const size_t nv = out.grid.size(), nq = out.inv_dist.size();
     const size_t need = 2 * nv + (n + 1) + steps.size() + (reg ? 2 * static_cast<size_t>(n) : 0) +
-                        times.size() * static_cast<size_t>(n);
+                        times.size() * static_cast<size_t>(n) * (curved ? 3 : 1) + nq;
     if (need > scr_tab_cap_) {
         if (scr_tab_) scheck(hipFree(scr_tab_), "free screen tables");
         if (scr_tab_host_) scheck(hipHostFree(scr_tab_host_), "free screen tables");
@@ -468,9 +520,24 @@ void DeviceSearch::screen_tables(const SearchConfig& cfg, const SearchOutput& ou
         for (double d : cfg.reg_dy) *q++ = d;
     }
     for (const auto& t : times) q = std::copy(t.begin(), t.end(), q);
+    if (curved) {
+        std::vector<double> px, py;
+        for (const auto& t : times) {
+            pass_parallax(cfg, time_sources(out.t_h, t), px, py);
+            q = std::copy(px.begin(), px.end(), q);
+        }
+        for (const auto& t : times) {
+            pass_parallax(cfg, time_sources(out.t_h, t), px, py);
+            q = std::copy(py.begin(), py.end(), q);
+        }
+    }
+    q = std::copy(out.inv_dist.begin(), out.inv_dist.end(), q);
     scheck(hipMemcpyAsync(scr_tab_, scr_tab_host_, sizeof(double) * need, hipMemcpyHostToDevice, s),
            "upload screen tables");
     scr_nv_ = static_cast<int>(nv);
+    scr_nq_ = static_cast<int>(nq);
+    scr_passes_ = static_cast<int>(times.size());
+    scr_curved_ = curved;
     scr_nsteps_ = static_cast<int>(steps.size());
     scr_edge_r_ = static_cast<int>(std::ceil(2.0 * cfg.fwhm_px)) + 1;
     scr_reg_ = reg;
@@ -516,15 +583,20 @@ std::vector<std::vector<unsigned char>> DeviceSearch::screen_passes(const Search
     const double* steps = sqrtc + (n + 1);
     const double* reg = steps + scr_nsteps_;
     const double* times = scr_reg_ ? reg + 2 * static_cast<size_t>(n) : reg;
+    const size_t tn = static_cast<size_t>(scr_passes_) * n;
+    const double* par_x = scr_curved_ ? times + tn : nullptr;
+    const double* par_y = scr_curved_ ? times + 2 * tn : nullptr;
+    const double* qs = times + (scr_curved_ ? 3 : 1) * tn;
     auto* hdesc = reinterpret_cast<ScreenFrames*>(scr_pk_host_);
     int* hint = reinterpret_cast<int*>(scr_pk_host_ + desc);
     for (size_t k = 0; k < sp.size(); ++k) {
-        hdesc[k] = ScreenFrames{sp[k].frames, sp[k].fstride, sp[k].pitch, 0, sp[k].scale,
-                                times + static_cast<size_t>(sp[k].pass) * n};
+        const size_t at = static_cast<size_t>(sp[k].pass) * n;
+        hdesc[k] = ScreenFrames{sp[k].frames, sp[k].fstride, sp[k].pitch, 0, sp[k].scale, times + at,
+                                par_x ? par_x + at : nullptr, par_y ? par_y + at : nullptr};
         const PeakList& pk = *sp[k].peaks;
         for (size_t j = 0; j < off[k + 1] - off[k]; ++j) {
             const int v = pk.vidx[j];
-            if (v < 0 || v >= scr_nv_) throw std::runtime_error("null screen: velocity index out of range");
+            if (v < 0 || v >= scr_nv_ * scr_nq_) throw std::runtime_error("null screen: motion index out of range");
             hint[off[k] + j] = pk.x[j];
             hint[total + off[k] + j] = pk.y[j];
             hint[2 * total + off[k] + j] = v;
@@ -543,6 +615,8 @@ std::vector<std::vector<unsigned char>> DeviceSearch::screen_passes(const Search
     a.reg_dx = scr_reg_ ? reg : nullptr;
     a.reg_dy = scr_reg_ ? reg + n : nullptr;
     a.vel = vel;
+    a.nv = scr_nv_;
+    a.qs = scr_curved_ ? qs : nullptr;
     a.sqrtc = sqrtc;
     a.steps = steps;
     a.nsteps = scr_nsteps_;
@@ -607,17 +681,34 @@ StageParams stage_params(const SearchConfig& cfg) {
 
 int min_count_of(const SearchConfig& cfg, int n) { return std::max(3, static_cast<int>(std::ceil(cfg.min_coverage * n))); }
 
-void shifts_for(const std::vector<std::pair<double, double>>& grid, const std::vector<double>& t, int n,
+// A pass's shift table for its frame times t and parallax displacements (par_x,
+// par_y; read only for q != 0): motion j = k nv + iv, frame i: lround(v t_i + q par_i)
+// (run_pass's curved tables, search.cpp:73-95), which for q = 0 is the linear table
+// lround(v t_i) (linear_shift_table, src/backend.cpp:114-123).
+void shifts_for(const SearchOutput& out, const double* par_x, const double* par_y, const std::vector<double>& t, int n,
                 std::vector<int32_t>& dx, std::vector<int32_t>& dy) {
-    const size_t nv = grid.size();
-    dx.resize(nv * n);
-    dy.resize(nv * n);
-    for (size_t j = 0; j < nv; ++j)
-        for (int i = 0; i < n; ++i) {
-            dx[j * n + i] = static_cast<int32_t>(std::lround(grid[j].first * t[static_cast<size_t>(i)]));
-            dy[j * n + i] = static_cast<int32_t>(std::lround(grid[j].second * t[static_cast<size_t>(i)]));
+    const size_t nv = out.grid.size(), nm = out.motions();
+    dx.resize(nm * n);
+    dy.resize(nm * n);
+    for (size_t k = 0; k < out.inv_dist.size(); ++k) {
+        const double q = out.inv_dist[k];
+        for (size_t iv = 0; iv < nv; ++iv) {
+            const size_t j = k * nv + iv;
+            const auto& v = out.grid[iv];
+            for (int i = 0; i < n; ++i) {
+                const double ti = t[static_cast<size_t>(i)];
+                double ddx = v.first * ti, ddy = v.second * ti;
+                if (q != 0.0) {
+                    ddx += q * par_x[i];
+                    ddy += q * par_y[i];
+                }
+                dx[j * n + i] = static_cast<int32_t>(std::lround(ddx));
+                dy[j * n + i] = static_cast<int32_t>(std::lround(ddy));
+            }
         }
+    }
 }
+
 
 void set_segments(StackPlan& p, int segments) {
     if (segments <= 0) return;
@@ -649,6 +740,9 @@ SearchOutput DeviceSearch::run(const float* raw, long long stride, const double*
     if (out.span_h <= 0.0) throw std::runtime_error("frames must span a non-zero time interval");
     for (int i = 0; i < n; ++i) out.t_h.push_back((mjd[i] - out.t_ref_mjd) * 24.0);
     out.grid = velocity_grid(out.span_h, cfg.vmin_pxh, cfg.vmax_pxh, cfg.grid_tol_px, &out.dv);
+    out.inv_dist = inverse_distance_grid(cfg, n, &out.dq);
+    if (out.motions() > 5000000)
+        throw std::runtime_error("near-Earth search too large; narrow the velocity range or raise --neo");
     try {
         if (cfg.passes_per_launch == 1) return run_sequential(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
         return run_batched(raw, stride, cfg, std::move(out), d_max_snr, d_best_v, s);
@@ -677,7 +771,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
                  sky_, nullptr, cfg.refines() ? rmask_ : nullptr, nullptr, s, cfg.refines() ? res_ : nullptr);
     lap("prepare");
 
-    const int nv = static_cast<int>(out.grid.size());
+    const int nm = static_cast<int>(out.motions());
     const int min_count = min_count_of(cfg, n);
     const bool auto_thr = cfg.threshold_snr <= 0.0;
     const int runs = auto_thr ? std::max(0, cfg.null_runs) : 0;
@@ -694,8 +788,9 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     };
     using PlanPtr = std::unique_ptr<ftk_plan, PlanDel>;
     // one pass with a plan of its own; the plan (and its 16-bit frames) is returned
-    auto pass = [&](const std::vector<double>& t, const float* frames, float* m, int* b) -> PlanPtr {
-        shifts_for(out.grid, t, n, dx, dy);
+    auto pass = [&](const std::vector<double>& t, const double* par_x, const double* par_y, const float* frames,
+                    float* m, int* b) -> PlanPtr {
+        shifts_for(out, par_x, par_y, t, n, dx, dy);
         ftk_options opt;
         opt.device = -1;
         opt.max_mb = 0;
@@ -705,7 +800,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         opt.queue = ask_queue;
         ftk_plan* plan = nullptr;
         char err[512] = {0};
-        if (ftk_plan_create(n, w, h, nv, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
+        if (ftk_plan_create(n, w, h, nm, dx.data(), dy.data(), min_count, &opt, &plan, err, sizeof err) != FTK_OK)
             throw std::runtime_error(err);
         PlanPtr held(plan);
         int st = FTK_OK;
@@ -725,13 +820,13 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
     int* b_real = d_best_v ? d_best_v : best_;
     {
         // the shape the plans resolve (the same rule for every pass of the run)
-        shifts_for(out.grid, out.t_h, n, dx, dy);
-        const StackPlan probe = make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count);
+        shifts_for(out, cfg.par_x.data(), cfg.par_y.data(), out.t_h, n, dx, dy);
+        const StackPlan probe = make_stack_plan(n, w, h, nm, dx.data(), dy.data(), min_count);
         const StackShape sh = resolve_shape(probe, cfg.storage, ask_rows, ask_queue, h);
         out.tile_rows = sh.tile_rows;
         out.queue = sh.queue;
     }
-    pass(out.t_h, real_frames, m_real, b_real);
+    pass(out.t_h, cfg.par_x.data(), cfg.par_y.data(), real_frames, m_real, b_real);
     lap("stack (real)");
     const float floor = static_cast<float>(auto_thr ? cfg.min_snr_floor : cfg.threshold_snr);
     out.peaks = peaks(m_real, b_real, floor, nms, edge, 100000, s);
@@ -746,7 +841,9 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         stage_.build(sub_, static_cast<long long>(plane), t.data(), prm, filt_, static_cast<long long>(plane), nullptr,
                      nullptr, nullptr, nullptr, s);
         lap("null rebuild");
-        const PlanPtr held = pass(t, filt_, mn, bn);
+        std::vector<double> ppx, ppy;
+        pass_parallax(cfg, time_sources(out.t_h, t), ppx, ppy);
+        const PlanPtr held = pass(t, ppx.data(), ppy.data(), filt_, mn, bn);
         lap("stack (null)");
         const PeakList npk = peaks(mn, bn, 3.0f, nms, edge, 20000, s);
         // the artefact tests read the frames the pass stacked: fp32 as built, or the
@@ -763,7 +860,7 @@ SearchOutput DeviceSearch::run_sequential(const float* raw, long long stride, co
         pool_null(npk, art[0], out);
         lap("peaks (null)");
     }
-    finish(cfg, runs, nv, out, s);
+    finish(cfg, runs, nm, out, s);
     lap("threshold fit");
     if (cfg.refines()) {
         refine(cfg, out, s);
@@ -786,7 +883,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     const long long plane = static_cast<long long>(w) * h;
     Lap lap(out, s);
     const StageParams prm = stage_params(cfg);
-    const int nv = static_cast<int>(out.grid.size());
+    const int nm = static_cast<int>(out.motions());
     const int min_count = min_count_of(cfg, n);
     const bool auto_thr = cfg.threshold_snr <= 0.0;
     const int runs = auto_thr ? std::max(0, cfg.null_runs) : 0;
@@ -800,12 +897,12 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
 
     // the plan of the real pass, shared by all passes
     std::vector<int32_t> dx, dy;
-    shifts_for(out.grid, out.t_h, n, dx, dy);
+    shifts_for(out, cfg.par_x.data(), cfg.par_y.data(), out.t_h, n, dx, dy);
     // the host plan: the previous run's when its shift tables and coverage count are the
     // same (the plan depends on nothing else; its tiles are set below), else a new one
     if (!plan_cache_ || plan_key_mc_ != min_count || plan_key_dx_ != dx || plan_key_dy_ != dy) {
         plan_cache_.reset();
-        plan_cache_ = std::make_unique<StackPlan>(make_stack_plan(n, w, h, nv, dx.data(), dy.data(), min_count));
+        plan_cache_ = std::make_unique<StackPlan>(make_stack_plan(n, w, h, nm, dx.data(), dy.data(), min_count));
         plan_key_dx_ = dx;
         plan_key_dy_ = dy;
         plan_key_mc_ = min_count;
@@ -890,14 +987,18 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     perms.emplace_back(static_cast<size_t>(n));
     std::iota(perms[0].begin(), perms[0].end(), 0);
     for (auto& p : null_time_perms(n, runs, cfg.seed)) perms.push_back(std::move(p));
+    // all_perm: per pass, the frame whose table row each frame takes: the first frame
+    // with its time (a null run's frame i has the real pass's shifts of that frame:
+    // the same time and, for curved tracks, the same parallax)
     std::vector<int> all_perm, all_rank(static_cast<size_t>(passes) * n);
-    for (const auto& p : perms) all_perm.insert(all_perm.end(), p.begin(), p.end());
     std::vector<std::vector<double>> ptimes(static_cast<size_t>(passes), std::vector<double>(static_cast<size_t>(n)));
     for (int p = 0; p < passes; ++p) {
         std::vector<double>& t = ptimes[static_cast<size_t>(p)];
         for (int i = 0; i < n; ++i)
             t[static_cast<size_t>(i)] = out.t_h[static_cast<size_t>(perms[static_cast<size_t>(p)][static_cast<size_t>(i)])];
         time_ranks(t.data(), n, all_rank.data() + static_cast<size_t>(p) * n);
+        const std::vector<int> src = time_sources(out.t_h, t);
+        all_perm.insert(all_perm.end(), src.begin(), src.end());
     }
 
     // per-run tables in one pinned block and one copy: the shared tables (the ones that
@@ -1250,7 +1351,7 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
 
     out.peaks = std::move(results[0]);
     for (int p = 1; p < passes; ++p) pool_null(results[static_cast<size_t>(p)], arts[static_cast<size_t>(p)], out);
-    finish(cfg, runs, nv, out, s);
+    finish(cfg, runs, nm, out, s);
     lap("threshold fit");
     if (cfg.refines()) {
         refine(cfg, out, s);
@@ -1261,9 +1362,9 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
 
 // The null-tail threshold, with the FP64 tail-fit sums on the GPU for 16-bit storage
 // (or as asked), and select_peaks (search.cpp:614-620).
-void DeviceSearch::finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s) {
+void DeviceSearch::finish(const SearchConfig& cfg, int runs, int motions, SearchOutput& out, hipStream_t s) {
     const bool gpu_fit = cfg.fit_on_device > 0 || (cfg.fit_on_device < 0 && cfg.storage != kStoreF32);
-    fit_threshold(cfg, runs, nv, gpu_fit, out, s);
+    fit_threshold(cfg, runs, motions, gpu_fit, out, s);
     for (size_t q = 0; q < out.peaks.snr.size(); ++q) {
         if (!(out.peaks.snr[q] >= out.threshold) || out.peaks.vidx[q] < 0) continue;
         if (static_cast<int>(out.selected.x.size()) >= cfg.max_candidates) break;
@@ -1279,12 +1380,12 @@ double DeviceSearch::refit(const SearchConfig& cfg, const SearchOutput& last, bo
     SearchOutput o;
     o.pooled = last.pooled;
     const int runs = cfg.threshold_snr <= 0.0 ? std::max(0, cfg.null_runs) : 0;
-    fit_threshold(cfg, runs, static_cast<int>(last.grid.size()), on_device, o, s);
+    fit_threshold(cfg, runs, static_cast<int>(last.motions()), on_device, o, s);
     if (method) *method = o.method;
     return o.threshold;
 }
 
-void DeviceSearch::fit_threshold(const SearchConfig& cfg, int runs, int nv, bool on_device, SearchOutput& out,
+void DeviceSearch::fit_threshold(const SearchConfig& cfg, int runs, int motions, bool on_device, SearchOutput& out,
                                  hipStream_t s) {
     double sums[10] = {0};
     const double* dsums = nullptr;
@@ -1314,7 +1415,7 @@ void DeviceSearch::fit_threshold(const SearchConfig& cfg, int runs, int nv, bool
             (void)hipFree(bad);
         }
     }
-    set_threshold(cfg, w, h, cfg.fwhm_px, static_cast<size_t>(nv), runs, out.pooled, out, dsums);
+    set_threshold(cfg, w, h, cfg.fwhm_px, static_cast<size_t>(motions), runs, out.pooled, out, dsums);
 }
 
 void DeviceSearch::reserve_real() {
@@ -1340,8 +1441,9 @@ void DeviceSearch::refine(const SearchConfig& cfg, SearchOutput& out, hipStream_
     const double span = std::max(2.0, 2.0 * cfg.fwhm_px);
     for (double d = -span; d <= span; d += 0.5) steps.push_back(d);
     const bool reg = !cfg.reg_dx.empty();
+    const bool curved = out.dq > 0.0;
     const size_t n_tab = static_cast<size_t>(n) + (n + 1) + steps.size() + (reg ? 2 * static_cast<size_t>(n) : 0) +
-                         static_cast<size_t>(size) * size + 4 * nc;
+                         static_cast<size_t>(size) * size + 5 * nc + (curved ? 2 * static_cast<size_t>(n) : 0);
     const size_t n_fig = nc * kRefOut;
     const size_t bytes = 8 * (n_tab + n_fig) + 4 * nc * static_cast<size_t>(n);
     if (bytes > ref_cap_) {
@@ -1367,14 +1469,21 @@ void DeviceSearch::refine(const SearchConfig& cfg, SearchOutput& out, hipStream_
     for (int y = 0; y < size; ++y)
         for (int x = 0; x < size; ++x)
             *q++ = std::exp(-0.5 * ((x - R) * (x - R) + (y - R) * (y - R)) / (s_psf * s_psf));
-    const int nv = static_cast<int>(out.grid.size());
+    // per candidate: grid position, then velocity and inverse distance of motion
+    // j = k nv + iv (search.cpp:732-733)
+    const int nv = static_cast<int>(out.grid.size()), nm = static_cast<int>(out.motions());
     for (size_t k = 0; k < nc; ++k) {
         const int v = out.selected.vidx[k];
-        if (v < 0 || v >= nv) throw std::runtime_error("refine: velocity index out of range");
+        if (v < 0 || v >= nm) throw std::runtime_error("refine: motion index out of range");
         *q++ = out.selected.x[k];
         *q++ = out.selected.y[k];
-        *q++ = out.grid[static_cast<size_t>(v)].first;
-        *q++ = out.grid[static_cast<size_t>(v)].second;
+        *q++ = out.grid[static_cast<size_t>(v % nv)].first;
+        *q++ = out.grid[static_cast<size_t>(v % nv)].second;
+        *q++ = out.inv_dist[static_cast<size_t>(v / nv)];
+    }
+    if (curved) {
+        for (double d : cfg.par_x) *q++ = d;
+        for (double d : cfg.par_y) *q++ = d;
     }
     scheck(hipMemcpyAsync(ref_dev_, ref_host_, 8 * n_tab, hipMemcpyHostToDevice, s), "upload candidate tables");
     const double* d = reinterpret_cast<const double*>(ref_dev_);
@@ -1395,6 +1504,11 @@ void DeviceSearch::refine(const SearchConfig& cfg, SearchOutput& out, hipStream_
     a.R = R;
     a.cand = a.kern + static_cast<size_t>(size) * size;
     a.count = static_cast<int>(nc);
+    if (curved) {
+        a.par_x = a.cand + 5 * nc;
+        a.par_y = a.par_x + n;
+        a.dq = out.dq;
+    }
     // the two local-search levels (search.cpp:258-261)
     const double dv = out.dv;
     const double levels[2][4] = {{dv / 4, dv, 0.5, 1.0}, {dv / 8, dv / 4, 0.25, 0.5}};

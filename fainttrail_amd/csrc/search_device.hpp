@@ -77,6 +77,16 @@ struct SearchConfig {
     // selection; half_tolerance is the halves test's allowed shortfall (sigma).
     int candidates = 1;
     double half_tolerance = 3.0;
+    // Near-Earth search (SearchParams::neo_min_dist_au > 0): curved tracks x + v t + q
+    // par(t) over the inverse distances q of inverse_distance_grid (src/parallax.cpp:
+    // 167-185) times the velocity grid, motion j = k nv + iv for q = Q[k], v = grid[iv]
+    // (search.cpp:727-733). par_x / par_y: each frame's parallax displacement at 1 AU
+    // in pixels, less its least-squares line in t (Prepared::par_x / par_y,
+    // set_parallax_model, src/parallax.cpp:132-156), one per frame; needed when
+    // neo_min_dist_au > 0.
+    double neo_min_dist_au = 0.0;
+    int neo_max_trials = 64;
+    std::vector<double> par_x, par_y;
     // whether a run has a candidate stage (candidates on and room for at least one)
     bool refines() const { return candidates != 0 && max_candidates > 0; }
 };
@@ -107,6 +117,10 @@ struct SearchOutput {
     double dv = 0.0, t_ref_mjd = 0.0, span_h = 0.0;
     std::vector<double> t_h;
     std::vector<std::pair<double, double>> grid;
+    std::vector<double> inv_dist{0.0};  // the inverse distances (1/AU) of the motions; {0}: straight tracks
+    double dq = 0.0;                    // their step (0 for straight tracks)
+    // motions of a pass: every (inverse distance, velocity) pair
+    size_t motions() const { return grid.size() * inv_dist.size(); }
     std::vector<float> pooled;  // null peak S/N values, run after run (artefacts left out)
     int screened = 0, artefacts = 0;  // null peaks through the artefact tests, and rejected
     PeakList peaks;             // real-pass peaks, sorted, at most 100000
@@ -182,8 +196,8 @@ private:
                                 float* d_max_snr, int* d_best_v, hipStream_t s);
     SearchOutput run_batched(const float* raw, long long stride, const SearchConfig& cfg, SearchOutput out,
                              float* d_max_snr, int* d_best_v, hipStream_t s);
-    void finish(const SearchConfig& cfg, int runs, int nv, SearchOutput& out, hipStream_t s);
-    void fit_threshold(const SearchConfig& cfg, int runs, int nv, bool on_device, SearchOutput& out, hipStream_t s);
+    void finish(const SearchConfig& cfg, int runs, int motions, SearchOutput& out, hipStream_t s);
+    void fit_threshold(const SearchConfig& cfg, int runs, int motions, bool on_device, SearchOutput& out, hipStream_t s);
     // The candidate stage: the selected peaks refined on the device against the real
     // pass's frames, residuals and mask, then finished on the host.
     void refine(const SearchConfig& cfg, SearchOutput& out, hipStream_t s);
@@ -235,7 +249,9 @@ private:
     unsigned char* scr_art_host_ = nullptr;
     size_t scr_cap_ = 0, scr_art_cap_ = 0;
     int scr_nv_ = 0, scr_nsteps_ = 0, scr_edge_r_ = 0;
+    int scr_nq_ = 1, scr_passes_ = 0;  // inverse distances; passes whose times (and parallax) the tables hold
     bool scr_reg_ = false;
+    bool scr_curved_ = false;          // near-Earth: per-pass parallax and the inverse distances in the tables
     PassSet sets_[2];
     hipStream_t build_stream_ = nullptr, copy_stream_ = nullptr;
     // more build lanes for small frames (each its own stage scratch, build target and
@@ -278,6 +294,10 @@ int build_lanes(int n, long long plane, int passes, int asked = 0);
 
 // Host pieces of the loop, restated from the reference.
 std::vector<std::pair<double, double>> velocity_grid(double span_h, double vmin, double vmax, double tol, double* dv);
+// inverse_distance_grid (src/parallax.cpp:167-185) for cfg's near-Earth settings and
+// parallax displacements: {0} when the search is off or the curvature is below the
+// grid tolerance; dq (may be null) receives the step (0 for {0}).
+std::vector<double> inverse_distance_grid(const SearchConfig& cfg, int n, double* dq);
 std::vector<std::vector<double>> null_time_orders(const std::vector<double>& t_h, int runs, uint64_t seed);
 // The same shuffles as index permutations: null_time_orders(t, runs, seed)[k][i] ==
 // t[null_time_perms(n, runs, seed)[k][i]].

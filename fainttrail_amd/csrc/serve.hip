@@ -102,7 +102,9 @@ long long SearchServer::submit(const float* raw, long long stride, int n, int w,
     if (j->cfg.queue < 0) j->cfg.queue = 1;
     const double span = (j->mjd.back() - j->mjd.front()) * 24.0;
     double dv = 0.0;
-    j->nv = static_cast<long long>(velocity_grid(span, j->cfg.vmin_pxh, j->cfg.vmax_pxh, j->cfg.grid_tol_px, &dv).size());
+    // motions: every (inverse distance, velocity) pair
+    j->nv = static_cast<long long>(velocity_grid(span, j->cfg.vmin_pxh, j->cfg.vmax_pxh, j->cfg.grid_tol_px, &dv).size()) *
+            static_cast<long long>(inverse_distance_grid(j->cfg, n, nullptr).size());
     j->passes = 1 + (j->cfg.threshold_snr > 0.0 ? 0 : std::max(0, j->cfg.null_runs));
     // group size: as asked, else up to 8 passes per launch, fewer if the budget needs it
     int G = j->cfg.passes_per_launch > 0 ? j->cfg.passes_per_launch : std::min(8, j->passes);

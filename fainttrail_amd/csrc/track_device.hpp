@@ -49,18 +49,34 @@ __device__ inline float bilinear(const ScreenFrames& f, int storage, int w, int 
     return static_cast<float>((1 - fy) * ((1 - fx) * r00 + fx * r01) + fy * ((1 - fx) * r10 + fx * r11));
 }
 
+// track_pos (src/internal.hpp:107-116): frame i's position of the track through (x, y)
+// at t = 0 with motion (vx, vy) and inverse distance q (a curved track when q != 0:
+// plus q times the frame's parallax displacement).
+__device__ inline void track_at(const double* t_h, const double* par_x, const double* par_y, double q, int i,
+                                double x, double y, double vx, double vy, double& ox, double& oy) {
+    const double t = t_h[i];
+    ox = x + vx * t;
+    oy = y + vy * t;
+    if (q != 0.0) {
+        ox += q * par_x[i];
+        oy += q * par_y[i];
+    }
+}
+
 // track_snr without the per-frame values, by one thread: the frames in order (which: 0
 // all, 1 those with t < 0, 2 those with t >= 0), the finite samples summed in double;
 // sqrtc[c] = sqrt(c).
 __device__ inline double track_snr(const ScreenFrames& f, int storage, int w, int h, int n, const double* sqrtc,
-                                   double x, double y, double vx, double vy, int which) {
+                                   double x, double y, double vx, double vy, int which, double q) {
     double sum = 0.0;
     int count = 0;
     for (int i = 0; i < n; ++i) {
         const double t = f.t[i];
         if (which == 1 && t >= 0.0) continue;
         if (which == 2 && t < 0.0) continue;
-        const float v = bilinear(f, storage, w, h, i, x + vx * t, y + vy * t);
+        double px, py;
+        track_at(f.t, f.par_x, f.par_y, q, i, x, y, vx, vy, px, py);
+        const float v = bilinear(f, storage, w, h, i, px, py);
         if (!isfinite(v)) continue;
         sum += v;
         ++count;
@@ -74,14 +90,15 @@ __device__ inline double track_snr(const ScreenFrames& f, int storage, int w, in
 // where the subtracted frame or the static sky is undefined, within r of it. blank
 // (n ints) must be zero on entry; the caller synchronises before reading it.
 __device__ inline void edge_rows(const float* sub, long long sub_stride, const float* sky, int w, int h, int n,
-                                 const double* t_h, double x, double y, double vx, double vy, int r, int lane, int nl,
-                                 int* blank) {
+                                 const double* t_h, const double* par_x, const double* par_y, double q, double x,
+                                 double y, double vx, double vy, int r, int lane, int nl, int* blank) {
     const int side = 2 * r + 1;
     for (int e = lane; e < n * side; e += nl) {
         const int i = e / side;
-        const double t = t_h[i];
-        const int xi = static_cast<int>(lround(x + vx * t));
-        const int yy = static_cast<int>(lround(y + vy * t)) + (e - i * side) - r;
+        double px, py;
+        track_at(t_h, par_x, par_y, q, i, x, y, vx, vy, px, py);
+        const int xi = static_cast<int>(lround(px));
+        const int yy = static_cast<int>(lround(py)) + (e - i * side) - r;
         // a row that leaves the frame holds an outside pixel
         int any = yy < 0 || yy >= h || xi - r < 0 || xi + r >= w;
         if (!any) {
@@ -97,14 +114,16 @@ __device__ inline void edge_rows(const float* sub, long long sub_stride, const f
 }
 
 // The track's mean detector position (detector_flag's mx, my), by one thread.
-__device__ inline void detector_mean(int n, const double* t_h, const double* reg_dx, const double* reg_dy, double x,
-                                     double y, double vx, double vy, double& mx, double& my) {
+__device__ inline void detector_mean(int n, const double* t_h, const double* par_x, const double* par_y, double q,
+                                     const double* reg_dx, const double* reg_dy, double x, double y, double vx,
+                                     double vy, double& mx, double& my) {
     mx = 0.0;
     my = 0.0;
     for (int i = 0; i < n; ++i) {
-        const double t = t_h[i];
-        mx += (x + vx * t) + reg_dx[i];
-        my += (y + vy * t) + reg_dy[i];
+        double px, py;
+        track_at(t_h, par_x, par_y, q, i, x, y, vx, vy, px, py);
+        mx += px + reg_dx[i];
+        my += py + reg_dy[i];
     }
     mx /= n;
     my /= n;

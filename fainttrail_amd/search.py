@@ -28,7 +28,8 @@ class _Params(C.Structure):
                 ("null_tested", C.c_int), ("max_frame_fraction", C.c_double),
                 ("reg_dx", C.POINTER(C.c_double)), ("reg_dy", C.POINTER(C.c_double)),
                 ("candidates", C.c_int), ("half_tolerance", C.c_double), ("tile_rows", C.c_int),
-                ("queue", C.c_int), ("build_lanes", C.c_int)]
+                ("queue", C.c_int), ("build_lanes", C.c_int), ("neo_min_dist_au", C.c_double),
+                ("neo_max_trials", C.c_int), ("par_x", C.POINTER(C.c_double)), ("par_y", C.POINTER(C.c_double))]
 
 
 class _Summary(C.Structure):
@@ -40,7 +41,8 @@ class _Summary(C.Structure):
                 ("t_peaks", C.c_double), ("t_threshold", C.c_double), ("t_plan", C.c_double),
                 ("t_passes", C.c_double), ("passes_per_launch", C.c_int), ("segments", C.c_int),
                 ("n_screened", C.c_int), ("n_artefacts", C.c_int), ("n_candidates", C.c_int),
-                ("t_refine", C.c_double), ("tile_rows", C.c_int), ("queue", C.c_int)]
+                ("t_refine", C.c_double), ("tile_rows", C.c_int), ("queue", C.c_int), ("n_distances", C.c_int),
+                ("dq_inv_au", C.c_double)]
 
 
 # Candidate's fields (include/fainttrail/search.hpp:131-154) that the record carries, in
@@ -137,6 +139,14 @@ class SearchParams:
     queue: int = -1
     # stage builds of a group in parallel on this many streams (1-4; 0 = by frame size)
     build_lanes: int = 0
+    # near-Earth search (neo_min_dist_au > 0): curved tracks over the inverse distances
+    # of inverse_distance_grid times the velocity grid; par_x / par_y: each frame's
+    # parallax displacement at 1 AU in pixels, less its least-squares line in t
+    # (Prepared::par_x / par_y), one per frame
+    neo_min_dist_au: float = 0.0
+    neo_max_trials: int = 64
+    par_x: Optional[Sequence[float]] = None
+    par_y: Optional[Sequence[float]] = None
 
     def c(self) -> _Params:
         from . import STORAGES
@@ -151,15 +161,26 @@ class SearchParams:
                     -1 if self.fit_on_device is None else int(bool(self.fit_on_device)),
                     self.passes_per_launch, self.segments, STORAGES[self.storage], self.null_tested,
                     self.max_frame_fraction, None, None, int(bool(self.candidates)), self.half_tolerance,
-                    self.tile_rows, int(self.queue), int(self.build_lanes))
+                    self.tile_rows, int(self.queue), int(self.build_lanes), float(self.neo_min_dist_au),
+                    int(self.neo_max_trials), None, None)
+        keep = []  # the arrays live as long as the struct
+        for a, b, what in ((self.reg_dx, self.reg_dy, "registration offsets (reg_dx, reg_dy)"),
+                           (self.par_x, self.par_y, "parallax displacements (par_x, par_y)")):
+            if (a is None) != (b is None):
+                raise FtkError(f"give both {what} or none")
+            if a is None:
+                continue
+            ax = np.ascontiguousarray(a, dtype=np.float64)
+            ay = np.ascontiguousarray(b, dtype=np.float64)
+            if ax.shape != ay.shape or ax.ndim != 1:
+                raise FtkError(f"{what}: one value per frame in each")
+            keep += [ax, ay]
+        it = iter(keep)
         if self.reg_dx is not None:
-            rx = np.ascontiguousarray(self.reg_dx, dtype=np.float64)
-            ry = np.ascontiguousarray(self.reg_dy, dtype=np.float64)
-            if rx.shape != ry.shape or rx.ndim != 1:
-                raise FtkError("registration offsets: one value per frame in each of reg_dx, reg_dy")
-            p.reg_dx = rx.ctypes.data_as(C.POINTER(C.c_double))
-            p.reg_dy = ry.ctypes.data_as(C.POINTER(C.c_double))
-            p._keep = (rx, ry)  # the arrays live as long as the struct
+            p.reg_dx, p.reg_dy = (next(it).ctypes.data_as(C.POINTER(C.c_double)) for _ in range(2))
+        if self.par_x is not None:
+            p.par_x, p.par_y = (next(it).ctypes.data_as(C.POINTER(C.c_double)) for _ in range(2))
+        p._keep = keep
         return p
 
 
@@ -237,6 +258,8 @@ class SearchResult:
     candidates: list = field(default_factory=list)  # Candidate records, the reference's order
     tile_rows: int = 16      # the stacking work shape the run used
     queue: int = 0
+    n_distances: int = 1     # inverse distances of the motions (near-Earth search; 1: straight tracks)
+    dq_inv_au: float = 0.0   # their step
 
 
 class Searcher:
@@ -308,7 +331,7 @@ class Searcher:
              "plan": summ.t_plan, "passes (batched)": summ.t_passes, "refine": summ.t_refine},
             mx, bx, summ.passes_per_launch, summ.segments, summ.n_screened, summ.n_artefacts,
             read_candidates(lambda cap, recs: lib().ftk_searcher_candidates(self._h, cap, recs), summ.n_candidates),
-            summ.tile_rows, summ.queue)
+            summ.tile_rows, summ.queue, summ.n_distances, summ.dq_inv_au)
 
     def refit(self, params: SearchParams, on_device: bool, stream=None):
         """The last run's null-tail threshold fitted again with the FP64 fit sums on the
@@ -370,4 +393,7 @@ def search(frames, mjd, params: SearchParams, device: int = 0) -> SearchResult:
         # the offsets belong to the frames: same order
         params = replace(params, reg_dx=np.asarray(params.reg_dx, np.float64)[order],
                          reg_dy=np.asarray(params.reg_dy, np.float64)[order])
+    if params.par_x is not None and params.par_y is not None:
+        params = replace(params, par_x=np.asarray(params.par_x, np.float64)[order],
+                         par_y=np.asarray(params.par_y, np.float64)[order])
     return Searcher(n, w, h, device).run(dev, m[order], params)

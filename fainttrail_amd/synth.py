@@ -21,6 +21,22 @@ class Mover:
     vx_pxh: float
     vy_pxh: float
     flux: float
+    inv_dist_au: float = 0.0  # curved track: plus inv_dist_au times the frame's parallax displacement
+
+
+def diurnal_parallax(t_h, amp_px: float = 9.0, phase: float = 0.4):
+    """Parallax displacements at 1 AU (pixels) of a ground observer over a night: the
+    diurnal circle's projection, amp_px at most, less its least-squares line in t (as
+    set_parallax_model leaves them, src/parallax.cpp:138-155, so that the velocity grid
+    absorbs the straight part)."""
+    t = np.asarray(t_h, np.float64)
+    w = 2.0 * math.pi / 23.934  # sidereal rate, 1/h
+    px = amp_px * np.cos(w * t + phase)
+    py = 0.45 * amp_px * np.sin(w * t + phase)
+    A = np.stack([np.ones_like(t), t], 1)
+    for v in (px, py):
+        v -= A @ np.linalg.lstsq(A, v, rcond=None)[0]
+    return px, py
 
 
 def _gauss_pixel_weights(c: float, lo: int, hi: int, sigma: float) -> np.ndarray:
@@ -48,13 +64,15 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
                   star_flux=(1e3, 1e6), psf_sigma: float = 1.5, nmovers: int = 3, mover_snr=(1.5, 1.5),
                   rate_px_min=(0.2, 1.1), cadence_s: float = 60.0, exptime_s: float = 60.0, seed: int = 7,
                   device: str = "cpu", mjd0: float = 61000.0, saturate: float = 0.0, cr_frames: int = 0,
-                  cr_hits: int = 50, mover_box=None):
+                  cr_hits: int = 50, mover_box=None, parallax=None, mover_inv_dist=(0.0, 0.0)):
     """Returns (frames (n, h, w) float32 torch tensor on `device`, mjd (n,), movers).
 
     saturate > 0 clips every pixel at that level (a detector's full well, e.g. 65535
     ADU); cr_frames frames (chosen by the seed) get cr_hits cosmic-ray hits of 1-4
     pixels and 800-15000 ADU each. mover_box (x0, x1, y0, y1): where the movers are at
-    the reference epoch (default: 24 px inside the frame)."""
+    the reference epoch (default: 24 px inside the frame). parallax (par_x, par_y), one
+    per frame: movers then follow curved tracks x + v t + q par, q drawn from
+    mover_inv_dist (1/AU)."""
     import torch
     rng = np.random.default_rng(seed)
     static = np.zeros((h, w), np.float32)
@@ -73,7 +91,8 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
         rate = rng.uniform(*rate_px_min) * 60.0  # px/h
         pa = rng.uniform(0.0, 2 * math.pi)
         movers.append(Mover(rng.uniform(bx0, bx1), rng.uniform(by0, by1), rate * math.cos(pa), rate * math.sin(pa),
-                            snr * noise * math.sqrt(4 * math.pi) * psf_sigma))
+                            snr * noise * math.sqrt(4 * math.pi) * psf_sigma,
+                            float(rng.uniform(*mover_inv_dist)) if parallax is not None else 0.0))
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
     base = torch.from_numpy(static).to(device) + sky
@@ -82,9 +101,11 @@ def make_sequence(n: int, w: int, h: int, *, sky: float = 1000.0, noise: float =
     for i in range(n):
         mv = np.zeros((h, w), np.float32)
         for m in movers:
+            cx = m.inv_dist_au * parallax[0][i] if parallax is not None else 0.0
+            cy = m.inv_dist_au * parallax[1][i] if parallax is not None else 0.0
             for k in range(steps):
                 tt = t_h[i] + ((k + 0.5) / steps - 0.5) * exptime_s / 3600.0
-                _add_source(mv, m.x0 + m.vx_pxh * tt, m.y0 + m.vy_pxh * tt, m.flux / steps, psf_sigma)
+                _add_source(mv, m.x0 + m.vx_pxh * tt + cx, m.y0 + m.vy_pxh * tt + cy, m.flux / steps, psf_sigma)
         frames[i] = base + torch.from_numpy(mv).to(device) + noise * torch.randn((h, w), generator=gen, device=device)
     for i in rng.choice(n, size=min(cr_frames, n), replace=False) if cr_frames > 0 else []:
         for _ in range(cr_hits):

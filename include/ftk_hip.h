@@ -268,6 +268,19 @@ typedef struct ftk_search_params {
     /* Stage builds of a group of passes run in parallel on this many streams, each fed
      * by a host thread of its own (1-4; 0 = by frame size). No result bit depends on it. */
     int build_lanes;
+    /* Near-Earth search (SearchParams::neo_min_dist_au / neo_max_trials, search.hpp:
+     * 86-88; default 0 = off, 64): with neo_min_dist_au > 0 the search runs curved
+     * tracks x + v t + q par(t) over the inverse distances q of inverse_distance_grid
+     * (src/parallax.cpp:167-185) times the velocity grid, motion j = k nv + iv
+     * (search.cpp:727-733), and candidates carry inv_dist_au / inv_dist_err_au. par_x /
+     * par_y: each frame's parallax displacement at 1 AU in pixels, less its
+     * least-squares line in t (Prepared::par_x / par_y as set_parallax_model leaves
+     * them, src/parallax.cpp:132-156), n host doubles each, or NULL when off; copied by
+     * the call. */
+    double neo_min_dist_au;
+    int neo_max_trials;
+    const double* par_x;
+    const double* par_y;
 } ftk_search_params;
 
 typedef struct ftk_search_summary {
@@ -286,6 +299,8 @@ typedef struct ftk_search_summary {
     int n_candidates;            /* candidates after the duplicate test */
     double t_refine;             /* seconds of the candidate stage */
     int tile_rows, queue;        /* the stacking work shape the run used */
+    int n_distances;             /* inverse distances of the motions (1: straight tracks) */
+    double dq_inv_au;            /* their step (0: straight tracks) */
 } ftk_search_summary;
 
 /* A candidate: ft_candidate (include/fainttrail/fainttrail.h:56-69) with the position

@@ -311,8 +311,12 @@ METHODS = ["null-fit gauss", "null-fit linear", "null-quantile", "fixed", "gauss
 
 
 def _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, threshold, floor, target, backend,
-                    null_tested, reg, tile, max_candidates, extras=True):
+                    null_tested, reg, tile, max_candidates, extras=True, neo=None):
     L = ref()
+    if not getattr(L, "_neo_bound", False):
+        L.ref_set_neo.argtypes = [C.c_int, F64P, F64P, C.c_double, C.c_int]
+        L.ref_last_neo.argtypes = [F64P]
+        L._neo_bound = True
     if not getattr(L, "_search_bound", False):
         L.ref_search_run.argtypes = [C.c_int, C.c_int, C.c_int, F32P, F64P, F64P, C.c_int, C.c_uint64, C.c_char_p,
                                      C.c_int, F64P, C.c_int, C.c_int, C.c_int]
@@ -335,9 +339,17 @@ def _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, thresho
     if reg is not None:
         rg = f64(np.concatenate([np.asarray(reg[0], np.float64), np.asarray(reg[1], np.float64)]))
         assert rg.size == 2 * n
-    rc = L.ref_search_run(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
-                          backend.encode(), null_tested, _p(rg, F64P) if rg is not None else None, tile,
-                          max_candidates, int(extras))
+    if neo is not None:
+        px, py = f64(neo["par_x"]), f64(neo["par_y"])
+        assert px.size == n and py.size == n
+        L.ref_set_neo(n, _p(px, F64P), _p(py, F64P), float(neo["min_dist_au"]), int(neo.get("max_trials", 64)))
+    try:
+        rc = L.ref_search_run(n, w, h, _p(fr, F32P), _p(m, F64P), _p(prm, F64P), null_runs, C.c_uint64(seed),
+                              backend.encode(), null_tested, _p(rg, F64P) if rg is not None else None, tile,
+                              max_candidates, int(extras))
+    finally:
+        if neo is not None:
+            L.ref_set_neo(0, None, None, 0.0, 64)
     if rc != 0:
         raise RuntimeError(f"reference search failed: {L.ref_last_error().decode()}")
     cnt = np.zeros(5, np.int32)
@@ -368,13 +380,16 @@ def _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, thresho
            "n_velocities": int(summ[5]), "dv_pxh": summ[6], "t_ref_mjd": summ[7],
            "method": METHODS[int(summ[8])] if summ[8] >= 0 else "", "pooled": pooled[:int(cnt[0])], "max_snr": mx,
            "candidates": cands}
+    nq = np.zeros(2, np.float64)
+    L.ref_last_neo(_p(nq, F64P))
+    out.update(n_distances=int(nq[0]), dq_inv_au=float(nq[1]))
     if tile <= 0 and extras:
         out.update(best_v=bv, peaks=peaks(0, int(cnt[1])), selected=peaks(1, int(cnt[2])))
     return out
 
 
 def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, threshold=0.0, floor=5.0, target=0.02,
-               backend="cpu", null_tested=3000, reg=None, max_candidates=0, extras=True):
+               backend="cpu", null_tested=3000, reg=None, max_candidates=0, extras=True, neo=None):
     """The reference's search() on prepare(align off), with the given stacking backend
     ("cpu", or "hip" through the linked adapter), null_tested (the SearchParams
     default 3000 screens the strongest null peaks; 0 screens none) and optional
@@ -383,9 +398,10 @@ def ref_search(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed=12345, thresh
     peaks in the search's order, the real peak list (find_peaks order), the selected
     peaks and the refined candidates (dicts of Candidate's fields). extras=False skips
     the second real pass that exposes best_v and the peak lists (to time search()
-    alone)."""
+    alone). neo = {"par_x", "par_y", "min_dist_au", "max_trials"}: the near-Earth
+    search with these parallax displacements set on the prepared sequence."""
     return _ref_search_run(frames, mjd, vmin, vmax, tol, fwhm, null_runs, seed, threshold, floor, target, backend,
-                           null_tested, reg, 0, max_candidates, extras)
+                           null_tested, reg, 0, max_candidates, extras, neo)
 
 
 def ref_search_tiled(frames, mjd, vmin, vmax, tol, fwhm, null_runs, tile, seed=12345, threshold=0.0, floor=5.0,

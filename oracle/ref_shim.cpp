@@ -242,6 +242,25 @@ struct LastSearch {
 };
 static LastSearch g_last;
 
+// Near-Earth settings for the following searches: the frames' parallax displacements
+// (Prepared::par_x / par_y, set on the prepared sequence) and SearchParams'
+// neo_min_dist_au / neo_max_trials; n = 0 clears them.
+struct NeoSettings {
+    std::vector<double> par_x, par_y;
+    double min_dist_au = 0.0;
+    int max_trials = 64;
+};
+static NeoSettings g_neo;
+
+void ref_set_neo(int n, const double* par_x, const double* par_y, double min_dist_au, int max_trials) {
+    g_neo = NeoSettings{};
+    if (n <= 0) return;
+    g_neo.par_x.assign(par_x, par_x + n);
+    g_neo.par_y.assign(par_y, par_y + n);
+    g_neo.min_dist_au = min_dist_au;
+    g_neo.max_trials = max_trials;
+}
+
 static std::vector<Frame> to_frames(int n, int w, int h, const float* frames, const double* mjd, const double* reg) {
     std::vector<Frame> fr(static_cast<size_t>(n));
     const size_t plane = static_cast<size_t>(w) * h;
@@ -279,6 +298,10 @@ int ref_search_run(int n, int w, int h, const float* frames, const double* mjd, 
         s.null_tested = null_tested;
         s.seed = seed;
         if (max_candidates > 0) s.max_candidates = max_candidates;
+        if (!g_neo.par_x.empty()) {
+            s.neo_min_dist_au = g_neo.min_dist_au;
+            s.neo_max_trials = g_neo.max_trials;
+        }
         if (tile > 0) {
             const auto src = memory_frame_source(fr);
             TileOptions opt;
@@ -286,7 +309,12 @@ int ref_search_run(int n, int w, int h, const float* frames, const double* mjd, 
             g_last.r = search_tiled(*src, s, opt);
             g_last.tiled = true;
         } else {
-            const Prepared p = prepare(fr, s);
+            Prepared p = prepare(fr, s);
+            if (!g_neo.par_x.empty()) {
+                if (static_cast<int>(g_neo.par_x.size()) != p.nframes) throw std::runtime_error("parallax: one per frame");
+                p.par_x = g_neo.par_x;
+                p.par_y = g_neo.par_y;
+            }
             g_last.r = search(p, s);
             if (!extras) {
                 for (const auto& c : g_last.r.candidates) g_last.flags += c.flags + ";";
@@ -314,6 +342,12 @@ int ref_search_run(int n, int w, int h, const float* frames, const double* mjd, 
 }
 
 const char* ref_last_error() { return g_last.error.c_str(); }
+
+// n_distances and dq_inv_au of the last search
+void ref_last_neo(double* out) {
+    out[0] = static_cast<double>(g_last.r.n_distances);
+    out[1] = g_last.r.dq_inv_au;
+}
 
 // counts: pooled null peaks, real peaks, selected peaks, candidates, flags bytes (+1)
 void ref_last_counts(int* counts) {

@@ -213,6 +213,65 @@ def test_search_selects_the_references_peaks(hip, group):
     assert len(got.candidates) >= 5
 
 
+def _neo_sequence():
+    """24 frames of 160 x 128 over 2 h with a ground observer's diurnal parallax
+    (detrended: 0.17 px at 1 AU) and 6 movers on curved tracks at inverse distances
+    15-45 / AU (2.6-7.7 px of curvature), bright enough to be selected."""
+    from fainttrail_amd import synth
+    n, w, h, cad = 24, 160, 128, 300.0
+    mjd = 61000.0 + np.arange(n) * cad / 86400.0
+    par = synth.diurnal_parallax((mjd - 0.5 * (mjd[0] + mjd[-1])) * 24.0, amp_px=9.0)
+    fr, _, movers = synth.make_sequence(n, w, h, nstars=40, nmovers=6, mover_snr=(2.5, 4.0), rate_px_min=(0.2, 0.45),
+                                        cadence_s=cad, seed=31, mover_box=(55, 105, 45, 83), parallax=par,
+                                        mover_inv_dist=(15.0, 45.0))
+    return fr.numpy(), mjd, par, movers
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("form", ["one-pass", "batched", "server"])
+def test_near_earth_search_matches_reference(hip, form):
+    """The near-Earth search (neo_min_dist_au > 0, src/search.cpp:727-733): curved
+    tracks over the reference's inverse_distance_grid (8 distances here, the
+    neo_max_trials cap) times the velocity grid, each null run's tables with the
+    parallax of the frames whose times it takes (with_times, preprocess.cpp:503-515),
+    the artefact screen and the refinement along curved tracks (the inverse distance
+    searched and fitted, refine :274-285, 308-316), ghosts by curved tracks. Every
+    output equals the reference's search() with the same parallax model, bit for bit,
+    one pass at a time, batched, and as a server job."""
+    import torch
+    from fainttrail_amd.search import SearchParams, Searcher
+    frames, mjd, par, movers = _neo_sequence()
+    neo = {"par_x": par[0], "par_y": par[1], "min_dist_au": 0.02, "max_trials": 8}
+    want = po.ref_search(frames, mjd, 12.0, 30.0, 0.5, 3.53, 3, seed=21, neo=neo)
+    assert want["n_distances"] == 8 and want["dq_inv_au"] > 0.0
+    assert len(want["candidates"]) >= 5 and any(c["inv_dist_au"] > 10.0 for c in want["candidates"])
+    prm = SearchParams(vmin_pxh=12.0, vmax_pxh=30.0, grid_tol_px=0.5, fwhm_px=3.53, null_runs=3, seed=21,
+                       neo_min_dist_au=0.02, neo_max_trials=8, par_x=par[0], par_y=par[1],
+                       passes_per_launch=1 if form == "one-pass" else 0)
+    dev = torch.from_numpy(frames).cuda()
+    print(f"near-Earth search, {form}", flush=True)
+    if form == "server":
+        from fainttrail_amd.serve import Server
+        srv = Server(slots=2)
+        try:
+            res = srv.wait(srv.submit(dev, mjd, prm))
+        finally:
+            srv.close()
+        assert res.threshold_snr == want["threshold"] and res.threshold_method == want["method"]
+        assert same_bits(res.null_peaks, want["pooled"])
+        for f in ("x", "y", "vidx"):
+            assert np.array_equal(getattr(res.selected, f), want["selected"][f]), f
+        _same_candidates(res.candidates, want["candidates"])
+        return
+    n, h, w = frames.shape
+    got = Searcher(n, w, h).run(dev, mjd, prm)
+    torch.cuda.synchronize()
+    assert got.n_distances == want["n_distances"] and got.dq_inv_au == want["dq_inv_au"]
+    _same_as_reference(got, want)
+    # the velocity grid times the distances: motion indices up to nv * nq
+    assert int(got.best_v.max()) >= got.n_velocities
+
+
 def _plateau_map(seed, h, w):
     """A max-S/N map on a coarse value grid (steps of 0.5): many equal neighbours and
     flat plateaus, so the one-peak-per-plateau rule decides most peaks."""
