@@ -390,7 +390,8 @@ def serving_measure(count: int, rate_hz: float, seed: int, slots: int, sustain_c
                 "jobs_per_s": len(res) / span, "units_per_s": sum(units) / span, "units": sum(units),
                 "wall_s": span, "mean_job_s_in_slot": busy / len(res), "max_running": st["max_running"],
                 "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9,
-                "evicted_searchers": st["evicted"]}
+                "evicted_searchers": st["evicted"], "stack_launches": st["stack_launches"],
+                "fused_launches": st["fused_launches"], "fused_passes": st["fused_passes"]}
 
     specs = poisson_jobs(count, rate_hz, seed)
     over = run(specs)
@@ -490,7 +491,8 @@ def serving_stream_measure(count: int, rate_hz: float, seed: int, slots: int, bu
             "late_submissions": len(late), "max_late_s": max(late, default=0.0),
             "candidates": int(sum(len(r.candidates) for r in res)), "max_running": st["max_running"],
             "peak_reserved_gb": st["peak_reserved_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9,
-            "evicted_searchers": st["evicted"], "failed": st["failed"]}
+            "evicted_searchers": st["evicted"], "failed": st["failed"], "stack_launches": st["stack_launches"],
+            "fused_launches": st["fused_launches"], "fused_passes": st["fused_passes"]}
 
 
 def batched_nulls_measure(with_plain: bool, plain_nulls: int = 2):
