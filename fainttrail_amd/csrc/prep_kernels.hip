@@ -395,6 +395,160 @@ __global__ void mf_cols_kernel(const float* __restrict__ in, long long istride, 
     out[i * ostride + static_cast<long long>(y) * w + x] = o;
 }
 
+// z, cosmic-ray hits, kill + clip and the separable matched filter of one 64 x 32 tile
+// of up to `frames` frames in one pass (the chain z_kernel .. mf_cols_kernel above:
+// preprocess.cpp:413-451, 116-148), every value computed with the same operations in
+// the same order, so the output is the chain's bit for bit. Per tile the mask and
+// scatter give each halo pixel its divisor e (scatter where it exceeds weight_scatter,
+// else 1; 0 where masked or outside the frame) once for all its frames. Per frame the
+// residuals over the tile grown by R + 2 (R the filter half width) become z in LDS; hit
+// flags over the tile grown by R + 1 (each needs z at +-1); killed and clipped values
+// over the tile grown by R (the kill test reads flags at +-1); the row filter over the
+// tile's columns and R more rows each side; the column filter writes the tile. Positions
+// outside the frame hold zeros that no in-frame computation reads (the chain's own
+// bounds: hits only inside the 1-pixel border, neighbours and taps only inside the
+// frame). Hits are counted for the tile's own pixels only, once per pixel. 256 threads
+// as 64 columns x 4 rows. LDS: 2 (TX + 2R + 4)(TY + 2R + 4) + (TX + 2R)(TY + 2R) + 2 (2R + 1)
+// floats and (TX + 2R + 2)(TY + 2R + 2) bytes; the row-filtered values reuse the z area.
+constexpr int kZfTileX = 64, kZfTileY = 32, kZfMaxR = 10;  // R <= 10: at most 62 KB of LDS
+
+__global__ __launch_bounds__(256) void zfilter_kernel(const float* __restrict__ resid, long long rstride, int n,
+                                                      int frames, int w, int h, const float* __restrict__ mask,
+                                                      const float* __restrict__ scatter,
+                                                      const float* __restrict__ inv_sigma, float wscat, float hit,
+                                                      float ratio, float clip, const float* __restrict__ taps, int R,
+                                                      int* __restrict__ cr, float* __restrict__ out,
+                                                      long long ostride) {
+    extern __shared__ float zf_lds[];
+    constexpr int TX = kZfTileX, TY = kZfTileY;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int i0 = blockIdx.z * frames, i1 = min(n, i0 + frames);
+    const int zw = TX + 2 * R + 4, zh = TY + 2 * R + 4;  // e and z: from (x0 - R - 2, y0 - R - 2)
+    const int kw = TX + 2 * R, kh = TY + 2 * R;          // killed / clipped: from (x0 - R, y0 - R)
+    const int hw = TX + 2 * R + 2, hh = TY + 2 * R + 2;  // hit flags: from (x0 - R - 1, y0 - R - 1)
+    const int nt = 2 * R + 1;
+    float* const E = zf_lds;
+    float* const Z = E + zw * zh;
+    float* const K = Z + zw * zh;
+    float* const T = K + kw * kh;  // row taps, then the normalised column taps
+    unsigned char* const H = reinterpret_cast<unsigned char*>(T + 2 * nt);
+    float* const RF = Z;           // row-filtered values, TX x kh (after K is built)
+    __shared__ int hits;
+    const int tid = threadIdx.x, lx = tid & 63, ly = tid >> 6;
+    if (tid == 0) hits = 0;
+    for (int e = tid; e < 2 * nt; e += blockDim.x) T[e] = taps[e];
+    for (int ey = ly; ey < zh; ey += 4)
+        for (int ex = lx; ex < zw; ex += 64) {
+            const int gy = y0 - R - 2 + ey, gx = x0 - R - 2 + ex;
+            float ee = 0.0f;
+            if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+                const long long p = static_cast<long long>(gy) * w + gx;
+                if (mask[p] != 0.0f) {
+                    const float s = scatter[p];
+                    ee = s > wscat ? s : 1.0f;  // >= 1, or +inf
+                }
+            }
+            E[ey * zw + ex] = ee;
+        }
+    __syncthreads();
+    auto zat = [&](int gx, int gy) { return Z[(gy - (y0 - R - 2)) * zw + (gx - (x0 - R - 2))]; };
+    auto hat = [&](int gx, int gy) { return H[(gy - (y0 - R - 1)) * hw + (gx - (x0 - R - 1))]; };
+    for (int i = i0; i < i1; ++i) {
+        const float* r = resid + i * rstride;
+        const float inv = inv_sigma[i];
+        // z (preprocess.cpp:413-425): r float(1/sigma_i) / e where unmasked and finite, else 0
+        for (int ey = ly; ey < zh; ey += 4)
+            for (int ex = lx; ex < zw; ex += 64) {
+                const float ee = E[ey * zw + ex];
+                float z = 0.0f;
+                if (ee != 0.0f) {
+                    const float v = r[static_cast<long long>(y0 - R - 2 + ey) * w + (x0 - R - 2 + ex)];
+                    if (finitef(v)) z = v * inv / ee;
+                }
+                Z[ey * zw + ex] = z;
+            }
+        __syncthreads();
+        // hits (preprocess.cpp:428-440): interior, z >= hit and the 4-neighbour mean below ratio z
+        int mine = 0;
+        for (int ey = ly; ey < hh; ey += 4)
+            for (int ex = lx; ex < hw; ex += 64) {
+                const int gy = y0 - R - 1 + ey, gx = x0 - R - 1 + ex;
+                unsigned char f = 0;
+                if (gx >= 1 && gx < w - 1 && gy >= 1 && gy < h - 1) {
+                    const float v = zat(gx, gy);
+                    if (!(v < hit)) {
+                        const float n4 =
+                            0.25f * (zat(gx - 1, gy) + zat(gx + 1, gy) + zat(gx, gy - 1) + zat(gx, gy + 1));
+                        if (!(n4 >= ratio * v)) {
+                            f = 1;
+                            mine += (gx >= x0 && gx < x0 + TX && gy >= y0 && gy < y0 + TY) ? 1 : 0;
+                        }
+                    }
+                }
+                H[ey * hw + ex] = f;
+            }
+        if (mine) atomicAdd(&hits, mine);
+        __syncthreads();
+        // kill the hits and their 8-neighbours above 2.5, then clamp to +-clip (:441-451)
+        for (int ey = ly; ey < kh; ey += 4)
+            for (int ex = lx; ex < kw; ex += 64) {
+                const int gy = y0 - R + ey, gx = x0 - R + ex;
+                float o = 0.0f;
+                if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+                    const float v = zat(gx, gy);
+                    bool kill = hat(gx, gy) != 0;
+                    if (!kill && v > 2.5f) {
+                        for (int dy = -1; dy <= 1 && !kill; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                const int xx = gx + dx, yy = gy + dy;
+                                if ((dx || dy) && xx >= 0 && xx < w && yy >= 0 && yy < h && hat(xx, yy)) {
+                                    kill = true;
+                                    break;
+                                }
+                            }
+                    }
+                    o = kill ? 0.0f : v;
+                    o = o < -clip ? -clip : (clip < o ? clip : o);
+                }
+                K[ey * kw + ex] = o;
+            }
+        __syncthreads();
+        if (tid == 0) {
+            if (hits) atomicAdd(&cr[i], hits);
+            hits = 0;  // the next frame's first hit count comes after two more barriers
+        }
+        // row filter over the tile's columns and R more rows each side (:116-131): taps
+        // truncated at the frame's edges, multiply then add in ascending tap order
+        {
+            const int gx = x0 + lx;
+            const int a = max(0, gx - R), b = min(w - 1, gx + R);
+            for (int ey = ly; ey < kh; ey += 4) {
+                const int gy = y0 - R + ey;
+                float acc = 0.0f;
+                if (gx < w && gy >= 0 && gy < h) {
+                    const float* krow = K + ey * kw;  // frame column j at krow[j - (x0 - R)]
+                    for (int j = a; j <= b; ++j) acc += krow[j - (x0 - R)] * T[j - gx + R];
+                }
+                RF[ey * TX + lx] = acc;
+            }
+        }
+        __syncthreads();
+        // column filter with the normalised taps (:132-148), the tile's pixels
+        {
+            const int gx = x0 + lx;
+            for (int cy = ly; cy < TY; cy += 4) {
+                const int gy = y0 + cy;
+                if (gx >= w || gy >= h) continue;
+                const int a = max(0, gy - R), b = min(h - 1, gy + R);
+                float o = 0.0f;
+                for (int j = a; j <= b; ++j) o += T[nt + j - gy + R] * RF[(j - (y0 - R)) * TX + lx];
+                out[i * ostride + static_cast<long long>(gy) * w + gx] = o;
+            }
+        }
+        __syncthreads();  // the next frame's z overwrites the row-filtered values
+    }
+}
+
 // g *= float(1 / sg) where sg > 0 (preprocess.cpp:474-477).
 __global__ void scale_kernel(float* __restrict__ g, long long gstride, long long plane, const SelectSeg* __restrict__ seg) {
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -797,20 +951,35 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
     hipLaunchKernelGGL(mask_kernel, dim3(blocks_for(w, 128), h), dim3(128), 0, s, scatter_, w, h, dil,
                        prm.mask_scatter, mask);
     // z, cosmic rays, clip
-    hipLaunchKernelGGL(z_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, resid_, plane, plane, mask,
-                       scatter_, inv_, static_cast<float>(prm.weight_scatter));
     hcheck(hipMemsetAsync(cr_, 0, sizeof(int) * n, s), "clear counts");
     const float ratio = static_cast<float>(0.5 * std::exp(-0.5 / (psf_sigma * psf_sigma)));
-    hipLaunchKernelGGL(hit_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
-                       static_cast<float>(prm.cr_sigma), ratio, flags_, cr_);
-    hipLaunchKernelGGL(kill_clip_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, flags_,
-                       static_cast<float>(prm.clip_sigma), filtered, filt_stride);
     // matched filter: taps from the host (glibc exp, set_taps), truncated at the edges
     const int R = std::max(1, static_cast<int>(std::ceil(3.0 * psf_sigma)));
-    hipLaunchKernelGGL(mf_rows_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, filtered, filt_stride, w, R,
-                       taps_, resid_, plane);
-    hipLaunchKernelGGL(mf_cols_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, R,
-                       taps_ + (2 * R + 1), filtered, filt_stride);
+    if (R <= kZfMaxR) {
+        // z, hits, kill + clip and both filter passes in one tiled pass
+        const size_t lds = sizeof(float) * (2 * static_cast<size_t>(kZfTileX + 2 * R + 4) * (kZfTileY + 2 * R + 4) +
+                                             static_cast<size_t>(kZfTileX + 2 * R) * (kZfTileY + 2 * R) + 2 * (2 * R + 1)) +
+                           static_cast<size_t>(kZfTileX + 2 * R + 2) * (kZfTileY + 2 * R + 2);
+        // frames per workgroup: up to 8 (the tile's divisors shared), while the grid keeps
+        // at least ~2048 workgroups
+        const long long tiles = static_cast<long long>(blocks_for(w, kZfTileX)) * blocks_for(h, kZfTileY);
+        const int frames = static_cast<int>(std::max(1LL, std::min<long long>(8, tiles * n / 2048)));
+        hipLaunchKernelGGL(zfilter_kernel, dim3(blocks_for(w, kZfTileX), blocks_for(h, kZfTileY), blocks_for(n, frames)),
+                           dim3(256), lds, s, resid_, plane, n, frames, w, h, mask, scatter_, inv_,
+                           static_cast<float>(prm.weight_scatter), static_cast<float>(prm.cr_sigma), ratio,
+                           static_cast<float>(prm.clip_sigma), taps_, R, cr_, filtered, filt_stride);
+    } else {
+        hipLaunchKernelGGL(z_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, resid_, plane, plane, mask,
+                           scatter_, inv_, static_cast<float>(prm.weight_scatter));
+        hipLaunchKernelGGL(hit_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
+                           static_cast<float>(prm.cr_sigma), ratio, flags_, cr_);
+        hipLaunchKernelGGL(kill_clip_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h,
+                           flags_, static_cast<float>(prm.clip_sigma), filtered, filt_stride);
+        hipLaunchKernelGGL(mf_rows_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, filtered, filt_stride, w,
+                           R, taps_, resid_, plane);
+        hipLaunchKernelGGL(mf_cols_kernel, dim3(blocks_for(w, 128), h, n), dim3(128), 0, s, resid_, plane, w, h, R,
+                           taps_ + (2 * R + 1), filtered, filt_stride);
+    }
     // normalisation: robust sigma of the masked grid samples, border ceil(3 sigma) + 1
     const int fborder = static_cast<int>(std::ceil(3 * psf_sigma)) + 1;
     select(filtered, filt_stride, fborder, mask, false, 0, s);

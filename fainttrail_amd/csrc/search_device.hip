@@ -1198,6 +1198,16 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
         // turn kept the later lanes waiting for the host. The lanes use disjoint stage
         // scratch, streams and pass slots; the device is the caller's.
         const int used = std::min(lanes, cnt);
+        if (shape.queue && grp > 0) {
+            // Under the persistent launch the builds wait for the previous group's
+            // stacking: started beside it they get a few CUs for its whole length and slow
+            // it by more than they take alone (2048^2, 256 nulls: 1.27 s per group's
+            // stacking beside builds, 1.13 s alone plus 0.12 s of builds after it).
+            const hipEvent_t prev = ev_stacked_[(grp - 1) % nsets];
+            scheck(hipStreamWaitEvent(build_stream_, prev, 0), "stream wait");
+            for (int l = 1; l < used; ++l)
+                scheck(hipStreamWaitEvent(lanes_[static_cast<size_t>(l) - 1].stream, prev, 0), "stream wait");
+        }
         if (used > 1) {
             int device = 0;
             scheck(hipGetDevice(&device), "current device");
