@@ -1278,19 +1278,15 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
             if (d_best_v) scheck(hipMemcpyAsync(d_best_v, S.best, sizeof(int) * plane, hipMemcpyDeviceToDevice, s), "copy map");
         }
         scheck(hipEventRecord(ev_stacked_[q], s), "record event");
-    };
-    // The group's peak counts to the host on the copy stream once it has stacked. Queued
-    // only after the previous group's collect: that collect synchronises the copy stream,
-    // and a wait for this group's stacking ahead of it there would hold the host (and the
-    // next group's builds) until this group had stacked.
-    auto post = [&](int grp) {
-        const int q = grp % nsets;
-        PassSet& S = sets_[q];
-        const int cnt = gstart[static_cast<size_t>(grp) + 1] - gstart[static_cast<size_t>(grp)];
-        scheck(hipStreamWaitEvent(copy_stream_, ev_stacked_[q], 0), "stream wait");
-        scheck(hipMemcpyAsync(S.h_counts, S.counts, sizeof(int) * cnt, hipMemcpyDeviceToHost, copy_stream_),
+        // The group's peak counts come back on the stacking stream itself: the copy stream
+        // never waits for another stream's work, because streams share the 4 hardware
+        // queues and such a wait would hold every stream on its queue (a build lane on the
+        // copy stream's queue would not start the next group's builds before this group
+        // had stacked). The host reads the counts after ev_counts_, so the peak downloads
+        // and the screen on the copy stream need no device-side wait.
+        scheck(hipMemcpyAsync(S.h_counts, S.counts, sizeof(int) * cnt, hipMemcpyDeviceToHost, s),
                "download peak counts");
-        scheck(hipEventRecord(ev_counts_[q], copy_stream_), "record event");
+        scheck(hipEventRecord(ev_counts_[q], s), "record event");
     };
     auto collect = [&](int grp) {
         const int q = grp % nsets;
@@ -1350,7 +1346,6 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
     for (int grp = 0; grp < groups; ++grp) {
         enqueue(grp);
         if (grp > 0) collect(grp - 1);
-        post(grp);
     }
     collect(groups - 1);
     scheck(hipStreamSynchronize(build_stream_), "stage builds");
