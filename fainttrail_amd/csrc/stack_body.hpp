@@ -187,6 +187,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    __builtin_assume(wave >= 0 && wave < NW);
     int tx, ty;
     tile_xy(rect, RING, tile, tx, ty);
     const int x0 = tx * kTileX;
@@ -242,6 +243,11 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
         // bx: the frame's box (x and y offset from the tile, extra columns, extra rows),
         // ft: the frame at the tile origin (frame i + (y0 - row0) pitch + x0)
         auto stage = [&](int i, const int (&bx)[4], const float* ft, float* dst) {
+            // the plan's extra rows are max - min of the block's vertical shifts (at most
+            // kLatSyMax for a lattice block): the first TY rows are always staged, so only
+            // the rows past them need the count test
+            __builtin_assume(bx[3] >= 0);
+            if constexpr (LAT) __builtin_assume(bx[3] <= kLatSyMax);
             const int sx = bx[2], R = TY + bx[3];
             const int bx0 = x0 + bx[0], by0 = y0 + bx[1];
             if (RING && !PADDED && !interior) {
