@@ -117,34 +117,93 @@ __device__ __forceinline__ float fma_q16(uint32_t w, bool hi, float s, float acc
     }
 }
 
-// Row class B of a lattice frame: motion (a, B) adds rows c .. c + 3 of column a. The
-// column values sit in v40 .. v71 (column a, row q in v(40 + 8 a + q)); s_set_gpr_idx_on
-// adds c to the source register number of the adds that follow, so one instruction
-// sequence serves every row offset without a branch.
-template <int B>
-__device__ __forceinline__ void lat_add_idx(float (&acc)[kVel][kRowsPerWave], const float (&col)[kLatSide][8], int c) {
+// All four row classes of a lattice frame in one GPR-indexed sequence: class B adds rows
+// c_B .. c_B + 3 of column a to motion (a, B). The column values sit in v40 .. v71 (column
+// a, row q in v(40 + 8 a + q)); s_set_gpr_idx_on adds c_B to the source register number
+// of the adds that follow, so one instruction sequence serves every row offset without a
+// branch. The index is set once and changed with s_set_gpr_idx_idx between classes (one
+// mode switch on and off per frame, not four). Each index setting writes M0 and is
+// followed by one wait state before the first indexed add: without it an add can take
+// its index from M0's previous contents (a direct-to-LDS load address), a register
+// outside the wavefront's allocation.
+__device__ __forceinline__ void lat_add_all(float (&acc)[kVel][kRowsPerWave], const float (&col)[kLatSide][8], int c0,
+                                            int c1, int c2, int c3) {
     asm volatile(
-        "s_set_gpr_idx_on %16, gpr_idx(SRC0)\n\t"
+        "s_set_gpr_idx_on %64, gpr_idx(SRC0)\n\t"
         "s_nop 0\n\t"
         "v_add_f32 %0, v40, %0\n\t"
         "v_add_f32 %1, v41, %1\n\t"
         "v_add_f32 %2, v42, %2\n\t"
         "v_add_f32 %3, v43, %3\n\t"
-        "v_add_f32 %4, v48, %4\n\t"
-        "v_add_f32 %5, v49, %5\n\t"
-        "v_add_f32 %6, v50, %6\n\t"
-        "v_add_f32 %7, v51, %7\n\t"
-        "v_add_f32 %8, v56, %8\n\t"
-        "v_add_f32 %9, v57, %9\n\t"
-        "v_add_f32 %10, v58, %10\n\t"
-        "v_add_f32 %11, v59, %11\n\t"
-        "v_add_f32 %12, v64, %12\n\t"
-        "v_add_f32 %13, v65, %13\n\t"
-        "v_add_f32 %14, v66, %14\n\t"
-        "v_add_f32 %15, v67, %15\n\t"
+        "v_add_f32 %16, v48, %16\n\t"
+        "v_add_f32 %17, v49, %17\n\t"
+        "v_add_f32 %18, v50, %18\n\t"
+        "v_add_f32 %19, v51, %19\n\t"
+        "v_add_f32 %32, v56, %32\n\t"
+        "v_add_f32 %33, v57, %33\n\t"
+        "v_add_f32 %34, v58, %34\n\t"
+        "v_add_f32 %35, v59, %35\n\t"
+        "v_add_f32 %48, v64, %48\n\t"
+        "v_add_f32 %49, v65, %49\n\t"
+        "v_add_f32 %50, v66, %50\n\t"
+        "v_add_f32 %51, v67, %51\n\t"
+        "s_set_gpr_idx_idx %65\n\t"
+        "s_nop 0\n\t"
+        "v_add_f32 %4, v40, %4\n\t"
+        "v_add_f32 %5, v41, %5\n\t"
+        "v_add_f32 %6, v42, %6\n\t"
+        "v_add_f32 %7, v43, %7\n\t"
+        "v_add_f32 %20, v48, %20\n\t"
+        "v_add_f32 %21, v49, %21\n\t"
+        "v_add_f32 %22, v50, %22\n\t"
+        "v_add_f32 %23, v51, %23\n\t"
+        "v_add_f32 %36, v56, %36\n\t"
+        "v_add_f32 %37, v57, %37\n\t"
+        "v_add_f32 %38, v58, %38\n\t"
+        "v_add_f32 %39, v59, %39\n\t"
+        "v_add_f32 %52, v64, %52\n\t"
+        "v_add_f32 %53, v65, %53\n\t"
+        "v_add_f32 %54, v66, %54\n\t"
+        "v_add_f32 %55, v67, %55\n\t"
+        "s_set_gpr_idx_idx %66\n\t"
+        "s_nop 0\n\t"
+        "v_add_f32 %8, v40, %8\n\t"
+        "v_add_f32 %9, v41, %9\n\t"
+        "v_add_f32 %10, v42, %10\n\t"
+        "v_add_f32 %11, v43, %11\n\t"
+        "v_add_f32 %24, v48, %24\n\t"
+        "v_add_f32 %25, v49, %25\n\t"
+        "v_add_f32 %26, v50, %26\n\t"
+        "v_add_f32 %27, v51, %27\n\t"
+        "v_add_f32 %40, v56, %40\n\t"
+        "v_add_f32 %41, v57, %41\n\t"
+        "v_add_f32 %42, v58, %42\n\t"
+        "v_add_f32 %43, v59, %43\n\t"
+        "v_add_f32 %56, v64, %56\n\t"
+        "v_add_f32 %57, v65, %57\n\t"
+        "v_add_f32 %58, v66, %58\n\t"
+        "v_add_f32 %59, v67, %59\n\t"
+        "s_set_gpr_idx_idx %67\n\t"
+        "s_nop 0\n\t"
+        "v_add_f32 %12, v40, %12\n\t"
+        "v_add_f32 %13, v41, %13\n\t"
+        "v_add_f32 %14, v42, %14\n\t"
+        "v_add_f32 %15, v43, %15\n\t"
+        "v_add_f32 %28, v48, %28\n\t"
+        "v_add_f32 %29, v49, %29\n\t"
+        "v_add_f32 %30, v50, %30\n\t"
+        "v_add_f32 %31, v51, %31\n\t"
+        "v_add_f32 %44, v56, %44\n\t"
+        "v_add_f32 %45, v57, %45\n\t"
+        "v_add_f32 %46, v58, %46\n\t"
+        "v_add_f32 %47, v59, %47\n\t"
+        "v_add_f32 %60, v64, %60\n\t"
+        "v_add_f32 %61, v65, %61\n\t"
+        "v_add_f32 %62, v66, %62\n\t"
+        "v_add_f32 %63, v67, %63\n\t"
         "s_set_gpr_idx_off"
-        : "+v"(acc[0+B][0]), "+v"(acc[0+B][1]), "+v"(acc[0+B][2]), "+v"(acc[0+B][3]), "+v"(acc[4+B][0]), "+v"(acc[4+B][1]), "+v"(acc[4+B][2]), "+v"(acc[4+B][3]), "+v"(acc[8+B][0]), "+v"(acc[8+B][1]), "+v"(acc[8+B][2]), "+v"(acc[8+B][3]), "+v"(acc[12+B][0]), "+v"(acc[12+B][1]), "+v"(acc[12+B][2]), "+v"(acc[12+B][3])
-        : "s"(c), "{v40}"(col[0][0]), "{v41}"(col[0][1]), "{v42}"(col[0][2]), "{v43}"(col[0][3]), "{v44}"(col[0][4]), "{v45}"(col[0][5]), "{v46}"(col[0][6]), "{v47}"(col[0][7]), "{v48}"(col[1][0]), "{v49}"(col[1][1]), "{v50}"(col[1][2]), "{v51}"(col[1][3]), "{v52}"(col[1][4]), "{v53}"(col[1][5]), "{v54}"(col[1][6]), "{v55}"(col[1][7]), "{v56}"(col[2][0]), "{v57}"(col[2][1]), "{v58}"(col[2][2]), "{v59}"(col[2][3]), "{v60}"(col[2][4]), "{v61}"(col[2][5]), "{v62}"(col[2][6]), "{v63}"(col[2][7]), "{v64}"(col[3][0]), "{v65}"(col[3][1]), "{v66}"(col[3][2]), "{v67}"(col[3][3]), "{v68}"(col[3][4]), "{v69}"(col[3][5]), "{v70}"(col[3][6]), "{v71}"(col[3][7])
+        : "+v"(acc[0][0]), "+v"(acc[0][1]), "+v"(acc[0][2]), "+v"(acc[0][3]), "+v"(acc[1][0]), "+v"(acc[1][1]), "+v"(acc[1][2]), "+v"(acc[1][3]), "+v"(acc[2][0]), "+v"(acc[2][1]), "+v"(acc[2][2]), "+v"(acc[2][3]), "+v"(acc[3][0]), "+v"(acc[3][1]), "+v"(acc[3][2]), "+v"(acc[3][3]), "+v"(acc[4][0]), "+v"(acc[4][1]), "+v"(acc[4][2]), "+v"(acc[4][3]), "+v"(acc[5][0]), "+v"(acc[5][1]), "+v"(acc[5][2]), "+v"(acc[5][3]), "+v"(acc[6][0]), "+v"(acc[6][1]), "+v"(acc[6][2]), "+v"(acc[6][3]), "+v"(acc[7][0]), "+v"(acc[7][1]), "+v"(acc[7][2]), "+v"(acc[7][3]), "+v"(acc[8][0]), "+v"(acc[8][1]), "+v"(acc[8][2]), "+v"(acc[8][3]), "+v"(acc[9][0]), "+v"(acc[9][1]), "+v"(acc[9][2]), "+v"(acc[9][3]), "+v"(acc[10][0]), "+v"(acc[10][1]), "+v"(acc[10][2]), "+v"(acc[10][3]), "+v"(acc[11][0]), "+v"(acc[11][1]), "+v"(acc[11][2]), "+v"(acc[11][3]), "+v"(acc[12][0]), "+v"(acc[12][1]), "+v"(acc[12][2]), "+v"(acc[12][3]), "+v"(acc[13][0]), "+v"(acc[13][1]), "+v"(acc[13][2]), "+v"(acc[13][3]), "+v"(acc[14][0]), "+v"(acc[14][1]), "+v"(acc[14][2]), "+v"(acc[14][3]), "+v"(acc[15][0]), "+v"(acc[15][1]), "+v"(acc[15][2]), "+v"(acc[15][3])
+        : "s"(c0), "s"(c1), "s"(c2), "s"(c3), "{v40}"(col[0][0]), "{v41}"(col[0][1]), "{v42}"(col[0][2]), "{v43}"(col[0][3]), "{v44}"(col[0][4]), "{v45}"(col[0][5]), "{v46}"(col[0][6]), "{v47}"(col[0][7]), "{v48}"(col[1][0]), "{v49}"(col[1][1]), "{v50}"(col[1][2]), "{v51}"(col[1][3]), "{v52}"(col[1][4]), "{v53}"(col[1][5]), "{v54}"(col[1][6]), "{v55}"(col[1][7]), "{v56}"(col[2][0]), "{v57}"(col[2][1]), "{v58}"(col[2][2]), "{v59}"(col[2][3]), "{v60}"(col[2][4]), "{v61}"(col[2][5]), "{v62}"(col[2][6]), "{v63}"(col[2][7]), "{v64}"(col[3][0]), "{v65}"(col[3][1]), "{v66}"(col[3][2]), "{v67}"(col[3][3]), "{v68}"(col[3][4]), "{v69}"(col[3][5]), "{v70}"(col[3][6]), "{v71}"(col[3][7])
         : "m0");
 }
 
@@ -349,10 +408,7 @@ __device__ __forceinline__ void stack_body(const StackArgs& a, int tile, int seg
                         col[c][7] = q[c][7 * kPitch];
                     }
                 }
-                lat_add_idx<0>(acc, col, wy & 0xf);
-                lat_add_idx<1>(acc, col, (wy >> 4) & 0xf);
-                lat_add_idx<2>(acc, col, (wy >> 8) & 0xf);
-                lat_add_idx<3>(acc, col, (wy >> 12) & 0xf);
+                lat_add_all(acc, col, wy & 0xf, (wy >> 4) & 0xf, (wy >> 8) & 0xf, (wy >> 12) & 0xf);
             } else {
                 cint* lo = c_off + (static_cast<long long>(blk) * n + i) * V;
 #pragma unroll
