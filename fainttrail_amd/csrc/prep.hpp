@@ -57,9 +57,15 @@ public:
     // The same with the time ranks already on the device (d_rank: n ints, time_ranks of
     // the frame times). Enqueues work only (no host copy) once the filter taps for
     // prm.fwhm_px are in place, so several builds can be queued on several streams.
+    // defer_scale: the frames are left unnormalised, for copy_scaled to normalise while it
+    // copies them into their stacking layout (one pass over the frames instead of two).
     void build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
                       float* filtered, long long filt_stride, float* static_sky, double* sigma_host, float* mask_out,
-                      int* cr_host, hipStream_t s, float* resid_out = nullptr);
+                      int* cr_host, hipStream_t s, float* resid_out = nullptr, bool defer_scale = false);
+    // The last build_ranked(defer_scale) frames (src, src_stride floats apart, w-float rows)
+    // normalised into dst (frame i at dst + i dst_stride, rows dst_pitch apart), on s.
+    void copy_scaled(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
+                     hipStream_t s);
     // Matched-filter taps for prm.fwhm_px (uploaded when they change; synchronises s then).
     void set_taps(const StageParams& prm, hipStream_t s);
     // Device bytes of the stage's scratch (the sizes asked of hipMalloc).

@@ -286,6 +286,82 @@ __global__ __launch_bounds__(64) void scatter_kernel(const float* __restrict__ r
     scatter[p] = static_cast<float>(1.4826 * med);
 }
 
+// A bitonic sorting network over a register array (compile-time indices only, so the array
+// stays in registers): ascending order by min / max, exact for values without NaN.
+template <int N, int K, int J>
+__device__ __forceinline__ void bitonic_pass(float (&u)[N]) {
+    static_for<0, N>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int l = i ^ J;
+        if constexpr (l > i) {
+            const float a = u[i], b = u[l];
+            if constexpr ((i & K) == 0) {
+                u[i] = __builtin_fminf(a, b);
+                u[l] = __builtin_fmaxf(a, b);
+            } else {
+                u[i] = __builtin_fmaxf(a, b);
+                u[l] = __builtin_fminf(a, b);
+            }
+        }
+    });
+}
+template <int N, int K, int J>
+__device__ __forceinline__ void bitonic_merge(float (&u)[N]) {
+    if constexpr (J > 0) {
+        bitonic_pass<N, K, J>(u);
+        bitonic_merge<N, K, J / 2>(u);
+    }
+}
+template <int N, int K = 2>
+__device__ __forceinline__ void bitonic_sort(float (&u)[N]) {
+    if constexpr (K <= N) {
+        bitonic_merge<N, K, K / 2>(u);
+        bitonic_sort<N, K * 2>(u);
+    }
+}
+
+// scatter_kernel for n <= NMAX (a power of two, at most 64): the pixel's values
+// float(|r_i| / sigma_i) in registers, +inf in the slots of non-finite or missing samples
+// (the finite values can be +inf only by overflow, and then sort with them to the top of
+// the c valid ones), sorted by the network; the middle elements are then picked with
+// compile-time indices. Same values and the same double arithmetic as scatter_kernel.
+template <int NMAX>
+__global__ __launch_bounds__(64) void scatter_sort_kernel(const float* __restrict__ resid, long long rstride, int n,
+                                                          long long plane, const double* __restrict__ sigma,
+                                                          float* __restrict__ scatter) {
+    const long long p = static_cast<long long>(blockIdx.x) * 64 + threadIdx.x;
+    if (p >= plane) return;
+    float u[NMAX];
+    int c = 0;
+    static_for<0, NMAX>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        float x = __builtin_inff();
+        if (i < n) {
+            const float v = resid[i * rstride + p];
+            if (finitef(v)) {
+                x = static_cast<float>(static_cast<double>(__builtin_fabsf(v)) / sigma[i]);
+                ++c;
+            }
+        }
+        u[i] = x;
+    });
+    if (c < n / 2) {
+        scatter[p] = 1e9f;
+        return;
+    }
+    bitonic_sort(u);
+    const int mid = c / 2;
+    float vmid = 0.0f, vlow = 0.0f;
+    static_for<0, NMAX>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        vmid = k == mid ? u[k] : vmid;
+        vlow = k == mid - 1 ? u[k] : vlow;
+    });
+    double med = vmid;
+    if (c % 2 == 0) med = 0.5 * (med + static_cast<double>(vlow));
+    scatter[p] = static_cast<float>(1.4826 * med);
+}
+
 // mask = 0 within dil pixels (square) of any pixel whose scatter exceeds mask_scatter
 // (preprocess.cpp:393-400; the comparison is float against double).
 __global__ void mask_kernel(const float* __restrict__ scatter, int w, int h, int dil, double mask_scatter,
@@ -556,6 +632,19 @@ __global__ void scale_kernel(float* __restrict__ g, long long gstride, long long
     if (p >= plane) return;
     const double sg = seg[i].result;
     if (sg > 0.0) g[i * gstride + p] *= static_cast<float>(1.0 / sg);
+}
+
+// scale_kernel's normalisation applied while copying the frames into another layout:
+// row y of frame i goes to dst + i dstride + y dpitch, times float(1 / sg) where sg > 0 (the
+// same float product as scaling in place, then copying).
+__global__ void scale_copy_kernel(const float* __restrict__ src, long long sstride, float* __restrict__ dst,
+                                  long long dstride, int dpitch, int w, int y0, const SelectSeg* __restrict__ seg) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = y0 + blockIdx.y, i = blockIdx.z;
+    if (x >= w) return;
+    const double sg = seg[i].result;
+    float v = src[i * sstride + static_cast<long long>(y) * w + x];
+    if (sg > 0.0) v *= static_cast<float>(1.0 / sg);
+    dst[i * dstride + static_cast<long long>(y) * dpitch + x] = v;
 }
 
 // sub = float(v - bg) for finite v, NaN otherwise (preprocess.cpp:233-236)
@@ -874,6 +963,14 @@ void DeviceStage::background(const float* raw, long long raw_stride, float* sub,
     }
 }
 
+void DeviceStage::copy_scaled(const float* src, long long src_stride, float* dst, long long dst_stride, int dst_pitch,
+                              hipStream_t s) {
+    for (int y0 = 0; y0 < h; y0 += 32768)
+        hipLaunchKernelGGL(scale_copy_kernel, dim3(blocks_for(w, 256), std::min(32768, h - y0), n), dim3(256), 0, s, src,
+                           src_stride, dst, dst_stride, dst_pitch, w, y0, seg_);
+    hcheck(hipGetLastError(), "scale and copy frames");
+}
+
 void time_ranks(const double* t_h, int n, int* rank) {
     // stable sort of the frame times (preprocess.cpp:319-331)
     std::vector<int> order(static_cast<size_t>(n));
@@ -917,7 +1014,7 @@ void DeviceStage::set_taps(const StageParams& prm, hipStream_t s) {
 
 void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int* d_rank, const StageParams& prm,
                                float* filtered, long long filt_stride, float* static_sky, double* sigma_host,
-                               float* mask_out, int* cr_host, hipStream_t s, float* resid_out) {
+                               float* mask_out, int* cr_host, hipStream_t s, float* resid_out, bool defer_scale) {
     const int exclude = prm.static_exclude >= 0 ? prm.static_exclude : std::max(1, static_cast<int>(std::lround(n / 8.0)));
     const int min_finite = static_cast<int>(std::max<size_t>(5, static_cast<size_t>(0.6 * n)));
     const double psf_sigma = prm.fwhm_px * kFwhmToSigmaD;
@@ -944,8 +1041,13 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
     select(resid_, plane, 2, nullptr, true, 1, s);
     hipLaunchKernelGGL(inv_sigma_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, s, seg_, n, inv_, sigma_);
     // scatter map and mask
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(plane, 64)), dim3(64), static_cast<size_t>(n) * 64 * 4, s,
-                       resid_, plane, n, plane, sigma_, scatter_);
+    if (n <= 64) {
+        auto sc_fn = n <= 16 ? scatter_sort_kernel<16> : n <= 32 ? scatter_sort_kernel<32> : scatter_sort_kernel<64>;
+        hipLaunchKernelGGL(sc_fn, dim3(blocks_for(plane, 64)), dim3(64), 0, s, resid_, plane, n, plane, sigma_, scatter_);
+    } else {
+        hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(plane, 64)), dim3(64), static_cast<size_t>(n) * 64 * 4, s,
+                           resid_, plane, n, plane, sigma_, scatter_);
+    }
     const int dil = std::max(1, static_cast<int>(std::lround(0.5 * prm.fwhm_px)));
     float* mask = mask_out ? mask_out : mask_;
     hipLaunchKernelGGL(mask_kernel, dim3(blocks_for(w, 128), h), dim3(128), 0, s, scatter_, w, h, dil,
@@ -984,7 +1086,9 @@ void DeviceStage::build_ranked(const float* sub, long long sub_stride, const int
     const int fborder = static_cast<int>(std::ceil(3 * psf_sigma)) + 1;
     select(filtered, filt_stride, fborder, mask, false, 0, s);
     select(filtered, filt_stride, fborder, mask, true, 0, s);
-    hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, filtered, filt_stride, plane, seg_);
+    if (!defer_scale)
+        hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(plane, 256), n), dim3(256), 0, s, filtered, filt_stride, plane,
+                           seg_);
     hcheck(hipGetLastError(), "build_stage kernels");
     if (sigma_host || cr_host) {
         if (sigma_host) hcheck(hipMemcpyAsync(sigma_host, sigma_, sizeof(double) * n, hipMemcpyDeviceToHost, s), "download sigma");

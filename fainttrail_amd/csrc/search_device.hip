@@ -1185,9 +1185,10 @@ SearchOutput DeviceSearch::run_batched(const float* raw, long long stride, const
                                     S.maxb + static_cast<long long>(g) * n, S.scl + static_cast<long long>(g) * n, bs),
                        "store frames");
             } else if (use_pad) {
+                // normalised while copied into the padded layout
                 stg.build_ranked(sub_, plane, rk, prm, scratch, plane, sky, nullptr, nullptr,
-                                 nullptr, bs);
-                scheck(launch_copy_rows(scratch, plane, slot + forigin, fs, pad.pitch, w, h, n, bs), "pad frames");
+                                 nullptr, bs, nullptr, true);
+                stg.copy_scaled(scratch, plane, slot + forigin, fs, pad.pitch, bs);
             } else {
                 stg.build_ranked(sub_, plane, rk, prm, slot, plane, sky, nullptr, nullptr,
                                  nullptr, bs);
