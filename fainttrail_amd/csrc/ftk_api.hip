@@ -102,7 +102,11 @@ struct DeviceBuffer {
     }
     void upload(const std::vector<T>& v, const char* what) {
         reserve(v.size(), what);
-        if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+        if (v.empty()) return;
+        check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+        // a copy from pageable memory may return before its DMA has landed on the null
+        // stream, which the launches' non-blocking streams do not wait for
+        check(hipStreamSynchronize(nullptr), what);
     }
 };
 
@@ -359,7 +363,9 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
                               "upload frames");
                     }
                 }
-                if (use_pad) check(hipStreamSynchronize(nullptr), "pad frames");
+                // the row copies (and the direct uploads, whose DMA may land after
+                // hipMemcpy returns) are on the null stream: complete before the launches
+                check(hipStreamSynchronize(nullptr), "upload frames");
                 up_to = r;
                 t_up += std::chrono::duration<double>(clk::now() - u0).count();
             };
@@ -477,6 +483,7 @@ extern "C" int ftk_plan_create(int n, int w, int h, int nv, const int32_t* dx, c
                 const size_t count = static_cast<size_t>(n) * pl->pad.rows * pl->pad.pitch;
                 pl->padded.reserve(count, "allocate padded frames");
                 check(hipMemset(pl->padded.p, 0, count * sizeof(float)), "clear padded frames");
+                check(hipStreamSynchronize(nullptr), "clear padded frames");  // before non-blocking streams read it
             }
         } catch (...) {
             delete pl;

@@ -905,6 +905,9 @@ DeviceStage::DeviceStage(int n_, int w_, int h_) : n(n_), w(w_), h(h_), plane(st
     seg_ = dalloc<SelectSeg>(n, "allocate selection state");
     hist_ = dalloc<unsigned int>(static_cast<size_t>(n) * 512, "allocate histograms");
     hcheck(hipMemset(hist_, 0, sizeof(unsigned int) * n * 512), "clear histograms");
+    // the clear runs on the null stream, which the builds' non-blocking streams do not
+    // wait for: it is complete before any of them can read the histograms
+    hcheck(hipStreamSynchronize(nullptr), "clear histograms");
     inv_ = dalloc<float>(n, "allocate 1/sigma");
     sigma_ = dalloc<double>(n, "allocate sigma");
     cr_ = dalloc<int>(n, "allocate cosmic-ray counts");
@@ -985,6 +988,7 @@ void DeviceStage::build(const float* sub, long long sub_stride, const double* t_
     std::vector<int> rank(static_cast<size_t>(n));
     time_ranks(t_h, n, rank.data());
     hcheck(hipMemcpyAsync(rank_, rank.data(), sizeof(int) * n, hipMemcpyHostToDevice, s), "upload ranks");
+    hcheck(hipStreamSynchronize(s), "upload ranks");  // `rank` is read before it goes
     build_ranked(sub, sub_stride, rank_, prm, filtered, filt_stride, static_sky, sigma_host, mask_out, cr_host, s,
                  resid_out);
 }
@@ -1005,10 +1009,13 @@ void DeviceStage::set_taps(const StageParams& prm, hipStream_t s) {
     for (size_t q = 0; q < taps.size(); ++q) taps_norm[q] = taps[q] * norm;
     std::vector<float> both(taps);
     both.insert(both.end(), taps_norm.begin(), taps_norm.end());
-    // earlier builds on this stream may still read the old taps; the copy is complete
-    // when it returns, so builds on any stream see the new ones
+    // earlier builds on this stream may still read the old taps. A copy from pageable
+    // memory may return before its DMA has landed, and it runs on the null stream, which
+    // non-blocking streams do not wait for: the null stream is synchronised before any
+    // build can read the taps
     hcheck(hipStreamSynchronize(s), "build_stage");
     hcheck(hipMemcpy(taps_, both.data(), sizeof(float) * both.size(), hipMemcpyHostToDevice), "upload taps");
+    hcheck(hipStreamSynchronize(nullptr), "upload taps");
     taps_sigma_ = psf_sigma;
 }
 
