@@ -340,6 +340,7 @@ extern "C" int ftk_stack_pass(const ftk_pass* pass, const ftk_options* opt_in, f
             const size_t chunk_cap = static_cast<size_t>(rows) * w;
             if (use_pad) {
                 check(hipMemset(d_frames.p, 0, sizeof(float) * n * stride), "clear frames");
+                check(hipStreamSynchronize(nullptr), "clear frames");  // before non-blocking streams read it
                 d_stage.reserve(chunk_cap, "allocate upload buffer");
             }
             int up_to = b.r0;  // frame rows [b.r0, up_to) are on the device
